@@ -1,0 +1,534 @@
+"""agent-ray on MI355X: host-side Python mirror of the reference renderer's
+scene / camera / integrator API, driving libagentray.so (hand-written gfx950
+HIP kernels behind a C ABI, ``include/agentray.h``).
+
+Names follow the reference (shiinamiyuki/agent-ray):
+
+* ``TriangleMesh``, ``SceneObject``, ``Scene``            (src/scene.rs, src/primitives/mesh.rs)
+* ``Lambertian``, ``ConductorBsdf``, ``DielectricBsdf``   (src/surfaces/*)
+* ``PointLight``, ``QuadLight``                            (src/lights/*; QuadLight is an extension)
+* ``PinholeCamera.from_lookat / from_eye_angle``           (src/cameras.rs)
+* ``PathTracer(PathTracerConfig)``                         (src/integrators/path_tracer.rs)
+* ``BidirectionalPathTracer(BdptConfig)``, ``MisMode``     (src/integrators/bidirectional_path_tracer.rs)
+* ``GradientDomainPathTracer(GdptConfig)``                 (src/integrators/gradient_domain_pt.rs)
+* ``Film``                                                 (src/film.rs)
+
+Every compute call goes through the HIP library; there is no CPU fallback.
+If the shared library is missing the import raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from enum import IntEnum
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import workloads  # noqa: F401  (re-exported)
+from .workloads import SceneDesc
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libagentray.so")
+
+AR_OK = 0
+AR_HOST_PTRS = 0x1
+AR_ASYNC = 0x2
+AR_REF_ORDER = 0x4
+NO_HIT = 0xFFFFFFFF
+
+
+class ArError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"agentray error {code}: {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------------------
+# ctypes mirrors of include/agentray.h
+# ---------------------------------------------------------------------------
+class ArMaterial(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("albedo", C.c_float * 3), ("roughness", C.c_float), ("eta", C.c_float)]
+
+
+class ArLight(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("position", C.c_float * 3), ("intensity", C.c_float * 3),
+                ("edge_u", C.c_float * 3), ("edge_v", C.c_float * 3)]
+
+
+class ArCamera(C.Structure):
+    _fields_ = [("origin", C.c_float * 3), ("lower_left", C.c_float * 3), ("horizontal", C.c_float * 3),
+                ("vertical", C.c_float * 3), ("forward", C.c_float * 3), ("area", C.c_float)]
+
+
+class ArRenderConfig(C.Structure):
+    _fields_ = [("integrator", C.c_uint32), ("spp", C.c_uint32), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("seed", C.c_uint64), ("schedule", C.c_uint32),
+                ("sample_chunk", C.c_uint32), ("mis_mode", C.c_uint32), ("mis_beta", C.c_float),
+                ("alpha", C.c_float), ("poisson_iterations", C.c_uint32), ("poisson_sign", C.c_int32),
+                ("gdpt_outputs", C.c_uint32)]
+
+
+class ArRenderStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("paths", C.c_uint64), ("rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("queue_overflow", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class ArSceneInfo(C.Structure):
+    _fields_ = [("n_objects", C.c_uint32), ("n_triangles", C.c_uint32), ("n_nodes", C.c_uint32),
+                ("n_lights", C.c_uint32), ("max_depth", C.c_uint32), ("device_bytes", C.c_uint64)]
+
+
+class ArServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_jobs", C.c_uint32), ("path_slots", C.c_uint32)]
+
+
+class ArJobDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("camera", ArCamera), ("config", ArRenderConfig)]
+
+
+class ArJobStats(C.Structure):
+    _fields_ = [("submit_time", C.c_double), ("start_time", C.c_double), ("finish_time", C.c_double),
+                ("paths", C.c_uint64), ("rays", C.c_uint64), ("state", C.c_int32)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C agent-ray_amd/csrc)")
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    u32, u64, i32, f = C.c_uint32, C.c_uint64, C.c_int32, C.c_float
+    sig = {
+        "ar_last_error": (C.c_char_p, []),
+        "ar_version": (C.c_char_p, []),
+        "ar_init": (i32, [C.c_int]),
+        "ar_scene_create": (i32, [C.POINTER(P)]),
+        "ar_scene_destroy": (i32, [P]),
+        "ar_scene_add_material": (i32, [P, C.POINTER(ArMaterial), C.POINTER(u32)]),
+        "ar_scene_add_mesh": (i32, [P, P, u32, P, u32, P, P, u32, P, u32, P, C.POINTER(u32)]),
+        "ar_scene_add_light": (i32, [P, C.POINTER(ArLight), C.POINTER(u32)]),
+        "ar_scene_set_light_distribution": (i32, [P, u32]),
+        "ar_scene_commit": (i32, [P]),
+        "ar_scene_get_info": (i32, [P, C.POINTER(ArSceneInfo)]),
+        "ar_trace": (i32, [P, P, u64, P, P, u32, P]),
+        "ar_occluded": (i32, [P, P, u64, P, u32, P]),
+        "ar_camera_lookat": (i32, [P, P, P, f, f, C.POINTER(ArCamera)]),
+        "ar_camera_eye_angle": (i32, [P, f, f, f, f, C.POINTER(ArCamera)]),
+        "ar_render": (i32, [P, C.POINTER(ArCamera), u32, u32, C.POINTER(ArRenderConfig), P, P,
+                            C.POINTER(ArRenderStats), u32, P]),
+        "ar_poisson_solve": (i32, [P, P, P, u32, u32, f, u32, i32, P, C.POINTER(C.c_double), u32, P]),
+        "ar_server_create": (i32, [P, C.POINTER(ArServerConfig), C.POINTER(P)]),
+        "ar_server_destroy": (i32, [P]),
+        "ar_job_submit": (i32, [P, C.POINTER(ArJobDesc), C.POINTER(u64)]),
+        "ar_job_poll": (i32, [P, u64, C.POINTER(i32)]),
+        "ar_job_wait": (i32, [P, u64, P, C.POINTER(ArJobStats)]),
+        "ar_job_cancel": (i32, [P, u64]),
+        "ar_server_clock": (C.c_double, [P]),
+        "ar_bvh_build_host": (i32, [P, u32, P, P, P, C.POINTER(u32), C.POINTER(u32)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = _load()
+_initialized = False
+
+
+def lib():
+    return _lib
+
+
+def _check(code):
+    if code != AR_OK:
+        raise ArError(code, _lib.ar_last_error().decode(errors="replace"))
+
+
+def init(device: int = 0):
+    global _initialized
+    if not _initialized:
+        _check(_lib.ar_init(device))
+        _initialized = True
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# ---------------------------------------------------------------------------
+# Scene description mirrors of the reference types
+# ---------------------------------------------------------------------------
+@dataclass
+class Lambertian:
+    """surfaces/lambertian.rs:17-65 (constant albedo texture)."""
+    albedo: Sequence[float]
+
+    def to_c(self):
+        return ArMaterial(0, (C.c_float * 3)(*self.albedo), 1.0, 1.0)
+
+
+@dataclass
+class ConductorBsdf:
+    """surfaces/microfacet.rs:86-153."""
+    f0: Sequence[float]
+    roughness: float
+
+    def to_c(self):
+        return ArMaterial(1, (C.c_float * 3)(*self.f0), self.roughness, 1.0)
+
+
+@dataclass
+class DielectricBsdf:
+    """surfaces/microfacet.rs:163-307."""
+    eta: float
+    roughness: float
+
+    def to_c(self):
+        return ArMaterial(2, (C.c_float * 3)(1.0, 1.0, 1.0), self.roughness, self.eta)
+
+
+def _mat_from_desc(m):
+    if m.type == 0:
+        return Lambertian(m.albedo)
+    if m.type == 1:
+        return ConductorBsdf(m.albedo, m.roughness)
+    return DielectricBsdf(m.eta, m.roughness)
+
+
+@dataclass
+class PointLight:
+    """lights/point.rs:22-100."""
+    position: Sequence[float]
+    intensity: Sequence[float]
+
+    def to_c(self):
+        z = (C.c_float * 3)(0, 0, 0)
+        return ArLight(0, (C.c_float * 3)(*self.position), (C.c_float * 3)(*self.intensity), z, z)
+
+
+@dataclass
+class QuadLight:
+    """One-sided quad area light (extension); emits `radiance` along
+    normalize(cross(edge_u, edge_v)) and adds its own geometry."""
+    corner: Sequence[float]
+    edge_u: Sequence[float]
+    edge_v: Sequence[float]
+    radiance: Sequence[float]
+
+    def to_c(self):
+        return ArLight(1, (C.c_float * 3)(*self.corner), (C.c_float * 3)(*self.radiance),
+                       (C.c_float * 3)(*self.edge_u), (C.c_float * 3)(*self.edge_v))
+
+
+@dataclass
+class TriangleMesh:
+    """primitives/mesh.rs:7-16."""
+    positions: np.ndarray
+    indices: np.ndarray
+    normals: Optional[np.ndarray] = None
+    material_slots: Optional[np.ndarray] = None
+
+
+@dataclass
+class SceneObject:
+    """scene.rs:16-35: mesh, local-to-world transform (column-major 4x4), BSDFs by slot."""
+    mesh: TriangleMesh
+    transform: Optional[np.ndarray]
+    materials: List[object]
+
+
+LIGHTDIST = {"none": 0, "uniform": 1, "power": 2}
+
+
+class Scene:
+    """scene.rs:113-168.  Builds the BLASes and TLAS on the host with the
+    reference's SAH, uploads the flattened scene to HBM and keeps it there."""
+
+    def __init__(self, objects: List[SceneObject], lights: List[object], light_dist: str = "power"):
+        init()
+        self._h = C.c_void_p()
+        _check(_lib.ar_scene_create(C.byref(self._h)))
+        self._keep = []
+        mat_ids = {}
+        for obj in objects:
+            ids = []
+            for m in obj.materials:
+                key = id(m)
+                if key not in mat_ids:
+                    out = C.c_uint32()
+                    _check(_lib.ar_scene_add_material(self._h, C.byref(m.to_c()), C.byref(out)))
+                    mat_ids[key] = out.value
+                ids.append(mat_ids[key])
+            self._add_mesh(obj.mesh, ids, obj.transform)
+        for l in lights:
+            _check(_lib.ar_scene_add_light(self._h, C.byref(l.to_c()), None))
+        _check(_lib.ar_scene_set_light_distribution(self._h, LIGHTDIST[light_dist]))
+        _check(_lib.ar_scene_commit(self._h))
+        self.info = ArSceneInfo()
+        _check(_lib.ar_scene_get_info(self._h, C.byref(self.info)))
+
+    def _add_mesh(self, mesh: TriangleMesh, mat_ids, transform):
+        pos = np.ascontiguousarray(mesh.positions, np.float32).reshape(-1, 3)
+        idx = np.ascontiguousarray(mesh.indices, np.uint32).reshape(-1, 3)
+        nrm = None if mesh.normals is None else np.ascontiguousarray(mesh.normals, np.float32).reshape(-1, 3)
+        slots = np.zeros(1, np.uint32) if mesh.material_slots is None else \
+            np.ascontiguousarray(mesh.material_slots, np.uint32).reshape(-1)
+        mats = np.asarray(mat_ids, np.uint32)
+        xf = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+        _check(_lib.ar_scene_add_mesh(self._h, _ptr(pos), pos.shape[0], _ptr(idx), idx.shape[0], _ptr(nrm),
+                                      _ptr(slots), slots.shape[0], _ptr(mats), mats.shape[0], _ptr(xf), None))
+
+    @classmethod
+    def from_desc(cls, desc: SceneDesc) -> "Scene":
+        mats = [_mat_from_desc(m) for m in desc.materials]
+        objs = []
+        for m in desc.meshes:
+            objs.append(SceneObject(TriangleMesh(m.positions, m.indices, m.normals, m.slots), m.transform,
+                                    [mats[i] for i in m.materials]))
+        lights = []
+        for l in desc.lights:
+            if l.type == 0:
+                lights.append(PointLight(l.position, l.intensity))
+            else:
+                lights.append(QuadLight(l.position, l.edge_u, l.edge_v, l.intensity))
+        return cls(objs, lights, {0: "none", 1: "uniform", 2: "power"}[desc.light_dist])
+
+    @property
+    def handle(self):
+        return self._h
+
+    def intersect(self, rays, ref_order: bool = False):
+        """TLAS::intersect over a batch (bvh.rs:413-447).  `rays`: (n, 8) float32
+        (origin, t_min, unit dir, t_max) as numpy (host) or a torch CUDA tensor.
+        Returns (t, prim_id, inst_id)."""
+        flags = AR_REF_ORDER if ref_order else 0
+        if _is_torch_cuda(rays):
+            import torch
+            n = rays.shape[0]
+            hits = torch.empty((n, 2), dtype=torch.float32, device=rays.device)
+            inst = torch.empty((n,), dtype=torch.int32, device=rays.device)
+            _check(_lib.ar_trace(self._h, C.c_void_p(rays.data_ptr()), n, C.c_void_p(hits.data_ptr()),
+                                 C.c_void_p(inst.data_ptr()), flags, None))
+            return hits[:, 0], hits[:, 1].view(torch.int32), inst
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = r.shape[0]
+        hits = np.empty((n, 2), np.float32)
+        inst = np.empty(n, np.uint32)
+        _check(_lib.ar_trace(self._h, _ptr(r), n, _ptr(hits), _ptr(inst), flags | AR_HOST_PTRS, None))
+        return hits[:, 0].copy(), hits[:, 1].view(np.uint32).copy(), inst
+
+    def occluded(self, rays, ref_order: bool = False):
+        """Scene::occluded (scene.rs:166-168) as an any-hit query."""
+        flags = AR_REF_ORDER if ref_order else 0
+        if _is_torch_cuda(rays):
+            import torch
+            n = rays.shape[0]
+            occ = torch.empty((n,), dtype=torch.int32, device=rays.device)
+            _check(_lib.ar_occluded(self._h, C.c_void_p(rays.data_ptr()), n, C.c_void_p(occ.data_ptr()), flags,
+                                    None))
+            return occ
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        occ = np.empty(r.shape[0], np.uint32)
+        _check(_lib.ar_occluded(self._h, _ptr(r), r.shape[0], _ptr(occ), flags | AR_HOST_PTRS, None))
+        return occ.astype(bool)
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.ar_scene_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+
+def _is_torch_cuda(x):
+    try:
+        import torch
+        return isinstance(x, torch.Tensor) and x.is_cuda
+    except ImportError:
+        return False
+
+
+# ---------------------------------------------------------------------------
+# Camera
+# ---------------------------------------------------------------------------
+class PinholeCamera:
+    """cameras.rs:78-249."""
+
+    def __init__(self, c: ArCamera):
+        self.c = c
+
+    @classmethod
+    def from_lookat(cls, lookfrom, lookat, vup, vfov, aspect_ratio):
+        c = ArCamera()
+        a = [np.asarray(v, np.float32) for v in (lookfrom, lookat, vup)]
+        _check(_lib.ar_camera_lookat(_ptr(a[0]), _ptr(a[1]), _ptr(a[2]), vfov, aspect_ratio, C.byref(c)))
+        return cls(c)
+
+    @classmethod
+    def from_eye_angle(cls, eye, yaw, pitch, vfov, aspect_ratio):
+        c = ArCamera()
+        e = np.asarray(eye, np.float32)
+        _check(_lib.ar_camera_eye_angle(_ptr(e), yaw, pitch, vfov, aspect_ratio, C.byref(c)))
+        return cls(c)
+
+    @classmethod
+    def from_array(cls, cam16):
+        c = ArCamera()
+        C.memmove(C.addressof(c), np.ascontiguousarray(cam16, np.float32).ctypes.data, 64)
+        return cls(c)
+
+    def to_array(self) -> np.ndarray:
+        return np.frombuffer(bytes(self.c), dtype=np.float32).copy()
+
+
+def camera_for(desc: SceneDesc) -> PinholeCamera:
+    cd = desc.camera
+    return PinholeCamera.from_array(
+        workloads.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, desc.width / desc.height))
+
+
+# ---------------------------------------------------------------------------
+# Film and integrators
+# ---------------------------------------------------------------------------
+class Film:
+    """film.rs:190-274: linear HDR RGB, row-major, top row first."""
+
+    def __init__(self, width, height, pixels: np.ndarray):
+        self.width, self.height = width, height
+        self.pixels = pixels.reshape(height, width, 3)
+
+    def get_pixel(self, x, y):
+        return self.pixels[y, x]
+
+    def to_hdr_vec(self):
+        return self.pixels.reshape(-1, 3)
+
+
+class Schedule(IntEnum):
+    WAVEFRONT = 0
+    MEGAKERNEL = 1
+
+
+@dataclass
+class PathTracerConfig:
+    """path_tracer.rs:18-31."""
+    spp: int = 64
+    max_depth: int = 8
+    rr_depth: int = 3
+
+
+class MisMode(IntEnum):
+    """bidirectional_path_tracer.rs:91-99."""
+    Power = 0
+    Uniform = 1
+
+
+@dataclass
+class BdptConfig:
+    """bidirectional_path_tracer.rs:102-133 (debug_strategy_images is not offered)."""
+    spp: int = 64
+    max_depth: int = 8
+    rr_depth: int = 3
+    mis_mode: MisMode = MisMode.Power
+    mis_beta: float = 2.0
+
+
+@dataclass
+class GdptConfig:
+    """gradient_domain_pt.rs:47-76; the Poisson solve is conjugate gradient
+    (poisson_iterations CG steps) instead of SOR, `sor_omega` is unused."""
+    spp: int = 64
+    max_depth: int = 8
+    rr_depth: int = 3
+    alpha: float = 0.2
+    poisson_iterations: int = 50
+    sor_omega: float = 1.6
+    poisson_sign: int = 1
+
+
+class _Integrator:
+    integrator_id = 0
+
+    def __init__(self, config, seed: int = 0, schedule: Schedule = Schedule.WAVEFRONT, sample_chunk: int = 0):
+        self.config = config
+        self.seed = seed
+        self.schedule = schedule
+        self.sample_chunk = sample_chunk
+        self.last_stats = None
+        self.aux = None
+
+    def _cfg(self):
+        c = ArRenderConfig()
+        c.integrator = self.integrator_id
+        c.spp = self.config.spp
+        c.max_depth = self.config.max_depth
+        c.rr_depth = self.config.rr_depth
+        c.seed = self.seed
+        c.schedule = int(self.schedule)
+        c.sample_chunk = self.sample_chunk
+        c.mis_mode = int(getattr(self.config, "mis_mode", 0))
+        c.mis_beta = float(getattr(self.config, "mis_beta", 2.0))
+        c.alpha = float(getattr(self.config, "alpha", 0.2))
+        c.poisson_iterations = int(getattr(self.config, "poisson_iterations", 50))
+        c.poisson_sign = int(getattr(self.config, "poisson_sign", 1))
+        c.gdpt_outputs = 1 if self.integrator_id == 2 else 0
+        return c
+
+    def render(self, scene: Scene, camera: PinholeCamera, width: int, height: int) -> Film:
+        """Integrator::render (integrators/mod.rs:19-27)."""
+        out = np.zeros((height, width, 3), np.float32)
+        aux = np.zeros((3, height, width, 3), np.float32) if self.integrator_id == 2 else None
+        st = ArRenderStats()
+        cfg = self._cfg()
+        _check(_lib.ar_render(scene.handle, C.byref(camera.c), width, height, C.byref(cfg), _ptr(out),
+                              _ptr(aux) if aux is not None else None, C.byref(st), AR_HOST_PTRS, None))
+        self.last_stats = st
+        self.aux = aux
+        return Film(width, height, out)
+
+
+class PathTracer(_Integrator):
+    integrator_id = 0
+
+
+class BidirectionalPathTracer(_Integrator):
+    integrator_id = 1
+
+
+class GradientDomainPathTracer(_Integrator):
+    integrator_id = 2
+
+
+def screened_poisson_reconstruct(primal, dx, dy, alpha: float, iterations: int = 50, sign: int = 1):
+    """gradient_domain_pt.rs:449-630 as a GPU conjugate-gradient solve.
+    Inputs (H, W, 3) float32; returns (image, final residual norm)."""
+    p = np.ascontiguousarray(primal, np.float32)
+    gx = np.ascontiguousarray(dx, np.float32)
+    gy = np.ascontiguousarray(dy, np.float32)
+    h, w = p.shape[0], p.shape[1]
+    out = np.empty_like(p)
+    res = C.c_double()
+    init()
+    _check(_lib.ar_poisson_solve(_ptr(p), _ptr(gx), _ptr(gy), w, h, alpha, iterations, sign, _ptr(out),
+                                 C.byref(res), AR_HOST_PTRS, None))
+    return out, res.value
+
+
+def bvh_build_host(boxes: np.ndarray):
+    """Host SAH build (the library's builder) for topology tests: returns
+    (boxes (m,6), meta (m,3) [left_or_start, right_or_end, count], prims)."""
+    b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+    n = b.shape[0]
+    cap = max(1, 2 * n)
+    ob = np.zeros((cap, 6), np.float32)
+    om = np.zeros((cap, 3), np.uint32)
+    op = np.zeros(max(1, n), np.uint32)
+    nn = C.c_uint32()
+    dep = C.c_uint32()
+    _check(_lib.ar_bvh_build_host(_ptr(b), n, _ptr(ob), _ptr(om), _ptr(op), C.byref(nn), C.byref(dep)))
+    m = nn.value
+    return ob[:m], om[:m], op[:n], dep.value

@@ -1,0 +1,151 @@
+// Host-side internals of libagentray: scene handle, device buffers, errors.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/agentray.h"
+#include "ar_device.h"
+
+namespace ar {
+
+// ---- errors -----------------------------------------------------------------
+void set_error(const std::string& msg);
+ar_status fail(ar_status code, const std::string& msg);
+#define AR_HIP(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::ar::fail(AR_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- device buffer (owned) ---------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  hipError_t alloc(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    release();
+    if (n == 0) return hipSuccess;
+    hipError_t e = hipMalloc(&p, n);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  template <class T>
+  T* as() const {
+    return (T*)p;
+  }
+};
+
+// ---- SAH BVH over a set of boxes --------------------------------------------
+struct HBox {
+  float lo[3], hi[3];
+};
+
+// Binary SAH tree in the reference's form (bvh.rs:32-51): preorder node
+// array, leaf = (start, count) into `prims`.
+struct BinNode {
+  HBox box;
+  uint32_t left_or_start, right_or_end, count;
+};
+struct BinTree {
+  std::vector<BinNode> nodes;
+  std::vector<uint32_t> prims;
+  uint32_t depth = 0;
+};
+// Exact full-sweep SAH (same split decisions as bvh.rs:71-150) computed with
+// per-axis presorted lists and stable partitioning: O(n log n) per level.
+BinTree build_sah(const std::vector<HBox>& boxes, int nthreads);
+
+// ---- materials / lights as stored on the device -----------------------------
+struct alignas(16) DMaterial {
+  uint32_t type;
+  int32_t light_id;  // >= 0 for an emitter (quad light geometry)
+  float roughness, eta;
+  float albedo[3];
+  float pad0;
+  float emission[3];
+  float pad1;
+};
+static_assert(sizeof(DMaterial) == 48, "material layout");
+
+struct alignas(16) DLight {
+  uint32_t type;
+  float area;
+  float pmf;  // selection probability (light distribution)
+  float cdf;  // inclusive CDF value (power distribution)
+  float pos[3];
+  float pad0;
+  float intensity[3];
+  float pad1;
+  float eu[3];
+  float pad2;
+  float ev[3];
+  float pad3;
+  float n[3];
+  float pad4;
+};
+static_assert(sizeof(DLight) == 96, "light layout");
+
+// Per-object shading record (scene.rs:16-88).
+struct alignas(16) DObject {
+  float nmat[9];        // normal matrix, column-major 3x3
+  uint32_t idx_base;    // first index (3 per triangle) in the global index array
+  int32_t nrm_base;     // first vertex normal, or -1 when the mesh has none
+  uint32_t slot_base;   // first material slot entry
+  uint32_t n_slots;     // 1 or triangle count
+  uint32_t mat_base;    // first entry of this object's material-id list
+  uint32_t n_mats;
+  uint32_t tri_base;    // first Tri record (leaf order)
+  uint32_t n_tris;
+  uint32_t pad[3];
+};
+static_assert(sizeof(DObject) == 80, "object layout");
+
+struct HostMesh {
+  std::vector<float> pos, nrm;
+  std::vector<uint32_t> idx, slots, mats;
+  float xf[16];
+};
+
+}  // namespace ar
+
+struct ar_scene {
+  std::vector<ar::DMaterial> materials;
+  std::vector<ar::HostMesh> meshes;
+  std::vector<ar_light> lights;
+  uint32_t dist_mode = AR_LIGHTDIST_POWER;
+  bool committed = false;
+  // host copies of what was uploaded (used by tests through the info call)
+  ar_scene_info info{};
+  // device-resident scene
+  ar::DevBuf d_scratch;
+  hipEvent_t scratch_event = nullptr;
+  uint32_t tlas_depth = 0;
+  ar::DevBuf d_nodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
+      d_tri_obj;
+  ar::SceneView view{};
+  uint32_t n_lights = 0;
+  bool has_dist = false;
+  bool has_emitters = false;
+  std::vector<ar::DLight> h_lights;
+  std::vector<ar::DObject> h_objects;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+};
+
+namespace ar {
+hipStream_t pick_stream(const ar_scene* s, void* stream);
+}

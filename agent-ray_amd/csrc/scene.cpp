@@ -1,0 +1,478 @@
+// Scene handle: collects Scene::new's inputs (agent-ray src/scene.rs:125-139),
+// builds each object's BLAS and the TLAS on the host with the reference's
+// SAH (accel/bvh.rs:317-410), flattens them into the wide-node layout and
+// uploads everything once.  The committed scene stays resident in HBM.
+#include <algorithm>
+#include <cmath>
+
+#include "ar_internal.h"
+
+using namespace ar;
+
+namespace {
+
+bool is_identity16(const float* m) {
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 4; ++r)
+      if (m[4 * c + r] != (c == r ? 1.0f : 0.0f)) return false;
+  return true;
+}
+
+// 4x4 inverse by cofactors (column-major in and out).  Identity maps to
+// itself exactly.
+void inverse16(const float* m, float* out) {
+  if (is_identity16(m)) {
+    for (int i = 0; i < 16; ++i) out[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+    return;
+  }
+  float inv[16];
+  inv[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] +
+           m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  inv[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] -
+           m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  inv[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] +
+           m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  inv[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] -
+            m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  inv[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] -
+           m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  inv[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] +
+           m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  inv[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] -
+           m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  inv[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] +
+            m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  inv[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] +
+           m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  inv[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] -
+           m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  inv[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] +
+            m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  inv[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] -
+            m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  inv[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] -
+           m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  inv[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] +
+           m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  inv[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] -
+            m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  inv[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] +
+            m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  float det = m[0] * inv[0] + m[1] * inv[4] + m[2] * inv[8] + m[3] * inv[12];
+  float rd = 1.0f / det;
+  for (int i = 0; i < 16; ++i) out[i] = inv[i] * rd;
+}
+
+// Mat3::from_mat4(m).inverse().transpose(), column-major 3x3 out.
+void normal_matrix(const float* m, float* out) {
+  float a[3][3];  // a[col][row]
+  bool ident = true;
+  for (int c = 0; c < 3; ++c)
+    for (int r = 0; r < 3; ++r) {
+      a[c][r] = m[4 * c + r];
+      if (a[c][r] != (c == r ? 1.0f : 0.0f)) ident = false;
+    }
+  if (ident) {
+    for (int i = 0; i < 9; ++i) out[i] = (i % 4 == 0) ? 1.0f : 0.0f;
+    return;
+  }
+  auto e = [&](int row, int col) { return a[col][row]; };
+  float cof[3][3];
+  cof[0][0] = e(1, 1) * e(2, 2) - e(1, 2) * e(2, 1);
+  cof[0][1] = e(1, 2) * e(2, 0) - e(1, 0) * e(2, 2);
+  cof[0][2] = e(1, 0) * e(2, 1) - e(1, 1) * e(2, 0);
+  cof[1][0] = e(0, 2) * e(2, 1) - e(0, 1) * e(2, 2);
+  cof[1][1] = e(0, 0) * e(2, 2) - e(0, 2) * e(2, 0);
+  cof[1][2] = e(0, 1) * e(2, 0) - e(0, 0) * e(2, 1);
+  cof[2][0] = e(0, 1) * e(1, 2) - e(0, 2) * e(1, 1);
+  cof[2][1] = e(0, 2) * e(1, 0) - e(0, 0) * e(1, 2);
+  cof[2][2] = e(0, 0) * e(1, 1) - e(0, 1) * e(1, 0);
+  float det = e(0, 0) * cof[0][0] + e(0, 1) * cof[0][1] + e(0, 2) * cof[0][2];
+  float id = 1.0f / det;
+  for (int row = 0; row < 3; ++row)
+    for (int col = 0; col < 3; ++col) out[3 * col + row] = cof[row][col] * id;
+}
+
+// transform_point3a with the reference grouping.
+void xform_pt16(const float* m, const float* p, float* o) {
+  for (int k = 0; k < 3; ++k) {
+    float a = m[k] * p[0];
+    a = m[4 + k] * p[1] + a;
+    a = m[8 + k] * p[2] + a;
+    o[k] = m[12 + k] + a;
+  }
+}
+
+inline float smin(float a, float b) { return a < b ? a : b; }
+inline float smax(float a, float b) { return a > b ? a : b; }
+
+// AABB::transform (geometry.rs:66-84)
+HBox transform_box(const HBox& b, const float* m) {
+  HBox r;
+  for (int k = 0; k < 3; ++k) {
+    r.lo[k] = INFINITY;
+    r.hi[k] = -INFINITY;
+  }
+  for (int c = 0; c < 8; ++c) {
+    float p[3] = {(c & 1) ? b.hi[0] : b.lo[0], (c & 2) ? b.hi[1] : b.lo[1], (c & 4) ? b.hi[2] : b.lo[2]};
+    float o[3];
+    xform_pt16(m, p, o);
+    for (int k = 0; k < 3; ++k) {
+      r.lo[k] = smin(r.lo[k], o[k]);
+      r.hi[k] = smax(r.hi[k], o[k]);
+    }
+  }
+  return r;
+}
+
+// Flatten a binary tree: every internal node becomes one wide node holding
+// both child boxes.  Returns the root ref; leaf refs are ~(prim_base + start).
+struct Flat {
+  int root_ref;
+  HBox root_box;
+};
+Flat flatten(const BinTree& t, std::vector<WNode>& out, uint32_t prim_base) {
+  Flat f;
+  f.root_box = t.nodes[0].box;
+  std::vector<int> wide(t.nodes.size(), -1);
+  uint32_t base = (uint32_t)out.size();
+  uint32_t k = 0;
+  for (size_t i = 0; i < t.nodes.size(); ++i)
+    if (t.nodes[i].count == 0) wide[i] = (int)(base + k++);
+  out.resize(base + k);
+  auto ref = [&](uint32_t i) -> int {
+    const BinNode& n = t.nodes[i];
+    return n.count == 0 ? wide[i] : ~(int)(prim_base + n.left_or_start);
+  };
+  for (size_t i = 0; i < t.nodes.size(); ++i) {
+    const BinNode& n = t.nodes[i];
+    if (n.count != 0) continue;
+    const HBox& L = t.nodes[n.left_or_start].box;
+    const HBox& R = t.nodes[n.right_or_end].box;
+    WNode w;
+    w.a = make_float4(L.lo[0], L.lo[1], L.lo[2], L.hi[0]);
+    w.b = make_float4(L.hi[1], L.hi[2], R.lo[0], R.lo[1]);
+    w.c = make_float4(R.lo[2], R.hi[0], R.hi[1], R.hi[2]);
+    w.d = make_int4(ref(n.left_or_start), ref(n.right_or_end), 0, 0);
+    out[wide[i]] = w;
+  }
+  f.root_ref = ref(0);
+  return f;
+}
+
+template <class T>
+hipError_t upload(DevBuf& b, const std::vector<T>& v) {
+  size_t n = std::max<size_t>(v.size() * sizeof(T), 16);
+  hipError_t e = b.alloc(n);
+  if (e != hipSuccess) return e;
+  if (!v.empty()) e = hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+ar_status ar_scene_create(ar_scene** out) {
+  if (!out) return fail(AR_ERR_INVALID_ARG, "ar_scene_create: out is NULL");
+  *out = new ar_scene();
+  return AR_OK;
+}
+
+ar_status ar_scene_destroy(ar_scene* s) {
+  if (!s) return AR_OK;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  if (s->scratch_event) (void)hipEventDestroy(s->scratch_event);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return AR_OK;
+}
+
+ar_status ar_scene_add_material(ar_scene* s, const ar_material* m, uint32_t* out_id) {
+  if (!s || !m) return fail(AR_ERR_INVALID_ARG, "ar_scene_add_material: NULL argument");
+  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
+  if (m->type > AR_MAT_DIELECTRIC) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown material type");
+  DMaterial d{};
+  d.type = m->type;
+  d.light_id = -1;
+  // ConductorBsdf::new / DielectricBsdf::new clamp alpha (microfacet.rs:94-99,171-176)
+  d.roughness = std::min(std::max(m->roughness, 1e-3f), 1.0f);
+  d.eta = m->eta;
+  for (int k = 0; k < 3; ++k) d.albedo[k] = m->albedo[k];
+  s->materials.push_back(d);
+  if (out_id) *out_id = (uint32_t)s->materials.size() - 1;
+  return AR_OK;
+}
+
+ar_status ar_scene_add_mesh(ar_scene* s, const float* positions, uint32_t nv, const uint32_t* indices,
+                            uint32_t nt, const float* normals, const uint32_t* slots, uint32_t n_slots,
+                            const uint32_t* mats, uint32_t n_mats, const float* transform,
+                            uint32_t* out_object_id) {
+  if (!s || !positions || !indices || !slots || !mats)
+    return fail(AR_ERR_INVALID_ARG, "ar_scene_add_mesh: NULL argument");
+  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
+  if (nt == 0 || nv == 0) return fail(AR_ERR_INVALID_ARG, "ar_scene_add_mesh: empty mesh");
+  if (n_slots != 1 && n_slots != nt)
+    return fail(AR_ERR_INVALID_ARG, "material_slots must have length 1 or the triangle count");
+  if (n_mats == 0) return fail(AR_ERR_INVALID_ARG, "an object needs at least one material");
+  for (uint64_t i = 0; i < 3ull * nt; ++i)
+    if (indices[i] >= nv) return fail(AR_ERR_INVALID_ARG, "triangle index out of range");
+  for (uint32_t i = 0; i < n_mats; ++i)
+    if (mats[i] >= s->materials.size()) return fail(AR_ERR_INVALID_ARG, "material id out of range");
+  for (uint64_t i = 0; i < 3ull * nv; ++i)
+    if (!std::isfinite(positions[i])) return fail(AR_ERR_INVALID_ARG, "non-finite vertex position");
+  HostMesh m;
+  m.pos.assign(positions, positions + 3ull * nv);
+  if (normals) m.nrm.assign(normals, normals + 3ull * nv);
+  m.idx.assign(indices, indices + 3ull * nt);
+  m.slots.assign(slots, slots + n_slots);
+  m.mats.assign(mats, mats + n_mats);
+  if (transform) {
+    for (int i = 0; i < 16; ++i) m.xf[i] = transform[i];
+  } else {
+    for (int i = 0; i < 16; ++i) m.xf[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+  }
+  s->meshes.push_back(std::move(m));
+  if (out_object_id) *out_object_id = (uint32_t)s->meshes.size() - 1;
+  return AR_OK;
+}
+
+ar_status ar_scene_add_light(ar_scene* s, const ar_light* l, uint32_t* out_id) {
+  if (!s || !l) return fail(AR_ERR_INVALID_ARG, "ar_scene_add_light: NULL argument");
+  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
+  if (l->type > AR_LIGHT_QUAD) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown light type");
+  uint32_t id = (uint32_t)s->lights.size();
+  if (l->type == AR_LIGHT_QUAD) {
+    f3 eu = mk(l->edge_u[0], l->edge_u[1], l->edge_u[2]), ev = mk(l->edge_v[0], l->edge_v[1], l->edge_v[2]);
+    if (length(cross(eu, ev)) <= 0.0f) return fail(AR_ERR_INVALID_ARG, "degenerate quad light");
+    DMaterial d{};
+    d.type = AR_MAT_LAMBERTIAN;
+    d.light_id = (int32_t)id;
+    d.roughness = 1.0f;
+    d.eta = 1.0f;
+    for (int k = 0; k < 3; ++k) d.emission[k] = l->intensity[k];
+    s->materials.push_back(d);
+    uint32_t mid = (uint32_t)s->materials.size() - 1;
+    f3 a = mk(l->position[0], l->position[1], l->position[2]);
+    f3 b = a + eu, c = (a + eu) + ev, dd = a + ev;
+    float pos[12] = {a.x, a.y, a.z, b.x, b.y, b.z, c.x, c.y, c.z, dd.x, dd.y, dd.z};
+    uint32_t idx[6] = {0, 1, 2, 0, 2, 3};
+    uint32_t slot = 0;
+    ar_status st = ar_scene_add_mesh(s, pos, 4, idx, 2, nullptr, &slot, 1, &mid, 1, nullptr, nullptr);
+    if (st != AR_OK) return st;
+  }
+  s->lights.push_back(*l);
+  if (out_id) *out_id = id;
+  return AR_OK;
+}
+
+ar_status ar_scene_set_light_distribution(ar_scene* s, uint32_t mode) {
+  if (!s) return fail(AR_ERR_INVALID_ARG, "NULL scene");
+  if (mode > AR_LIGHTDIST_POWER) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown light distribution");
+  s->dist_mode = mode;
+  return AR_OK;
+}
+
+ar_status ar_scene_commit(ar_scene* s) {
+  if (!s) return fail(AR_ERR_INVALID_ARG, "NULL scene");
+  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
+  if (s->meshes.empty()) return fail(AR_ERR_INVALID_ARG, "scene has no objects");
+  std::vector<WNode> nodes;
+  std::vector<Tri> tris;
+  std::vector<Inst> inst_by_obj(s->meshes.size());
+  std::vector<HBox> world(s->meshes.size());
+  std::vector<DObject> objs(s->meshes.size());
+  std::vector<uint32_t> gidx, gslots, gmats, tri_obj;
+  std::vector<float> gnrm;
+  uint32_t max_depth = 0;
+  for (size_t oi = 0; oi < s->meshes.size(); ++oi) {
+    const HostMesh& m = s->meshes[oi];
+    uint32_t nt = (uint32_t)(m.idx.size() / 3);
+    std::vector<HBox> boxes(nt);
+    for (uint32_t i = 0; i < nt; ++i) {  // mesh.rs:81-94
+      const float* v0 = &m.pos[3 * m.idx[3 * i]];
+      const float* v1 = &m.pos[3 * m.idx[3 * i + 1]];
+      const float* v2 = &m.pos[3 * m.idx[3 * i + 2]];
+      for (int k = 0; k < 3; ++k) {
+        boxes[i].lo[k] = smin(smin(v0[k], v1[k]), v2[k]);
+        boxes[i].hi[k] = smax(smax(v0[k], v1[k]), v2[k]);
+      }
+    }
+    BinTree t = build_sah(boxes, 0);
+    max_depth = std::max(max_depth, t.depth);
+    uint32_t tri_base = (uint32_t)tris.size();
+    for (size_t i = 0; i < t.nodes.size(); ++i) {
+      const BinNode& n = t.nodes[i];
+      if (n.count == 0) continue;
+      for (uint32_t k = 0; k < n.count; ++k) {
+        uint32_t prim = t.prims[n.left_or_start + k];
+        const float* v0 = &m.pos[3 * m.idx[3 * prim]];
+        const float* v1 = &m.pos[3 * m.idx[3 * prim + 1]];
+        const float* v2 = &m.pos[3 * m.idx[3 * prim + 2]];
+        Tri tr;
+        uint32_t tag = prim | (k + 1 == n.count ? kLastBit : 0u);
+        float tagf;
+        memcpy(&tagf, &tag, 4);
+        tr.v0 = make_float4(v0[0], v0[1], v0[2], tagf);
+        tr.e1 = make_float4(v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2], 0.0f);
+        tr.e2 = make_float4(v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2], 0.0f);
+        if (tris.size() != tri_base + n.left_or_start + k) return fail(AR_ERR_STATE, "leaf order mismatch");
+        tris.push_back(tr);
+      }
+    }
+    for (uint32_t i = 0; i < nt; ++i) tri_obj.push_back((uint32_t)oi);
+    Flat f = flatten(t, nodes, tri_base);
+    Inst& in = inst_by_obj[oi];
+    memset(&in, 0, sizeof(in));
+    float inv[16];
+    inverse16(m.xf, inv);
+    for (int c = 0; c < 4; ++c)
+      for (int r = 0; r < 3; ++r) in.inv[3 * c + r] = inv[4 * c + r];
+    for (int k = 0; k < 3; ++k) {
+      in.box[k] = f.root_box.lo[k];
+      in.box[3 + k] = f.root_box.hi[k];
+    }
+    in.root = f.root_ref;
+    in.id = (uint32_t)oi;
+    in.identity = is_identity16(m.xf) ? 1u : 0u;
+    in.tri_base = tri_base;
+    world[oi] = transform_box(f.root_box, m.xf);  // Instance::new (bvh.rs:373-377)
+    DObject& d = objs[oi];
+    memset(&d, 0, sizeof(d));
+    normal_matrix(m.xf, d.nmat);
+    d.idx_base = (uint32_t)gidx.size();
+    gidx.insert(gidx.end(), m.idx.begin(), m.idx.end());
+    if (!m.nrm.empty()) {
+      d.nrm_base = (int32_t)(gnrm.size() / 3);
+      gnrm.insert(gnrm.end(), m.nrm.begin(), m.nrm.end());
+    } else {
+      d.nrm_base = -1;
+    }
+    d.slot_base = (uint32_t)gslots.size();
+    d.n_slots = (uint32_t)m.slots.size();
+    gslots.insert(gslots.end(), m.slots.begin(), m.slots.end());
+    d.mat_base = (uint32_t)gmats.size();
+    d.n_mats = (uint32_t)m.mats.size();
+    gmats.insert(gmats.end(), m.mats.begin(), m.mats.end());
+    d.tri_base = tri_base;
+    d.n_tris = nt;
+  }
+  // TLAS over instance world boxes (bvh.rs:395-410); instance records are
+  // stored in TLAS leaf order.
+  BinTree tl = build_sah(world, 0);
+  s->tlas_depth = tl.depth;
+  std::vector<Inst> insts;
+  for (size_t i = 0; i < tl.nodes.size(); ++i) {
+    const BinNode& n = tl.nodes[i];
+    if (n.count == 0) continue;
+    for (uint32_t k = 0; k < n.count; ++k) {
+      Inst in = inst_by_obj[tl.prims[n.left_or_start + k]];
+      if (k + 1 == n.count) in.id |= kLastBit;
+      insts.push_back(in);
+    }
+  }
+  uint32_t tlas_node_base = (uint32_t)nodes.size();
+  (void)tlas_node_base;
+  Flat tf = flatten(tl, nodes, 0);
+  // Materials / lights / distribution (lights.rs:115-235).
+  std::vector<DLight> dl(s->lights.size());
+  bool emit = false;
+  for (size_t i = 0; i < s->lights.size(); ++i) {
+    const ar_light& l = s->lights[i];
+    DLight& d = dl[i];
+    memset(&d, 0, sizeof(d));
+    d.type = l.type;
+    for (int k = 0; k < 3; ++k) {
+      d.pos[k] = l.position[k];
+      d.intensity[k] = l.intensity[k];
+      d.eu[k] = l.edge_u[k];
+      d.ev[k] = l.edge_v[k];
+    }
+    if (l.type == AR_LIGHT_QUAD) {
+      emit = true;
+      f3 c = cross(mk(d.eu[0], d.eu[1], d.eu[2]), mk(d.ev[0], d.ev[1], d.ev[2]));
+      d.area = length(c);
+      f3 n = normalize(c);
+      d.n[0] = n.x;
+      d.n[1] = n.y;
+      d.n[2] = n.z;
+    }
+  }
+  uint32_t nl = (uint32_t)dl.size();
+  s->has_dist = s->dist_mode != AR_LIGHTDIST_NONE && nl > 0;
+  if (s->has_dist) {
+    if (s->dist_mode == AR_LIGHTDIST_UNIFORM) {
+      for (uint32_t i = 0; i < nl; ++i) dl[i].pmf = 1.0f / (float)nl;
+    } else {
+      std::vector<float> w(nl);
+      float total = 0.0f;
+      for (uint32_t i = 0; i < nl; ++i) {
+        float p[3];
+        for (int k = 0; k < 3; ++k)
+          p[k] = dl[i].type == AR_LIGHT_POINT ? dl[i].intensity[k] * (4.0f * kPi)
+                                               : dl[i].intensity[k] * (kPi * dl[i].area);
+        w[i] = std::fmax(0.2126f * p[0] + 0.7152f * p[1] + 0.0722f * p[2], 0.0f);
+      }
+      for (uint32_t i = 0; i < nl; ++i) total = total + w[i];
+      if (total == 0.0f) {
+        float u = 1.0f / (float)nl;
+        for (uint32_t i = 0; i < nl; ++i) {
+          dl[i].pmf = u;
+          dl[i].cdf = (float)(i + 1) * u;
+        }
+      } else {
+        float acc = 0.0f;
+        for (uint32_t i = 0; i < nl; ++i) {
+          dl[i].pmf = w[i] / total;
+          acc += dl[i].pmf;
+          dl[i].cdf = acc;
+        }
+        dl[nl - 1].cdf = 1.0f;
+      }
+    }
+  }
+  AR_HIP(upload(s->d_nodes, nodes));
+  AR_HIP(upload(s->d_tris, tris));
+  AR_HIP(upload(s->d_insts, insts));
+  AR_HIP(upload(s->d_objects, objs));
+  AR_HIP(upload(s->d_materials, s->materials));
+  AR_HIP(upload(s->d_lights, dl));
+  AR_HIP(upload(s->d_idx, gidx));
+  AR_HIP(upload(s->d_nrm, gnrm));
+  AR_HIP(upload(s->d_slots, gslots));
+  AR_HIP(upload(s->d_objmats, gmats));
+  AR_HIP(upload(s->d_tri_obj, tri_obj));
+  s->view.nodes = s->d_nodes.as<WNode>();
+  s->view.tris = s->d_tris.as<Tri>();
+  s->view.insts = s->d_insts.as<Inst>();
+  s->view.tlas_root = tf.root_ref;
+  for (int k = 0; k < 3; ++k) {
+    s->view.tlas_box[k] = tf.root_box.lo[k];
+    s->view.tlas_box[3 + k] = tf.root_box.hi[k];
+  }
+  s->view.n_inst = (uint32_t)insts.size();
+  s->n_lights = nl;
+  s->has_emitters = emit;
+  s->h_lights = dl;
+  s->h_objects = objs;
+  s->info.n_objects = (uint32_t)s->meshes.size();
+  s->info.n_triangles = (uint32_t)tris.size();
+  s->info.n_nodes = (uint32_t)nodes.size();
+  s->info.n_lights = nl;
+  s->info.max_depth = max_depth;
+  s->info.device_bytes = s->d_nodes.bytes + s->d_tris.bytes + s->d_insts.bytes + s->d_objects.bytes +
+                         s->d_materials.bytes + s->d_lights.bytes + s->d_idx.bytes + s->d_nrm.bytes +
+                         s->d_slots.bytes + s->d_objmats.bytes + s->d_tri_obj.bytes;
+  AR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  s->committed = true;
+  return AR_OK;
+}
+
+ar_status ar_scene_get_info(const ar_scene* s, ar_scene_info* info) {
+  if (!s || !info) return fail(AR_ERR_INVALID_ARG, "NULL argument");
+  if (!s->committed) return fail(AR_ERR_STATE, "scene not committed");
+  *info = s->info;
+  return AR_OK;
+}
+
+}  // extern "C"

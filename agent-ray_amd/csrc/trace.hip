@@ -1,0 +1,208 @@
+// Batch traversal kernels behind ar_trace / ar_occluded
+// (TLAS::intersect, agent-ray src/accel/bvh.rs:413-447; Scene::occluded,
+// src/scene.rs:166-168).
+//
+// One ray per lane, persistent grid: each workgroup walks ray blocks with a
+// grid stride.  The grid is a multiple of 8 and block ids are remapped so
+// that each XCD (blocks are dispatched round-robin over the 8 XCDs) sweeps a
+// contiguous range of rays, keeping neighbouring (coherent) rays in the same
+// L2.
+#include "ar_internal.h"
+#include "ar_traverse.h"
+#include "ar_kernels.h"
+
+namespace ar {
+
+__device__ __forceinline__ Ray load_ray(const ar_ray* rays, uint64_t i) {
+  const float4* p = reinterpret_cast<const float4*>(rays + i);
+  float4 a = p[0], b = p[1];
+  Ray r;
+  r.o = mk(a.x, a.y, a.z);
+  r.tmin = a.w;
+  r.d = mk(b.x, b.y, b.z);
+  r.tmax = b.w;
+  return r;
+}
+
+template <bool ANY, bool NEAR, int S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const ar_ray* __restrict__ rays, uint64_t n,
+                                                 ar_hit* __restrict__ hits, uint32_t* __restrict__ inst_ids,
+                                                 uint32_t* __restrict__ occ, int2* __restrict__ spill) {
+  __shared__ int2 lds[S * BLOCK];
+  const uint32_t G = gridDim.x;
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
+  const uint32_t gtid = lb * BLOCK + threadIdx.x;
+  Stack stk;
+  stk.lds = lds + threadIdx.x;
+  stk.stride = BLOCK;
+  stk.spill = spill + gtid;
+  stk.spill_stride = G * BLOCK;
+  stk.S = S;
+  stk.sp = 0;
+  for (uint64_t i = gtid; i < n; i += (uint64_t)G * BLOCK) {
+    Ray r = load_ray(rays, i);
+    WorldHit h;
+    bool any = trace_scene<ANY, NEAR>(sv, r, h, stk);
+    if (ANY) {
+      occ[i] = any ? 1u : 0u;
+    } else {
+      ar_hit o;
+      if (h.pos != kNoHit) {
+        o.t = h.t;
+        o.prim_id = __float_as_uint(sv.tris[h.pos].v0.w) & ~kLastBit;
+      } else {
+        o.t = INFINITY;
+        o.prim_id = kNoHit;
+      }
+      *reinterpret_cast<float2*>(hits + i) = make_float2(o.t, __uint_as_float(o.prim_id));
+      if (inst_ids) inst_ids[i] = h.pos != kNoHit ? (sv.insts[h.inst].id & ~kLastBit) : kNoHit;
+    }
+  }
+}
+
+// Stack sizing: a push happens only at an internal node with both children
+// hit, so the depth of the deepest BLAS plus the TLAS bounds the stack.
+static uint32_t stack_need(const ar_scene* s) { return s->info.max_depth + s->tlas_depth + 2; }
+
+template <bool ANY, bool NEAR>
+static ar_status launch_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst,
+                              uint32_t* occ, hipStream_t st) {
+  constexpr int S = kTraceLdsStack;
+  constexpr int BLOCK = kTraceBlock;
+  uint32_t G = trace_grid(n, BLOCK);
+  int2* spill = nullptr;
+  uint32_t need = stack_need(s);
+  size_t bytes = need > (uint32_t)S ? (size_t)(need - S) * G * BLOCK * sizeof(int2) : 0;
+  ar_scene* ms = const_cast<ar_scene*>(s);
+  ar_status e = scratch_acquire(ms, bytes, st, (void**)&spill);
+  if (e != AR_OK) return e;
+  hipLaunchKernelGGL((k_trace<ANY, NEAR, S, BLOCK>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits, inst,
+                     occ, spill);
+  hipError_t le = hipGetLastError();
+  scratch_release(ms, st);
+  if (le != hipSuccess) return fail(AR_ERR_DEVICE, std::string("k_trace launch: ") + hipGetErrorString(le));
+  return AR_OK;
+}
+
+uint32_t trace_grid(uint64_t n, int block) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  uint64_t want = (n + block - 1) / block;
+  uint64_t cap = (uint64_t)cus * kTraceBlocksPerCU;
+  uint64_t g = want < cap ? want : cap;
+  g = (g + 7) / 8 * 8;
+  if (g == 0) g = 8;
+  return (uint32_t)g;
+}
+
+ar_status scratch_acquire(ar_scene* s, size_t bytes, hipStream_t st, void** out) {
+  s->mu.lock();
+  if (!s->scratch_event) {
+    hipError_t e = hipEventCreateWithFlags(&s->scratch_event, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      s->mu.unlock();
+      return fail(AR_ERR_DEVICE, "hipEventCreate failed");
+    }
+  } else {
+    hipError_t e = hipStreamWaitEvent(st, s->scratch_event, 0);
+    if (e != hipSuccess) {
+      s->mu.unlock();
+      return fail(AR_ERR_DEVICE, "hipStreamWaitEvent failed");
+    }
+  }
+  if (bytes > 0 && s->d_scratch.bytes < bytes) {
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = s->d_scratch.alloc(bytes);
+    if (e != hipSuccess) {
+      s->mu.unlock();
+      return fail(AR_ERR_DEVICE, std::string("scratch alloc: ") + hipGetErrorString(e));
+    }
+  }
+  *out = s->d_scratch.p;
+  return AR_OK;
+}
+
+void scratch_release(ar_scene* s, hipStream_t st) {
+  (void)hipEventRecord(s->scratch_event, st);
+  s->mu.unlock();
+}
+
+}  // namespace ar
+
+using namespace ar;
+
+namespace {
+
+// Stages host buffers when AR_HOST_PTRS is set.
+struct Staged {
+  DevBuf rays, out, out2;
+};
+
+}  // namespace
+
+extern "C" {
+
+ar_status ar_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst_ids,
+                   uint32_t flags, void* stream) {
+  if (!s || (!rays && n) || (!hits && n)) return fail(AR_ERR_INVALID_ARG, "ar_trace: NULL argument");
+  if (!s->committed) return fail(AR_ERR_STATE, "ar_trace: scene not committed");
+  if (n == 0) return AR_OK;
+  hipStream_t st = pick_stream(s, stream);
+  const ar_ray* d_rays = rays;
+  ar_hit* d_hits = hits;
+  uint32_t* d_inst = inst_ids;
+  Staged sg;
+  bool host = flags & AR_HOST_PTRS;
+  if (host) {
+    AR_HIP(sg.rays.alloc(n * sizeof(ar_ray)));
+    AR_HIP(sg.out.alloc(n * sizeof(ar_hit)));
+    if (inst_ids) AR_HIP(sg.out2.alloc(n * sizeof(uint32_t)));
+    AR_HIP(hipMemcpyAsync(sg.rays.p, rays, n * sizeof(ar_ray), hipMemcpyHostToDevice, st));
+    d_rays = sg.rays.as<ar_ray>();
+    d_hits = sg.out.as<ar_hit>();
+    d_inst = inst_ids ? sg.out2.as<uint32_t>() : nullptr;
+  }
+  ar_status e = (flags & AR_REF_ORDER) ? launch_trace<false, false>(s, d_rays, n, d_hits, d_inst, nullptr, st)
+                                       : launch_trace<false, true>(s, d_rays, n, d_hits, d_inst, nullptr, st);
+  if (e != AR_OK) return e;
+  if (host) {
+    AR_HIP(hipMemcpyAsync(hits, d_hits, n * sizeof(ar_hit), hipMemcpyDeviceToHost, st));
+    if (inst_ids) AR_HIP(hipMemcpyAsync(inst_ids, d_inst, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  }
+  if (host || !(flags & AR_ASYNC)) AR_HIP(hipStreamSynchronize(st));
+  return AR_OK;
+}
+
+ar_status ar_occluded(const ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t* occluded, uint32_t flags,
+                      void* stream) {
+  if (!s || (!rays && n) || (!occluded && n)) return fail(AR_ERR_INVALID_ARG, "ar_occluded: NULL argument");
+  if (!s->committed) return fail(AR_ERR_STATE, "ar_occluded: scene not committed");
+  if (n == 0) return AR_OK;
+  hipStream_t st = pick_stream(s, stream);
+  const ar_ray* d_rays = rays;
+  uint32_t* d_occ = occluded;
+  Staged sg;
+  bool host = flags & AR_HOST_PTRS;
+  if (host) {
+    AR_HIP(sg.rays.alloc(n * sizeof(ar_ray)));
+    AR_HIP(sg.out.alloc(n * sizeof(uint32_t)));
+    AR_HIP(hipMemcpyAsync(sg.rays.p, rays, n * sizeof(ar_ray), hipMemcpyHostToDevice, st));
+    d_rays = sg.rays.as<ar_ray>();
+    d_occ = sg.out.as<uint32_t>();
+  }
+  ar_status e = (flags & AR_REF_ORDER) ? launch_trace<true, false>(s, d_rays, n, nullptr, nullptr, d_occ, st)
+                                       : launch_trace<true, true>(s, d_rays, n, nullptr, nullptr, d_occ, st);
+  if (e != AR_OK) return e;
+  if (host) AR_HIP(hipMemcpyAsync(occluded, d_occ, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  if (host || !(flags & AR_ASYNC)) AR_HIP(hipStreamSynchronize(st));
+  return AR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,250 @@
+/*
+ * agentray.h — C ABI of the MI355X-native ray-traversal and path-sampling
+ * library (libagentray.so).  Plain pointers and sizes only; every function
+ * returns an ar_status (0 = AR_OK) and leaves a thread-local message for
+ * ar_last_error() on failure.
+ *
+ * The entry points replace, one for one, the interfaces of the reference
+ * renderer (shiinamiyuki/agent-ray, Rust) that sit on the traversal and
+ * path-sampling hot path.  Each declaration cites the reference interface it
+ * stands in for.  A Rust `extern "C"` block (or ctypes, or cgo) binds this
+ * header directly; see INTEGRATION.md.
+ *
+ * Ownership: the caller owns every input/output buffer; the library owns the
+ * opaque handles.  Pointers are device pointers unless AR_HOST_PTRS is set
+ * in `flags` (then the library stages the copies itself).  A committed scene
+ * is immutable and may be used from several threads; each call runs on the
+ * HIP stream passed in (NULL = the library's own stream) and is synchronous
+ * unless AR_ASYNC is set.
+ */
+#ifndef AGENTRAY_H
+#define AGENTRAY_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t ar_status;
+#define AR_OK 0
+#define AR_ERR_INVALID_ARG (-1)
+#define AR_ERR_UNSUPPORTED_TYPE (-2)
+#define AR_ERR_OUT_OF_BUDGET (-3)
+#define AR_ERR_DEVICE (-4)
+#define AR_ERR_QUEUE_OVERFLOW (-5)
+#define AR_ERR_STATE (-6)
+#define AR_ERR_CANCELLED (-7)
+#define AR_ERR_NOT_FOUND (-8)
+
+/* flags */
+#define AR_HOST_PTRS 0x1u    /* buffers are host memory */
+#define AR_ASYNC 0x2u        /* do not synchronize the stream before returning */
+#define AR_REF_ORDER 0x4u    /* traverse left child first (reference DFS order) */
+
+/* ---- value types ------------------------------------------------------ */
+
+/* Ray, 32 bytes: origin, t_min, direction (unit length, as produced by the
+ * reference's Ray::new, geometry.rs:12-20), t_max. */
+typedef struct ar_ray {
+  float origin[3];
+  float t_min;
+  float dir[3];
+  float t_max;
+} ar_ray;
+
+/* Closest hit, 8 bytes: world-space t and primitive id (prim_id = UINT32_MAX
+ * and t = +inf on a miss).  Mirrors RayHit{t, prim_id} (accel/bvh.rs:10-20);
+ * instance ids go to an optional separate array. */
+typedef struct ar_hit {
+  float t;
+  uint32_t prim_id;
+} ar_hit;
+
+/* Material types: the closed set of Bsdf impls (surfaces.rs:25-29). */
+#define AR_MAT_LAMBERTIAN 0u /* surfaces/lambertian.rs:17-65 (albedo)          */
+#define AR_MAT_CONDUCTOR 1u  /* surfaces/microfacet.rs:86-153 (f0, roughness)  */
+#define AR_MAT_DIELECTRIC 2u /* surfaces/microfacet.rs:163-307 (eta, roughness) */
+typedef struct ar_material {
+  uint32_t type;
+  float albedo[3]; /* Lambertian albedo, or conductor F0 */
+  float roughness; /* GGX alpha, clamped to [1e-3, 1] as the reference does */
+  float eta;       /* dielectric interior IOR */
+} ar_material;
+
+/* Light types: PointLight (lights/point.rs:22-100) and a quad area light
+ * (extension; the reference has no emitters).  A quad light also inserts its
+ * emissive geometry (two triangles, normal = cross(edge_u, edge_v)). */
+#define AR_LIGHT_POINT 0u
+#define AR_LIGHT_QUAD 1u
+typedef struct ar_light {
+  uint32_t type;
+  float position[3];  /* point position, or quad corner */
+  float intensity[3]; /* point intensity I (W/sr), or quad radiance L */
+  float edge_u[3];
+  float edge_v[3];
+} ar_light;
+
+/* Light selection (lights.rs:115-235). */
+#define AR_LIGHTDIST_NONE 0u
+#define AR_LIGHTDIST_UNIFORM 1u
+#define AR_LIGHTDIST_POWER 2u
+
+/* Pinhole camera state (PinholeCamera, cameras.rs:78-87), 16 floats. */
+typedef struct ar_camera {
+  float origin[3];
+  float lower_left[3];
+  float horizontal[3];
+  float vertical[3];
+  float forward[3];
+  float area;
+} ar_camera;
+
+/* Integrators (integrators/mod.rs:19-27 implementors). */
+#define AR_INTEGRATOR_PT 0u   /* PathTracer, path_tracer.rs:159-191            */
+#define AR_INTEGRATOR_BDPT 1u /* BidirectionalPathTracer, bdpt.rs:1203-1357   */
+#define AR_INTEGRATOR_GDPT 2u /* GradientDomainPathTracer, gdpt.rs:345-415    */
+
+/* PT execution schedule. */
+#define AR_SCHED_WAVEFRONT 0u  /* persistent wavefront stages over device queues */
+#define AR_SCHED_MEGAKERNEL 1u /* one thread per path (the plain GPU kernel)     */
+
+typedef struct ar_render_config {
+  uint32_t integrator;
+  uint32_t spp;
+  uint32_t max_depth;
+  uint32_t rr_depth;
+  uint64_t seed;          /* counter-RNG key (Philox4x32-10) */
+  uint32_t schedule;      /* AR_SCHED_* */
+  uint32_t sample_chunk;  /* samples per pixel per wave (0 = auto) */
+  /* BDPT (BdptConfig, bidirectional_path_tracer.rs:102-133) */
+  uint32_t mis_mode;      /* 0 = power heuristic, 1 = uniform */
+  float mis_beta;
+  /* GDPT (GdptConfig, gradient_domain_pt.rs:47-76) */
+  float alpha;
+  uint32_t poisson_iterations; /* CG iterations */
+  int32_t poisson_sign;        /* +1: reference RHS sign; -1: textbook */
+  uint32_t gdpt_outputs;       /* 1: also return primal, dx, dy films */
+} ar_render_config;
+
+typedef struct ar_render_stats {
+  double seconds;        /* wall time of the render call */
+  uint64_t paths;        /* camera paths traced (W*H*spp, x3 for GDPT) */
+  uint64_t rays;         /* every traversal query: extension + shadow */
+  uint64_t shadow_rays;
+  uint32_t queue_overflow;
+  uint32_t reserved;
+} ar_render_stats;
+
+typedef struct ar_scene_info {
+  uint32_t n_objects;
+  uint32_t n_triangles;
+  uint32_t n_nodes;      /* wide nodes over all BLAS + TLAS */
+  uint32_t n_lights;
+  uint32_t max_depth;    /* deepest BLAS */
+  uint64_t device_bytes; /* scene bytes resident in HBM */
+} ar_scene_info;
+
+typedef struct ar_scene ar_scene;
+
+/* ---- library ----------------------------------------------------------- */
+const char* ar_last_error(void);
+ar_status ar_init(int device);
+const char* ar_version(void);
+
+/* ---- scene (scene.rs:113-168, accel/bvh.rs:317-448) --------------------- */
+/* Scene::new's inputs are collected, then committed once. */
+ar_status ar_scene_create(ar_scene** out);
+ar_status ar_scene_destroy(ar_scene* s);
+/* Material table entry; *out_id is the material id used by meshes. */
+ar_status ar_scene_add_material(ar_scene* s, const ar_material* m, uint32_t* out_id);
+/* SceneObject::new (scene.rs:29-35) + TriangleMesh (primitives/mesh.rs:7-16):
+ * positions (nv x 3 floats), indices (nt x 3), optional vertex normals
+ * (nv x 3 or NULL), material_slots (len 1 or nt), material ids by slot,
+ * and a column-major 4x4 local-to-world transform (NULL = identity). */
+ar_status ar_scene_add_mesh(ar_scene* s, const float* positions, uint32_t nv, const uint32_t* indices,
+                            uint32_t nt, const float* normals, const uint32_t* material_slots,
+                            uint32_t n_slots, const uint32_t* material_ids, uint32_t n_materials,
+                            const float* transform, uint32_t* out_object_id);
+ar_status ar_scene_add_light(ar_scene* s, const ar_light* l, uint32_t* out_light_id);
+ar_status ar_scene_set_light_distribution(ar_scene* s, uint32_t mode);
+/* BLASAccel::build per object + TLAS::build (bvh.rs:326-410): host SAH build,
+ * flatten to the device layout, upload.  Immutable afterwards. */
+ar_status ar_scene_commit(ar_scene* s);
+ar_status ar_scene_get_info(const ar_scene* s, ar_scene_info* info);
+/* Diagnostic, host only (no device needed): the library's SAH build over n
+ * boxes (6 floats: lo, hi) in the reference's binary form (bvh.rs:32-51):
+ * out_boxes (6 per node), out_meta (left_or_start, right_or_end, count per
+ * node; capacity 2n-1 nodes), out_prims (n), node count and tree depth. */
+ar_status ar_bvh_build_host(const float* boxes, uint32_t n, float* out_boxes, uint32_t* out_meta,
+                            uint32_t* out_prims, uint32_t* out_nodes, uint32_t* out_depth);
+
+/* ---- traversal (TLAS::intersect, bvh.rs:413-447; Scene::occluded,
+ * scene.rs:166-168) ------------------------------------------------------- */
+/* Closest hit for n rays.  inst_ids may be NULL. */
+ar_status ar_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst_ids,
+                   uint32_t flags, void* stream);
+/* Any hit (boolean equal to the reference's full closest-hit is_some()). */
+ar_status ar_occluded(const ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t* occluded,
+                      uint32_t flags, void* stream);
+
+/* ---- camera (cameras.rs:91-149) ----------------------------------------- */
+ar_status ar_camera_lookat(const float lookfrom[3], const float lookat[3], const float vup[3], float vfov_deg,
+                           float aspect, ar_camera* out);
+ar_status ar_camera_eye_angle(const float eye[3], float yaw_deg, float pitch_deg, float vfov_deg,
+                              float aspect, ar_camera* out);
+
+/* ---- rendering (Integrator::render, integrators/mod.rs:19-27) ----------- */
+/* out_rgb: width*height*3 floats, row-major, top row first (the Film the
+ * reference returns).  For GDPT with gdpt_outputs=1, out_aux receives the
+ * primal, dx and dy films (3 x width*height*3 floats). */
+ar_status ar_render(const ar_scene* s, const ar_camera* cam, uint32_t width, uint32_t height,
+                    const ar_render_config* cfg, float* out_rgb, float* out_aux, ar_render_stats* stats,
+                    uint32_t flags, void* stream);
+
+/* ---- screened Poisson (gradient_domain_pt.rs:449-630) ------------------- */
+/* Conjugate gradient on (alpha + n_neigh) I - sum(neigh) = alpha P + sign*div G,
+ * fp32 vectors, fp64 dot products and convergence test, warm start P, output
+ * clamped at 0.  Films are width*height*3 floats. */
+ar_status ar_poisson_solve(const float* primal, const float* dx, const float* dy, uint32_t width,
+                           uint32_t height, float alpha, uint32_t iterations, int32_t sign, float* out,
+                           double* final_residual, uint32_t flags, void* stream);
+
+/* ---- serving (no reference counterpart: the request loop above
+ * Integrator::render) ------------------------------------------------------ */
+typedef struct ar_server ar_server;
+typedef struct ar_server_config {
+  uint64_t hbm_budget_bytes; /* admission budget for per-job device state */
+  uint32_t max_active_jobs;  /* job slots rendering concurrently */
+  uint32_t path_slots;       /* in-flight path slots shared by all jobs (0 = auto) */
+} ar_server_config;
+typedef struct ar_job_desc {
+  uint32_t width, height;
+  ar_camera camera;
+  ar_render_config config;
+} ar_job_desc;
+typedef struct ar_job_stats {
+  double submit_time, start_time, finish_time; /* seconds, server clock */
+  uint64_t paths, rays;
+  int32_t state;
+} ar_job_stats;
+#define AR_JOB_QUEUED 0
+#define AR_JOB_RUNNING 1
+#define AR_JOB_DONE 2
+#define AR_JOB_CANCELLED 3
+#define AR_JOB_FAILED 4
+
+ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_server** out);
+ar_status ar_server_destroy(ar_server* srv);
+ar_status ar_job_submit(ar_server* srv, const ar_job_desc* job, uint64_t* out_job_id);
+ar_status ar_job_poll(ar_server* srv, uint64_t job_id, int32_t* out_state);
+/* Blocks until the job ends; copies its film (width*height*3 floats, host). */
+ar_status ar_job_wait(ar_server* srv, uint64_t job_id, float* out_rgb, ar_job_stats* stats);
+ar_status ar_job_cancel(ar_server* srv, uint64_t job_id);
+double ar_server_clock(const ar_server* srv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AGENTRAY_H */
