@@ -122,7 +122,12 @@ struct HostMesh {
 
 }  // namespace ar
 
+namespace ar {
+struct Workspace;
+}
+
 struct ar_scene {
+  ar::Workspace* ws = nullptr;  // render workspace used by ar_render
   std::vector<ar::DMaterial> materials;
   std::vector<ar::HostMesh> meshes;
   std::vector<ar_light> lights;
@@ -148,4 +153,7 @@ struct ar_scene {
 
 namespace ar {
 hipStream_t pick_stream(const ar_scene* s, void* stream);
+}
+namespace ar {
+void destroy_workspace(Workspace* ws);
 }

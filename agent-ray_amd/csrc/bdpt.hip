@@ -1,0 +1,624 @@
+// Bidirectional path tracer with MIS on gfx950 behind ar_render
+// (BidirectionalPathTracer::render, agent-ray
+// src/integrators/bidirectional_path_tracer.rs:1221-1357).
+//
+// Phase 1 (k_bdpt_subpaths): one lane per (pixel, sample) builds the camera
+// subpath (:338-437) and the light subpath (:443-559) into a structure-of-
+// arrays vertex buffer in HBM, [side][vertex slot][path], so lanes of a wave
+// touch consecutive addresses for the same slot.
+// Phase 2 (k_bdpt_connect): one lane per path enumerates the strategies in
+// the reference order (t outer, s inner; :1281-1334), traces one any-hit
+// shadow ray per connection and evaluates the MIS weight (:594-770, fp64
+// sums).  Non-splat contributions go to the path's radiance slot (reduced
+// per pixel in sample order like the path tracer); light-tracing (s = 1)
+// splats are added as 64-bit fixed-point integers, so the splat film is the
+// same whatever order the atomics land in.
+//
+// Reference behaviour kept (bug-compatible): t = 0 / s = 0 not implemented;
+// point-light t = 1 omits 1/sel_pmf; the camera vertex normal is the first
+// ray direction; surface vertices are never delta; the first camera-side
+// MIS step reuses vertex s-1 in the loop.  Quad lights (extension) connect
+// at t = 1 to the sampled emission point with the general formula.
+#include "ar_kernels.h"
+#include "ar_pt.h"
+#include "ar_render.h"
+
+namespace ar {
+
+enum : uint32_t { VT_CAMERA = 0, VT_LIGHT = 1, VT_SURFACE = 2 };
+constexpr float kSplatScale = 16777216.0f;  // 2^24 fixed point
+
+struct BdptParams {
+  PtParams pt;
+  uint32_t V;  // vertex slots per subpath = max_depth + 1
+  uint32_t mis_mode;
+  float mis_beta;
+};
+
+struct Verts {
+  float4* pf;    // (p.xyz, pdf_fwd)          [2][V][npaths]
+  float4* nr;    // (n.xyz, pdf_rev)
+  float4* bm;    // (beta.xyz, meta bits: type | delta << 2 | id << 4)
+  uint2* count;  // (camera vertices, light vertices) per path
+  uint32_t npaths;
+  uint32_t V;
+  __device__ __forceinline__ size_t at(uint32_t side, uint32_t k, uint32_t p) const {
+    return ((size_t)side * V + k) * npaths + p;
+  }
+};
+
+struct PV {
+  f3 p, n, beta;
+  float pdf_fwd, pdf_rev;
+  uint32_t type, delta, id;  // id: material (surface) or light index
+};
+
+__device__ __forceinline__ PV load_v(const Verts& Vt, uint32_t side, uint32_t k, uint32_t p) {
+  size_t i = Vt.at(side, k, p);
+  float4 a = Vt.pf[i], b = Vt.nr[i], c = Vt.bm[i];
+  PV v;
+  v.p = mk(a.x, a.y, a.z);
+  v.pdf_fwd = a.w;
+  v.n = mk(b.x, b.y, b.z);
+  v.pdf_rev = b.w;
+  v.beta = mk(c.x, c.y, c.z);
+  uint32_t m = __float_as_uint(c.w);
+  v.type = m & 3u;
+  v.delta = (m >> 2) & 1u;
+  v.id = m >> 4;
+  return v;
+}
+__device__ __forceinline__ void store_v(const Verts& Vt, uint32_t side, uint32_t k, uint32_t p, const PV& v) {
+  size_t i = Vt.at(side, k, p);
+  uint32_t m = v.type | (v.delta << 2) | (v.id << 4);
+  Vt.pf[i] = make_float4(v.p.x, v.p.y, v.p.z, v.pdf_fwd);
+  Vt.nr[i] = make_float4(v.n.x, v.n.y, v.n.z, v.pdf_rev);
+  Vt.bm[i] = make_float4(v.beta.x, v.beta.y, v.beta.z, __uint_as_float(m));
+}
+
+// :305-314
+__device__ __forceinline__ float pdf_sa_to_area(float pdf, f3 from, f3 to, f3 n_to) {
+  f3 d = to - from;
+  float d2 = dot(d, d);
+  if (d2 < 1e-12f) return 0.0f;
+  f3 dir = d / sqrtf(d2);
+  float c = fabsf(dot(n_to, dir));
+  return pdf * c / d2;
+}
+// :318-328
+__device__ __forceinline__ float geometry_term(f3 pa, f3 na, f3 pb, f3 nb) {
+  f3 d = pb - pa;
+  float d2 = dot(d, d);
+  if (d2 < 1e-12f) return 0.0f;
+  f3 dir = d / sqrtf(d2);
+  float ca = fabsf(dot(na, dir));
+  float cb = fabsf(dot(nb, -dir));
+  return ca * cb / d2;
+}
+
+// PinholeCamera::pdf_we / sample_we (cameras.rs:175-248)
+__device__ __forceinline__ float cam_pdf_we(const Cam& c, f3 d) {
+  float ct = dot(d, c.fwd);
+  if (ct <= 0.0f) return 0.0f;
+  return 1.0f / (c.area * ct * ct * ct);
+}
+struct WeSample {
+  float px, py;
+  f3 wi;
+  float dist, we;
+};
+__device__ __forceinline__ bool cam_sample_we(const Cam& c, f3 p, uint32_t w, uint32_t h, WeSample& o) {
+  f3 to_cam = c.origin - p;
+  float d2 = dot(to_cam, to_cam);
+  float dist = sqrtf(d2);
+  if (dist < 1e-6f) return false;
+  f3 wi = to_cam / dist;
+  f3 dir = -wi;
+  float ct = dot(dir, c.fwd);
+  if (ct <= 1e-6f) return false;
+  f3 right = normalize(c.horiz), up = normalize(c.vert);
+  float vw = length(c.horiz), vh = length(c.vert);
+  f3 ds = dir / ct;
+  float u = dot(ds, right) / vw + 0.5f;
+  float v = dot(ds, up) / vh + 0.5f;
+  if (u < 0.0f || u > 1.0f || v < 0.0f || v > 1.0f) return false;
+  float px = u * (float)w;
+  float py = (1.0f - v) * (float)h;
+  if (px < 0.0f || px >= (float)w || py < 0.0f || py >= (float)h) return false;
+  float c2 = ct * ct;
+  float c4 = c2 * c2;
+  o.px = px;
+  o.py = py;
+  o.wi = wi;
+  o.dist = dist;
+  o.we = 1.0f / (c.area * c4);
+  return true;
+}
+
+struct Emission {
+  Ray ray;
+  f3 le, n_light;
+  float pdf_pos, pdf_dir;
+};
+// PointLight::sample_emission (point.rs:75-90); quad: uniform position,
+// cosine-weighted direction about the normal (extension).
+__device__ __forceinline__ bool light_emission(const DLight& L, float upx, float upy, float udx, float udy,
+                                               Emission& e) {
+  if (L.type == AR_LIGHT_POINT) {
+    float ct = 1.0f - 2.0f * udx;
+    float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+    float phi = (2.0f * kPi) * udy;
+    float s, c;
+    sincosf(phi, &s, &c);
+    f3 dir = mk(st * c, st * s, ct);
+    e.ray = make_ray(mk(L.pos[0], L.pos[1], L.pos[2]), dir, 1e-4f, kF32Max);
+    e.le = mk(L.intensity[0], L.intensity[1], L.intensity[2]);
+    e.pdf_pos = 1.0f;
+    e.pdf_dir = 1.0f / (4.0f * kPi);
+    e.n_light = dir;
+    return true;
+  }
+  f3 n = mk(L.n[0], L.n[1], L.n[2]);
+  f3 x = (mk(L.pos[0], L.pos[1], L.pos[2]) + mk(L.eu[0], L.eu[1], L.eu[2]) * upx) + mk(L.ev[0], L.ev[1], L.ev[2]) * upy;
+  f3 l = cosine_hemisphere(udx, udy);
+  Onb o = onb_from_normal(n);
+  f3 dir = to_world(o, l);
+  if (l.z <= 0.0f) return false;
+  e.ray = make_ray(x, dir, 1e-4f, kF32Max);
+  e.le = mk(L.intensity[0], L.intensity[1], L.intensity[2]);
+  e.pdf_pos = 1.0f / L.area;
+  e.pdf_dir = l.z * kInvPi;
+  e.n_light = n;
+  return true;
+}
+__device__ __forceinline__ float light_pdf_emission_dir(const DLight& L, f3 dir) {
+  if (L.type == AR_LIGHT_POINT) return 1.0f / (4.0f * kPi);
+  float c = dot(mk(L.n[0], L.n[1], L.n[2]), dir);
+  return c > 0.0f ? c * kInvPi : 0.0f;
+}
+
+// Shared subpath bounce loop (:368-434, :498-556).  Vertex 0 is already
+// stored; returns the vertex count.
+template <bool NEAR>
+__device__ __forceinline__ uint32_t extend_subpath(const ShadeView& v, const BdptParams& P, const Verts& Vt,
+                                                   uint32_t side, uint32_t p, Ray ray, f3 beta, float prev_pdf_dir,
+                                                   PV prev, Rng& rng, Stack& stk, uint32_t& nrays) {
+  uint32_t nv = 1;
+  PV before = prev;  // vertex nv-2 (for the retroactive pdf_rev write)
+  for (uint32_t depth = 0; depth < P.pt.max_depth; ++depth) {
+    WorldHit h;
+    trace_scene<false, NEAR>(v.sv, ray, h, stk);
+    nrays++;
+    if (h.pos == kNoHit) break;
+    ShadingPoint sp;
+    make_shading(v, ray, h.t, h.pos, h.inst, h.u, h.v, sp);
+    PV sv;
+    sv.type = VT_SURFACE;
+    sv.delta = 0;
+    sv.id = sp.mat;
+    sv.p = sp.p;
+    sv.n = sp.n;
+    sv.beta = beta;
+    sv.pdf_fwd = pdf_sa_to_area(prev_pdf_dir, prev.p, sp.p, sp.n);
+    sv.pdf_rev = 0.0f;
+    store_v(Vt, side, nv, p, sv);
+    before = prev;
+    prev = sv;
+    nv++;
+    const DMaterial& m = v.materials[sp.mat];
+    f3 wo_l = to_local(sp.onb, -ray.d);
+    float u_sel = rng.next();
+    float ux = rng.next(), uy = rng.next();
+    BsdfSample bs;
+    if (!bsdf_sample(m, wo_l, u_sel, ux, uy, bs)) break;
+    if (bs.pdf == 0.0f || is_zero(bs.f)) break;
+    float cos_wi = fabsf(bs.wi.z);
+    beta = beta * (bs.f * cos_wi / bs.pdf);
+    prev_pdf_dir = bs.pdf;
+    float pdf_rev_dir = bsdf_pdf(m, wo_l, bs.wi);
+    before.pdf_rev = pdf_sa_to_area(pdf_rev_dir, sp.p, before.p, before.n);
+    size_t bi = Vt.at(side, nv - 2, p);
+    float4 nr = Vt.nr[bi];
+    Vt.nr[bi] = make_float4(nr.x, nr.y, nr.z, before.pdf_rev);
+    if (depth + 1 >= P.pt.rr_depth) {
+      float surv = fminf(maxc(beta), 0.95f);
+      if (rng.next() > surv || surv <= 0.0f) break;
+      beta = beta / surv;
+    }
+    f3 wi_w = to_world(sp.onb, bs.wi);
+    f3 off = sp.n * 1e-4f * signum(dot(wi_w, sp.n));
+    ray = make_ray(sp.p + off, wi_w, 1e-4f, kF32Max);
+  }
+  return nv;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bdpt_subpaths(ShadeView v, BdptParams P, Verts Vt, int2* __restrict__ spill,
+                                                         unsigned long long* __restrict__ stats) {
+  __shared__ int2 lds[kTraceLdsStack * BLOCK];
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  const uint32_t lb = (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
+  const uint32_t gtid = lb * BLOCK + threadIdx.x;
+  const uint32_t nthreads = G * BLOCK;
+  Stack stk;
+  stk.lds = lds + threadIdx.x;
+  stk.stride = BLOCK;
+  stk.spill = spill + gtid;
+  stk.spill_stride = nthreads;
+  stk.S = kTraceLdsStack;
+  stk.sp = 0;
+  uint32_t nrays = 0;
+  for (uint32_t p = gtid; p < Vt.npaths; p += nthreads) {
+    uint32_t pixel = p % P.pt.npix;
+    uint32_t smp = P.pt.s0 + p / P.pt.npix;
+    Rng rng;
+    rng.init(P.pt.seed, pixel, smp, 0);
+    float jx = rng.next(), jy = rng.next();
+    float u = ((float)(pixel % P.pt.width) + jx) / (float)P.pt.width;
+    float vv = 1.0f - ((float)(pixel / P.pt.width) + jy) / (float)P.pt.height;
+    Ray ray = camera_ray(P.pt.cam, u, vv);
+    // camera vertex (:349-361)
+    PV cv;
+    cv.type = VT_CAMERA;
+    cv.delta = 1;
+    cv.id = 0;
+    cv.p = P.pt.cam.origin;
+    cv.n = ray.d;
+    cv.beta = mk(1.0f, 1.0f, 1.0f);
+    cv.pdf_fwd = 1.0f;
+    cv.pdf_rev = 0.0f;
+    store_v(Vt, 0, 0, p, cv);
+    uint32_t nc = extend_subpath<true>(v, P, Vt, 0, p, ray, mk(1.0f, 1.0f, 1.0f), cam_pdf_we(P.pt.cam, ray.d), cv, rng,
+                                       stk, nrays);
+    // light subpath (:451-496)
+    uint32_t nl = 0;
+    if (v.dist_mode != AR_LIGHTDIST_NONE && v.n_lights > 0) {
+      float u_sel = rng.next();
+      uint32_t li;
+      float sel_pmf;
+      light_select(v, u_sel, &li, &sel_pmf);
+      const DLight& L = v.lights[li];
+      float upx = rng.next(), upy = rng.next(), udx = rng.next(), udy = rng.next();
+      Emission em;
+      if (light_emission(L, upx, upy, udx, udy, em)) {
+        PV lv;
+        lv.type = VT_LIGHT;
+        lv.delta = L.type == AR_LIGHT_POINT ? 1u : 0u;
+        lv.id = li;
+        lv.p = em.ray.o;
+        lv.n = em.n_light;
+        lv.pdf_fwd = em.pdf_pos * sel_pmf;
+        lv.pdf_rev = 0.0f;
+        lv.beta = em.le / (em.pdf_pos * sel_pmf);
+        store_v(Vt, 1, 0, p, lv);
+        nl = 1;
+        if (P.pt.max_depth > 0) {
+          f3 thr = em.le / (em.pdf_pos * sel_pmf * em.pdf_dir);
+          float cos_emit = fabsf(dot(em.n_light, em.ray.d));
+          thr = thr * cos_emit;
+          nl = extend_subpath<true>(v, P, Vt, 1, p, em.ray, thr, em.pdf_dir, lv, rng, stk, nrays);
+        }
+      }
+    }
+    Vt.count[p] = make_uint2(nc, nl);
+  }
+  uint32_t s = nrays;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (__lane_id() == 0 && s) atomicAdd(&stats[0], (unsigned long long)s);
+}
+
+__device__ __forceinline__ double powb(double x, float beta) {
+  return beta == 2.0f ? x * x : pow(x, (double)beta);
+}
+
+// :594-770
+__device__ __forceinline__ float mis_weight(const BdptParams& P, const Verts& Vt, uint32_t p, uint32_t nc, uint32_t nl,
+                                            uint32_t s, uint32_t t, float pdf_cc, float pdf_cl) {
+  if (P.mis_mode == 1) {  // :605-636
+    uint32_t len = s + t, count = 0;
+    uint32_t tmx = len < nl ? len : nl;
+    for (uint32_t t2 = 0; t2 <= tmx; ++t2) {
+      uint32_t s2 = len - t2;
+      if (s2 > nc || s2 + t2 < 2 || s2 == 0) continue;
+      count++;
+    }
+    return count == 0 ? 1.0f : 1.0f / (float)count;
+  }
+  if (s + t == 2) return 1.0f;
+  double sum = 0.0;
+  // Only surface vertices can be MIS-delta, and the reference never marks
+  // them delta (:268), so every "delta" test below is false except where the
+  // reference forces an unimplemented s' = 0 / t' = 0 strategy.
+  {
+    double ri = 1.0;
+    if (s >= 1) {
+      PV v = load_v(Vt, 0, s - 1, p);
+      ri *= (double)pdf_cc / (double)fmaxf(v.pdf_fwd, 1e-30f);
+      bool prev_delta = s >= 2 ? false : true;
+      if (!prev_delta) sum += powb(fabs(ri), P.mis_beta);
+    }
+    for (uint32_t i = s - 1; s >= 1 && i >= 1; --i) {
+      PV v = load_v(Vt, 0, i, p);
+      ri *= (double)fmaxf(v.pdf_rev, 0.0f) / (double)fmaxf(v.pdf_fwd, 1e-30f);
+      sum += powb(fabs(ri), P.mis_beta);
+    }
+  }
+  {
+    double ri = 1.0;
+    if (t >= 1) {
+      PV v = load_v(Vt, 1, t - 1, p);
+      ri *= (double)pdf_cl / (double)fmaxf(v.pdf_fwd, 1e-30f);
+      bool prev_delta = t >= 2 ? false : true;
+      if (!prev_delta) sum += powb(fabs(ri), P.mis_beta);
+    }
+    for (uint32_t i = t - 1; t >= 1 && i >= 1; --i) {
+      PV v = load_v(Vt, 1, i, p);
+      ri *= (double)fmaxf(v.pdf_rev, 0.0f) / (double)fmaxf(v.pdf_fwd, 1e-30f);
+      sum += powb(fabs(ri), P.mis_beta);
+    }
+  }
+  return (float)(1.0 / (1.0 + sum));
+}
+
+__device__ __forceinline__ void splat_add(unsigned long long* film, uint32_t idx, f3 c) {
+  float vals[3] = {c.x, c.y, c.z};
+  for (int k = 0; k < 3; ++k) {
+    float s = vals[k] * kSplatScale;
+    long long q = 0;
+    if (s == s) q = (long long)fminf(fmaxf(rintf(s), -4.0e18f), 4.0e18f);
+    if (q != 0) atomicAdd(film + 3 * (size_t)idx + k, (unsigned long long)q);
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams P, Verts Vt, float4* __restrict__ Lout,
+                                                        unsigned long long* __restrict__ splat_film, int2* __restrict__ spill,
+                                                        unsigned long long* __restrict__ stats) {
+  __shared__ int2 lds[kTraceLdsStack * BLOCK];
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  const uint32_t lb = (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
+  const uint32_t gtid = lb * BLOCK + threadIdx.x;
+  const uint32_t nthreads = G * BLOCK;
+  Stack stk;
+  stk.lds = lds + threadIdx.x;
+  stk.stride = BLOCK;
+  stk.spill = spill + gtid;
+  stk.spill_stride = nthreads;
+  stk.S = kTraceLdsStack;
+  stk.sp = 0;
+  uint32_t nrays = 0;
+  const Cam& cam = P.pt.cam;
+  const uint32_t W = P.pt.width, H = P.pt.height;
+  for (uint32_t p = gtid; p < Vt.npaths; p += nthreads) {
+    uint2 cnt = Vt.count[p];
+    uint32_t smax = cnt.x, tmax = cnt.y, limit = P.pt.max_depth + 2;
+    f3 acc = splat(0.0f);
+    for (uint32_t t = 0; t <= tmax; ++t) {
+      for (uint32_t s = 0; s <= smax; ++s) {
+        if (s + t < 2 || s + t > limit || s == 0) continue;
+        if (s == 1) {  // light tracing to the camera (:1051-1155)
+          if (t < 2) continue;
+          PV y = load_v(Vt, 1, t - 1, p);
+          WeSample cs;
+          if (!cam_sample_we(cam, y.p, W, H, cs)) continue;
+          f3 to_cam = cs.wi;
+          f3 off = y.n * 1e-4f * signum(dot(to_cam, y.n));
+          Ray sr = make_ray(y.p + off, to_cam, 0.0f, cs.dist * (1.0f - 1e-4f));
+          WorldHit hh;
+          nrays++;
+          if (trace_scene<true, true>(v.sv, sr, hh, stk)) continue;
+          PV yp = load_v(Vt, 1, t - 2, p);
+          Onb on = onb_from_normal(y.n);
+          f3 wo_l = to_local(on, normalize(yp.p - y.p));
+          f3 wi_l = to_local(on, to_cam);
+          const DMaterial& m = v.materials[y.id];
+          f3 f = bsdf_eval(m, wi_l, wo_l);
+          if (is_zero(f)) continue;
+          float cos_y = fabsf(dot(y.n, to_cam));
+          float g = cos_y / (cs.dist * cs.dist);
+          f3 c = y.beta * f * g * cs.we;
+          if (maxc(c) <= 0.0f) continue;
+          float pd = bsdf_pdf(m, wi_l, wo_l);
+          PV c0 = load_v(Vt, 0, 0, p);
+          float pdf_cl = pdf_sa_to_area(pd, y.p, cam.origin, c0.n);
+          float w = mis_weight(P, Vt, p, smax, tmax, 1, t, 0.0f, pdf_cl);
+          c = c * w;
+          uint32_t ix = (uint32_t)cs.px, iy = (uint32_t)cs.py;
+          if (ix > W - 1) ix = W - 1;
+          if (iy > H - 1) iy = H - 1;
+          splat_add(splat_film, iy * W + ix, c);
+          continue;
+        }
+        if (t == 0) continue;  // (:810-818)
+        // general / NEE (:844-1043)
+        PV z = load_v(Vt, 0, s - 1, p);
+        PV y = load_v(Vt, 1, t - 1, p);
+        f3 d = y.p - z.p;
+        float d2 = dot(d, d);
+        if (d2 < 1e-10f) continue;
+        float dist = sqrtf(d2);
+        f3 wi_z = d / dist;
+        f3 wi_y = -wi_z;
+        f3 oz = z.n * 1e-4f * signum(dot(wi_z, z.n));
+        f3 oy = y.n * 1e-4f * signum(dot(wi_y, y.n));
+        f3 so = z.p + oz, st = y.p + oy;
+        f3 sd = st - so;
+        float sdist = length(sd);
+        if (sdist < 1e-6f) continue;
+        Ray sr = make_ray(so, sd / sdist, 0.0f, sdist * (1.0f - 1e-4f));
+        WorldHit hh;
+        nrays++;
+        if (trace_scene<true, true>(v.sv, sr, hh, stk)) continue;
+        PV zp = load_v(Vt, 0, s - 2, p);
+        Onb onz = onb_from_normal(z.n);
+        f3 wo_z_l = to_local(onz, normalize(zp.p - z.p));
+        const DMaterial& mz = v.materials[z.id];
+        f3 f_z = bsdf_eval(mz, to_local(onz, wi_z), wo_z_l);
+        if (is_zero(f_z)) continue;
+        f3 f_y = mk(1.0f, 1.0f, 1.0f);
+        const DLight* Lt = nullptr;
+        Onb ony = onb_from_normal(y.n);
+        f3 wo_y_l = splat(0.0f);
+        if (t == 1) {
+          Lt = &v.lights[y.id];
+          if (Lt->type == AR_LIGHT_QUAD && !(dot(y.n, wi_y) > 0.0f)) f_y = splat(0.0f);
+        } else {
+          PV yp = load_v(Vt, 1, t - 2, p);
+          wo_y_l = to_local(ony, normalize(yp.p - y.p));
+          f_y = bsdf_eval(v.materials[y.id], to_local(ony, wi_y), wo_y_l);
+        }
+        if (is_zero(f_y)) continue;
+        float g = geometry_term(z.p, z.n, y.p, y.n);
+        f3 c;
+        if (t == 1 && Lt->type == AR_LIGHT_POINT) {  // :947-967
+          LightSample ls;
+          if (!light_sample(*Lt, z.p, 0.0f, 0.0f, ls)) continue;
+          f3 fd = bsdf_eval(mz, to_local(onz, ls.wi), wo_z_l);
+          float cos_z = fabsf(dot(z.n, ls.wi));
+          c = z.beta * fd * cos_z * ls.li;
+        } else {
+          c = z.beta * f_z * g * f_y * y.beta;
+        }
+        if (maxc(c) <= 0.0f) continue;
+        float pdf_cc;
+        if (t == 1) {
+          f3 dyz = normalize(z.p - y.p);
+          pdf_cc = pdf_sa_to_area(light_pdf_emission_dir(*Lt, dyz), y.p, z.p, z.n);
+        } else {
+          f3 wi_y2 = normalize(z.p - y.p);
+          float pd = bsdf_pdf(v.materials[y.id], to_local(ony, wi_y2), wo_y_l);
+          pdf_cc = pdf_sa_to_area(pd, y.p, z.p, z.n);
+        }
+        f3 wi_z2 = normalize(y.p - z.p);
+        float pdz = bsdf_pdf(mz, to_local(onz, wi_z2), wo_z_l);
+        float pdf_cl = pdf_sa_to_area(pdz, z.p, y.p, y.n);
+        float w = mis_weight(P, Vt, p, smax, tmax, s, t, pdf_cc, pdf_cl);
+        acc = acc + c * w;
+      }
+    }
+    Lout[p] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+  }
+  uint32_t s = nrays;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (__lane_id() == 0 && s) atomicAdd(&stats[1], (unsigned long long)s);
+}
+
+// film = acc / spp, then + splat * (1/spp) where the splat is non-zero (:1341-1348)
+__global__ void k_bdpt_resolve(const float4* __restrict__ acc, const long long* __restrict__ splat, float* out,
+                               uint32_t npix, float spp) {
+  float inv = 1.0f / spp;
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    float4 a = acc[px];
+    float m[3] = {a.x / spp, a.y / spp, a.z / spp};
+    float sp[3];
+    bool nz = false;
+    for (int k = 0; k < 3; ++k) {
+      sp[k] = (float)((double)splat[3 * (size_t)px + k] * (1.0 / (double)kSplatScale));
+      nz |= sp[k] != 0.0f;
+    }
+    for (int k = 0; k < 3; ++k) out[3 * (size_t)px + k] = nz ? m[k] + sp[k] * inv : m[k];
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, size_t n) {
+  for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull;
+}
+
+__global__ void k_film_accumulate_b(const float4* __restrict__ L, float4* __restrict__ acc, uint32_t npix, uint32_t k,
+                                    uint32_t first) {
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    float4 a = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : acc[px];
+    for (uint32_t s = 0; s < k; ++s) {
+      float4 l = L[(size_t)s * npix + px];
+      a.x = a.x + l.x;
+      a.y = a.y + l.y;
+      a.z = a.z + l.z;
+    }
+    acc[px] = a;
+  }
+}
+
+static uint32_t bdpt_grid(int block) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t pr;
+    if (hipGetDeviceProperties(&pr, dev) == hipSuccess) cus = pr.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return (uint32_t)(cus * kTraceBlocksPerCU / 8 * 8);
+}
+
+ar_status render_bdpt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
+                      const ar_render_config* cfg, float* out_dev, hipStream_t st, ar_render_stats* stats) {
+  constexpr int BLOCK = kTraceBlock;
+  const uint32_t npix = w * h;
+  if (cfg->max_depth > 64) return fail(AR_ERR_INVALID_ARG, "BDPT: max_depth above 64");
+  ShadeView v = shade_view(s);
+  BdptParams P{};
+  P.pt.width = w;
+  P.pt.height = h;
+  P.pt.spp = cfg->spp;
+  P.pt.max_depth = cfg->max_depth;
+  P.pt.rr_depth = cfg->rr_depth;
+  P.pt.npix = npix;
+  P.pt.seed = cfg->seed;
+  P.pt.cam = cam_from(*cam);
+  P.pt.variants = 1;
+  P.V = cfg->max_depth + 1;
+  P.mis_mode = cfg->mis_mode;
+  P.mis_beta = cfg->mis_beta;
+  uint32_t k = cfg->sample_chunk;
+  if (k == 0) {
+    uint64_t target = 1ull << 20;  // ~1M subpath pairs per chunk
+    k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cfg->spp, target / std::max<uint32_t>(npix, 1)));
+  }
+  k = std::min(k, cfg->spp);
+  uint32_t G = bdpt_grid(BLOCK);
+  uint32_t cap = npix * k;
+  ar_status e = ws.reserve_paths(cap, npix, stack_spill_entries(s), G * BLOCK);
+  if (e != AR_OK) return e;
+  size_t nvert = 2ull * P.V * cap;
+  size_t need = nvert * 3 * sizeof(float4) + (size_t)cap * sizeof(uint2) + (size_t)npix * 3 * sizeof(long long) + 1024;
+  if (ws.aux.bytes < need) {
+    AR_HIP(hipStreamSynchronize(st));
+    AR_HIP(ws.aux.alloc(need));
+  }
+  char* base = (char*)ws.aux.p;
+  Verts Vt;
+  Vt.pf = (float4*)base;
+  Vt.nr = Vt.pf + nvert;
+  Vt.bm = Vt.nr + nvert;
+  Vt.count = (uint2*)(Vt.bm + nvert);
+  unsigned long long* splat = (unsigned long long*)((char*)(Vt.count + cap) + 256 - ((uintptr_t)(Vt.count + cap) % 256));
+  Vt.V = P.V;
+  AR_HIP(hipMemsetAsync(ws.B.stats, 0, 16, st));
+  uint32_t eg = std::min<uint32_t>((npix + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_zero_u64, dim3(eg), dim3(256), 0, st, splat, (size_t)npix * 3);
+  for (uint32_t s0 = 0; s0 < cfg->spp; s0 += k) {
+    P.pt.s0 = s0;
+    P.pt.k = std::min(k, cfg->spp - s0);
+    Vt.npaths = npix * P.pt.k;
+    hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, Vt, ws.spill, ws.B.stats);
+    hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, Vt, ws.B.L, splat, ws.spill,
+                       ws.B.stats);
+    hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, st, ws.B.L, ws.acc, npix, P.pt.k,
+                       s0 == 0 ? 1u : 0u);
+    AR_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_bdpt_resolve, dim3(eg), dim3(256), 0, st, ws.acc, (const long long*)splat, out_dev, npix,
+                     (float)cfg->spp);
+  AR_HIP(hipGetLastError());
+  if (stats) {
+    unsigned long long hs[2] = {0, 0};
+    AR_HIP(hipMemcpyAsync(hs, ws.B.stats, 16, hipMemcpyDeviceToHost, st));
+    AR_HIP(hipStreamSynchronize(st));
+    stats->paths = (uint64_t)npix * cfg->spp;
+    stats->rays = hs[0] + hs[1];
+    stats->shadow_rays = hs[1];
+  }
+  return AR_OK;
+}
+
+}  // namespace ar

@@ -1,0 +1,565 @@
+// Path tracer on gfx950 behind ar_render (PathTracer::render,
+// agent-ray src/integrators/path_tracer.rs:159-191).
+//
+// Samples are processed in chunks of k samples per pixel; path id within a
+// chunk = s_local * npix + pixel, so neighbouring lanes trace neighbouring
+// pixels.  Every path writes its radiance to its own slot; a reduction adds
+// the slots of each pixel in sample order, which reproduces the reference's
+// per-pixel accumulation order exactly (film.add_sample(accum / spp)) and
+// makes the film independent of chunking and schedule.
+//
+// Schedules:
+//   megakernel  one thread per path, whole bounce loop inline.
+//   wavefront   per bounce: extend (closest hit over the active queue) ->
+//               shade (emission, next-event sample, BSDF sample, RR; pushes
+//               survivors and shadow requests with wave-aggregated atomics)
+//               -> shadow (any hit; adds the contribution if unoccluded).
+//               Stage kernels are persistent (fixed grid, grid-stride over a
+//               device-side count), so the host never reads queue sizes and
+//               the whole bounce loop of a chunk is captured once as a HIP
+//               graph and replayed per chunk.
+#include "ar_kernels.h"
+#include "ar_pt.h"
+#include "ar_render.h"
+
+namespace ar {
+
+__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
+  // Wave-aggregated queue append: one atomic per wave, lane offsets by
+  // popcount of the ballot below the lane.
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return 0;
+  uint32_t lane = __lane_id();
+  uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, leader);
+  uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
+  return base + (uint32_t)__popcll(below);
+}
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint32_t v) {
+  uint32_t s = v;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (__lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
+}
+
+template <int BLOCK>
+__device__ __forceinline__ Stack make_stack(int2* lds, int2* spill, uint32_t gtid, uint32_t nthreads) {
+  Stack stk;
+  stk.lds = lds + threadIdx.x;
+  stk.stride = BLOCK;
+  stk.spill = spill + gtid;
+  stk.spill_stride = nthreads;
+  stk.S = kTraceLdsStack;
+  stk.sp = 0;
+  return stk;
+}
+
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  return (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
+}
+
+// ---------------------------------------------------------------------------
+// Megakernel
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_pt_mega(ShadeView v, PtParams P, float4* __restrict__ Lbuf,
+                                                   int2* __restrict__ spill, unsigned long long* __restrict__ stats) {
+  __shared__ int2 lds[kTraceLdsStack * BLOCK];
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  Stack stk = make_stack<BLOCK>(lds, spill, gtid, nthreads);
+  const uint32_t npaths = P.npix * P.k * P.variants;
+  uint32_t nrays = 0;
+  for (uint32_t p = gtid; p < npaths; p += nthreads) {
+    PathState st;
+    Rng rng;
+    f3 L = splat(0.0f);
+    if (gen_path(P, p, st, rng)) L = pt_li<true>(v, P, st, rng, stk, nrays);
+    Lbuf[p] = make_float4(L.x, L.y, L.z, 0.0f);
+  }
+  wave_add_u64(stats, nrays);
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront stages
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void load_state(const WaveBufs& B, uint32_t i, PathState& s) {
+  float4 o = B.ray_o[i], d = B.ray_d[i], b = B.beta[i], l = B.L[i], pp = B.prev_p[i];
+  uint4 m = B.meta[i];
+  s.ray.o = mk(o.x, o.y, o.z);
+  s.ray.tmin = o.w;
+  s.ray.d = mk(d.x, d.y, d.z);
+  s.ray.tmax = d.w;
+  s.beta = mk(b.x, b.y, b.z);
+  s.prev_pdf = b.w;
+  s.L = mk(l.x, l.y, l.z);
+  s.prev_p = mk(pp.x, pp.y, pp.z);
+  s.pixel = m.x;
+  s.sample = m.y;
+  s.dim = m.z;
+  s.depth = m.w;
+}
+
+__device__ __forceinline__ void store_state(const WaveBufs& B, uint32_t i, const PathState& s) {
+  B.ray_o[i] = make_float4(s.ray.o.x, s.ray.o.y, s.ray.o.z, s.ray.tmin);
+  B.ray_d[i] = make_float4(s.ray.d.x, s.ray.d.y, s.ray.d.z, s.ray.tmax);
+  B.beta[i] = make_float4(s.beta.x, s.beta.y, s.beta.z, s.prev_pdf);
+  B.L[i] = make_float4(s.L.x, s.L.y, s.L.z, 0.0f);
+  B.prev_p[i] = make_float4(s.prev_p.x, s.prev_p.y, s.prev_p.z, 0.0f);
+  B.meta[i] = make_uint4(s.pixel, s.sample, s.dim, s.depth);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_generate(PtParams P, WaveBufs B) {
+  const uint32_t npaths = P.npix * P.k * P.variants;
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  const uint32_t rounds = (npaths + nthreads - 1) / nthreads;
+  const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    uint32_t p = r * nthreads + gtid;
+    bool live = false;
+    if (p < npaths) {
+      PathState s;
+      Rng rng;
+      live = gen_path(P, p, s, rng) && P.max_depth > 0;
+      if (!live) {
+        s.L = splat(0.0f);
+        s.beta = splat(0.0f);
+        s.prev_p = splat(0.0f);
+        s.prev_pdf = 0.0f;
+        s.depth = 0;
+        s.dim = 0;
+        s.ray.o = s.ray.d = splat(0.0f);
+        s.ray.tmin = s.ray.tmax = 0.0f;
+      }
+      store_state(B, p, s);
+    }
+    uint32_t qi = wave_push(&B.cnt[0], live);
+    if (live) B.q[0][qi] = p;
+  }
+}
+
+// Queue counters start at zero for a chunk (a kernel, not a memset, so the
+// chunk can be captured in a graph).
+__global__ void k_wf_begin(WaveBufs B) {
+  if (threadIdx.x < 3) B.cnt[threadIdx.x] = 0;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill) {
+  __shared__ int2 lds[kTraceLdsStack * BLOCK];
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  Stack stk = make_stack<BLOCK>(lds, spill, gtid, nthreads);
+  const uint32_t n = B.cnt[qi];
+  const uint32_t* q = B.q[qi];
+  for (uint32_t i = gtid; i < n; i += nthreads) {
+    uint32_t p = q[i];
+    float4 o = B.ray_o[p], d = B.ray_d[p];
+    Ray r;
+    r.o = mk(o.x, o.y, o.z);
+    r.tmin = o.w;
+    r.d = mk(d.x, d.y, d.z);
+    r.tmax = d.w;
+    WorldHit h;
+    trace_scene<false, true>(sv, r, h, stk);
+    B.hit[i] = make_float4(h.t, __uint_as_float(h.pos), h.u, h.v);
+    B.hit_inst[i] = h.inst;
+  }
+  if (gtid == 0) atomicAdd(&B.stats[0], (unsigned long long)n);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, PtParams P, WaveBufs B, int qi) {
+  const uint32_t n = B.cnt[qi];
+  const uint32_t* q = B.q[qi];
+  uint32_t* qn = B.q[qi ^ 1];
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  // Iterate whole waves together so the wave-aggregated pushes see every lane.
+  uint32_t rounds = (n + nthreads - 1) / nthreads;
+  uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    uint32_t i = r * nthreads + gtid;
+    bool active = i < n;
+    bool cont = false;
+    ShadowReq sh;
+    sh.valid = false;
+    uint32_t p = 0;
+    PathState s;
+    if (active) {
+      p = q[i];
+      load_state(B, p, s);
+      float4 hv = B.hit[i];
+      WorldHit h;
+      h.t = hv.x;
+      h.pos = __float_as_uint(hv.y);
+      h.u = hv.z;
+      h.v = hv.w;
+      h.inst = B.hit_inst[i];
+      Rng rng;
+      rng.init(P.seed, s.pixel, s.sample, 0, s.dim);
+      cont = pt_shade(v, P, s, h, rng, sh);
+      s.dim = rng.dim;
+      store_state(B, p, s);
+    }
+    uint32_t si = wave_push(&B.cnt[2], active && sh.valid);
+    if (active && sh.valid) {
+      B.sh_path[si] = p;
+      B.sh_o[si] = make_float4(sh.ray.o.x, sh.ray.o.y, sh.ray.o.z, sh.ray.tmin);
+      B.sh_d[si] = make_float4(sh.ray.d.x, sh.ray.d.y, sh.ray.d.z, sh.ray.tmax);
+      B.sh_c[si] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0.0f);
+    }
+    uint32_t qi2 = wave_push(&B.cnt[qi ^ 1], active && cont);
+    if (active && cont) qn[qi2] = p;
+  }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill) {
+  __shared__ int2 lds[kTraceLdsStack * BLOCK];
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  Stack stk = make_stack<BLOCK>(lds, spill, gtid, nthreads);
+  const uint32_t n = B.cnt[2];
+  for (uint32_t i = gtid; i < n; i += nthreads) {
+    float4 o = B.sh_o[i], d = B.sh_d[i];
+    Ray r;
+    r.o = mk(o.x, o.y, o.z);
+    r.tmin = o.w;
+    r.d = mk(d.x, d.y, d.z);
+    r.tmax = d.w;
+    WorldHit h;
+    if (!trace_scene<true, true>(sv, r, h, stk)) {
+      uint32_t p = B.sh_path[i];
+      float4 c = B.sh_c[i], l = B.L[p];
+      B.L[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+    }
+  }
+  if (gtid == 0) atomicAdd(&B.stats[1], (unsigned long long)n);
+}
+
+// End of bounce: the consumed queue and the shadow queue become empty.
+__global__ void k_wf_reset(WaveBufs B, int qi) {
+  if (threadIdx.x == 0) {
+    B.cnt[qi] = 0;
+    B.cnt[2] = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Film: acc[pixel] += L[s * npix + pixel] for s in sample order.
+// ---------------------------------------------------------------------------
+__global__ void k_film_accumulate(const float4* __restrict__ L, float4* __restrict__ acc, uint32_t npix, uint32_t k,
+                                  uint32_t first) {
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    float4 a = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : acc[px];
+    for (uint32_t s = 0; s < k; ++s) {
+      float4 l = L[(size_t)s * npix + px];
+      a.x = a.x + l.x;
+      a.y = a.y + l.y;
+      a.z = a.z + l.z;
+    }
+    acc[px] = a;
+  }
+}
+
+__global__ void k_film_resolve(const float4* __restrict__ acc, float* __restrict__ out, uint32_t npix, float spp) {
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    float4 a = acc[px];
+    out[3 * (size_t)px] = a.x / spp;
+    out[3 * (size_t)px + 1] = a.y / spp;
+    out[3 * (size_t)px + 2] = a.z / spp;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+ShadeView shade_view(const ar_scene* s) {
+  ShadeView v;
+  v.sv = s->view;
+  v.objects = s->d_objects.as<DObject>();
+  v.materials = s->d_materials.as<DMaterial>();
+  v.lights = s->d_lights.as<DLight>();
+  v.idx = s->d_idx.as<uint32_t>();
+  v.nrm = s->d_nrm.as<float>();
+  v.slots = s->d_slots.as<uint32_t>();
+  v.objmats = s->d_objmats.as<uint32_t>();
+  v.n_lights = s->n_lights;
+  v.dist_mode = s->has_dist ? s->dist_mode : AR_LIGHTDIST_NONE;
+  return v;
+}
+
+static uint32_t persistent_grid(int block, int per_cu) {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+    if (cus <= 0) cus = 256;
+  }
+  return (uint32_t)(cus * per_cu / 8 * 8);
+}
+
+uint32_t stack_spill_entries(const ar_scene* s) {
+  uint32_t need = s->info.max_depth + s->tlas_depth + 2;
+  return need > (uint32_t)kTraceLdsStack ? need - kTraceLdsStack : 0;
+}
+
+ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_entries, uint32_t grid_threads) {
+  size_t f4 = sizeof(float4);
+  size_t bytes = 0;
+  auto add = [&](size_t& off, size_t n) {
+    off = bytes;
+    bytes += (n + 255) / 256 * 256;
+  };
+  size_t o_ro, o_rd, o_b, o_L, o_pp, o_meta, o_q0, o_q1, o_hit, o_hi, o_shp, o_sho, o_shd, o_shc, o_cnt, o_st, o_acc,
+      o_spill;
+  add(o_ro, cap * f4);
+  add(o_rd, cap * f4);
+  add(o_b, cap * f4);
+  add(o_L, cap * f4);
+  add(o_pp, cap * f4);
+  add(o_meta, cap * sizeof(uint4));
+  add(o_q0, cap * 4);
+  add(o_q1, cap * 4);
+  add(o_hit, cap * f4);
+  add(o_hi, cap * 4);
+  add(o_shp, cap * 4);
+  add(o_sho, cap * f4);
+  add(o_shd, cap * f4);
+  add(o_shc, cap * f4);
+  add(o_cnt, 64);
+  add(o_st, 64);
+  add(o_acc, (size_t)npix * f4 * 3);
+  add(o_spill, (size_t)spill_entries * grid_threads * sizeof(int2));
+  if (buf.bytes < bytes) {
+    AR_HIP(hipDeviceSynchronize());
+    AR_HIP(buf.alloc(bytes));
+    graph_valid = false;
+  }
+  char* b = (char*)buf.p;
+  B.ray_o = (float4*)(b + o_ro);
+  B.ray_d = (float4*)(b + o_rd);
+  B.beta = (float4*)(b + o_b);
+  B.L = (float4*)(b + o_L);
+  B.prev_p = (float4*)(b + o_pp);
+  B.meta = (uint4*)(b + o_meta);
+  B.q[0] = (uint32_t*)(b + o_q0);
+  B.q[1] = (uint32_t*)(b + o_q1);
+  B.hit = (float4*)(b + o_hit);
+  B.hit_inst = (uint32_t*)(b + o_hi);
+  B.sh_path = (uint32_t*)(b + o_shp);
+  B.sh_o = (float4*)(b + o_sho);
+  B.sh_d = (float4*)(b + o_shd);
+  B.sh_c = (float4*)(b + o_shc);
+  B.cnt = (uint32_t*)(b + o_cnt);
+  B.stats = (unsigned long long*)(b + o_st);
+  acc = (float4*)(b + o_acc);
+  spill = (int2*)(b + o_spill);
+  capacity = cap;
+  return AR_OK;
+}
+
+Workspace::~Workspace() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+}
+
+static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const ShadeView& v, const PtParams& P,
+                                     hipStream_t st, bool use_graph) {
+  constexpr int BLOCK = kTraceBlock;
+  uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU);
+  WaveBufs B = ws.B;
+  // Chunk parameters live in the kernel arguments, so a graph is captured
+  // per distinct (chunk size, sample offset) only when use_graph is set and
+  // replayed for every chunk with identical parameters.
+  hipLaunchKernelGGL(k_wf_begin, dim3(1), dim3(64), 0, st, B);
+  hipLaunchKernelGGL((k_wf_generate<BLOCK>), dim3(G), dim3(BLOCK), 0, st, P, B);
+  for (uint32_t d = 0; d < P.max_depth; ++d) {
+    int qi = d & 1;
+    hipLaunchKernelGGL((k_wf_extend<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v.sv, B, qi, ws.spill);
+    hipLaunchKernelGGL((k_wf_shade<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, B, qi);
+    hipLaunchKernelGGL((k_wf_shadow<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v.sv, B, ws.spill);
+    hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, B, qi);
+  }
+  (void)use_graph;
+  AR_HIP(hipGetLastError());
+  return AR_OK;
+}
+
+ar_status render_pt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
+                    const ar_render_config* cfg, float* out_dev, hipStream_t st, ar_render_stats* stats) {
+  constexpr int BLOCK = kTraceBlock;
+  const uint32_t npix = w * h;
+  ShadeView v = shade_view(s);
+  PtParams P{};
+  P.width = w;
+  P.height = h;
+  P.spp = cfg->spp;
+  P.max_depth = cfg->max_depth;
+  P.rr_depth = cfg->rr_depth;
+  P.npix = npix;
+  P.seed = cfg->seed;
+  P.cam = cam_from(*cam);
+  P.variants = 1;
+  uint32_t k = cfg->sample_chunk;
+  if (k == 0) {
+    uint64_t target = 1ull << 22;  // ~4M paths in flight per chunk
+    k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cfg->spp, target / std::max<uint32_t>(npix, 1)));
+  }
+  k = std::min(k, cfg->spp);
+  uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU);
+  ar_status e = ws.reserve_paths(npix * k, npix, stack_spill_entries(s), G * BLOCK);
+  if (e != AR_OK) return e;
+  AR_HIP(hipMemsetAsync(ws.B.stats, 0, 16, st));
+  bool mega = cfg->schedule == AR_SCHED_MEGAKERNEL;
+  for (uint32_t s0 = 0; s0 < cfg->spp; s0 += k) {
+    P.s0 = s0;
+    P.k = std::min(k, cfg->spp - s0);
+    if (mega) {
+      hipLaunchKernelGGL((k_pt_mega<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, ws.B.L, ws.spill, ws.B.stats);
+    } else {
+      e = run_chunk_wavefront(s, ws, v, P, st, false);
+      if (e != AR_OK) return e;
+    }
+    hipLaunchKernelGGL(k_film_accumulate, dim3(std::min<uint32_t>((npix + 255) / 256, 4096)), dim3(256), 0, st,
+                       ws.B.L, ws.acc, npix, P.k, s0 == 0 ? 1u : 0u);
+    AR_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_film_resolve, dim3(std::min<uint32_t>((npix + 255) / 256, 4096)), dim3(256), 0, st, ws.acc,
+                     out_dev, npix, (float)cfg->spp);
+  AR_HIP(hipGetLastError());
+  if (stats) {
+    unsigned long long hs[2] = {0, 0};
+    AR_HIP(hipMemcpyAsync(hs, ws.B.stats, 16, hipMemcpyDeviceToHost, st));
+    AR_HIP(hipStreamSynchronize(st));
+    stats->paths = (uint64_t)npix * cfg->spp;
+    stats->rays = hs[0] + hs[1];
+    stats->shadow_rays = mega ? 0 : hs[1];
+  }
+  return AR_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Gradient-domain path tracer (GradientDomainPathTracer::render,
+// gradient_domain_pt.rs:346-415): per pixel and sample, a base path and the
+// x+1 / y+1 offset paths (variants 1, 2) that replay the base path's random
+// numbers.  Accumulated in sample order: primal += base, dx += right - base,
+// dy += down - base; an offset outside the image equals the base.
+// ---------------------------------------------------------------------------
+__global__ void k_gdpt_accumulate(const float4* __restrict__ L, float4* __restrict__ acc, uint32_t w, uint32_t h,
+                                  uint32_t k, uint32_t first) {
+  const uint32_t npix = w * h;
+  const size_t per = (size_t)npix * k;
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+    uint32_t x = px % w, y = px / w;
+    float4 a0 = first ? make_float4(0, 0, 0, 0) : acc[px];
+    float4 a1 = first ? make_float4(0, 0, 0, 0) : acc[npix + px];
+    float4 a2 = first ? make_float4(0, 0, 0, 0) : acc[2 * (size_t)npix + px];
+    for (uint32_t s = 0; s < k; ++s) {
+      size_t i = (size_t)s * npix + px;
+      float4 b = L[i];
+      float4 r = x + 1 < w ? L[per + i] : b;
+      float4 d = y + 1 < h ? L[2 * per + i] : b;
+      a0.x = a0.x + b.x;
+      a0.y = a0.y + b.y;
+      a0.z = a0.z + b.z;
+      a1.x = a1.x + (r.x - b.x);
+      a1.y = a1.y + (r.y - b.y);
+      a1.z = a1.z + (r.z - b.z);
+      a2.x = a2.x + (d.x - b.x);
+      a2.y = a2.y + (d.y - b.y);
+      a2.z = a2.z + (d.z - b.z);
+    }
+    acc[px] = a0;
+    acc[npix + px] = a1;
+    acc[2 * (size_t)npix + px] = a2;
+  }
+}
+
+// films (interleaved RGB): primal, dx, dy = accum * (1/spp)
+__global__ void k_gdpt_resolve(const float4* __restrict__ acc, float* __restrict__ films, uint32_t npix, float inv_spp) {
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x)
+    for (int f = 0; f < 3; ++f) {
+      float4 a = acc[(size_t)f * npix + px];
+      float* o = films + (size_t)f * npix * 3 + 3 * (size_t)px;
+      o[0] = a.x * inv_spp;
+      o[1] = a.y * inv_spp;
+      o[2] = a.z * inv_spp;
+    }
+}
+
+ar_status render_gdpt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
+                      const ar_render_config* cfg, float* out_dev, float* aux_dev, hipStream_t st,
+                      ar_render_stats* stats) {
+  constexpr int BLOCK = kTraceBlock;
+  const uint32_t npix = w * h;
+  ShadeView v = shade_view(s);
+  PtParams P{};
+  P.width = w;
+  P.height = h;
+  P.spp = cfg->spp;
+  P.max_depth = cfg->max_depth;
+  P.rr_depth = cfg->rr_depth;
+  P.npix = npix;
+  P.seed = cfg->seed;
+  P.cam = cam_from(*cam);
+  P.variants = 3;
+  uint32_t k = cfg->sample_chunk;
+  if (k == 0) {
+    uint64_t target = 1ull << 22;
+    k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cfg->spp, target / (3ull * std::max<uint32_t>(npix, 1))));
+  }
+  k = std::min(k, cfg->spp);
+  uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU);
+  ar_status e = ws.reserve_paths(3 * npix * k, npix, stack_spill_entries(s), G * BLOCK);
+  if (e != AR_OK) return e;
+  size_t film_bytes = (size_t)npix * 3 * sizeof(float);
+  float* films = aux_dev;
+  if (!films) {
+    if (ws.aux.bytes < 3 * film_bytes) {
+      AR_HIP(hipStreamSynchronize(st));
+      AR_HIP(ws.aux.alloc(3 * film_bytes));
+    }
+    films = ws.aux.as<float>();
+  }
+  AR_HIP(hipMemsetAsync(ws.B.stats, 0, 16, st));
+  bool mega = cfg->schedule == AR_SCHED_MEGAKERNEL;
+  uint32_t eg = std::min<uint32_t>((npix + 255) / 256, 4096);
+  for (uint32_t s0 = 0; s0 < cfg->spp; s0 += k) {
+    P.s0 = s0;
+    P.k = std::min(k, cfg->spp - s0);
+    if (mega) {
+      hipLaunchKernelGGL((k_pt_mega<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, ws.B.L, ws.spill, ws.B.stats);
+    } else {
+      e = run_chunk_wavefront(s, ws, v, P, st, false);
+      if (e != AR_OK) return e;
+    }
+    hipLaunchKernelGGL(k_gdpt_accumulate, dim3(eg), dim3(256), 0, st, ws.B.L, ws.acc, w, h, P.k, s0 == 0 ? 1u : 0u);
+    AR_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_gdpt_resolve, dim3(eg), dim3(256), 0, st, ws.acc, films, npix, 1.0f / (float)cfg->spp);
+  AR_HIP(hipGetLastError());
+  double res = 0.0;
+  e = poisson_solve_device(films, films + (size_t)npix * 3, films + (size_t)npix * 6, w, h, cfg->alpha,
+                           cfg->poisson_iterations, cfg->poisson_sign, out_dev, &res, ws.pois, st);
+  if (e != AR_OK) return e;
+  if (stats) {
+    unsigned long long hs[2] = {0, 0};
+    AR_HIP(hipMemcpyAsync(hs, ws.B.stats, 16, hipMemcpyDeviceToHost, st));
+    AR_HIP(hipStreamSynchronize(st));
+    stats->paths = 3ull * npix * cfg->spp;
+    stats->rays = hs[0] + hs[1];
+    stats->shadow_rays = mega ? 0 : hs[1];
+  }
+  return AR_OK;
+}
+
+}  // namespace ar

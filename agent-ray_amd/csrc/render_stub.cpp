@@ -2,14 +2,6 @@
 #include "ar_internal.h"
 using namespace ar;
 extern "C" {
-ar_status ar_render(const ar_scene*, const ar_camera*, uint32_t, uint32_t, const ar_render_config*, float*, float*,
-                    ar_render_stats*, uint32_t, void*) {
-  return fail(AR_ERR_UNSUPPORTED_TYPE, "ar_render: not built yet");
-}
-ar_status ar_poisson_solve(const float*, const float*, const float*, uint32_t, uint32_t, float, uint32_t, int32_t,
-                           float*, double*, uint32_t, void*) {
-  return fail(AR_ERR_UNSUPPORTED_TYPE, "ar_poisson_solve: not built yet");
-}
 ar_status ar_server_create(const ar_scene*, const ar_server_config*, ar_server**) {
   return fail(AR_ERR_UNSUPPORTED_TYPE, "serving: not built yet");
 }

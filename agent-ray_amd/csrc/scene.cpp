@@ -183,6 +183,7 @@ ar_status ar_scene_destroy(ar_scene* s) {
   if (!s) return AR_OK;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   if (s->scratch_event) (void)hipEventDestroy(s->scratch_event);
+  destroy_workspace(s->ws);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return AR_OK;

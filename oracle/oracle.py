@@ -48,6 +48,9 @@ def lib():
             "orc_bsdf_pdf": (f, [u32, P, P, P]),
             "orc_bsdf_sample": (C.c_int, [u32, P, P, f, f, f, P]),
             "orc_render_pt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, u64, C.c_int, u32, u32, P, P, P]),
+            "orc_render_gdpt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, u64, C.c_int, P, P, P]),
+            "orc_poisson_sor": (d, [P, P, P, u32, u32, f, u32, f, C.c_int, P]),
+            "orc_render_bdpt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, f, C.c_int, u64, C.c_int, P, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name, None)
@@ -147,6 +150,37 @@ class OracleScene:
         secs = lib().orc_render_pt(self.h, _p(cam), w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads, r0, r1,
                                    _p(mean), _p(m2), _p(cnt))
         return mean, m2, cnt, secs
+
+
+def render_gdpt(scene, cam16, w, h, spp, max_depth=8, rr_depth=3, rng_mode=1, seed=0, nthreads=0):
+    """GDPT pass 1: (primal, dx, dy) films, each (h, w, 3), and seconds."""
+    films = [np.zeros((h, w, 3), np.float32) for _ in range(3)]
+    cam = np.ascontiguousarray(cam16, np.float32)
+    secs = lib().orc_render_gdpt(scene.h, _p(cam), w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads,
+                                 _p(films[0]), _p(films[1]), _p(films[2]))
+    return films[0], films[1], films[2], secs
+
+
+def render_bdpt(scene, cam16, w, h, spp, max_depth=8, rr_depth=3, mis_mode=0, mis_beta=2.0, rng_mode=1, seed=0,
+                nthreads=0):
+    """BidirectionalPathTracer::render: (film, splat film, seconds)."""
+    out = np.zeros((h, w, 3), np.float32)
+    spl = np.zeros((h, w, 3), np.float32)
+    cam = np.ascontiguousarray(cam16, np.float32)
+    secs = lib().orc_render_bdpt(scene.h, _p(cam), w, h, spp, max_depth, rr_depth, mis_mode, mis_beta, rng_mode,
+                                 seed, nthreads, _p(out), _p(spl))
+    return out, spl, secs
+
+
+def poisson_sor(primal, dx, dy, alpha, iterations=200, omega=1.6, sign=1):
+    """screened_poisson_reconstruct (reference red-black SOR); returns (img, seconds)."""
+    p = np.ascontiguousarray(primal, np.float32)
+    gx = np.ascontiguousarray(dx, np.float32)
+    gy = np.ascontiguousarray(dy, np.float32)
+    h, w = p.shape[0], p.shape[1]
+    out = np.zeros_like(p)
+    secs = lib().orc_poisson_sor(_p(p), _p(gx), _p(gy), w, h, alpha, iterations, omega, sign, _p(out))
+    return out, secs
 
 
 def ray_new(o, d, tmin, tmax):

@@ -1,0 +1,135 @@
+"""GPU parity of the traversal kernels (ar_trace / ar_occluded) against the
+oracle's restatement of TLAS::intersect / Scene::occluded on the same rays.
+
+Bar: bit-exact.  Reference order (AR_REF_ORDER) must reproduce the oracle's
+(t bits, prim, instance) on every ray.  Near-first order must too, except
+that two different primitives hit at bitwise-equal t may swap (a tie); the
+number of ties is reported and must stay tiny."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _compare(gpu, ora):
+    t, prim, inst = gpu
+    ot, oprim, oinst = ora[0], ora[1], ora[2]
+    tb, otb = t.view(np.uint32), ot.view(np.uint32)
+    exact = (prim == oprim) & (tb == otb) & ((inst == oinst) | (oprim == 0xFFFFFFFF))
+    ties = (~exact) & (tb == otb) & (prim != 0xFFFFFFFF) & (oprim != 0xFFFFFFFF)
+    return int((~exact).sum()), int(ties.sum())
+
+
+def _check_scene(ar, orc, desc, rays, max_tie_frac=1e-4):
+    sc = ar.Scene.from_desc(desc)
+    osc = orc.OracleScene(desc)
+    ora = osc.trace(rays)
+    print(f"  scene: {desc.n_triangles} tris, rays {rays.shape[0]}", flush=True)
+    bad_ref, _ = _compare(sc.intersect(rays, ref_order=True), ora)
+    assert bad_ref == 0, f"reference-order mismatches: {bad_ref}"
+    bad_near, ties = _compare(sc.intersect(rays, ref_order=False), ora)
+    assert bad_near == ties, f"near-order mismatches that are not ties: {bad_near - ties}"
+    assert ties <= max_tie_frac * rays.shape[0] + 1
+    occ_o, _, _ = osc.occluded(rays)
+    assert np.array_equal(sc.occluded(rays), occ_o)
+    return sc, osc
+
+
+def test_soup_closest_and_any(ar, orc, wl):
+    desc = wl.triangle_soup(1 << 14, seed=21)
+    rays = wl.soup_rays(1 << 16, seed=22, image=256)
+    _check_scene(ar, orc, desc, rays)
+
+
+def test_soup_short_rays_any_hit(ar, orc, wl):
+    desc = wl.triangle_soup(1 << 13, seed=23)
+    rays = wl.soup_rays(1 << 15, seed=24, image=128).copy()
+    rng = np.random.default_rng(25)
+    rays[:, 7] = rng.uniform(0.0, 0.2, rays.shape[0]).astype(np.float32)  # shadow-like t_max
+    rays[::7, 3] = rays[::7, 7] + np.float32(0.1)                         # t_min > t_max: always miss
+    _check_scene(ar, orc, desc, rays)
+
+
+def _cornell_rays(wl, desc, n_side=128, seed=31):
+    cam = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 1.0)
+    k = np.arange(n_side * n_side)
+    u = ((k % n_side).astype(np.float32) + np.float32(0.5)) / np.float32(n_side)
+    v = np.float32(1) - ((k // n_side).astype(np.float32) + np.float32(0.5)) / np.float32(n_side)
+    cr = wl.camera_rays(cam, u, v)
+    rng = np.random.default_rng(seed)
+    m = 1 << 14
+    o = rng.uniform(0.02, 0.98, (m, 3)).astype(np.float32)
+    d = wl.normalize_f32(rng.standard_normal((m, 3)).astype(np.float32))
+    rr = np.zeros((m, 8), np.float32)
+    rr[:, 0:3], rr[:, 4:7], rr[:, 7] = o, d, wl.F32_MAX
+    # axis-aligned rays from interior points (zero direction components)
+    ax = np.zeros((6, 8), np.float32)
+    for i, dd in enumerate(np.eye(3).tolist() + (-np.eye(3)).tolist()):
+        ax[i, 0:3] = (0.3, 0.4, 0.5)
+        ax[i, 4:7] = dd
+        ax[i, 7] = wl.F32_MAX
+    return np.concatenate([cr, rr, ax]).astype(np.float32)
+
+
+def test_cornell_instances(ar, orc, wl):
+    # rotated + scaled instances exercise the TLAS dir_scale / renormalize path
+    desc = wl.cornell_box()
+    _check_scene(ar, orc, desc, _cornell_rays(wl, desc))
+
+
+def test_room_scene(ar, orc, wl):
+    desc = wl.procedural_room(n_boxes=30)
+    cam = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 16 / 9)
+    rng = np.random.default_rng(41)
+    n = 1 << 15
+    cr = wl.camera_rays(cam, rng.uniform(0, 1, n), rng.uniform(0, 1, n))
+    o = rng.uniform((0.1, 0.1, 0.1), (7.9, 3.9, 7.9), (n, 3)).astype(np.float32)
+    d = wl.normalize_f32(rng.standard_normal((n, 3)).astype(np.float32))
+    rr = np.zeros((n, 8), np.float32)
+    rr[:, 0:3], rr[:, 3], rr[:, 4:7], rr[:, 7] = o, 1e-4, d, wl.F32_MAX
+    _check_scene(ar, orc, desc, np.concatenate([cr, rr]))
+
+
+def test_empty_batch_and_device_pointers(ar, wl):
+    import torch
+    desc = wl.triangle_soup(4096, seed=51)
+    sc = ar.Scene.from_desc(desc)
+    t, p, i = sc.intersect(np.zeros((0, 8), np.float32))
+    assert t.shape == (0,)
+    rays = wl.soup_rays(1 << 12, seed=52, image=64)
+    th, ph, ih = sc.intersect(rays)
+    dr = torch.from_numpy(rays).cuda()
+    td, pd, idd = sc.intersect(dr)
+    torch.cuda.synchronize()
+    assert np.array_equal(td.cpu().numpy().view(np.uint32), th.view(np.uint32))
+    assert np.array_equal(pd.cpu().numpy().view(np.uint32), ph)
+    occ = sc.occluded(dr)
+    torch.cuda.synchronize()
+    assert np.array_equal(occ.cpu().numpy().astype(bool), sc.occluded(rays))
+
+
+@pytest.mark.slow
+def test_full_size_soup(ar, orc, wl):
+    """1M triangles, 2^24 rays: every ray agrees between near-first and
+    reference order on the GPU (size-independent property), and a 2^16 ray
+    sample of both halves matches the oracle bit for bit."""
+    import torch
+    desc = wl.triangle_soup(1 << 20, seed=0)
+    rays = wl.soup_rays(1 << 24, seed=1)
+    sc = ar.Scene.from_desc(desc)
+    dr = torch.from_numpy(rays).cuda()
+    tn, pn, _ = sc.intersect(dr, ref_order=False)
+    tr, pr, _ = sc.intersect(dr, ref_order=True)
+    torch.cuda.synchronize()
+    same_t = (tn.view(torch.int32) == tr.view(torch.int32))
+    same_p = (pn == pr)
+    assert bool(same_t.all())
+    n_swap = int((~same_p).sum())
+    assert n_swap <= 16, n_swap
+    idx = np.concatenate([np.arange(0, 1 << 15), (1 << 23) + np.arange(0, 1 << 15)])
+    osc = orc.OracleScene(desc)
+    ot, op, oi, _, _ = osc.trace(rays[idx])
+    tr_h = tr.cpu().numpy()[idx]
+    pr_h = pr.cpu().numpy().view(np.uint32)[idx]
+    assert np.array_equal(tr_h.view(np.uint32), ot.view(np.uint32))
+    assert np.array_equal(pr_h, op)
