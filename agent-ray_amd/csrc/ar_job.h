@@ -1,0 +1,48 @@
+// A render in flight, issued chunk by chunk on one stream.  ar_render runs
+// one job to completion; the server interleaves the chunks of many jobs on
+// their own streams (continuous batching).  A job's film depends only on its
+// own description: chunking and interleaving never change a bit.
+#pragma once
+#include <memory>
+
+#include "ar_render.h"
+
+namespace ar {
+
+struct JobSpec {
+  uint32_t width = 0, height = 0;
+  ar_camera camera{};
+  ar_render_config cfg{};
+};
+
+class JobRun {
+ public:
+  virtual ~JobRun() {}
+  // Reserve device memory and enqueue the set-up work.
+  virtual ar_status begin() = 0;
+  // Enqueue the next chunk; *done is set once every sample has been issued.
+  virtual ar_status step(bool* done) = 0;
+  // Enqueue the resolve (and the Poisson solve for GDPT) into out/aux.
+  virtual ar_status finish() = 0;
+  // After the stream has drained: rays/paths counters.
+  virtual ar_status read_stats(ar_render_stats* st) = 0;
+
+  const ar_scene* scene = nullptr;
+  JobSpec spec;
+  Workspace* ws = nullptr;
+  hipStream_t stream = nullptr;
+  float* out_dev = nullptr;  // width*height*3 floats
+  float* aux_dev = nullptr;  // GDPT: primal, dx, dy films (optional)
+};
+
+std::unique_ptr<JobRun> make_job(const ar_scene* s, const JobSpec& spec, Workspace* ws, hipStream_t st, float* out_dev,
+                                 float* aux_dev);
+std::unique_ptr<JobRun> make_pt_job(const ar_scene* s, const JobSpec& spec, Workspace* ws, hipStream_t st,
+                                    float* out_dev, float* aux_dev);
+std::unique_ptr<JobRun> make_bdpt_job(const ar_scene* s, const JobSpec& spec, Workspace* ws, hipStream_t st,
+                                      float* out_dev);
+// Device bytes a job of this shape needs (admission control).
+size_t job_footprint(const ar_scene* s, const JobSpec& spec);
+ar_status validate_job(const ar_scene* s, const JobSpec& spec);
+
+}  // namespace ar

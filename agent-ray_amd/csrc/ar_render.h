@@ -48,12 +48,8 @@ ar_status poisson_solve_device(const float* P_rgb, const float* gx_rgb, const fl
                                float alpha, uint32_t iters, int32_t sign, float* out_rgb, double* final_residual,
                                DevBuf& scratch, hipStream_t st);
 
-ar_status render_pt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
-                    const ar_render_config* cfg, float* out_dev, hipStream_t st, ar_render_stats* stats);
-ar_status render_gdpt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
-                      const ar_render_config* cfg, float* out_dev, float* aux_dev, hipStream_t st,
-                      ar_render_stats* stats);
-ar_status render_bdpt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
-                      const ar_render_config* cfg, float* out_dev, hipStream_t st, ar_render_stats* stats);
+struct JobSpec;
+// Samples per pixel issued per chunk (bounds in-flight path state).
+uint32_t chunk_samples(const JobSpec& spec);
 
 }  // namespace ar

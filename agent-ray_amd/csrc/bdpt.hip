@@ -21,7 +21,7 @@
 // at t = 1 to the sampled emission point with the general formula.
 #include "ar_kernels.h"
 #include "ar_pt.h"
-#include "ar_render.h"
+#include "ar_job.h"
 
 namespace ar {
 
@@ -550,75 +550,107 @@ static uint32_t bdpt_grid(int block) {
   return (uint32_t)(cus * kTraceBlocksPerCU / 8 * 8);
 }
 
-ar_status render_bdpt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
-                      const ar_render_config* cfg, float* out_dev, hipStream_t st, ar_render_stats* stats) {
-  constexpr int BLOCK = kTraceBlock;
-  const uint32_t npix = w * h;
-  if (cfg->max_depth > 64) return fail(AR_ERR_INVALID_ARG, "BDPT: max_depth above 64");
-  ShadeView v = shade_view(s);
+__global__ void k_bdpt_zero_stats(unsigned long long* st) {
+  if (threadIdx.x < 2) st[threadIdx.x] = 0ull;
+}
+
+class BdptJob : public JobRun {
+ public:
   BdptParams P{};
-  P.pt.width = w;
-  P.pt.height = h;
-  P.pt.spp = cfg->spp;
-  P.pt.max_depth = cfg->max_depth;
-  P.pt.rr_depth = cfg->rr_depth;
-  P.pt.npix = npix;
-  P.pt.seed = cfg->seed;
-  P.pt.cam = cam_from(*cam);
-  P.pt.variants = 1;
-  P.V = cfg->max_depth + 1;
-  P.mis_mode = cfg->mis_mode;
-  P.mis_beta = cfg->mis_beta;
-  uint32_t k = cfg->sample_chunk;
-  if (k == 0) {
-    uint64_t target = 1ull << 20;  // ~1M subpath pairs per chunk
-    k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cfg->spp, target / std::max<uint32_t>(npix, 1)));
+  ShadeView v{};
+  Verts Vt{};
+  unsigned long long* splat = nullptr;
+  uint32_t k = 1, s0 = 0, G = 0, eg = 0;
+
+  ar_status begin() override {
+    constexpr int BLOCK = kTraceBlock;
+    const ar_render_config* cfg = &spec.cfg;
+    const uint32_t w = spec.width, h = spec.height, npix = w * h;
+    v = shade_view(scene);
+    P.pt.width = w;
+    P.pt.height = h;
+    P.pt.spp = cfg->spp;
+    P.pt.max_depth = cfg->max_depth;
+    P.pt.rr_depth = cfg->rr_depth;
+    P.pt.npix = npix;
+    P.pt.seed = cfg->seed;
+    P.pt.cam = cam_from(spec.camera);
+    P.pt.variants = 1;
+    P.V = cfg->max_depth + 1;
+    P.mis_mode = cfg->mis_mode;
+    P.mis_beta = cfg->mis_beta;
+    k = chunk_samples(spec);
+    G = bdpt_grid(BLOCK);
+    uint32_t cap = npix * k;
+    ar_status e = ws->reserve_paths(cap, npix, stack_spill_entries(scene), G * BLOCK);
+    if (e != AR_OK) return e;
+    size_t nvert = 2ull * P.V * cap;
+    size_t need =
+        nvert * 3 * sizeof(float4) + (size_t)cap * sizeof(uint2) + (size_t)npix * 3 * sizeof(long long) + 1024;
+    if (ws->aux.bytes < need) {
+      AR_HIP(hipStreamSynchronize(stream));
+      AR_HIP(ws->aux.alloc(need));
+    }
+    char* base = (char*)ws->aux.p;
+    Vt.pf = (float4*)base;
+    Vt.nr = Vt.pf + nvert;
+    Vt.bm = Vt.nr + nvert;
+    Vt.count = (uint2*)(Vt.bm + nvert);
+    uintptr_t tail = (uintptr_t)(Vt.count + cap);
+    splat = (unsigned long long*)(tail + 256 - tail % 256);
+    Vt.V = P.V;
+    eg = std::min<uint32_t>((npix + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_bdpt_zero_stats, dim3(1), dim3(64), 0, stream, ws->B.stats);
+    hipLaunchKernelGGL(k_zero_u64, dim3(eg), dim3(256), 0, stream, splat, (size_t)npix * 3);
+    s0 = 0;
+    return AR_OK;
   }
-  k = std::min(k, cfg->spp);
-  uint32_t G = bdpt_grid(BLOCK);
-  uint32_t cap = npix * k;
-  ar_status e = ws.reserve_paths(cap, npix, stack_spill_entries(s), G * BLOCK);
-  if (e != AR_OK) return e;
-  size_t nvert = 2ull * P.V * cap;
-  size_t need = nvert * 3 * sizeof(float4) + (size_t)cap * sizeof(uint2) + (size_t)npix * 3 * sizeof(long long) + 1024;
-  if (ws.aux.bytes < need) {
-    AR_HIP(hipStreamSynchronize(st));
-    AR_HIP(ws.aux.alloc(need));
+
+  ar_status step(bool* done) override {
+    constexpr int BLOCK = kTraceBlock;
+    if (s0 < P.pt.spp) {
+      P.pt.s0 = s0;
+      P.pt.k = std::min(k, P.pt.spp - s0);
+      Vt.npaths = P.pt.npix * P.pt.k;
+      hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->spill,
+                         ws->B.stats);
+      hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->B.L, splat,
+                         ws->spill, ws->B.stats);
+      hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, P.pt.npix, P.pt.k,
+                         s0 == 0 ? 1u : 0u);
+      AR_HIP(hipGetLastError());
+      s0 += P.pt.k;
+    }
+    *done = s0 >= P.pt.spp;
+    return AR_OK;
   }
-  char* base = (char*)ws.aux.p;
-  Verts Vt;
-  Vt.pf = (float4*)base;
-  Vt.nr = Vt.pf + nvert;
-  Vt.bm = Vt.nr + nvert;
-  Vt.count = (uint2*)(Vt.bm + nvert);
-  unsigned long long* splat = (unsigned long long*)((char*)(Vt.count + cap) + 256 - ((uintptr_t)(Vt.count + cap) % 256));
-  Vt.V = P.V;
-  AR_HIP(hipMemsetAsync(ws.B.stats, 0, 16, st));
-  uint32_t eg = std::min<uint32_t>((npix + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_zero_u64, dim3(eg), dim3(256), 0, st, splat, (size_t)npix * 3);
-  for (uint32_t s0 = 0; s0 < cfg->spp; s0 += k) {
-    P.pt.s0 = s0;
-    P.pt.k = std::min(k, cfg->spp - s0);
-    Vt.npaths = npix * P.pt.k;
-    hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, Vt, ws.spill, ws.B.stats);
-    hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, Vt, ws.B.L, splat, ws.spill,
-                       ws.B.stats);
-    hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, st, ws.B.L, ws.acc, npix, P.pt.k,
-                       s0 == 0 ? 1u : 0u);
+
+  ar_status finish() override {
+    hipLaunchKernelGGL(k_bdpt_resolve, dim3(eg), dim3(256), 0, stream, ws->acc, (const long long*)splat, out_dev,
+                       P.pt.npix, (float)P.pt.spp);
     AR_HIP(hipGetLastError());
+    return AR_OK;
   }
-  hipLaunchKernelGGL(k_bdpt_resolve, dim3(eg), dim3(256), 0, st, ws.acc, (const long long*)splat, out_dev, npix,
-                     (float)cfg->spp);
-  AR_HIP(hipGetLastError());
-  if (stats) {
+
+  ar_status read_stats(ar_render_stats* st) override {
     unsigned long long hs[2] = {0, 0};
-    AR_HIP(hipMemcpyAsync(hs, ws.B.stats, 16, hipMemcpyDeviceToHost, st));
-    AR_HIP(hipStreamSynchronize(st));
-    stats->paths = (uint64_t)npix * cfg->spp;
-    stats->rays = hs[0] + hs[1];
-    stats->shadow_rays = hs[1];
+    AR_HIP(hipMemcpy(hs, ws->B.stats, 16, hipMemcpyDeviceToHost));
+    st->paths = (uint64_t)P.pt.npix * P.pt.spp;
+    st->rays = hs[0] + hs[1];
+    st->shadow_rays = hs[1];
+    return AR_OK;
   }
-  return AR_OK;
+};
+
+std::unique_ptr<JobRun> make_bdpt_job(const ar_scene* s, const JobSpec& spec, Workspace* ws, hipStream_t st,
+                                      float* out_dev) {
+  std::unique_ptr<BdptJob> j(new BdptJob());
+  j->scene = s;
+  j->spec = spec;
+  j->ws = ws;
+  j->stream = st;
+  j->out_dev = out_dev;
+  return j;
 }
 
 }  // namespace ar

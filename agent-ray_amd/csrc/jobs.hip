@@ -1,0 +1,50 @@
+// Job plumbing shared by ar_render and the server: chunk sizing, footprint
+// estimate for admission, validation, and the integrator dispatch.
+#include "ar_job.h"
+#include "ar_kernels.h"
+
+namespace ar {
+
+uint32_t chunk_samples(const JobSpec& spec) {
+  const ar_render_config& c = spec.cfg;
+  uint64_t npix = (uint64_t)spec.width * spec.height;
+  uint64_t per_sample = npix * (c.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
+  uint64_t target = c.integrator == AR_INTEGRATOR_BDPT ? (1ull << 20) : (1ull << 22);
+  uint32_t k = c.sample_chunk;
+  if (k == 0) k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.spp, target / std::max<uint64_t>(per_sample, 1)));
+  return std::min(k, c.spp);
+}
+
+ar_status validate_job(const ar_scene* s, const JobSpec& spec) {
+  const ar_render_config& c = spec.cfg;
+  if (!s->committed) return fail(AR_ERR_STATE, "scene not committed");
+  if (spec.width == 0 || spec.height == 0 || c.spp == 0) return fail(AR_ERR_INVALID_ARG, "empty image or spp");
+  if ((uint64_t)spec.width * spec.height > (1ull << 26)) return fail(AR_ERR_INVALID_ARG, "image too large");
+  if (c.integrator > AR_INTEGRATOR_GDPT) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown integrator");
+  if (c.schedule > AR_SCHED_MEGAKERNEL) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown schedule");
+  if (c.integrator == AR_INTEGRATOR_BDPT && c.max_depth > 64) return fail(AR_ERR_INVALID_ARG, "BDPT max_depth > 64");
+  if (c.integrator == AR_INTEGRATOR_GDPT && !(c.alpha > 0.0f)) return fail(AR_ERR_INVALID_ARG, "GDPT alpha <= 0");
+  if (c.mis_mode > 1) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown MIS mode");
+  return AR_OK;
+}
+
+size_t job_footprint(const ar_scene* s, const JobSpec& spec) {
+  const ar_render_config& c = spec.cfg;
+  uint64_t npix = (uint64_t)spec.width * spec.height;
+  uint64_t k = chunk_samples(spec);
+  uint64_t paths = npix * k * (c.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
+  size_t bytes = paths * (5 * 16 + 16 + 8 + 16 + 4 + 4 + 48) + npix * 48 + (64ull << 20);  // state, queues, film, spill
+  if (c.integrator == AR_INTEGRATOR_GDPT) bytes += npix * 12 * 3 + npix * 12 * 9 + (64 << 10);
+  if (c.integrator == AR_INTEGRATOR_BDPT) bytes += paths * (2ull * (c.max_depth + 1) * 48 + 8) + npix * 24;
+  bytes += npix * 12 * 4;  // output staging
+  (void)s;
+  return bytes;
+}
+
+std::unique_ptr<JobRun> make_job(const ar_scene* s, const JobSpec& spec, Workspace* ws, hipStream_t st, float* out_dev,
+                                 float* aux_dev) {
+  if (spec.cfg.integrator == AR_INTEGRATOR_BDPT) return make_bdpt_job(s, spec, ws, st, out_dev);
+  return make_pt_job(s, spec, ws, st, out_dev, aux_dev);
+}
+
+}  // namespace ar

@@ -20,7 +20,7 @@
 //               graph and replayed per chunk.
 #include "ar_kernels.h"
 #include "ar_pt.h"
-#include "ar_render.h"
+#include "ar_job.h"
 
 namespace ar {
 
@@ -242,6 +242,10 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(SceneView sv, WaveBufs B, i
   if (gtid == 0) atomicAdd(&B.stats[1], (unsigned long long)n);
 }
 
+__global__ void k_zero_stats(unsigned long long* st) {
+  if (threadIdx.x < 2) st[threadIdx.x] = 0ull;
+}
+
 // End of bounce: the consumed queue and the shadow queue become empty.
 __global__ void k_wf_reset(WaveBufs B, int qi) {
   if (threadIdx.x == 0) {
@@ -393,60 +397,6 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   return AR_OK;
 }
 
-ar_status render_pt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
-                    const ar_render_config* cfg, float* out_dev, hipStream_t st, ar_render_stats* stats) {
-  constexpr int BLOCK = kTraceBlock;
-  const uint32_t npix = w * h;
-  ShadeView v = shade_view(s);
-  PtParams P{};
-  P.width = w;
-  P.height = h;
-  P.spp = cfg->spp;
-  P.max_depth = cfg->max_depth;
-  P.rr_depth = cfg->rr_depth;
-  P.npix = npix;
-  P.seed = cfg->seed;
-  P.cam = cam_from(*cam);
-  P.variants = 1;
-  uint32_t k = cfg->sample_chunk;
-  if (k == 0) {
-    uint64_t target = 1ull << 22;  // ~4M paths in flight per chunk
-    k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cfg->spp, target / std::max<uint32_t>(npix, 1)));
-  }
-  k = std::min(k, cfg->spp);
-  uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU);
-  ar_status e = ws.reserve_paths(npix * k, npix, stack_spill_entries(s), G * BLOCK);
-  if (e != AR_OK) return e;
-  AR_HIP(hipMemsetAsync(ws.B.stats, 0, 16, st));
-  bool mega = cfg->schedule == AR_SCHED_MEGAKERNEL;
-  for (uint32_t s0 = 0; s0 < cfg->spp; s0 += k) {
-    P.s0 = s0;
-    P.k = std::min(k, cfg->spp - s0);
-    if (mega) {
-      hipLaunchKernelGGL((k_pt_mega<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, ws.B.L, ws.spill, ws.B.stats);
-    } else {
-      e = run_chunk_wavefront(s, ws, v, P, st, false);
-      if (e != AR_OK) return e;
-    }
-    hipLaunchKernelGGL(k_film_accumulate, dim3(std::min<uint32_t>((npix + 255) / 256, 4096)), dim3(256), 0, st,
-                       ws.B.L, ws.acc, npix, P.k, s0 == 0 ? 1u : 0u);
-    AR_HIP(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_film_resolve, dim3(std::min<uint32_t>((npix + 255) / 256, 4096)), dim3(256), 0, st, ws.acc,
-                     out_dev, npix, (float)cfg->spp);
-  AR_HIP(hipGetLastError());
-  if (stats) {
-    unsigned long long hs[2] = {0, 0};
-    AR_HIP(hipMemcpyAsync(hs, ws.B.stats, 16, hipMemcpyDeviceToHost, st));
-    AR_HIP(hipStreamSynchronize(st));
-    stats->paths = (uint64_t)npix * cfg->spp;
-    stats->rays = hs[0] + hs[1];
-    stats->shadow_rays = mega ? 0 : hs[1];
-  }
-  return AR_OK;
-}
-
-
 // ---------------------------------------------------------------------------
 // Gradient-domain path tracer (GradientDomainPathTracer::render,
 // gradient_domain_pt.rs:346-415): per pixel and sample, a base path and the
@@ -496,70 +446,113 @@ __global__ void k_gdpt_resolve(const float4* __restrict__ acc, float* __restrict
     }
 }
 
-ar_status render_gdpt(const ar_scene* s, Workspace& ws, const ar_camera* cam, uint32_t w, uint32_t h,
-                      const ar_render_config* cfg, float* out_dev, float* aux_dev, hipStream_t st,
-                      ar_render_stats* stats) {
-  constexpr int BLOCK = kTraceBlock;
-  const uint32_t npix = w * h;
-  ShadeView v = shade_view(s);
+// ---------------------------------------------------------------------------
+// Job interface: PT (variants = 1) and GDPT (variants = 3).
+// ---------------------------------------------------------------------------
+class PtJob : public JobRun {
+ public:
   PtParams P{};
-  P.width = w;
-  P.height = h;
-  P.spp = cfg->spp;
-  P.max_depth = cfg->max_depth;
-  P.rr_depth = cfg->rr_depth;
-  P.npix = npix;
-  P.seed = cfg->seed;
-  P.cam = cam_from(*cam);
-  P.variants = 3;
-  uint32_t k = cfg->sample_chunk;
-  if (k == 0) {
-    uint64_t target = 1ull << 22;
-    k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(cfg->spp, target / (3ull * std::max<uint32_t>(npix, 1))));
-  }
-  k = std::min(k, cfg->spp);
-  uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU);
-  ar_status e = ws.reserve_paths(3 * npix * k, npix, stack_spill_entries(s), G * BLOCK);
-  if (e != AR_OK) return e;
-  size_t film_bytes = (size_t)npix * 3 * sizeof(float);
-  float* films = aux_dev;
-  if (!films) {
-    if (ws.aux.bytes < 3 * film_bytes) {
-      AR_HIP(hipStreamSynchronize(st));
-      AR_HIP(ws.aux.alloc(3 * film_bytes));
+  ShadeView v{};
+  bool gdpt = false, mega = false;
+  uint32_t k = 1, s0 = 0, G = 0, eg = 0;
+  float* films = nullptr;
+
+  ar_status begin() override {
+    constexpr int BLOCK = kTraceBlock;
+    const ar_render_config* cfg = &spec.cfg;
+    const uint32_t w = spec.width, h = spec.height, npix = w * h;
+    gdpt = cfg->integrator == AR_INTEGRATOR_GDPT;
+    mega = cfg->schedule == AR_SCHED_MEGAKERNEL;
+    v = shade_view(scene);
+    P.width = w;
+    P.height = h;
+    P.spp = cfg->spp;
+    P.max_depth = cfg->max_depth;
+    P.rr_depth = cfg->rr_depth;
+    P.npix = npix;
+    P.seed = cfg->seed;
+    P.cam = cam_from(spec.camera);
+    P.variants = gdpt ? 3 : 1;
+    k = chunk_samples(spec);
+    G = persistent_grid(BLOCK, kTraceBlocksPerCU);
+    eg = std::min<uint32_t>((npix + 255) / 256, 4096);
+    ar_status e = ws->reserve_paths(P.variants * npix * k, npix, stack_spill_entries(scene), G * BLOCK);
+    if (e != AR_OK) return e;
+    if (gdpt) {
+      films = aux_dev;
+      size_t film_bytes = (size_t)npix * 3 * sizeof(float);
+      if (!films) {
+        if (ws->aux.bytes < 3 * film_bytes) {
+          AR_HIP(hipStreamSynchronize(stream));
+          AR_HIP(ws->aux.alloc(3 * film_bytes));
+        }
+        films = ws->aux.as<float>();
+      }
     }
-    films = ws.aux.as<float>();
+    hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, stream, ws->B.stats);
+    s0 = 0;
+    return AR_OK;
   }
-  AR_HIP(hipMemsetAsync(ws.B.stats, 0, 16, st));
-  bool mega = cfg->schedule == AR_SCHED_MEGAKERNEL;
-  uint32_t eg = std::min<uint32_t>((npix + 255) / 256, 4096);
-  for (uint32_t s0 = 0; s0 < cfg->spp; s0 += k) {
-    P.s0 = s0;
-    P.k = std::min(k, cfg->spp - s0);
-    if (mega) {
-      hipLaunchKernelGGL((k_pt_mega<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, ws.B.L, ws.spill, ws.B.stats);
-    } else {
-      e = run_chunk_wavefront(s, ws, v, P, st, false);
-      if (e != AR_OK) return e;
+
+  ar_status step(bool* done) override {
+    constexpr int BLOCK = kTraceBlock;
+    if (s0 < P.spp) {
+      P.s0 = s0;
+      P.k = std::min(k, P.spp - s0);
+      if (mega) {
+        hipLaunchKernelGGL((k_pt_mega<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, ws->B.L, ws->spill,
+                           ws->B.stats);
+      } else {
+        ar_status e = run_chunk_wavefront(scene, *ws, v, P, stream, false);
+        if (e != AR_OK) return e;
+      }
+      if (gdpt)
+        hipLaunchKernelGGL(k_gdpt_accumulate, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, P.width, P.height,
+                           P.k, s0 == 0 ? 1u : 0u);
+      else
+        hipLaunchKernelGGL(k_film_accumulate, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, P.npix, P.k,
+                           s0 == 0 ? 1u : 0u);
+      AR_HIP(hipGetLastError());
+      s0 += P.k;
     }
-    hipLaunchKernelGGL(k_gdpt_accumulate, dim3(eg), dim3(256), 0, st, ws.B.L, ws.acc, w, h, P.k, s0 == 0 ? 1u : 0u);
+    *done = s0 >= P.spp;
+    return AR_OK;
+  }
+
+  ar_status finish() override {
+    if (!gdpt) {
+      hipLaunchKernelGGL(k_film_resolve, dim3(eg), dim3(256), 0, stream, ws->acc, out_dev, P.npix, (float)P.spp);
+      AR_HIP(hipGetLastError());
+      return AR_OK;
+    }
+    hipLaunchKernelGGL(k_gdpt_resolve, dim3(eg), dim3(256), 0, stream, ws->acc, films, P.npix, 1.0f / (float)P.spp);
     AR_HIP(hipGetLastError());
+    const ar_render_config* cfg = &spec.cfg;
+    return poisson_solve_device(films, films + (size_t)P.npix * 3, films + (size_t)P.npix * 6, P.width, P.height,
+                                cfg->alpha, cfg->poisson_iterations, cfg->poisson_sign, out_dev, nullptr, ws->pois,
+                                stream);
   }
-  hipLaunchKernelGGL(k_gdpt_resolve, dim3(eg), dim3(256), 0, st, ws.acc, films, npix, 1.0f / (float)cfg->spp);
-  AR_HIP(hipGetLastError());
-  double res = 0.0;
-  e = poisson_solve_device(films, films + (size_t)npix * 3, films + (size_t)npix * 6, w, h, cfg->alpha,
-                           cfg->poisson_iterations, cfg->poisson_sign, out_dev, &res, ws.pois, st);
-  if (e != AR_OK) return e;
-  if (stats) {
+
+  ar_status read_stats(ar_render_stats* st) override {
     unsigned long long hs[2] = {0, 0};
-    AR_HIP(hipMemcpyAsync(hs, ws.B.stats, 16, hipMemcpyDeviceToHost, st));
-    AR_HIP(hipStreamSynchronize(st));
-    stats->paths = 3ull * npix * cfg->spp;
-    stats->rays = hs[0] + hs[1];
-    stats->shadow_rays = mega ? 0 : hs[1];
+    AR_HIP(hipMemcpy(hs, ws->B.stats, 16, hipMemcpyDeviceToHost));
+    st->paths = (uint64_t)P.variants * P.npix * P.spp;
+    st->rays = hs[0] + hs[1];
+    st->shadow_rays = mega ? 0 : hs[1];
+    return AR_OK;
   }
-  return AR_OK;
+};
+
+std::unique_ptr<JobRun> make_pt_job(const ar_scene* s, const JobSpec& spec, Workspace* ws, hipStream_t st,
+                                    float* out_dev, float* aux_dev) {
+  std::unique_ptr<PtJob> j(new PtJob());
+  j->scene = s;
+  j->spec = spec;
+  j->ws = ws;
+  j->stream = st;
+  j->out_dev = out_dev;
+  j->aux_dev = aux_dev;
+  return j;
 }
 
 }  // namespace ar

@@ -1,9 +1,9 @@
 // ar_render: Integrator::render (agent-ray src/integrators/mod.rs:19-27) for
-// the three integrators, with host/device buffer handling.
+// the three integrators, as one job run to completion on the caller's stream.
 #include <chrono>
 
+#include "ar_job.h"
 #include "ar_kernels.h"
-#include "ar_render.h"
 
 using namespace ar;
 
@@ -11,16 +11,18 @@ extern "C" ar_status ar_render(const ar_scene* s, const ar_camera* cam, uint32_t
                                const ar_render_config* cfg, float* out_rgb, float* out_aux, ar_render_stats* stats,
                                uint32_t flags, void* stream) {
   if (!s || !cam || !cfg || !out_rgb) return fail(AR_ERR_INVALID_ARG, "ar_render: NULL argument");
-  if (!s->committed) return fail(AR_ERR_STATE, "ar_render: scene not committed");
-  if (width == 0 || height == 0 || cfg->spp == 0) return fail(AR_ERR_INVALID_ARG, "ar_render: empty image or spp");
-  if ((uint64_t)width * height > (1ull << 26)) return fail(AR_ERR_INVALID_ARG, "ar_render: image too large");
-  if (cfg->integrator > AR_INTEGRATOR_GDPT) return fail(AR_ERR_UNSUPPORTED_TYPE, "ar_render: unknown integrator");
-  if (cfg->schedule > AR_SCHED_MEGAKERNEL) return fail(AR_ERR_UNSUPPORTED_TYPE, "ar_render: unknown schedule");
+  JobSpec spec;
+  spec.width = width;
+  spec.height = height;
+  spec.camera = *cam;
+  spec.cfg = *cfg;
+  ar_status e = validate_job(s, spec);
+  if (e != AR_OK) return e;
   auto t0 = std::chrono::steady_clock::now();
   ar_scene* ms = const_cast<ar_scene*>(s);
   hipStream_t st = pick_stream(s, stream);
   void* unused = nullptr;
-  ar_status e = scratch_acquire(ms, 0, st, &unused);
+  e = scratch_acquire(ms, 0, st, &unused);
   if (e != AR_OK) return e;
   if (!ms->ws) ms->ws = new Workspace();
   Workspace& ws = *ms->ws;
@@ -42,12 +44,11 @@ extern "C" ar_status ar_render(const ar_scene* s, const ar_camera* cam, uint32_t
     out_dev = ws.out.as<float>();
     aux_dev = out_aux ? out_dev + npix * 3 : nullptr;
   }
-  ar_render_stats local{};
-  switch (cfg->integrator) {
-    case AR_INTEGRATOR_PT: e = render_pt(s, ws, cam, width, height, cfg, out_dev, st, &local); break;
-    case AR_INTEGRATOR_GDPT: e = render_gdpt(s, ws, cam, width, height, cfg, out_dev, aux_dev, st, &local); break;
-    default: e = render_bdpt(s, ws, cam, width, height, cfg, out_dev, st, &local); break;
-  }
+  std::unique_ptr<JobRun> job = make_job(s, spec, &ws, st, out_dev, aux_dev);
+  e = job->begin();
+  bool done = false;
+  while (e == AR_OK && !done) e = job->step(&done);
+  if (e == AR_OK) e = job->finish();
   if (e == AR_OK && host) {
     hipError_t he = hipMemcpyAsync(out_rgb, out_dev, npix * 3 * sizeof(float), hipMemcpyDeviceToHost, st);
     if (he == hipSuccess && out_aux)
@@ -56,7 +57,14 @@ extern "C" ar_status ar_render(const ar_scene* s, const ar_camera* cam, uint32_t
   }
   scratch_release(ms, st);
   if (e != AR_OK) return e;
-  if (host || !(flags & AR_ASYNC)) AR_HIP(hipStreamSynchronize(st));
+  ar_render_stats local{};
+  if (host || !(flags & AR_ASYNC) || stats) {
+    AR_HIP(hipStreamSynchronize(st));
+    if (stats) {
+      e = job->read_stats(&local);
+      if (e != AR_OK) return e;
+    }
+  }
   local.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = local;
   return AR_OK;
@@ -64,5 +72,4 @@ extern "C" ar_status ar_render(const ar_scene* s, const ar_camera* cam, uint32_t
 
 namespace ar {
 void destroy_workspace(Workspace* ws) { delete ws; }
-
 }  // namespace ar
