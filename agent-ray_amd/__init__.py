@@ -532,3 +532,65 @@ def bvh_build_host(boxes: np.ndarray):
     _check(_lib.ar_bvh_build_host(_ptr(b), n, _ptr(ob), _ptr(om), _ptr(op), C.byref(nn), C.byref(dep)))
     m = nn.value
     return ob[:m], om[:m], op[:n], dep.value
+
+
+# ---------------------------------------------------------------------------
+# Serving: continuous-batching render server over a resident scene
+# ---------------------------------------------------------------------------
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED, JOB_FAILED = 0, 1, 2, 3, 4
+
+
+class RenderServer:
+    """Accepts render jobs of differing resolution, spp and integrator against
+    one resident scene; jobs are admitted into slots by HBM budget and run
+    concurrently (see include/agentray.h, ar_server_*)."""
+
+    def __init__(self, scene: Scene, hbm_budget_bytes: int = 200 << 30, max_active_jobs: int = 8):
+        self.scene = scene
+        cfg = ArServerConfig(hbm_budget_bytes, max_active_jobs, 0)
+        self._h = C.c_void_p()
+        _check(_lib.ar_server_create(scene.handle, C.byref(cfg), C.byref(self._h)))
+        self._shapes = {}
+
+    def submit(self, integrator: "_Integrator", camera: PinholeCamera, width: int, height: int) -> int:
+        d = ArJobDesc()
+        d.width, d.height = width, height
+        d.camera = camera.c
+        d.config = integrator._cfg()
+        d.config.gdpt_outputs = 0
+        jid = C.c_uint64()
+        _check(_lib.ar_job_submit(self._h, C.byref(d), C.byref(jid)))
+        self._shapes[jid.value] = (height, width)
+        return jid.value
+
+    def poll(self, job_id: int) -> int:
+        st = C.c_int32()
+        _check(_lib.ar_job_poll(self._h, job_id, C.byref(st)))
+        return st.value
+
+    def wait(self, job_id: int):
+        """Blocks until the job ends; returns (Film, stats dict)."""
+        h, w = self._shapes.pop(job_id)
+        out = np.zeros((h, w, 3), np.float32)
+        st = ArJobStats()
+        _check(_lib.ar_job_wait(self._h, job_id, _ptr(out), C.byref(st)))
+        stats = {"submit": st.submit_time, "start": st.start_time, "finish": st.finish_time,
+                 "latency": st.finish_time - st.submit_time, "paths": st.paths, "rays": st.rays}
+        return Film(w, h, out), stats
+
+    def cancel(self, job_id: int):
+        _check(_lib.ar_job_cancel(self._h, job_id))
+
+    def clock(self) -> float:
+        return _lib.ar_server_clock(self._h)
+
+    def close(self):
+        if self._h:
+            _lib.ar_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

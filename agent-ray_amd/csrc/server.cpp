@@ -1,0 +1,380 @@
+// Render server: the request loop above Integrator::render (no counterpart
+// in the reference, whose callers render one image per process).
+//
+// Continuous batching: jobs wait in a FIFO and are admitted into job slots
+// as soon as a slot is free and the HBM budget covers the job's footprint.
+// Every slot owns a HIP stream, a grow-only workspace (path state, queues,
+// film, stack spill) and a pinned output buffer.  One scheduler thread issues
+// the admitted jobs' chunks round-robin, keeping up to two chunks in flight
+// per job, so the chunks of different jobs run concurrently on the device
+// and new arrivals start while others are mid-render.  A job's film depends
+// only on its own description (per-job seed, deterministic accumulation),
+// never on what it shared the device with.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+
+#include "ar_job.h"
+
+namespace ar {
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  Workspace ws;
+  float* pinned = nullptr;
+  size_t pinned_bytes = 0;
+  bool busy = false;
+};
+
+struct Job {
+  uint64_t id = 0;
+  JobSpec spec;
+  int32_t state = AR_JOB_QUEUED;
+  ar_status error = AR_OK;
+  std::string error_msg;
+  double t_submit = 0, t_start = 0, t_finish = 0;
+  size_t footprint = 0;
+  int slot = -1;
+  bool cancel = false;
+  bool issued_all = false;   // every chunk + resolve + copy-out enqueued
+  std::unique_ptr<JobRun> run;
+  std::deque<hipEvent_t> inflight;
+  std::vector<float> film;
+  ar_render_stats stats{};
+};
+
+}  // namespace ar
+
+using namespace ar;
+
+struct ar_server {
+  const ar_scene* scene = nullptr;
+  ar_server_config cfg{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<uint64_t> pending;
+  std::map<uint64_t, std::shared_ptr<Job>> jobs;
+  std::vector<std::shared_ptr<Job>> active;
+  std::vector<std::unique_ptr<Slot>> slots;
+  std::vector<hipEvent_t> event_pool;
+  size_t used_bytes = 0;
+  bool stop = false;
+  uint64_t next_id = 1;
+  std::thread worker;
+  std::chrono::steady_clock::time_point t0;
+  int device = 0;
+
+  double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+
+  hipEvent_t get_event() {
+    if (!event_pool.empty()) {
+      hipEvent_t e = event_pool.back();
+      event_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreateWithFlags(&e, hipEventDisableTiming);
+    return e;
+  }
+
+  void release(Job& j) {  // caller holds mu
+    if (j.slot >= 0) {
+      slots[j.slot]->busy = false;
+      j.slot = -1;
+    }
+    used_bytes -= std::min(used_bytes, j.footprint);
+    j.run.reset();
+  }
+
+  void fail_job(Job& j, ar_status code, const std::string& msg) {  // caller holds mu
+    j.state = AR_JOB_FAILED;
+    j.error = code;
+    j.error_msg = msg;
+    j.t_finish = now();
+  }
+
+  // Admit queued jobs into free slots within the budget (caller holds mu).
+  void admit() {
+    while (!pending.empty()) {
+      auto it = jobs.find(pending.front());
+      if (it == jobs.end() || it->second->state != AR_JOB_QUEUED) {
+        pending.pop_front();
+        continue;
+      }
+      Job& j = *it->second;
+      if (j.footprint > cfg.hbm_budget_bytes) {
+        pending.pop_front();
+        fail_job(j, AR_ERR_OUT_OF_BUDGET, "job footprint exceeds the HBM budget");
+        cv_done.notify_all();
+        continue;
+      }
+      if (used_bytes + j.footprint > cfg.hbm_budget_bytes) break;
+      int free_slot = -1;
+      for (size_t s = 0; s < slots.size(); ++s)
+        if (!slots[s]->busy) {
+          free_slot = (int)s;
+          break;
+        }
+      if (free_slot < 0) break;
+      pending.pop_front();
+      Slot& sl = *slots[free_slot];
+      size_t npix = (size_t)j.spec.width * j.spec.height;
+      size_t out_bytes = npix * 3 * sizeof(float);
+      if (sl.ws.out.bytes < out_bytes && sl.ws.out.alloc(out_bytes) != hipSuccess) {
+        fail_job(j, AR_ERR_DEVICE, "output buffer allocation failed");
+        cv_done.notify_all();
+        continue;
+      }
+      if (sl.pinned_bytes < out_bytes) {
+        if (sl.pinned) (void)hipHostFree(sl.pinned);
+        sl.pinned = nullptr;
+        sl.pinned_bytes = 0;
+        if (hipHostMalloc((void**)&sl.pinned, out_bytes, hipHostMallocDefault) != hipSuccess) {
+          fail_job(j, AR_ERR_DEVICE, "pinned buffer allocation failed");
+          cv_done.notify_all();
+          continue;
+        }
+        sl.pinned_bytes = out_bytes;
+      }
+      sl.busy = true;
+      j.slot = free_slot;
+      used_bytes += j.footprint;
+      j.state = AR_JOB_RUNNING;
+      j.t_start = now();
+      j.run = make_job(scene, j.spec, &sl.ws, sl.stream, sl.ws.out.as<float>(), nullptr);
+      ar_status e = j.run->begin();
+      if (e != AR_OK) {
+        fail_job(j, e, std::string("job begin: ") + ar_last_error());
+        release(j);
+        cv_done.notify_all();
+        continue;
+      }
+      active.push_back(it->second);
+    }
+  }
+
+  // One pass over the active jobs: retire finished chunks, issue new ones.
+  // Returns true if any progress was made.  Called without mu held; takes it
+  // only for bookkeeping.
+  bool pump() {
+    std::vector<std::shared_ptr<Job>> act;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      act = active;
+    }
+    bool progress = false;
+    for (auto& jp : act) {
+      Job& j = *jp;
+      while (!j.inflight.empty() && hipEventQuery(j.inflight.front()) == hipSuccess) {
+        event_pool.push_back(j.inflight.front());
+        j.inflight.pop_front();
+        progress = true;
+      }
+      bool cancel;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        cancel = j.cancel;
+      }
+      if (j.issued_all || cancel) {
+        if (!j.inflight.empty()) continue;
+        // drained: complete the job
+        std::lock_guard<std::mutex> lk(mu);
+        if (cancel && !j.issued_all) {
+          j.state = AR_JOB_CANCELLED;
+          j.t_finish = now();
+        } else {
+          Slot& sl = *slots[j.slot];
+          size_t n = (size_t)j.spec.width * j.spec.height * 3;
+          j.film.assign(sl.pinned, sl.pinned + n);
+          (void)j.run->read_stats(&j.stats);
+          j.state = cancel ? AR_JOB_CANCELLED : AR_JOB_DONE;
+          j.t_finish = now();
+        }
+        release(j);
+        active.erase(std::find(active.begin(), active.end(), jp));
+        cv_done.notify_all();
+        cv_work.notify_all();
+        progress = true;
+        continue;
+      }
+      if (j.inflight.size() >= 2) continue;
+      bool done = false;
+      ar_status e = j.run->step(&done);
+      if (e == AR_OK && done) {
+        e = j.run->finish();
+        if (e == AR_OK) {
+          Slot& sl = *slots[j.slot];
+          size_t bytes = (size_t)j.spec.width * j.spec.height * 3 * sizeof(float);
+          if (hipMemcpyAsync(sl.pinned, sl.ws.out.p, bytes, hipMemcpyDeviceToHost, sl.stream) != hipSuccess)
+            e = fail(AR_ERR_DEVICE, "copy-out failed");
+        }
+        j.issued_all = true;
+      }
+      if (e != AR_OK) {
+        (void)hipStreamSynchronize(slots[j.slot]->stream);
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto ev : j.inflight) event_pool.push_back(ev);
+        j.inflight.clear();
+        fail_job(j, e, ar_last_error());
+        release(j);
+        active.erase(std::find(active.begin(), active.end(), jp));
+        cv_done.notify_all();
+        continue;
+      }
+      hipEvent_t ev = get_event();
+      (void)hipEventRecord(ev, slots[j.slot]->stream);
+      j.inflight.push_back(ev);
+      progress = true;
+    }
+    return progress;
+  }
+
+  void loop() {
+    (void)hipSetDevice(device);
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        if (stop && active.empty() && pending.empty()) break;
+        admit();
+        if (active.empty()) {
+          if (stop) continue;
+          cv_work.wait_for(lk, std::chrono::milliseconds(5));
+          continue;
+        }
+      }
+      if (!pump()) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+  }
+};
+
+extern "C" {
+
+ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_server** out) {
+  if (!s || !out) return fail(AR_ERR_INVALID_ARG, "ar_server_create: NULL argument");
+  if (!s->committed) return fail(AR_ERR_STATE, "ar_server_create: scene not committed");
+  std::unique_ptr<ar_server> srv(new ar_server());
+  srv->scene = s;
+  if (cfg) srv->cfg = *cfg;
+  if (srv->cfg.hbm_budget_bytes == 0) srv->cfg.hbm_budget_bytes = 200ull << 30;
+  if (srv->cfg.max_active_jobs == 0) srv->cfg.max_active_jobs = 8;
+  if (srv->cfg.max_active_jobs > 64) return fail(AR_ERR_INVALID_ARG, "max_active_jobs above 64");
+  (void)hipGetDevice(&srv->device);
+  for (uint32_t i = 0; i < srv->cfg.max_active_jobs; ++i) {
+    srv->slots.emplace_back(new Slot());
+    AR_HIP(hipStreamCreateWithFlags(&srv->slots.back()->stream, hipStreamNonBlocking));
+  }
+  srv->t0 = std::chrono::steady_clock::now();
+  ar_server* p = srv.release();
+  p->worker = std::thread([p]() { p->loop(); });
+  *out = p;
+  return AR_OK;
+}
+
+ar_status ar_server_destroy(ar_server* srv) {
+  if (!srv) return AR_OK;
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    for (uint64_t id : srv->pending) {
+      auto it = srv->jobs.find(id);
+      if (it != srv->jobs.end() && it->second->state == AR_JOB_QUEUED) it->second->state = AR_JOB_CANCELLED;
+    }
+    srv->pending.clear();
+    for (auto& j : srv->active) j->cancel = true;
+    srv->stop = true;
+  }
+  srv->cv_work.notify_all();
+  srv->worker.join();
+  for (auto& sp : srv->slots) {
+    Slot& sl = *sp;
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.pinned) (void)hipHostFree(sl.pinned);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  for (auto e : srv->event_pool) (void)hipEventDestroy(e);
+  delete srv;
+  return AR_OK;
+}
+
+ar_status ar_job_submit(ar_server* srv, const ar_job_desc* desc, uint64_t* out_id) {
+  if (!srv || !desc) return fail(AR_ERR_INVALID_ARG, "ar_job_submit: NULL argument");
+  auto j = std::make_shared<Job>();
+  j->spec.width = desc->width;
+  j->spec.height = desc->height;
+  j->spec.camera = desc->camera;
+  j->spec.cfg = desc->config;
+  ar_status e = validate_job(srv->scene, j->spec);
+  if (e != AR_OK) return e;
+  j->footprint = job_footprint(srv->scene, j->spec);
+  if (j->footprint > srv->cfg.hbm_budget_bytes) return fail(AR_ERR_OUT_OF_BUDGET, "job exceeds the HBM budget");
+  {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    if (srv->stop) return fail(AR_ERR_STATE, "server is shutting down");
+    j->id = srv->next_id++;
+    j->t_submit = srv->now();
+    srv->jobs[j->id] = j;
+    srv->pending.push_back(j->id);
+    if (out_id) *out_id = j->id;
+  }
+  srv->cv_work.notify_all();
+  return AR_OK;
+}
+
+ar_status ar_job_poll(ar_server* srv, uint64_t id, int32_t* state) {
+  if (!srv || !state) return fail(AR_ERR_INVALID_ARG, "ar_job_poll: NULL argument");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->jobs.find(id);
+  if (it == srv->jobs.end()) return fail(AR_ERR_NOT_FOUND, "unknown job id");
+  *state = it->second->state;
+  return AR_OK;
+}
+
+ar_status ar_job_wait(ar_server* srv, uint64_t id, float* out_rgb, ar_job_stats* stats) {
+  if (!srv) return fail(AR_ERR_INVALID_ARG, "ar_job_wait: NULL server");
+  std::shared_ptr<Job> j;
+  {
+    std::unique_lock<std::mutex> lk(srv->mu);
+    auto it = srv->jobs.find(id);
+    if (it == srv->jobs.end()) return fail(AR_ERR_NOT_FOUND, "unknown job id");
+    j = it->second;
+    srv->cv_done.wait(lk, [&]() {
+      return j->state == AR_JOB_DONE || j->state == AR_JOB_CANCELLED || j->state == AR_JOB_FAILED;
+    });
+    srv->jobs.erase(id);
+  }
+  if (stats) {
+    stats->submit_time = j->t_submit;
+    stats->start_time = j->t_start;
+    stats->finish_time = j->t_finish;
+    stats->paths = j->stats.paths;
+    stats->rays = j->stats.rays;
+    stats->state = j->state;
+  }
+  if (j->state == AR_JOB_CANCELLED) return fail(AR_ERR_CANCELLED, "job was cancelled");
+  if (j->state == AR_JOB_FAILED) return fail(j->error, j->error_msg);
+  if (out_rgb) memcpy(out_rgb, j->film.data(), j->film.size() * sizeof(float));
+  return AR_OK;
+}
+
+ar_status ar_job_cancel(ar_server* srv, uint64_t id) {
+  if (!srv) return fail(AR_ERR_INVALID_ARG, "ar_job_cancel: NULL server");
+  std::lock_guard<std::mutex> lk(srv->mu);
+  auto it = srv->jobs.find(id);
+  if (it == srv->jobs.end()) return fail(AR_ERR_NOT_FOUND, "unknown job id");
+  Job& j = *it->second;
+  if (j.state == AR_JOB_QUEUED) {
+    j.state = AR_JOB_CANCELLED;
+    j.t_finish = srv->now();
+    srv->cv_done.notify_all();
+  } else if (j.state == AR_JOB_RUNNING) {
+    j.cancel = true;
+  }
+  return AR_OK;
+}
+
+double ar_server_clock(const ar_server* srv) { return srv ? srv->now() : 0.0; }
+
+}  // extern "C"
