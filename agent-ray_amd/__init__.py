@@ -88,6 +88,11 @@ class ArJobDesc(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("camera", ArCamera), ("config", ArRenderConfig)]
 
 
+class ArTraceConfig(C.Structure):
+    _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("ms", C.c_float)]
+
+
 class ArJobStats(C.Structure):
     _fields_ = [("submit_time", C.c_double), ("start_time", C.c_double), ("finish_time", C.c_double),
                 ("paths", C.c_uint64), ("rays", C.c_uint64), ("state", C.c_int32)]
@@ -126,6 +131,9 @@ def _load():
         "ar_job_cancel": (i32, [P, u64]),
         "ar_server_clock": (C.c_double, [P]),
         "ar_bvh_build_host": (i32, [P, u32, P, P, P, C.POINTER(u32), C.POINTER(u32)]),
+        "ar_scene_set_trace_config": (i32, [P, C.POINTER(ArTraceConfig)]),
+        "ar_scene_get_trace_config": (i32, [P, C.POINTER(ArTraceConfig)]),
+        "ar_scene_autotune": (i32, [P, P, u64, u32, P, C.POINTER(ArTraceConfig)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -319,6 +327,25 @@ class Scene:
         inst = np.empty(n, np.uint32)
         _check(_lib.ar_trace(self._h, _ptr(r), n, _ptr(hits), _ptr(inst), flags | AR_HOST_PTRS, None))
         return hits[:, 0].copy(), hits[:, 1].view(np.uint32).copy(), inst
+
+    def trace_config(self) -> dict:
+        c = ArTraceConfig()
+        _check(_lib.ar_scene_get_trace_config(self._h, C.byref(c)))
+        return {"workgroup_size": c.workgroup_size, "lds_stack": c.lds_stack, "blocks_per_cu": c.blocks_per_cu,
+                "ms": c.ms}
+
+    def set_trace_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int):
+        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, 0.0)
+        _check(_lib.ar_scene_set_trace_config(self._h, C.byref(c)))
+
+    def autotune(self, rays, stream=None) -> dict:
+        """Times the traversal launch variants on a device ray sample (torch
+        CUDA tensor, (n, 8) float32) and keeps the fastest for this scene."""
+        c = ArTraceConfig()
+        sh = C.c_void_p(stream.cuda_stream) if stream is not None else None
+        _check(_lib.ar_scene_autotune(self._h, C.c_void_p(rays.data_ptr()), rays.shape[0], 0, sh, C.byref(c)))
+        return {"workgroup_size": c.workgroup_size, "lds_stack": c.lds_stack, "blocks_per_cu": c.blocks_per_cu,
+                "ms": c.ms}
 
     def occluded(self, rays, ref_order: bool = False):
         """Scene::occluded (scene.rs:166-168) as an any-hit query."""

@@ -139,6 +139,7 @@ struct ar_scene {
   ar::DevBuf d_scratch;
   hipEvent_t scratch_event = nullptr;
   uint32_t tlas_depth = 0;
+  ar_trace_config tcfg{128, 16, 8, 0.0f};  // traversal launch configuration (autotuned)
   ar::DevBuf d_nodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
       d_tri_obj;
   ar::SceneView view{};

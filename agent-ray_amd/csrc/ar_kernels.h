@@ -9,6 +9,12 @@ constexpr int kTraceLdsStack = 16;     // LDS stack entries per lane (8 B each)
 constexpr int kTraceBlocksPerCU = 8;   // persistent grid: 16 waves per CU
 
 uint32_t trace_grid(uint64_t n, int block);
+uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c);
+int device_cus();
+
+// Traversal variants compiled: (workgroup size, LDS stack entries per lane).
+#define AR_TRACE_VARIANTS(X) X(64, 8) X(64, 16) X(128, 8) X(128, 16) X(256, 8)
+bool trace_variant_supported(uint32_t block, uint32_t stack);
 
 // Scene-level scratch (traversal stack spill).  acquire() orders the caller's
 // stream after the previous user of the scratch (an event, no host wait) and

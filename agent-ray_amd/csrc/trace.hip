@@ -65,27 +65,47 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const ar_ray* __r
 // hit, so the depth of the deepest BLAS plus the TLAS bounds the stack.
 static uint32_t stack_need(const ar_scene* s) { return s->info.max_depth + s->tlas_depth + 2; }
 
+template <bool ANY, bool NEAR, int BLOCK, int S>
+static void launch_variant(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits,
+                           uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st) {
+  hipLaunchKernelGGL((k_trace<ANY, NEAR, S, BLOCK>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits, inst, occ,
+                     spill);
+}
+
 template <bool ANY, bool NEAR>
 static ar_status launch_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst,
-                              uint32_t* occ, hipStream_t st) {
-  constexpr int S = kTraceLdsStack;
-  constexpr int BLOCK = kTraceBlock;
-  uint32_t G = trace_grid(n, BLOCK);
+                              uint32_t* occ, hipStream_t st, const ar_trace_config& c) {
+  uint32_t G = trace_grid_cfg(n, c);
   int2* spill = nullptr;
   uint32_t need = stack_need(s);
-  size_t bytes = need > (uint32_t)S ? (size_t)(need - S) * G * BLOCK * sizeof(int2) : 0;
+  size_t bytes = need > c.lds_stack ? (size_t)(need - c.lds_stack) * G * c.workgroup_size * sizeof(int2) : 0;
   ar_scene* ms = const_cast<ar_scene*>(s);
   ar_status e = scratch_acquire(ms, bytes, st, (void**)&spill);
   if (e != AR_OK) return e;
-  hipLaunchKernelGGL((k_trace<ANY, NEAR, S, BLOCK>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits, inst,
-                     occ, spill);
+  bool ok = false;
+#define AR_LAUNCH(B, SS)                                                         \
+  if (c.workgroup_size == B && c.lds_stack == SS) {                              \
+    launch_variant<ANY, NEAR, B, SS>(s, G, rays, n, hits, inst, occ, spill, st); \
+    ok = true;                                                                   \
+  }
+  AR_TRACE_VARIANTS(AR_LAUNCH)
+#undef AR_LAUNCH
   hipError_t le = hipGetLastError();
   scratch_release(ms, st);
+  if (!ok) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
   if (le != hipSuccess) return fail(AR_ERR_DEVICE, std::string("k_trace launch: ") + hipGetErrorString(le));
   return AR_OK;
 }
 
-uint32_t trace_grid(uint64_t n, int block) {
+bool trace_variant_supported(uint32_t block, uint32_t stack) {
+  bool ok = false;
+#define AR_CHECK(B, SS) ok |= (block == B && stack == SS);
+  AR_TRACE_VARIANTS(AR_CHECK)
+#undef AR_CHECK
+  return ok;
+}
+
+int device_cus() {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
@@ -94,12 +114,21 @@ uint32_t trace_grid(uint64_t n, int block) {
     if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
     if (cus <= 0) cus = 256;
   }
-  uint64_t want = (n + block - 1) / block;
-  uint64_t cap = (uint64_t)cus * kTraceBlocksPerCU;
+  return cus;
+}
+
+uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c) {
+  uint64_t want = (n + c.workgroup_size - 1) / c.workgroup_size;
+  uint64_t cap = (uint64_t)device_cus() * c.blocks_per_cu;
   uint64_t g = want < cap ? want : cap;
   g = (g + 7) / 8 * 8;
   if (g == 0) g = 8;
   return (uint32_t)g;
+}
+
+uint32_t trace_grid(uint64_t n, int block) {
+  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, 0.0f};
+  return trace_grid_cfg(n, c);
 }
 
 ar_status scratch_acquire(ar_scene* s, size_t bytes, hipStream_t st, void** out) {
@@ -169,8 +198,8 @@ ar_status ar_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hi
     d_hits = sg.out.as<ar_hit>();
     d_inst = inst_ids ? sg.out2.as<uint32_t>() : nullptr;
   }
-  ar_status e = (flags & AR_REF_ORDER) ? launch_trace<false, false>(s, d_rays, n, d_hits, d_inst, nullptr, st)
-                                       : launch_trace<false, true>(s, d_rays, n, d_hits, d_inst, nullptr, st);
+  ar_status e = (flags & AR_REF_ORDER) ? launch_trace<false, false>(s, d_rays, n, d_hits, d_inst, nullptr, st, s->tcfg)
+                                       : launch_trace<false, true>(s, d_rays, n, d_hits, d_inst, nullptr, st, s->tcfg);
   if (e != AR_OK) return e;
   if (host) {
     AR_HIP(hipMemcpyAsync(hits, d_hits, n * sizeof(ar_hit), hipMemcpyDeviceToHost, st));
@@ -197,11 +226,86 @@ ar_status ar_occluded(const ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
     d_rays = sg.rays.as<ar_ray>();
     d_occ = sg.out.as<uint32_t>();
   }
-  ar_status e = (flags & AR_REF_ORDER) ? launch_trace<true, false>(s, d_rays, n, nullptr, nullptr, d_occ, st)
-                                       : launch_trace<true, true>(s, d_rays, n, nullptr, nullptr, d_occ, st);
+  ar_status e = (flags & AR_REF_ORDER) ? launch_trace<true, false>(s, d_rays, n, nullptr, nullptr, d_occ, st, s->tcfg)
+                                       : launch_trace<true, true>(s, d_rays, n, nullptr, nullptr, d_occ, st, s->tcfg);
   if (e != AR_OK) return e;
   if (host) AR_HIP(hipMemcpyAsync(occluded, d_occ, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   if (host || !(flags & AR_ASYNC)) AR_HIP(hipStreamSynchronize(st));
+  return AR_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c) {
+  if (!s || !c) return fail(AR_ERR_INVALID_ARG, "ar_scene_set_trace_config: NULL argument");
+  if (!trace_variant_supported(c->workgroup_size, c->lds_stack))
+    return fail(AR_ERR_UNSUPPORTED_TYPE, "no traversal variant for this workgroup size / stack depth");
+  if (c->blocks_per_cu == 0 || c->blocks_per_cu > 64) return fail(AR_ERR_INVALID_ARG, "blocks_per_cu out of range");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->tcfg = *c;
+  s->tcfg.ms = 0.0f;
+  return AR_OK;
+}
+
+ar_status ar_scene_get_trace_config(const ar_scene* s, ar_trace_config* c) {
+  if (!s || !c) return fail(AR_ERR_INVALID_ARG, "ar_scene_get_trace_config: NULL argument");
+  *c = s->tcfg;
+  return AR_OK;
+}
+
+// Times every (variant, blocks per CU) candidate on the sample (closest hit,
+// near-first order, one warm-up launch each) and keeps the fastest.
+ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t flags, void* stream,
+                            ar_trace_config* best) {
+  if (!s || !rays || n == 0) return fail(AR_ERR_INVALID_ARG, "ar_scene_autotune: NULL or empty sample");
+  if (!s->committed) return fail(AR_ERR_STATE, "ar_scene_autotune: scene not committed");
+  if (flags & AR_HOST_PTRS) return fail(AR_ERR_INVALID_ARG, "ar_scene_autotune: sample must be on the device");
+  hipStream_t st = pick_stream(s, stream);
+  DevBuf hits;
+  AR_HIP(hits.alloc(n * sizeof(ar_hit)));
+  hipEvent_t e0, e1;
+  AR_HIP(hipEventCreate(&e0));
+  AR_HIP(hipEventCreate(&e1));
+  ar_trace_config winner = s->tcfg;
+  float best_ms = INFINITY;
+  const uint32_t per_cu[3] = {4, 8, 16};
+  const uint32_t variants[][2] = {
+#define AR_LIST(B, SS) {B, SS},
+      AR_TRACE_VARIANTS(AR_LIST)
+#undef AR_LIST
+  };
+  ar_status err = AR_OK;
+  for (auto& vv : variants) {
+    for (uint32_t pc : per_cu) {
+      ar_trace_config c{vv[0], vv[1], pc, 0.0f};
+      err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
+      if (err != AR_OK) break;
+      (void)hipEventRecord(e0, st);
+      err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
+      if (err != AR_OK) break;
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (ms < best_ms) {
+        best_ms = ms;
+        winner = c;
+      }
+    }
+    if (err != AR_OK) break;
+  }
+  (void)hipStreamSynchronize(st);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != AR_OK) return err;
+  winner.ms = best_ms;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->tcfg = winner;
+  }
+  if (best) *best = winner;
   return AR_OK;
 }
 

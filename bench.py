@@ -107,6 +107,14 @@ def bench_traversal(args, ar, torch):
     lib = ar.lib()
     stream = torch.cuda.current_stream()
     sh = C_void(stream.cuda_stream)
+    tuned = None
+    if not args.no_autotune:
+        # per-scene autotuning of the traversal launch on 2^20 rays from both halves
+        idx = torch.cat([torch.arange(0, 1 << 19), (n_rays // 2) + torch.arange(0, 1 << 19)]).cuda()
+        sample = dr.index_select(0, idx).contiguous()
+        tuned = sc.autotune(sample, stream)
+        log(f"autotuned traversal: {tuned}")
+        del sample
 
     def closest():
         ar._check(lib.ar_trace(sc.handle, C_void(dr.data_ptr()), n_rays, C_void(hits.data_ptr()), None,
@@ -151,6 +159,7 @@ def bench_traversal(args, ar, torch):
         "closest_kernel_ms": round(float(np.mean(t_closest)), 3),
         "anyhit_kernel_ms": round(float(np.mean(t_any)), 3),
         "scene_build_s": round(build_s, 2),
+        "launch_config": sc.trace_config(),
     }
     # per-half rates (coherent / incoherent), one extra timed call each
     for name, sl in (("coherent", slice(0, n_rays // 2)), ("incoherent", slice(n_rays // 2, n_rays))):
@@ -188,6 +197,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="traversal", choices=["traversal"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "

@@ -180,6 +180,24 @@ ar_status ar_scene_get_info(const ar_scene* s, ar_scene_info* info);
 ar_status ar_bvh_build_host(const float* boxes, uint32_t n, float* out_boxes, uint32_t* out_meta,
                             uint32_t* out_prims, uint32_t* out_nodes, uint32_t* out_depth);
 
+/* ---- launch configuration and autotuning (no reference counterpart: the
+ * reference runs on rayon's CPU pool) ---------------------------------------
+ * Traversal kernels come in (workgroup size, LDS stack depth) variants
+ * (64/8, 64/16, 128/8, 128/16, 256/8) and a persistent grid of blocks_per_cu
+ * workgroups per CU.  The autotuner times the candidates on a ray sample
+ * (device pointers) and keeps the fastest for this scene; ar_trace,
+ * ar_occluded and the wavefront stages then use it. */
+typedef struct ar_trace_config {
+  uint32_t workgroup_size;
+  uint32_t lds_stack;
+  uint32_t blocks_per_cu;
+  float ms; /* measured time of the winning candidate (autotune only) */
+} ar_trace_config;
+ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c);
+ar_status ar_scene_get_trace_config(const ar_scene* s, ar_trace_config* c);
+ar_status ar_scene_autotune(ar_scene* s, const ar_ray* sample_rays, uint64_t n, uint32_t flags, void* stream,
+                            ar_trace_config* best);
+
 /* ---- traversal (TLAS::intersect, bvh.rs:413-447; Scene::occluded,
  * scene.rs:166-168) ------------------------------------------------------- */
 /* Closest hit for n rays.  inst_ids may be NULL. */

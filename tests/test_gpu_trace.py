@@ -108,6 +108,27 @@ def test_empty_batch_and_device_pointers(ar, wl):
     assert np.array_equal(occ.cpu().numpy().astype(bool), sc.occluded(rays))
 
 
+def test_every_launch_variant_same_answer(ar, wl):
+    """All (workgroup, LDS stack, blocks/CU) variants, including stacks that
+    spill to HBM, give identical hits; the autotuner picks one of them."""
+    import torch
+    desc = wl.triangle_soup(1 << 15, seed=61)
+    rays = wl.soup_rays(1 << 15, seed=62, image=128)
+    sc = ar.Scene.from_desc(desc)
+    ref_t, ref_p, _ = sc.intersect(rays)
+    ref_o = sc.occluded(rays)
+    for (b, s) in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8)]:
+        for pc in (4, 16):
+            sc.set_trace_config(b, s, pc)
+            t, p, _ = sc.intersect(rays)
+            assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p), (b, s, pc)
+            assert np.array_equal(sc.occluded(rays), ref_o)
+    cfg = sc.autotune(torch.from_numpy(rays).cuda())
+    assert cfg["ms"] > 0 and cfg["workgroup_size"] in (64, 128, 256)
+    with pytest.raises(ar.ArError):
+        sc.set_trace_config(96, 8, 4)
+
+
 @pytest.mark.slow
 def test_full_size_soup(ar, orc, wl):
     """1M triangles, 2^24 rays: every ray agrees between near-first and
