@@ -190,14 +190,115 @@ def C_void(p):
     return ctypes.c_void_p(p)
 
 
+def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
+    """Path tracer throughput on the Cornell box (256^2, 16 spp, depth 8) or
+    the procedural room (1920x1080, 64 spp, depth 16).  Mpaths/s = W*H*spp /
+    render wall time (scene upload excluded, film readback included)."""
+    wl = ar.workloads
+    desc = wl.cornell_box() if which == "cornell" else wl.procedural_room()
+    t0 = time.time()
+    sc = ar.Scene.from_desc(desc)
+    build = time.time() - t0
+    cd = desc.camera
+    cam = ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov,
+                                                       desc.width / desc.height))
+    out = {"width": desc.width, "height": desc.height, "spp": desc.spp, "max_depth": desc.max_depth,
+           "triangles": desc.n_triangles, "scene_build_s": round(build, 2)}
+    films = {}
+    for sched in schedules:
+        pt = ar.PathTracer(ar.PathTracerConfig(spp=desc.spp, max_depth=desc.max_depth, rr_depth=3), seed=0,
+                           schedule=ar.Schedule.MEGAKERNEL if sched == "megakernel" else ar.Schedule.WAVEFRONT)
+        pt.render(sc, cam, desc.width, desc.height)  # warm-up (allocations)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        film = pt.render(sc, cam, desc.width, desc.height)
+        dt = time.perf_counter() - t0
+        st = pt.last_stats
+        films[sched] = film.pixels
+        out[sched] = {"mpaths_s": round(st.paths / dt / 1e6, 2), "mrays_s": round(st.rays / dt / 1e6, 1),
+                      "seconds": round(dt, 4), "rays": int(st.rays)}
+        log(f"pt {which} {sched}: {out[sched]}")
+    if len(films) == 2:
+        a, b = films.values()
+        out["schedules_bitwise_equal"] = bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
+        out["wavefront_speedup"] = round(out["megakernel"]["seconds"] / out["wavefront"]["seconds"], 3)
+    return out
+
+
+def bench_serving(args, ar, torch):
+    """Poisson arrivals against the resident room scene; per job resolution
+    U{64..1024 step 64}^2, spp round(exp(U[0, ln 256])), PT 60% / BDPT 25% /
+    GDPT 15%, seed = job index.  Reports p50/p99 latency, offered load and the
+    p99 of latency / solo render time over a sample of jobs."""
+    wl = ar.workloads
+    desc = wl.procedural_room()
+    sc = ar.Scene.from_desc(desc)
+    jobs = wl.serving_jobs(rate=args.rate, duration=args.duration, seed=0)
+    cd = desc.camera
+
+    def integ(j):
+        if j.integrator == 0:
+            return ar.PathTracer(ar.PathTracerConfig(spp=j.spp, max_depth=16, rr_depth=3), seed=j.seed)
+        if j.integrator == 1:
+            return ar.BidirectionalPathTracer(ar.BdptConfig(spp=j.spp, max_depth=10, rr_depth=3), seed=j.seed)
+        return ar.GradientDomainPathTracer(ar.GdptConfig(spp=j.spp, max_depth=16, poisson_iterations=50),
+                                           seed=j.seed)
+
+    def cam(j):
+        return ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov,
+                                                            j.width / j.height))
+    srv = ar.RenderServer(sc, hbm_budget_bytes=200 << 30, max_active_jobs=args.slots)
+    # warm the slot workspaces so first-touch allocation is not on the clock
+    warm = [srv.submit(integ(j), cam(j), j.width, j.height) for j in jobs[: args.slots]]
+    for w in warm:
+        srv.wait(w)
+    t0 = time.perf_counter()
+    ids = []
+    for j in jobs:
+        now = time.perf_counter() - t0
+        if j.arrival > now:
+            time.sleep(j.arrival - now)
+        ids.append((j, srv.submit(integ(j), cam(j), j.width, j.height)))
+    lat, paths = [], 0
+    for j, jid in ids:
+        _, st = srv.wait(jid)
+        lat.append(st["latency"])
+        paths += st["paths"]
+    wall = time.perf_counter() - t0
+    srv.close()
+    lat = np.asarray(lat)
+    # solo render times of an evenly spaced sample of the jobs
+    picks = list(range(0, len(ids), max(1, len(ids) // 24)))
+    ratios, solo_total = [], 0.0
+    for k in picks:
+        j = ids[k][0]
+        it = integ(j)
+        it.render(sc, cam(j), j.width, j.height)
+        solo = it.last_stats.seconds
+        solo_total += solo
+        ratios.append(lat[k] / max(solo, 1e-6))
+    sample = picks
+    return {"jobs": len(jobs), "rate_per_s": args.rate, "duration_s": args.duration, "slots": args.slots,
+            "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+            "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+            "p99_latency_over_solo": round(float(np.percentile(np.asarray(ratios), 99)), 3),
+            "median_latency_over_solo": round(float(np.median(np.asarray(ratios))), 3),
+            "offered_load": round(solo_total / len(sample) * args.rate, 3),
+            "mpaths_s": round(paths / wall / 1e6, 2), "wall_s": round(wall, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="traversal", choices=["traversal"])
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the path-tracer throughput fields")
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -209,7 +310,18 @@ def main():
         sys.exit(1)
     ar = load_pkg()
     ar.init(0)
+    if args.workload == "serving":
+        srv = bench_serving(args, ar, torch)
+        out = {"metric": METRIC, "value": srv["p99_latency_s"], "unit": "s (p99 job latency)", "n_gpus": 1,
+               "steps": len(srv), "warmup": args.slots, "ms_per_step": None, "higher_is_better": False,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+               "config": {"workload": "serving_room_280k_tris", "parallelism": "single-gpu"}, "serving": srv}
+        print(json.dumps(out), flush=True)
+        return
     out = bench_traversal(args, ar, torch)
+    if not args.no_extra:
+        out["extra"] = {"pt_cornell_256x256x16spp": bench_pt(ar, torch, "cornell"),
+                        "pt_room_1920x1080x64spp": bench_pt(ar, torch, "room")}
     print(json.dumps(out), flush=True)
 
 
