@@ -44,14 +44,14 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint32
   if (__lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
 }
 
-template <int BLOCK>
+template <int BLOCK, int S = kTraceLdsStack>
 __device__ __forceinline__ Stack make_stack(int2* lds, int2* spill, uint32_t gtid, uint32_t nthreads) {
   Stack stk;
   stk.lds = lds + threadIdx.x;
   stk.stride = BLOCK;
   stk.spill = spill + gtid;
   stk.spill_stride = nthreads;
-  stk.S = kTraceLdsStack;
+  stk.S = S;
   stk.sp = 0;
   return stk;
 }
@@ -149,12 +149,12 @@ __global__ void k_wf_begin(WaveBufs B) {
   if (threadIdx.x < 3) B.cnt[threadIdx.x] = 0;
 }
 
-template <int BLOCK>
+template <int BLOCK, int S>
 __global__ __launch_bounds__(BLOCK) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill) {
-  __shared__ int2 lds[kTraceLdsStack * BLOCK];
+  __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
-  Stack stk = make_stack<BLOCK>(lds, spill, gtid, nthreads);
+  Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
   for (uint32_t i = gtid; i < n; i += nthreads) {
@@ -218,12 +218,12 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, PtParams P, Wav
   }
 }
 
-template <int BLOCK>
+template <int BLOCK, int S>
 __global__ __launch_bounds__(BLOCK) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill) {
-  __shared__ int2 lds[kTraceLdsStack * BLOCK];
+  __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
-  Stack stk = make_stack<BLOCK>(lds, spill, gtid, nthreads);
+  Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = B.cnt[2];
   for (uint32_t i = gtid; i < n; i += nthreads) {
     float4 o = B.sh_o[i], d = B.sh_d[i];
@@ -383,13 +383,24 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   // Chunk parameters live in the kernel arguments, so a graph is captured
   // per distinct (chunk size, sample offset) only when use_graph is set and
   // replayed for every chunk with identical parameters.
+  // Traversal stages use the scene's (autotuned) launch configuration.
+  const ar_trace_config& tc = s->tcfg;
+  uint32_t Gt = (uint32_t)(device_cus() * tc.blocks_per_cu / 8 * 8);
   hipLaunchKernelGGL(k_wf_begin, dim3(1), dim3(64), 0, st, B);
   hipLaunchKernelGGL((k_wf_generate<BLOCK>), dim3(G), dim3(BLOCK), 0, st, P, B);
   for (uint32_t d = 0; d < P.max_depth; ++d) {
     int qi = d & 1;
-    hipLaunchKernelGGL((k_wf_extend<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v.sv, B, qi, ws.spill);
+#define AR_EXT(BB, SS)                                                                                         \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                           \
+    hipLaunchKernelGGL((k_wf_extend<BB, SS>), dim3(Gt), dim3(BB), 0, st, v.sv, B, qi, ws.spill);
+    AR_TRACE_VARIANTS(AR_EXT)
+#undef AR_EXT
     hipLaunchKernelGGL((k_wf_shade<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, B, qi);
-    hipLaunchKernelGGL((k_wf_shadow<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v.sv, B, ws.spill);
+#define AR_SHD(BB, SS)                                                                                         \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                           \
+    hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(Gt), dim3(BB), 0, st, v.sv, B, ws.spill);
+    AR_TRACE_VARIANTS(AR_SHD)
+#undef AR_SHD
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, B, qi);
   }
   (void)use_graph;
@@ -476,7 +487,16 @@ class PtJob : public JobRun {
     k = chunk_samples(spec);
     G = persistent_grid(BLOCK, kTraceBlocksPerCU);
     eg = std::min<uint32_t>((npix + 255) / 256, 4096);
-    ar_status e = ws->reserve_paths(P.variants * npix * k, npix, stack_spill_entries(scene), G * BLOCK);
+    // Spill area: covers the megakernel's (128, 16) launch and the traversal
+    // stages' tuned launch, whichever runs.
+    uint32_t need = scene->info.max_depth + scene->tlas_depth + 2;
+    const ar_trace_config& tc = scene->tcfg;
+    uint64_t gt = (uint64_t)(device_cus() * tc.blocks_per_cu / 8 * 8) * tc.workgroup_size;
+    uint64_t e_t = need > tc.lds_stack ? need - tc.lds_stack : 0;
+    uint64_t e_m = stack_spill_entries(scene);
+    uint64_t spill_words = std::max<uint64_t>(e_t * gt, e_m * (uint64_t)G * BLOCK);
+    ar_status e = ws->reserve_paths(P.variants * npix * k, npix, (uint32_t)((spill_words + G * BLOCK - 1) / (G * BLOCK)),
+                                    G * BLOCK);
     if (e != AR_OK) return e;
     if (gdpt) {
       films = aux_dev;
