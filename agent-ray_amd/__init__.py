@@ -101,6 +101,14 @@ class ArJobStats(C.Structure):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() (make -C agent-ray_amd/csrc)")
+    # PyTorch ships its own HIP runtime (same soname, libamdhip64.so.7).  If
+    # torch is present it is loaded first so the library binds to that copy:
+    # two HIP/HSA runtimes in one process cannot both own the GPU, and device
+    # pointers from torch tensors are only valid in the runtime that made them.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     P = C.c_void_p
     u32, u64, i32, f = C.c_uint32, C.c_uint64, C.c_int32, C.c_float
