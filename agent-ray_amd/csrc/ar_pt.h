@@ -103,7 +103,12 @@ __device__ __forceinline__ bool pt_shade(const ShadeView& v, const PtParams& P, 
         }
         if (!is_zero(c)) {
           f3 off = sp.n * 1e-4f * signum(dot(ls.wi, sp.n));
-          sh.ray = make_ray(sp.p + off, ls.wi, 0.0f, ls.dist * (1.0f - 1e-4f));
+          // path_tracer.rs:88-92 for a point light.  A quad stops a relative
+          // 1e-4 short of its emitter plane measured from the offset origin,
+          // so the emitter's own triangles never decide occlusion by rounding.
+          float t_end = ls.dist;
+          if (Lt.type != AR_LIGHT_POINT) t_end = ls.dist + dot(off, mk(Lt.n[0], Lt.n[1], Lt.n[2])) / ls.cos_l;
+          sh.ray = make_ray(sp.p + off, ls.wi, 0.0f, t_end * (1.0f - 1e-4f));
           sh.c = c;
           sh.valid = true;
         }

@@ -292,6 +292,7 @@ struct LightSample {
   float dist;
   f3 li;
   float pdf;
+  float cos_l;  // quad: cosine at the light (0 for a point light)
 };
 
 __device__ __forceinline__ bool light_sample(const DLight& L, f3 p, float ux, float uy, LightSample& o) {
@@ -304,6 +305,7 @@ __device__ __forceinline__ bool light_sample(const DLight& L, f3 p, float ux, fl
     o.dist = d;
     o.li = mk(L.intensity[0], L.intensity[1], L.intensity[2]) / d2;
     o.pdf = 1.0f;
+    o.cos_l = 0.0f;
     return true;
   }
   f3 x = (mk(L.pos[0], L.pos[1], L.pos[2]) + mk(L.eu[0], L.eu[1], L.eu[2]) * ux) + mk(L.ev[0], L.ev[1], L.ev[2]) * uy;
@@ -318,6 +320,7 @@ __device__ __forceinline__ bool light_sample(const DLight& L, f3 p, float ux, fl
   o.dist = d;
   o.li = mk(L.intensity[0], L.intensity[1], L.intensity[2]);
   o.pdf = d2 / (L.area * cl);
+  o.cos_l = cl;
   return true;
 }
 

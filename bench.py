@@ -105,10 +105,17 @@ def bench_traversal(args, ar, torch):
     hits = torch.empty((n_rays, 2), dtype=torch.float32, device="cuda")
     occ = torch.empty((n_rays,), dtype=torch.int32, device="cuda")
     lib = ar.lib()
-    stream = torch.cuda.current_stream()
+    # A stream of our own (default priority): torch's default stream has the
+    # NULL handle, which the library reads as "the scene's own stream", and
+    # the events below must sit on the stream the kernels run on.
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     sh = C_void(stream.cuda_stream)
+    assert stream.cuda_stream != 0
     tuned = None
-    if not args.no_autotune:
+    if args.trace_config:
+        sc.set_trace_config(*[int(v) for v in args.trace_config.split(",")])
+    elif not args.no_autotune:
         # per-scene autotuning of the traversal launch on 2^20 rays from both halves
         idx = torch.cat([torch.arange(0, 1 << 19), (n_rays // 2) + torch.arange(0, 1 << 19)]).cuda()
         sample = dr.index_select(0, idx).contiguous()
@@ -183,6 +190,26 @@ def bench_traversal(args, ar, torch):
                            "kernel": "k_trace<closest, near-first>",
                            "bytes_per_ray": round(b_ray, 1), "n_node": round(nn, 2), "n_tri": round(nt, 2)}
     return out
+
+
+def bench_copy(args, torch):
+    """1 GiB fp32 device-to-device copy (torch's elementwise copy kernel):
+    reads and writes exactly 2^30 bytes per launch."""
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(max(args.warmup, 1)):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    return {"workload": "copy_1GiB_f32", "bytes_read": 4 * n, "bytes_written": 4 * n, "ms_per_copy": round(ms, 4),
+            "gbs": round(8 * n / (ms / 1e3) / 1e9, 1)}
 
 
 def C_void(p):
@@ -292,7 +319,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving"])
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "copy"],
+                    help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--trace-config", default=None,
+                    help="WORKGROUP,LDS_STACK,BLOCKS_PER_CU: fixed traversal launch instead of autotuning")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the path-tracer throughput fields")
@@ -310,6 +341,9 @@ def main():
         sys.exit(1)
     ar = load_pkg()
     ar.init(0)
+    if args.workload == "copy":
+        print(json.dumps(bench_copy(args, torch)), flush=True)
+        return
     if args.workload == "serving":
         srv = bench_serving(args, ar, torch)
         out = {"metric": METRIC, "value": srv["p99_latency_s"], "unit": "s (p99 job latency)", "n_gpus": 1,

@@ -277,6 +277,7 @@ bool Light::sample(V3 p, float ux, float uy, LightSample* out) const {
   out->dist = dist;
   out->li = intensity;
   out->pdf = dist2 / (area * cos_l);
+  out->cos_l = cos_l;
   return true;
 }
 
@@ -586,7 +587,14 @@ V3 pt_li(const Scene& sc, Ray ray, Sampler& smp, const PtConfig& cfg, TraceCount
       LightSample ls;
       if (L.sample(hit.p, ux, uy, &ls)) {
         V3 off = hit.n * 1e-4f * rs_signum(dot(ls.wi, hit.n));
-        Ray sr = Ray::make(hit.p + off, ls.wi, 0.0f, ls.dist * (1.0f - 1e-4f));
+        float t_end = ls.dist;  // point light: path_tracer.rs:88-92
+        // Quad (extension): distance from the offset origin to the emitter
+        // plane, so the ray stops a relative 1e-4 short of the emitter's own
+        // triangles.  With t_end = dist the origin offset and the shortening
+        // cancel wherever the light is ~1 unit away, and occlusion by the
+        // emitter is then decided by the last bit.
+        if (!L.is_delta()) t_end = ls.dist + dot(off, L.n) / ls.cos_l;
+        Ray sr = Ray::make(hit.p + off, ls.wi, 0.0f, t_end * (1.0f - 1e-4f));
         if (!sc.occluded(sr, c)) {
           V3 wi_l = hit.onb.to_local(ls.wi);
           float cos_wi = std::fabs(wi_l.z);

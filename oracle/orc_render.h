@@ -46,6 +46,7 @@ struct LightSample {
   float dist;
   V3 li;
   float pdf;
+  float cos_l = 0.0f;  // quad: cosine at the light (0 for a point light)
 };
 
 struct EmissionSample {

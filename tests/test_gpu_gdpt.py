@@ -1,8 +1,10 @@
 """GPU gradient-domain path tracer (GradientDomainPathTracer::render,
 gradient_domain_pt.rs:346-415) against the oracle restatement.
 
-* shared counter RNG: primal/dx/dy films agree with the oracle's to rounding
-  (rare branch flips allowed, fraction bounded);
+* shared counter RNG: primal/dx/dy films agree with the oracle's to rounding:
+  at most 0.5% of pixel-channels off by more than 1e-4 relative (a branch
+  flipped by a last-bit sin/cos difference) and total |diff| below 1e-4 of
+  total |film|;
 * schedules give bitwise-identical films;
 * the base paths are the path tracer's paths: primal == PT film up to the
   reference's different normalisation (accum * (1/spp) vs accum / spp);
@@ -37,9 +39,12 @@ def test_films_match_oracle(ar, orc, setup):
     img, aux, st = gdpt(ar, sc, cam, w, h, spp, seed=11)
     op, ox, oy, _ = orc.render_gdpt(osc, cam16, w, h, spp, 8, 3, rng_mode=1, seed=11)
     for g, o, name in ((aux[0], op, "primal"), (aux[1], ox, "dx"), (aux[2], oy, "dy")):
-        diff = np.abs(g - o) > 1e-4 * np.maximum(1.0, np.abs(o))
-        print(f"  {name}: frac off {diff.mean():.4%}")
-        assert diff.mean() < 0.03
+        ad = np.abs(g - o)
+        diff = ad > 1e-4 * np.maximum(1.0, np.abs(o))
+        l1 = float(ad.sum() / max(np.abs(o).sum(), 1e-30))
+        print(f"  {name}: frac off {diff.mean():.4%}, l1 rel {l1:.3g}")
+        assert diff.mean() <= 0.005
+        assert l1 < 1e-4
     assert st.paths == 3 * w * h * spp
 
 

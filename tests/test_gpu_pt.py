@@ -4,7 +4,8 @@
   not depend on the sample chunking (deterministic accumulation);
 * shared counter RNG: GPU and oracle trace the same random numbers, so the
   films agree to rounding except on the rare path whose branch flips on a
-  last-bit difference of sin/cos (tolerance stated below);
+  last-bit difference of sin/cos: at most 0.5% of pixel-channels off by more
+  than 1e-4 relative, total |diff| below 1e-4 of the total film;
 * independent RNG: per-pixel means agree within 3 sigma except for a
   binomial fraction of pixels; relMSE against a high-spp oracle render is
   within 5% of the oracle's own at equal spp;
@@ -53,9 +54,11 @@ def test_counter_rng_matches_oracle(ar, cornell):
     diff = np.abs(g - o)
     tol = 1e-4 * np.maximum(1.0, np.abs(o))
     frac = float((diff > tol).mean())
-    print(f"  pixel-channels off by > 1e-4 rel: {frac:.4%}, max diff {diff.max():.3g}")
-    assert frac < 0.02
-    assert abs(g.mean() - o.mean()) < 1e-3 * o.mean()
+    l1 = float(diff.sum() / o.sum())
+    print(f"  pixel-channels off by > 1e-4 rel: {frac:.4%}, max diff {diff.max():.3g}, l1 rel {l1:.3g}")
+    assert frac <= 0.005
+    assert l1 < 1e-4
+    assert abs(g.mean() - o.mean()) < 1e-4 * o.mean()
 
 
 def test_independent_rng_three_sigma(ar, cornell):
