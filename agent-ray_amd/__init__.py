@@ -90,7 +90,11 @@ class ArJobDesc(C.Structure):
 
 class ArTraceConfig(C.Structure):
     _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("ms", C.c_float)]
+                ("refill_min", C.c_uint32), ("ms", C.c_float)]
+
+    def as_dict(self) -> dict:
+        return {"workgroup_size": self.workgroup_size, "lds_stack": self.lds_stack,
+                "blocks_per_cu": self.blocks_per_cu, "refill_min": self.refill_min, "ms": self.ms}
 
 
 class ArJobStats(C.Structure):
@@ -339,11 +343,10 @@ class Scene:
     def trace_config(self) -> dict:
         c = ArTraceConfig()
         _check(_lib.ar_scene_get_trace_config(self._h, C.byref(c)))
-        return {"workgroup_size": c.workgroup_size, "lds_stack": c.lds_stack, "blocks_per_cu": c.blocks_per_cu,
-                "ms": c.ms}
+        return c.as_dict()
 
-    def set_trace_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int):
-        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, 0.0)
+    def set_trace_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16):
+        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, 0.0)
         _check(_lib.ar_scene_set_trace_config(self._h, C.byref(c)))
 
     def autotune(self, rays, stream=None) -> dict:
@@ -352,8 +355,7 @@ class Scene:
         c = ArTraceConfig()
         sh = C.c_void_p(stream.cuda_stream) if stream is not None else None
         _check(_lib.ar_scene_autotune(self._h, C.c_void_p(rays.data_ptr()), rays.shape[0], 0, sh, C.byref(c)))
-        return {"workgroup_size": c.workgroup_size, "lds_stack": c.lds_stack, "blocks_per_cu": c.blocks_per_cu,
-                "ms": c.ms}
+        return c.as_dict()
 
     def occluded(self, rays, ref_order: bool = False):
         """Scene::occluded (scene.rs:166-168) as an any-hit query."""

@@ -12,6 +12,12 @@
 
 #define AR_HD __host__ __device__ __forceinline__
 
+// Bit casts usable in host and device code.
+AR_HD float bits_f(int v) { return __builtin_bit_cast(float, v); }
+AR_HD float bits_f(uint32_t v) { return __builtin_bit_cast(float, v); }
+AR_HD int bits_i(float v) { return __builtin_bit_cast(int, v); }
+AR_HD uint32_t bits_u(float v) { return __builtin_bit_cast(uint32_t, v); }
+
 namespace ar {
 
 constexpr float kPi = 3.14159265358979323846f;

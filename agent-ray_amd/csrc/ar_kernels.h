@@ -7,10 +7,16 @@ namespace ar {
 constexpr int kTraceBlock = 128;       // 2 waves per workgroup
 constexpr int kTraceLdsStack = 16;     // LDS stack entries per lane (8 B each)
 constexpr int kTraceBlocksPerCU = 8;   // persistent grid: 16 waves per CU
+constexpr uint32_t kTraceRefillMin = 16;  // idle lanes a wave waits for before taking new rays
 
 uint32_t trace_grid(uint64_t n, int block);
 uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c);
 int device_cus();
+// Persistent grids hand every wave a static share of the work, so blocks
+// that cannot be resident at once would only run after the others: clamps
+// G to (resident blocks per CU of `kernel` at `block` threads) x CUs,
+// rounded down to a multiple of 8.
+uint32_t clamp_resident(uint32_t G, const void* kernel, int block);
 
 // Traversal variants compiled: (workgroup size, LDS stack entries per lane).
 #define AR_TRACE_VARIANTS(X) X(64, 8) X(64, 16) X(128, 8) X(128, 16) X(256, 8)

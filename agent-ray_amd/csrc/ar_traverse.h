@@ -31,15 +31,15 @@ struct Stack {
   uint32_t spill_stride;
   int S;
   int sp;
-  __device__ __forceinline__ void push(int ref, float tnear) {
-    int2 e = make_int2(ref, __float_as_int(tnear));
+  AR_HD void push(int ref, float tnear) {
+    int2 e = make_int2(ref, bits_i(tnear));
     if (sp < S)
       lds[sp * stride] = e;
     else
       spill[(size_t)(sp - S) * spill_stride] = e;
     ++sp;
   }
-  __device__ __forceinline__ int2 pop() {
+  AR_HD int2 pop() {
     --sp;
     return sp < S ? lds[sp * stride] : spill[(size_t)(sp - S) * spill_stride];
   }
@@ -55,7 +55,7 @@ struct LocalHit {
 // with t < tmax (strict), shrinking tmax.  Any mode: returns true at the
 // first hit whose world t (t / scale) is < world_tmax.
 template <bool ANY, bool NEAR>
-__device__ __forceinline__ bool blas_traverse(const WNode* __restrict__ nodes, const Tri* __restrict__ tris,
+AR_HD bool blas_traverse(const WNode* __restrict__ nodes, const Tri* __restrict__ tris,
                                               int root, const float* rb, f3 o, f3 d, float tmin, float tmax,
                                               LocalHit& best, Stack& stk, float scale, float world_tmax) {
   f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
@@ -91,7 +91,7 @@ __device__ __forceinline__ bool blas_traverse(const WNode* __restrict__ nodes, c
       for (;;) {
         const Tri* tp = tris + k;
         float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
-        uint32_t tag = __float_as_uint(v0.w);
+        uint32_t tag = bits_u(v0.w);
         float th, uh, vh;
         if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tmax, &th,
                     &uh, &vh)) {
@@ -114,7 +114,7 @@ __device__ __forceinline__ bool blas_traverse(const WNode* __restrict__ nodes, c
     for (;;) {
       if (stk.sp == base) return found;
       int2 e = stk.pop();
-      if (__int_as_float(e.y) <= tmax) {
+      if (bits_f(e.y) <= tmax) {
         cur = e.x;
         break;
       }
@@ -131,7 +131,7 @@ struct WorldHit {
 
 // Instance leaf visit: transform, BLAS traverse, world acceptance.
 template <bool ANY, bool NEAR>
-__device__ __forceinline__ bool visit_instance(const SceneView& sv, uint32_t k, const Ray& r, WorldHit& best,
+AR_HD bool visit_instance(const SceneView& sv, uint32_t k, const Ray& r, WorldHit& best,
                                                Stack& stk) {
   const Inst* in = sv.insts + k;
   f3 lo, ld;
@@ -169,7 +169,7 @@ __device__ __forceinline__ bool visit_instance(const SceneView& sv, uint32_t k, 
 // Full query.  Closest: fills best (best.pos == kNoHit on miss).  Any:
 // returns whether anything is hit in [tmin, tmax).
 template <bool ANY, bool NEAR>
-__device__ __forceinline__ bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack& stk) {
+AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack& stk) {
   best.t = r.tmax;
   best.pos = kNoHit;
   best.inst = kNoHit;
@@ -221,10 +221,243 @@ __device__ __forceinline__ bool trace_scene(const SceneView& sv, const Ray& r, W
     for (;;) {
       if (stk.sp == base) return found;
       int2 e = stk.pop();
-      if (__int_as_float(e.y) <= best.t) {
+      if (bits_f(e.y) <= best.t) {
         cur = e.x;
         break;
       }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Resumable form of trace_scene.  The whole state of one query lives in the
+// Walker (registers) and one call of step() does one unit of work: a node
+// visit, a leaf's triangles, or an instance entry, followed by the pops it
+// needs.  A lane can therefore finish its ray and start another while the
+// other lanes of its wave are still walking (walk_items below), instead of
+// idling until the wave's slowest ray ends.
+//
+// Arithmetic and decisions are those of trace_scene / visit_instance /
+// blas_traverse, operation for operation, so results are bit-identical:
+//   TLAS level: ray (wo, 1/wd, wtmin, bt), pops accepted when tnear <= bt;
+//   BLAS level: local ray (M^-1 o, normalize(M^-1 d)), range (wtmin*s, bt*s)
+//   at entry, local acceptance by strict < (NEAR ties: lower leaf position),
+//   world acceptance of the local best as t/s < bt (NEAR ties: lower
+//   instance record); any-hit ends at the first hit with t/s < bt.
+// ---------------------------------------------------------------------------
+template <bool ANY, bool NEAR>
+struct Walker {
+  f3 wo, wd;              // world ray
+  float wtmin;
+  float bt;               // world best t (closest) or world tmax (any)
+  uint32_t bpos, binst;   // world best: Tri record, instance record
+  float bu, bv;
+  f3 o, d, inv;           // ray of the current level
+  float tmin, tmax;       // range of the current level (BLAS: tmax = local best once found)
+  float s;                // BLAS: |M^-1 d|
+  uint32_t lpos;          // BLAS-local best Tri record
+  float lu, lv;
+  uint32_t ik;            // instance record being walked
+  int base;               // stack depth at instance entry
+  int cur;                // node ref to process next
+  bool in_blas, lfound;
+
+  // Starts a query.  Returns true when it is already finished (the TLAS
+  // root box is missed).
+  AR_HD bool start(const SceneView& sv, const Ray& r) {
+    wo = r.o;
+    wd = r.d;
+    wtmin = r.tmin;
+    bt = r.tmax;
+    bpos = kNoHit;
+    binst = kNoHit;
+    bu = bv = 0.0f;
+    o = wo;
+    d = wd;
+    inv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+    tmin = wtmin;
+    tmax = bt;
+    in_blas = false;
+    cur = sv.tlas_root;
+    float tn;
+    const float* tb = sv.tlas_box;
+    return !slab(tb[0], tb[1], tb[2], tb[3], tb[4], tb[5], o, inv, tmin, tmax, &tn);
+  }
+
+  // Enters the first instance from record k on (same TLAS leaf) whose BLAS
+  // root box the local ray hits.  False when the leaf has none left.
+  AR_HD bool enter(const SceneView& sv, uint32_t k, const Stack& stk) {
+    for (;;) {
+      const Inst* in = sv.insts + k;
+      f3 lo, ld;
+      if (in->identity) {
+        lo = wo;
+        ld = wd;
+      } else {
+        lo = xform_point(in->inv, wo);
+        ld = xform_vector(in->inv, wd);
+      }
+      float sc = length(ld);
+      if (sc >= kF32Eps) {
+        f3 ldn = normalize(ld);
+        f3 li = mk(1.0f / ldn.x, 1.0f / ldn.y, 1.0f / ldn.z);
+        float a = wtmin * sc, b = bt * sc;
+        float tn;
+        const float* rb = in->box;
+        if (slab(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], lo, li, a, b, &tn)) {
+          o = lo;
+          d = ldn;
+          inv = li;
+          tmin = a;
+          tmax = b;
+          s = sc;
+          ik = k;
+          base = stk.sp;
+          cur = in->root;
+          in_blas = true;
+          lfound = false;
+          lpos = kNoHit;
+          lu = lv = 0.0f;
+          return true;
+        }
+      }
+      if (in->id & kLastBit) return false;
+      ++k;
+    }
+  }
+
+  // Pops until a deferred node is still in range.  Finishes instances on the
+  // way; returns true when the query is done.
+  AR_HD bool pop(const SceneView& sv, Stack& stk) {
+    for (;;) {
+      if (in_blas) {
+        if (stk.sp == base) {
+          if (!ANY && lfound) {
+            float tw = tmax / s;
+            if (tw < bt || (NEAR && bpos != kNoHit && tw == bt && ik < binst)) {
+              bt = tw;
+              bpos = lpos;
+              binst = ik;
+              bu = lu;
+              bv = lv;
+            }
+          }
+          in_blas = false;
+          o = wo;
+          d = wd;
+          inv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+          tmin = wtmin;
+          tmax = bt;
+          if (!(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
+          continue;
+        }
+      } else if (stk.sp == 0) {
+        return true;
+      }
+      int2 e = stk.pop();
+      if (bits_f(e.y) <= tmax) {
+        cur = e.x;
+        return false;
+      }
+    }
+  }
+
+  // One unit of work.  Returns true when the query is done.
+  AR_HD bool step(const SceneView& sv, Stack& stk) {
+    if (cur >= 0) {
+      const WNode* n = sv.nodes + cur;
+      float4 a = n->a, b = n->b, c = n->c;
+      int4 ch = n->d;
+      float tl, tr;
+      bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
+      bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
+      if (hl && hr) {
+        bool swap = NEAR && (tr < tl);
+        stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
+        cur = swap ? ch.y : ch.x;
+        return false;
+      }
+      if (hl) {
+        cur = ch.x;
+        return false;
+      }
+      if (hr) {
+        cur = ch.y;
+        return false;
+      }
+    } else if (in_blas) {
+      uint32_t k = (uint32_t)~cur;
+      for (;;) {
+        const Tri* tp = sv.tris + k;
+        float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
+        uint32_t tag = bits_u(v0.w);
+        float th, uh, vh;
+        if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tmax, &th,
+                    &uh, &vh)) {
+          if (ANY) {
+            if (th / s < bt) {
+              bpos = k;
+              binst = ik;
+              stk.sp = 0;
+              return true;
+            }
+          } else if (th < tmax || (NEAR && lfound && th == tmax && k < lpos)) {
+            tmax = th;
+            lpos = k;
+            lu = uh;
+            lv = vh;
+            lfound = true;
+          }
+        }
+        if (tag & kLastBit) break;
+        ++k;
+      }
+    } else if (enter(sv, (uint32_t)~cur, stk)) {
+      return false;
+    }
+    return pop(sv, stk);
+  }
+};
+
+// Items [0, n) are split in chunks of 64; wave `wave` of `waves` owns chunks
+// wave, wave + waves, ... and walks their items in order.  A lane that
+// finishes takes the next unassigned item of its wave; idle lanes are
+// refilled together once at least `refill_min` of them wait, so items
+// fetched together are neighbours (coherent rays stay together).  Every
+// item is started exactly once, and the loop ends when the wave's items are
+// exhausted and every lane is idle.
+//   src(i)      -> Ray for item i
+//   sink(i, w)  consumes the finished walker of item i
+template <bool ANY, bool NEAR, class Src, class Sink>
+__device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint64_t n, uint64_t wave,
+                                           uint64_t waves, uint32_t refill_min, Src src, Sink sink) {
+  const uint32_t lane = __lane_id();
+  const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+  uint32_t next = 0;  // wave-uniform: position of the next unassigned item
+  uint32_t item = 0;
+  bool alive = false;
+  Walker<ANY, NEAR> w;
+  for (;;) {
+    uint64_t idle = __ballot(!alive);
+    uint32_t nidle = (uint32_t)__popcll(idle);
+    uint32_t first = (uint32_t)((wave + (next >> 6) * waves) * 64 + (next & 63));
+    bool more = first < n;
+    if (!more && nidle == 64) break;
+    if (more && nidle >= refill_min) {
+      if (!alive) {
+        uint32_t p = next + (uint32_t)__popcll(idle & below);
+        uint32_t it = (uint32_t)((wave + (p >> 6) * waves) * 64 + (p & 63));
+        if (it < n) {
+          item = it;
+          alive = !w.start(sv, src(it));
+          if (!alive) sink(it, w);
+        }
+      }
+      next += nidle;
+    }
+    if (alive && w.step(sv, stk)) {
+      sink(item, w);
+      alive = false;
     }
   }
 }

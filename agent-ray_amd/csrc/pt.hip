@@ -149,27 +149,37 @@ __global__ void k_wf_begin(WaveBufs B) {
   if (threadIdx.x < 3) B.cnt[threadIdx.x] = 0;
 }
 
+__device__ __forceinline__ Ray make_ray4(float4 o, float4 d) {
+  Ray r;
+  r.o = mk(o.x, o.y, o.z);
+  r.tmin = o.w;
+  r.d = mk(d.x, d.y, d.z);
+  r.tmax = d.w;
+  return r;
+}
+
+// Extension rays of the live paths (queue qi): closest hit, near-first order.
+// Lanes take the queue's rays through walk_items, so a wave keeps all its
+// lanes busy until its share of the queue is done.
 template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill) {
+__global__ __launch_bounds__(BLOCK) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill,
+                                                     uint32_t refill_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
   Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
-  for (uint32_t i = gtid; i < n; i += nthreads) {
-    uint32_t p = q[i];
-    float4 o = B.ray_o[p], d = B.ray_d[p];
-    Ray r;
-    r.o = mk(o.x, o.y, o.z);
-    r.tmin = o.w;
-    r.d = mk(d.x, d.y, d.z);
-    r.tmax = d.w;
-    WorldHit h;
-    trace_scene<false, true>(sv, r, h, stk);
-    B.hit[i] = make_float4(h.t, __uint_as_float(h.pos), h.u, h.v);
-    B.hit_inst[i] = h.inst;
-  }
+  walk_items<false, true>(
+      sv, stk, n, gtid / 64, nthreads / 64, refill_min,
+      [&](uint64_t i) {
+        uint32_t p = q[i];
+        return make_ray4(B.ray_o[p], B.ray_d[p]);
+      },
+      [&](uint64_t i, const Walker<false, true>& w) {
+        B.hit[i] = make_float4(w.bt, __uint_as_float(w.bpos), w.bu, w.bv);
+        B.hit_inst[i] = w.binst;
+      });
   if (gtid == 0) atomicAdd(&B.stats[0], (unsigned long long)n);
 }
 
@@ -218,27 +228,26 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, PtParams P, Wav
   }
 }
 
+// Shadow rays of this bounce: any hit; an unoccluded ray adds its
+// contribution to its path's radiance (one shadow ray per path per bounce).
 template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill) {
+__global__ __launch_bounds__(BLOCK) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill,
+                                                     uint32_t refill_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
   Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = B.cnt[2];
-  for (uint32_t i = gtid; i < n; i += nthreads) {
-    float4 o = B.sh_o[i], d = B.sh_d[i];
-    Ray r;
-    r.o = mk(o.x, o.y, o.z);
-    r.tmin = o.w;
-    r.d = mk(d.x, d.y, d.z);
-    r.tmax = d.w;
-    WorldHit h;
-    if (!trace_scene<true, true>(sv, r, h, stk)) {
-      uint32_t p = B.sh_path[i];
-      float4 c = B.sh_c[i], l = B.L[p];
-      B.L[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
-    }
-  }
+  walk_items<true, true>(
+      sv, stk, n, gtid / 64, nthreads / 64, refill_min,
+      [&](uint64_t i) { return make_ray4(B.sh_o[i], B.sh_d[i]); },
+      [&](uint64_t i, const Walker<true, true>& w) {
+        if (w.bpos == kNoHit) {
+          uint32_t p = B.sh_path[i];
+          float4 c = B.sh_c[i], l = B.L[p];
+          B.L[p] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+        }
+      });
   if (gtid == 0) atomicAdd(&B.stats[1], (unsigned long long)n);
 }
 
@@ -392,13 +401,15 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
     int qi = d & 1;
 #define AR_EXT(BB, SS)                                                                                         \
   if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                           \
-    hipLaunchKernelGGL((k_wf_extend<BB, SS>), dim3(Gt), dim3(BB), 0, st, v.sv, B, qi, ws.spill);
+    hipLaunchKernelGGL((k_wf_extend<BB, SS>), dim3(clamp_resident(Gt, (const void*)&k_wf_extend<BB, SS>, BB)),  \
+                       dim3(BB), 0, st, v.sv, B, qi, ws.spill, tc.refill_min);
     AR_TRACE_VARIANTS(AR_EXT)
 #undef AR_EXT
     hipLaunchKernelGGL((k_wf_shade<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, B, qi);
 #define AR_SHD(BB, SS)                                                                                         \
   if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                           \
-    hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(Gt), dim3(BB), 0, st, v.sv, B, ws.spill);
+    hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(clamp_resident(Gt, (const void*)&k_wf_shadow<BB, SS>, BB)),  \
+                       dim3(BB), 0, st, v.sv, B, ws.spill, tc.refill_min);
     AR_TRACE_VARIANTS(AR_SHD)
 #undef AR_SHD
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, B, qi);

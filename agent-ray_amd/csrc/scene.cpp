@@ -274,17 +274,22 @@ ar_status ar_scene_set_light_distribution(ar_scene* s, uint32_t mode) {
   return AR_OK;
 }
 
-ar_status ar_scene_commit(ar_scene* s) {
-  if (!s) return fail(AR_ERR_INVALID_ARG, "NULL scene");
-  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
-  if (s->meshes.empty()) return fail(AR_ERR_INVALID_ARG, "scene has no objects");
-  std::vector<WNode> nodes;
-  std::vector<Tri> tris;
+}  // extern "C"
+
+namespace ar {
+
+// Host half of ar_scene_commit: BLAS per object, Tri records in leaf order,
+// instance records in TLAS leaf order, flattened node array, per-object
+// shading records.  Touches no device.
+ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
+  std::vector<WNode>& nodes = h.nodes;
+  std::vector<Tri>& tris = h.tris;
   std::vector<Inst> inst_by_obj(s->meshes.size());
   std::vector<HBox> world(s->meshes.size());
-  std::vector<DObject> objs(s->meshes.size());
-  std::vector<uint32_t> gidx, gslots, gmats, tri_obj;
-  std::vector<float> gnrm;
+  std::vector<DObject>& objs = h.objs;
+  objs.assign(s->meshes.size(), DObject{});
+  std::vector<uint32_t>&gidx = h.gidx, &gslots = h.gslots, &gmats = h.gmats, &tri_obj = h.tri_obj;
+  std::vector<float>& gnrm = h.gnrm;
   uint32_t max_depth = 0;
   for (size_t oi = 0; oi < s->meshes.size(); ++oi) {
     const HostMesh& m = s->meshes[oi];
@@ -361,8 +366,9 @@ ar_status ar_scene_commit(ar_scene* s) {
   // TLAS over instance world boxes (bvh.rs:395-410); instance records are
   // stored in TLAS leaf order.
   BinTree tl = build_sah(world, 0);
-  s->tlas_depth = tl.depth;
-  std::vector<Inst> insts;
+  h.tlas_depth = tl.depth;
+  h.max_depth = max_depth;
+  std::vector<Inst>& insts = h.insts;
   for (size_t i = 0; i < tl.nodes.size(); ++i) {
     const BinNode& n = tl.nodes[i];
     if (n.count == 0) continue;
@@ -372,9 +378,28 @@ ar_status ar_scene_commit(ar_scene* s) {
       insts.push_back(in);
     }
   }
-  uint32_t tlas_node_base = (uint32_t)nodes.size();
-  (void)tlas_node_base;
   Flat tf = flatten(tl, nodes, 0);
+  h.tlas_root = tf.root_ref;
+  for (int k = 0; k < 3; ++k) {
+    h.tlas_box[k] = tf.root_box.lo[k];
+    h.tlas_box[3 + k] = tf.root_box.hi[k];
+  }
+  return AR_OK;
+}
+
+}  // namespace ar
+
+extern "C" {
+
+ar_status ar_scene_commit(ar_scene* s) {
+  if (!s) return fail(AR_ERR_INVALID_ARG, "NULL scene");
+  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
+  if (s->meshes.empty()) return fail(AR_ERR_INVALID_ARG, "scene has no objects");
+  HostAccel h;
+  ar_status be = build_host_accel(s, h);
+  if (be != AR_OK) return be;
+  s->tlas_depth = h.tlas_depth;
+  const uint32_t max_depth = h.max_depth;
   // Materials / lights / distribution (lights.rs:115-235).
   std::vector<DLight> dl(s->lights.size());
   bool emit = false;
@@ -432,33 +457,30 @@ ar_status ar_scene_commit(ar_scene* s) {
       }
     }
   }
-  AR_HIP(upload(s->d_nodes, nodes));
-  AR_HIP(upload(s->d_tris, tris));
-  AR_HIP(upload(s->d_insts, insts));
-  AR_HIP(upload(s->d_objects, objs));
+  AR_HIP(upload(s->d_nodes, h.nodes));
+  AR_HIP(upload(s->d_tris, h.tris));
+  AR_HIP(upload(s->d_insts, h.insts));
+  AR_HIP(upload(s->d_objects, h.objs));
   AR_HIP(upload(s->d_materials, s->materials));
   AR_HIP(upload(s->d_lights, dl));
-  AR_HIP(upload(s->d_idx, gidx));
-  AR_HIP(upload(s->d_nrm, gnrm));
-  AR_HIP(upload(s->d_slots, gslots));
-  AR_HIP(upload(s->d_objmats, gmats));
-  AR_HIP(upload(s->d_tri_obj, tri_obj));
+  AR_HIP(upload(s->d_idx, h.gidx));
+  AR_HIP(upload(s->d_nrm, h.gnrm));
+  AR_HIP(upload(s->d_slots, h.gslots));
+  AR_HIP(upload(s->d_objmats, h.gmats));
+  AR_HIP(upload(s->d_tri_obj, h.tri_obj));
   s->view.nodes = s->d_nodes.as<WNode>();
   s->view.tris = s->d_tris.as<Tri>();
   s->view.insts = s->d_insts.as<Inst>();
-  s->view.tlas_root = tf.root_ref;
-  for (int k = 0; k < 3; ++k) {
-    s->view.tlas_box[k] = tf.root_box.lo[k];
-    s->view.tlas_box[3 + k] = tf.root_box.hi[k];
-  }
-  s->view.n_inst = (uint32_t)insts.size();
+  s->view.tlas_root = h.tlas_root;
+  for (int k = 0; k < 6; ++k) s->view.tlas_box[k] = h.tlas_box[k];
+  s->view.n_inst = (uint32_t)h.insts.size();
   s->n_lights = nl;
   s->has_emitters = emit;
   s->h_lights = dl;
-  s->h_objects = objs;
+  s->h_objects = h.objs;
   s->info.n_objects = (uint32_t)s->meshes.size();
-  s->info.n_triangles = (uint32_t)tris.size();
-  s->info.n_nodes = (uint32_t)nodes.size();
+  s->info.n_triangles = (uint32_t)h.tris.size();
+  s->info.n_nodes = (uint32_t)h.nodes.size();
   s->info.n_lights = nl;
   s->info.max_depth = max_depth;
   s->info.device_bytes = s->d_nodes.bytes + s->d_tris.bytes + s->d_insts.bytes + s->d_objects.bytes +

@@ -2,14 +2,17 @@
 // (TLAS::intersect, agent-ray src/accel/bvh.rs:413-447; Scene::occluded,
 // src/scene.rs:166-168).
 //
-// One ray per lane, persistent grid: each workgroup walks ray blocks with a
-// grid stride.  The grid is a multiple of 8 and block ids are remapped so
-// that each XCD (blocks are dispatched round-robin over the 8 XCDs) sweeps a
-// contiguous range of rays, keeping neighbouring (coherent) rays in the same
-// L2.
+// One ray per lane, persistent grid: each wave owns every W-th chunk of 64
+// rays (W = waves in the grid) and its lanes take the wave's rays in order as
+// they finish (walk_items), so a wave never waits for its slowest ray.  The
+// grid is a multiple of 8 and block ids are remapped so that each XCD
+// (blocks are dispatched round-robin over the 8 XCDs) sweeps a contiguous
+// range of rays, keeping neighbouring (coherent) rays in the same L2.
 #include "ar_internal.h"
 #include "ar_traverse.h"
 #include "ar_kernels.h"
+
+#include <map>
 
 namespace ar {
 
@@ -27,7 +30,8 @@ __device__ __forceinline__ Ray load_ray(const ar_ray* rays, uint64_t i) {
 template <bool ANY, bool NEAR, int S, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const ar_ray* __restrict__ rays, uint64_t n,
                                                  ar_hit* __restrict__ hits, uint32_t* __restrict__ inst_ids,
-                                                 uint32_t* __restrict__ occ, int2* __restrict__ spill) {
+                                                 uint32_t* __restrict__ occ, int2* __restrict__ spill,
+                                                 uint32_t refill_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t G = gridDim.x;
   const uint32_t b = blockIdx.x;
@@ -40,25 +44,24 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const ar_ray* __r
   stk.spill_stride = G * BLOCK;
   stk.S = S;
   stk.sp = 0;
-  for (uint64_t i = gtid; i < n; i += (uint64_t)G * BLOCK) {
-    Ray r = load_ray(rays, i);
-    WorldHit h;
-    bool any = trace_scene<ANY, NEAR>(sv, r, h, stk);
-    if (ANY) {
-      occ[i] = any ? 1u : 0u;
-    } else {
-      ar_hit o;
-      if (h.pos != kNoHit) {
-        o.t = h.t;
-        o.prim_id = __float_as_uint(sv.tris[h.pos].v0.w) & ~kLastBit;
-      } else {
-        o.t = INFINITY;
-        o.prim_id = kNoHit;
-      }
-      *reinterpret_cast<float2*>(hits + i) = make_float2(o.t, __uint_as_float(o.prim_id));
-      if (inst_ids) inst_ids[i] = h.pos != kNoHit ? (sv.insts[h.inst].id & ~kLastBit) : kNoHit;
-    }
-  }
+  const uint64_t waves = (uint64_t)G * (BLOCK / 64);
+  const uint64_t wave = (uint64_t)lb * (BLOCK / 64) + threadIdx.x / 64;
+  walk_items<ANY, NEAR>(
+      sv, stk, n, wave, waves, refill_min, [&](uint64_t i) { return load_ray(rays, i); },
+      [&](uint64_t i, const Walker<ANY, NEAR>& w) {
+        if (ANY) {
+          occ[i] = w.bpos != kNoHit ? 1u : 0u;
+        } else {
+          float t = INFINITY;
+          uint32_t prim = kNoHit;
+          if (w.bpos != kNoHit) {
+            t = w.bt;
+            prim = __float_as_uint(sv.tris[w.bpos].v0.w) & ~kLastBit;
+          }
+          *reinterpret_cast<float2*>(hits + i) = make_float2(t, __uint_as_float(prim));
+          if (inst_ids) inst_ids[i] = w.bpos != kNoHit ? (sv.insts[w.binst].id & ~kLastBit) : kNoHit;
+        }
+      });
 }
 
 // Stack sizing: a push happens only at an internal node with both children
@@ -67,9 +70,10 @@ static uint32_t stack_need(const ar_scene* s) { return s->info.max_depth + s->tl
 
 template <bool ANY, bool NEAR, int BLOCK, int S>
 static void launch_variant(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits,
-                           uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st) {
+                           uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st, uint32_t refill_min) {
+  G = clamp_resident(G, reinterpret_cast<const void*>(&k_trace<ANY, NEAR, S, BLOCK>), BLOCK);
   hipLaunchKernelGGL((k_trace<ANY, NEAR, S, BLOCK>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits, inst, occ,
-                     spill);
+                     spill, refill_min);
 }
 
 template <bool ANY, bool NEAR>
@@ -85,7 +89,7 @@ static ar_status launch_trace(const ar_scene* s, const ar_ray* rays, uint64_t n,
   bool ok = false;
 #define AR_LAUNCH(B, SS)                                                         \
   if (c.workgroup_size == B && c.lds_stack == SS) {                              \
-    launch_variant<ANY, NEAR, B, SS>(s, G, rays, n, hits, inst, occ, spill, st); \
+    launch_variant<ANY, NEAR, B, SS>(s, G, rays, n, hits, inst, occ, spill, st, c.refill_min); \
     ok = true;                                                                   \
   }
   AR_TRACE_VARIANTS(AR_LAUNCH)
@@ -103,6 +107,25 @@ bool trace_variant_supported(uint32_t block, uint32_t stack) {
   AR_TRACE_VARIANTS(AR_CHECK)
 #undef AR_CHECK
   return ok;
+}
+
+uint32_t clamp_resident(uint32_t G, const void* kernel, int block) {
+  static std::mutex mu;
+  static std::map<std::pair<const void*, int>, uint32_t> cache;
+  uint32_t nb;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find({kernel, block});
+    if (it == cache.end()) {
+      int v = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&v, kernel, block, 0) != hipSuccess || v <= 0) v = 1;
+      it = cache.emplace(std::make_pair(kernel, block), (uint32_t)v).first;
+    }
+    nb = it->second;
+  }
+  uint32_t cap = (uint32_t)device_cus() * nb / 8 * 8;
+  if (cap == 0) cap = 8;
+  return G < cap ? G : cap;
 }
 
 int device_cus() {
@@ -127,7 +150,7 @@ uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c) {
 }
 
 uint32_t trace_grid(uint64_t n, int block) {
-  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, 0.0f};
+  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, kTraceRefillMin, 0.0f};
   return trace_grid_cfg(n, c);
 }
 
@@ -243,6 +266,7 @@ ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c) {
   if (!trace_variant_supported(c->workgroup_size, c->lds_stack))
     return fail(AR_ERR_UNSUPPORTED_TYPE, "no traversal variant for this workgroup size / stack depth");
   if (c->blocks_per_cu == 0 || c->blocks_per_cu > 64) return fail(AR_ERR_INVALID_ARG, "blocks_per_cu out of range");
+  if (c->refill_min == 0 || c->refill_min > 64) return fail(AR_ERR_INVALID_ARG, "refill_min out of range (1..64)");
   std::lock_guard<std::mutex> lk(s->mu);
   s->tcfg = *c;
   s->tcfg.ms = 0.0f;
@@ -271,6 +295,7 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
   ar_trace_config winner = s->tcfg;
   float best_ms = INFINITY;
   const uint32_t per_cu[3] = {4, 8, 16};
+  const uint32_t refills[3] = {8, 24, 48};
   const uint32_t variants[][2] = {
 #define AR_LIST(B, SS) {B, SS},
       AR_TRACE_VARIANTS(AR_LIST)
@@ -279,20 +304,23 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
   ar_status err = AR_OK;
   for (auto& vv : variants) {
     for (uint32_t pc : per_cu) {
-      ar_trace_config c{vv[0], vv[1], pc, 0.0f};
-      err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
-      if (err != AR_OK) break;
-      (void)hipEventRecord(e0, st);
-      err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
-      if (err != AR_OK) break;
-      (void)hipEventRecord(e1, st);
-      (void)hipEventSynchronize(e1);
-      float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best_ms) {
-        best_ms = ms;
-        winner = c;
+      for (uint32_t rf : refills) {
+        ar_trace_config c{vv[0], vv[1], pc, rf, 0.0f};
+        err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
+        if (err != AR_OK) break;
+        (void)hipEventRecord(e0, st);
+        err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
+        if (err != AR_OK) break;
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best_ms) {
+          best_ms = ms;
+          winner = c;
+        }
       }
+      if (err != AR_OK) break;
     }
     if (err != AR_OK) break;
   }

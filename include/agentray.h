@@ -191,6 +191,7 @@ typedef struct ar_trace_config {
   uint32_t workgroup_size;
   uint32_t lds_stack;
   uint32_t blocks_per_cu;
+  uint32_t refill_min; /* 1..64: idle lanes a wave gathers before it takes new rays */
   float ms; /* measured time of the winning candidate (autotune only) */
 } ar_trace_config;
 ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c);

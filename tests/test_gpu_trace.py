@@ -118,15 +118,18 @@ def test_every_launch_variant_same_answer(ar, wl):
     ref_t, ref_p, _ = sc.intersect(rays)
     ref_o = sc.occluded(rays)
     for (b, s) in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8)]:
-        for pc in (4, 16):
-            sc.set_trace_config(b, s, pc)
+        for pc, rf in ((4, 1), (16, 64), (8, 24)):
+            sc.set_trace_config(b, s, pc, rf)
             t, p, _ = sc.intersect(rays)
-            assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p), (b, s, pc)
+            assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p), (b, s, pc, rf)
             assert np.array_equal(sc.occluded(rays), ref_o)
     cfg = sc.autotune(torch.from_numpy(rays).cuda())
     assert cfg["ms"] > 0 and cfg["workgroup_size"] in (64, 128, 256)
+    assert cfg["refill_min"] in (8, 24, 48)
     with pytest.raises(ar.ArError):
         sc.set_trace_config(96, 8, 4)
+    with pytest.raises(ar.ArError):
+        sc.set_trace_config(64, 8, 4, 65)
 
 
 @pytest.mark.slow

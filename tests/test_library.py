@@ -29,7 +29,7 @@ def test_header_structs_match_ctypes(ar, tmp_path):
     """Struct layouts of include/agentray.h (compiled by the C compiler) equal
     the ctypes mirrors used by the Python host layer."""
     names = ["ar_material", "ar_light", "ar_camera", "ar_render_config", "ar_render_stats", "ar_scene_info",
-             "ar_job_desc", "ar_job_stats", "ar_server_config"]
+             "ar_job_desc", "ar_job_stats", "ar_server_config", "ar_trace_config"]
     src = tmp_path / "sz.c"
     src.write_text('#include <stdio.h>\n#include "agentray.h"\nint main(){' +
                    "".join(f'printf("%zu\\n", sizeof({n}));' for n in names) + "return 0;}\n")
@@ -38,7 +38,7 @@ def test_header_structs_match_ctypes(ar, tmp_path):
     subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
     sizes = [int(x) for x in subprocess.check_output([str(exe)]).split()]
     mirrors = [ar.ArMaterial, ar.ArLight, ar.ArCamera, ar.ArRenderConfig, ar.ArRenderStats, ar.ArSceneInfo,
-               ar.ArJobDesc, ar.ArJobStats, ar.ArServerConfig]
+               ar.ArJobDesc, ar.ArJobStats, ar.ArServerConfig, ar.ArTraceConfig]
     assert sizes == [C.sizeof(m) for m in mirrors]
 
 
