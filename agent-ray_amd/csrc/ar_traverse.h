@@ -231,11 +231,14 @@ AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack&
 
 // ---------------------------------------------------------------------------
 // Resumable form of trace_scene.  The whole state of one query lives in the
-// Walker (registers) and one call of step() does one unit of work: a node
-// visit, a leaf's triangles, or an instance entry, followed by the pops it
-// needs.  A lane can therefore finish its ray and start another while the
-// other lanes of its wave are still walking (walk_items below), instead of
-// idling until the wave's slowest ray ends.
+// Walker (registers), so a lane can finish its ray and start another while
+// the other lanes of its wave are still walking (walk_items below) instead
+// of idling until the wave's slowest ray ends.
+//
+// Work is split in two phases so that a wave's lanes run the same code at
+// the same time: descend() visits internal nodes until the lane holds a
+// leaf (or its level's stack is exhausted); leaf() then does the rarer work
+// -- a leaf's triangles, entering or leaving an instance, ending the query.
 //
 // Arithmetic and decisions are those of trace_scene / visit_instance /
 // blas_traverse, operation for operation, so results are bit-identical:
@@ -245,6 +248,8 @@ AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack&
 //   world acceptance of the local best as t/s < bt (NEAR ties: lower
 //   instance record); any-hit ends at the first hit with t/s < bt.
 // ---------------------------------------------------------------------------
+constexpr int kLevelEnd = (int)0x80000000;  // cur: the current level's stack is exhausted
+
 template <bool ANY, bool NEAR>
 struct Walker {
   f3 wo, wd;              // world ray
@@ -255,12 +260,12 @@ struct Walker {
   f3 o, d, inv;           // ray of the current level
   float tmin, tmax;       // range of the current level (BLAS: tmax = local best once found)
   float s;                // BLAS: |M^-1 d|
-  uint32_t lpos;          // BLAS-local best Tri record
+  uint32_t lpos;          // BLAS-local best Tri record (kNoHit: none yet)
   float lu, lv;
   uint32_t ik;            // instance record being walked
   int base;               // stack depth at instance entry
-  int cur;                // node ref to process next
-  bool in_blas, lfound;
+  int cur;                // node ref, leaf ref, or kLevelEnd
+  bool in_blas;
 
   // Starts a query.  Returns true when it is already finished (the TLAS
   // root box is missed).
@@ -282,6 +287,46 @@ struct Walker {
     float tn;
     const float* tb = sv.tlas_box;
     return !slab(tb[0], tb[1], tb[2], tb[3], tb[4], tb[5], o, inv, tmin, tmax, &tn);
+  }
+
+  // Pops the deferred nodes of the current level until one is in range;
+  // kLevelEnd when the level has none left.
+  AR_HD void pop_level(Stack& stk) {
+    const int lo = in_blas ? base : 0;
+    for (;;) {
+      if (stk.sp == lo) {
+        cur = kLevelEnd;
+        return;
+      }
+      int2 e = stk.pop();
+      if (bits_f(e.y) <= tmax) {
+        cur = e.x;
+        return;
+      }
+    }
+  }
+
+  // Visits internal nodes until the lane holds a leaf or kLevelEnd.
+  AR_HD void descend(const SceneView& sv, Stack& stk) {
+    while (cur >= 0) {
+      const WNode* n = sv.nodes + cur;
+      float4 a = n->a, b = n->b, c = n->c;
+      int4 ch = n->d;
+      float tl, tr;
+      bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
+      bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
+      if (hl && hr) {
+        bool swap = NEAR && (tr < tl);
+        stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
+        cur = swap ? ch.y : ch.x;
+      } else if (hl) {
+        cur = ch.x;
+      } else if (hr) {
+        cur = ch.y;
+      } else {
+        pop_level(stk);
+      }
+    }
   }
 
   // Enters the first instance from record k on (same TLAS leaf) whose BLAS
@@ -315,7 +360,6 @@ struct Walker {
           base = stk.sp;
           cur = in->root;
           in_blas = true;
-          lfound = false;
           lpos = kNoHit;
           lu = lv = 0.0f;
           return true;
@@ -326,65 +370,29 @@ struct Walker {
     }
   }
 
-  // Pops until a deferred node is still in range.  Finishes instances on the
-  // way; returns true when the query is done.
-  AR_HD bool pop(const SceneView& sv, Stack& stk) {
-    for (;;) {
-      if (in_blas) {
-        if (stk.sp == base) {
-          if (!ANY && lfound) {
-            float tw = tmax / s;
-            if (tw < bt || (NEAR && bpos != kNoHit && tw == bt && ik < binst)) {
-              bt = tw;
-              bpos = lpos;
-              binst = ik;
-              bu = lu;
-              bv = lv;
-            }
-          }
-          in_blas = false;
-          o = wo;
-          d = wd;
-          inv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
-          tmin = wtmin;
-          tmax = bt;
-          if (!(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
-          continue;
+  // Leaf phase: the lane holds a leaf ref or kLevelEnd.  Returns true when
+  // the query is done; otherwise cur is a node, a leaf or kLevelEnd again.
+  AR_HD bool leaf(const SceneView& sv, Stack& stk) {
+    if (cur == kLevelEnd) {
+      if (!in_blas) return true;  // TLAS exhausted: query done
+      // instance finished: world acceptance of its local best
+      if (!ANY && lpos != kNoHit) {
+        float tw = tmax / s;
+        if (tw < bt || (NEAR && bpos != kNoHit && tw == bt && ik < binst)) {
+          bt = tw;
+          bpos = lpos;
+          binst = ik;
+          bu = lu;
+          bv = lv;
         }
-      } else if (stk.sp == 0) {
-        return true;
       }
-      int2 e = stk.pop();
-      if (bits_f(e.y) <= tmax) {
-        cur = e.x;
-        return false;
-      }
-    }
-  }
-
-  // One unit of work.  Returns true when the query is done.
-  AR_HD bool step(const SceneView& sv, Stack& stk) {
-    if (cur >= 0) {
-      const WNode* n = sv.nodes + cur;
-      float4 a = n->a, b = n->b, c = n->c;
-      int4 ch = n->d;
-      float tl, tr;
-      bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
-      bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
-      if (hl && hr) {
-        bool swap = NEAR && (tr < tl);
-        stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
-        cur = swap ? ch.y : ch.x;
-        return false;
-      }
-      if (hl) {
-        cur = ch.x;
-        return false;
-      }
-      if (hr) {
-        cur = ch.y;
-        return false;
-      }
+      in_blas = false;
+      o = wo;
+      d = wd;
+      inv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+      tmin = wtmin;
+      tmax = bt;
+      if (!(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
     } else if (in_blas) {
       uint32_t k = (uint32_t)~cur;
       for (;;) {
@@ -401,12 +409,11 @@ struct Walker {
               stk.sp = 0;
               return true;
             }
-          } else if (th < tmax || (NEAR && lfound && th == tmax && k < lpos)) {
+          } else if (th < tmax || (NEAR && lpos != kNoHit && th == tmax && k < lpos)) {
             tmax = th;
             lpos = k;
             lu = uh;
             lv = vh;
-            lfound = true;
           }
         }
         if (tag & kLastBit) break;
@@ -415,17 +422,18 @@ struct Walker {
     } else if (enter(sv, (uint32_t)~cur, stk)) {
       return false;
     }
-    return pop(sv, stk);
+    pop_level(stk);
+    return false;
   }
 };
 
 // Items [0, n) are split in chunks of 64; wave `wave` of `waves` owns chunks
 // wave, wave + waves, ... and walks their items in order.  A lane that
 // finishes takes the next unassigned item of its wave; idle lanes are
-// refilled together once at least `refill_min` of them wait, so items
-// fetched together are neighbours (coherent rays stay together).  Every
-// item is started exactly once, and the loop ends when the wave's items are
-// exhausted and every lane is idle.
+// refilled together once at least `refill_min` (1..64) of them wait, so
+// items fetched together are neighbours (coherent rays stay together).
+// Every item is started exactly once, and the loop ends when the wave's
+// items are exhausted and every lane is idle.  n < 2^32.
 //   src(i)      -> Ray for item i
 //   sink(i, w)  consumes the finished walker of item i
 template <bool ANY, bool NEAR, class Src, class Sink>
@@ -440,24 +448,27 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
   for (;;) {
     uint64_t idle = __ballot(!alive);
     uint32_t nidle = (uint32_t)__popcll(idle);
-    uint32_t first = (uint32_t)((wave + (next >> 6) * waves) * 64 + (next & 63));
+    uint64_t first = (wave + (next >> 6) * waves) * 64 + (next & 63);
     bool more = first < n;
     if (!more && nidle == 64) break;
     if (more && nidle >= refill_min) {
       if (!alive) {
         uint32_t p = next + (uint32_t)__popcll(idle & below);
-        uint32_t it = (uint32_t)((wave + (p >> 6) * waves) * 64 + (p & 63));
+        uint64_t it = (wave + (p >> 6) * waves) * 64 + (p & 63);
         if (it < n) {
-          item = it;
+          item = (uint32_t)it;
           alive = !w.start(sv, src(it));
           if (!alive) sink(it, w);
         }
       }
       next += nidle;
     }
-    if (alive && w.step(sv, stk)) {
-      sink(item, w);
-      alive = false;
+    if (alive) {
+      w.descend(sv, stk);
+      if (w.leaf(sv, stk)) {
+        sink(item, w);
+        alive = false;
+      }
     }
   }
 }

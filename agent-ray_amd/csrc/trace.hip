@@ -87,13 +87,23 @@ static ar_status launch_trace(const ar_scene* s, const ar_ray* rays, uint64_t n,
   ar_status e = scratch_acquire(ms, bytes, st, (void**)&spill);
   if (e != AR_OK) return e;
   bool ok = false;
-#define AR_LAUNCH(B, SS)                                                         \
-  if (c.workgroup_size == B && c.lds_stack == SS) {                              \
-    launch_variant<ANY, NEAR, B, SS>(s, G, rays, n, hits, inst, occ, spill, st, c.refill_min); \
-    ok = true;                                                                   \
+  // one launch handles < 2^32 rays (32-bit ray positions inside a wave)
+  const uint64_t kMaxLaunch = 1ull << 31;
+  for (uint64_t off = 0; off < n; off += kMaxLaunch) {
+    const uint64_t m = n - off < kMaxLaunch ? n - off : kMaxLaunch;
+    const ar_ray* r = rays + off;
+    ar_hit* h = hits ? hits + off : nullptr;
+    uint32_t* in = inst ? inst + off : nullptr;
+    uint32_t* oc = occ ? occ + off : nullptr;
+#define AR_LAUNCH(B, SS)                                                           \
+  if (c.workgroup_size == B && c.lds_stack == SS) {                                \
+    launch_variant<ANY, NEAR, B, SS>(s, G, r, m, h, in, oc, spill, st, c.refill_min); \
+    ok = true;                                                                     \
   }
-  AR_TRACE_VARIANTS(AR_LAUNCH)
+    AR_TRACE_VARIANTS(AR_LAUNCH)
 #undef AR_LAUNCH
+    if (!ok) break;
+  }
   hipError_t le = hipGetLastError();
   scratch_release(ms, st);
   if (!ok) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
