@@ -5,16 +5,22 @@ with 2^24 rays (half coherent 4096^2 pinhole primaries, half incoherent), all
 resident in HBM.  One step = one closest-hit pass + one any-hit pass over all
 2^24 rays; value = rays traced per second (both passes) in Mrays/s.
 
-Extra workloads (--workload pt-cornell | pt-room) time the path tracer and
-report Mpaths/s; they are reported under "extra" when --extra is given.
+Under "extra" (unless --no-extra): path-tracer throughput in Mpaths/s on the
+Cornell box (256^2, 16 spp) and the 280k-triangle room (1920x1080, 64 spp),
+megakernel and wavefront schedules.  --workload serving: Poisson job arrivals
+against the resident room scene (p50/p99 latency).
 
 Roofline: the dominant kernel is the closest-hit traversal.  Its algorithmic
 bytes per ray are B_ray = 32 (ray) + 8 (hit) + 32 * N_node + 36 * N_tri with
 N_node / N_tri the oracle's mean box tests and triangle tests per ray
 (reference left-first traversal) measured on the CPU-baseline sample of the
 same rays; achieved = B_ray * rays / average kernel duration (HIP events on
-the stream the kernel runs on).  CPU baseline: the oracle (C++ restatement of
-the reference CPU path) on the host cores, on a bounded sample of the rays.
+the stream the kernel runs on).  Node and triangle bytes re-read from cache
+count in B_ray, so achieved can exceed the HBM peak; `traffic` is the HBM
+traffic of one launch measured by two rocprofv3 --pmc child runs
+(FETCH_SIZE, WRITE_SIZE) calibrated on a 1 GiB copy.  CPU baseline: the
+oracle (C++ restatement of the reference CPU path) on the host cores, on a
+bounded sample of the rays.
 """
 import argparse
 import importlib.util
@@ -188,7 +194,18 @@ def bench_traversal(args, ar, torch):
         out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                            "kernel": "k_trace<closest, near-first>",
-                           "bytes_per_ray": round(b_ray, 1), "n_node": round(nn, 2), "n_tri": round(nt, 2)}
+                           "bytes_per_ray": round(b_ray, 1), "n_node": round(nn, 2), "n_tri": round(nt, 2),
+                           "kernel_ms": round(float(np.mean(t_closest)), 3)}
+        if not args.no_traffic:
+            c = sc.trace_config()
+            cfg = f"{c['workgroup_size']},{c['lds_stack']},{c['blocks_per_cu']},{c['refill_min']}"
+            traffic, detail = measure_traffic(cfg)
+            out["roofline"]["traffic"] = None if traffic is None else round(traffic)
+            out["roofline"]["traffic_detail"] = detail
+            if traffic is not None:
+                dram = traffic / (np.mean(t_closest) / 1e3) / 1e9
+                out["roofline"]["traffic_gbs"] = round(dram, 1)
+                out["roofline"]["traffic_frac"] = round(dram / HBM_PEAK_GBS, 4)
     return out
 
 
@@ -210,6 +227,81 @@ def bench_copy(args, torch):
     ms = e0.elapsed_time(e1) / args.steps
     return {"workload": "copy_1GiB_f32", "bytes_read": 4 * n, "bytes_written": 4 * n, "ms_per_copy": round(ms, 4),
             "gbs": round(8 * n / (ms / 1e3) / 1e9, 1)}
+
+
+def pmc_probe(args, ar, torch):
+    """Child of measure_traffic: one closest-hit launch over the full ray set
+    with the given launch configuration, then one 1 GiB device copy (the
+    calibration kernel), each followed by a synchronize."""
+    wl = ar.workloads
+    desc = wl.triangle_soup(1 << 20, seed=0)
+    rays = wl.soup_rays(1 << 24, seed=1)
+    sc = ar.Scene.from_desc(desc)
+    if args.trace_config:
+        sc.set_trace_config(*[int(v) for v in args.trace_config.split(",")])
+    dr = torch.from_numpy(rays).cuda()
+    hits = torch.empty((rays.shape[0], 2), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ar._check(ar.lib().ar_trace(sc.handle, C_void(dr.data_ptr()), rays.shape[0], C_void(hits.data_ptr()), None,
+                                ar.AR_ASYNC, None))
+    torch.cuda.synchronize()
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "ok", "copy_bytes": 4 * n}), flush=True)
+
+
+def measure_traffic(cfg: str, kernel_key: str = "k_trace<false, true"):
+    """HBM bytes of one closest-hit launch from two rocprofv3 --pmc child runs
+    of the probe (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch).  Each
+    counter is scaled by what it reads for the probe's 1 GiB copy, whose true
+    traffic is 2^30 bytes read and 2^30 written.  Returns (bytes, detail) or
+    (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    env = dict(os.environ, TMPDIR="/tmp")
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="ar_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "probe", "--",
+               os.path.realpath(sys.executable), os.path.join(ROOT, "bench.py"), "--workload", "pmc-probe",
+               "--trace-config", cfg]
+        log(f"traffic: rocprofv3 --pmc {counter}")
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                               timeout=240)
+        except subprocess.TimeoutExpired:
+            return None, f"rocprofv3 --pmc {counter} timed out"
+        files = [os.path.join(dp, f) for dp, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+        if not files:
+            return None, f"rocprofv3 --pmc {counter}: no counter file (rc {r.returncode})"
+        trav, copy = None, None
+        for row in csv.DictReader(open(files[0])):
+            v = float(row["Counter_Value"])
+            name = row["Kernel_Name"]
+            if kernel_key in name:
+                trav = v
+            elif "copyBuffer" in name or "copy_kernel" in name:
+                copy = v if copy is None else max(copy, v)
+        if trav is None or not copy:
+            return None, f"rocprofv3 --pmc {counter}: kernels not found"
+        out[counter] = (trav, copy)
+    f_fetch = (1 << 30) / (out["FETCH_SIZE"][1] * 1024.0)
+    f_write = (1 << 30) / (out["WRITE_SIZE"][1] * 1024.0)
+    fetch_b = out["FETCH_SIZE"][0] * 1024.0 * f_fetch
+    write_b = out["WRITE_SIZE"][0] * 1024.0 * f_write
+    return fetch_b + write_b, {"fetch_bytes": round(fetch_b), "write_bytes": round(write_b),
+                               "fetch_scale": round(f_fetch, 4), "write_scale": round(f_write, 4),
+                               "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on one closest-hit launch, "
+                                         "scaled by the same counters on a 1 GiB copy"}
 
 
 def C_void(p):
@@ -281,16 +373,23 @@ def bench_serving(args, ar, torch):
         srv.wait(w)
     t0 = time.perf_counter()
     ids = []
+    last_log = 0.0
     for j in jobs:
         now = time.perf_counter() - t0
         if j.arrival > now:
             time.sleep(j.arrival - now)
         ids.append((j, srv.submit(integ(j), cam(j), j.width, j.height)))
+        if now - last_log > 10.0:
+            log(f"serving: {len(ids)}/{len(jobs)} jobs submitted at t={now:.1f}s")
+            last_log = now
     lat, paths = [], 0
-    for j, jid in ids:
+    for k, (j, jid) in enumerate(ids):
         _, st = srv.wait(jid)
         lat.append(st["latency"])
         paths += st["paths"]
+        if time.perf_counter() - t0 - last_log > 10.0:
+            last_log = time.perf_counter() - t0
+            log(f"serving: {k + 1}/{len(ids)} jobs finished at t={last_log:.1f}s")
     wall = time.perf_counter() - t0
     srv.close()
     lat = np.asarray(lat)
@@ -319,14 +418,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "copy"],
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "copy", "pmc-probe"],
                     help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--trace-config", default=None,
-                    help="WORKGROUP,LDS_STACK,BLOCKS_PER_CU: fixed traversal launch instead of autotuning")
+                    help="WORKGROUP,LDS_STACK,BLOCKS_PER_CU[,REFILL_MIN]: fixed traversal launch instead of "
+                         "autotuning")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the path-tracer throughput fields")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs for roofline.traffic")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--slots", type=int, default=8)
@@ -344,10 +445,13 @@ def main():
     if args.workload == "copy":
         print(json.dumps(bench_copy(args, torch)), flush=True)
         return
+    if args.workload == "pmc-probe":
+        pmc_probe(args, ar, torch)
+        return
     if args.workload == "serving":
         srv = bench_serving(args, ar, torch)
         out = {"metric": METRIC, "value": srv["p99_latency_s"], "unit": "s (p99 job latency)", "n_gpus": 1,
-               "steps": len(srv), "warmup": args.slots, "ms_per_step": None, "higher_is_better": False,
+               "steps": srv["jobs"], "warmup": args.slots, "ms_per_step": None, "higher_is_better": False,
                "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
                "config": {"workload": "serving_room_280k_tris", "parallelism": "single-gpu"}, "serving": srv}
         print(json.dumps(out), flush=True)
