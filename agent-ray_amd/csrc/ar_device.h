@@ -69,6 +69,21 @@ struct alignas(16) WNode {
 };
 static_assert(sizeof(WNode) == 64, "node layout");
 
+// 4-wide node of the near-first traversal.  Child c's box on axis k is
+// [org[k] + qlo[k][c] * scl[k], org[k] + qhi[k][c] * scl[k]] (scl a power of
+// two, so q * scl is exact and one rounding remains); the builder rounds the
+// 8-bit planes outward until every dequantized box contains the child's
+// exact box.  Empty slots have ref == kEmptyRef.  Load as four 16 B words:
+//   a = (org.xyz, scl.x)  b = (scl.y, scl.z, qlo.x, qlo.y)
+//   c = (qlo.z, qhi.x, qhi.y, qhi.z)  d = ref[0..3]
+// where each q word packs children 0..3 in bytes 0..3.
+struct alignas(64) QNode {
+  float4 a, b, c;
+  int4 d;
+};
+static_assert(sizeof(QNode) == 64, "qnode layout");
+constexpr int kEmptyRef = (int)0x80000001;
+
 // Triangle in BLAS leaf order: v0, e1 = v1 - v0, e2 = v2 - v0 (computed in
 // fp32 exactly as the reference's Triangle::new).  v0.w = prim id | last bit.
 struct alignas(16) Tri {
@@ -84,16 +99,19 @@ struct alignas(16) Inst {
   uint32_t id;        // object id | last-in-TLAS-leaf bit
   uint32_t identity;  // 1 when the transform is the identity
   uint32_t tri_base;  // first Tri record of this BLAS (global index)
-  uint32_t pad[2];
+  int qroot;          // BLAS root ref in the 4-wide node array
+  uint32_t pad;
 };
 static_assert(sizeof(Inst) == 96, "instance layout");
 
 struct SceneView {
-  const WNode* nodes;
+  const WNode* nodes;   // binary trees (reference order)
+  const QNode* qnodes;  // 4-wide trees (near-first order)
   const Tri* tris;
   const Inst* insts;
-  int tlas_root;       // TLAS root ref (node index or ~first instance record)
-  float tlas_box[6];   // TLAS root box (world)
+  int tlas_root;        // TLAS root ref (node index or ~first instance record)
+  int qtlas_root;       // the same in the 4-wide node array
+  float tlas_box[6];    // TLAS root box (world)
   uint32_t n_inst;
 };
 
@@ -104,21 +122,40 @@ struct Ray {
   float tmax;
 };
 
-// Slab test with the reference's arithmetic (geometry.rs:86-98); inv = 1/d
-// precomputed (bitwise identical to recomputing per call).  Returns entry
-// distance in *tn; hit iff max(slab near, tmin) <= min(slab far, tmax).
+// glam's SSE2 Vec3A min / max lanes (minps / maxps): the second operand
+// when the comparison is false, so a NaN in either position follows the
+// instruction, not IEEE minNum.
+AR_HD float sel_min(float a, float b) { return a < b ? a : b; }
+AR_HD float sel_max(float a, float b) { return a > b ? a : b; }
+
+// Slab test with the reference's arithmetic and NaN behaviour
+// (geometry.rs:86-98): t0 = (lo - o) * inv, t1 = (hi - o) * inv per axis,
+// near = Vec3A::min(t0, t1), far = Vec3A::max(t0, t1), reduced as glam's
+// max_element / min_element (max(max(x, z), max(y, z))), then clamped with
+// f32::max(tmin) / f32::min(tmax) (a NaN operand ignored).  A ray lying in a
+// box plane with a zero direction component makes 0 * inf = NaN, and the
+// result then depends on exactly this order.  inv = 1/d precomputed
+// (bitwise identical to recomputing per call).  Returns the entry distance
+// in *tn.
 AR_HD bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz, f3 o, f3 inv,
                 float tmin, float tmax, float* tn) {
   float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
   float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
   float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
-  float nx = fminf(t0x, t1x), fx = fmaxf(t0x, t1x);
-  float ny = fminf(t0y, t1y), fy = fmaxf(t0y, t1y);
-  float nz = fminf(t0z, t1z), fz = fmaxf(t0z, t1z);
-  float a = fmaxf(fmaxf(fmaxf(nx, ny), nz), tmin);
-  float b = fminf(fminf(fminf(fx, fy), fz), tmax);
+  float nx = sel_min(t0x, t1x), fx = sel_max(t0x, t1x);
+  float ny = sel_min(t0y, t1y), fy = sel_max(t0y, t1y);
+  float nz = sel_min(t0z, t1z), fz = sel_max(t0z, t1z);
+  float a = fmaxf(sel_max(sel_max(nx, nz), sel_max(ny, nz)), tmin);
+  float b = fminf(sel_min(sel_min(fx, fz), sel_min(fy, fz)), tmax);
   *tn = a;
   return a <= b;
+}
+
+// True when no component of inv is infinite (no zero or subnormal direction
+// component): the slab test then never meets 0 * inf, and it is monotone in
+// the box planes.
+AR_HD bool finite_inv(f3 inv) {
+  return fabsf(inv.x) < INFINITY && fabsf(inv.y) < INFINITY && fabsf(inv.z) < INFINITY;
 }
 
 // Moller-Trumbore (geometry.rs:171-206); returns t, u, v on hit.

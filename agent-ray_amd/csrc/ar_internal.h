@@ -139,8 +139,9 @@ struct ar_scene {
   ar::DevBuf d_scratch;
   hipEvent_t scratch_event = nullptr;
   uint32_t tlas_depth = 0;
+  uint32_t stack_depth = 0;  // traversal stack entries a query can need (either tree form)
   ar_trace_config tcfg{128, 16, 8, 16, 0.0f};  // traversal launch configuration (autotuned)
-  ar::DevBuf d_nodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
+  ar::DevBuf d_nodes, d_qnodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
       d_tri_obj;
   ar::SceneView view{};
   uint32_t n_lights = 0;
@@ -159,14 +160,16 @@ hipStream_t pick_stream(const ar_scene* s, void* stream);
 // and shading records (what ar_scene_commit uploads).
 struct HostAccel {
   std::vector<WNode> nodes;
+  std::vector<QNode> qnodes;
   std::vector<Tri> tris;
   std::vector<Inst> insts;
   std::vector<DObject> objs;
   std::vector<uint32_t> gidx, gslots, gmats, tri_obj;
   std::vector<float> gnrm;
-  int tlas_root = 0;
+  int tlas_root = 0, qtlas_root = 0;
   float tlas_box[6] = {0, 0, 0, 0, 0, 0};
-  uint32_t max_depth = 0, tlas_depth = 0;
+  uint32_t max_depth = 0, tlas_depth = 0;    // binary trees
+  uint32_t qmax_depth = 0, qtlas_depth = 0;  // 4-wide trees
 };
 ar_status build_host_accel(const ar_scene* s, HostAccel& h);
 }

@@ -11,10 +11,10 @@
 //   * per instance: local ray = (M^-1 o, normalize(M^-1 d)), t range scaled
 //     by s = |M^-1 d|, BLAS hit t divided by s, accepted against the world
 //     best with strict `<`.
-// REF order (left child first) reproduces the reference's results bit for
-// bit, including which of two equal-t triangles wins.  NEAR order visits the
-// nearer child first and breaks equal-t ties by leaf position, which is the
-// reference's DFS order, so the same primitive wins.
+// REF order (binary trees, left child first) reproduces the reference's
+// results bit for bit, including which of two equal-t triangles wins.  NEAR
+// order (4-wide trees, nearest child first) breaks equal-t ties by leaf
+// position, which is the reference's DFS order, so the same primitive wins.
 //
 // Stack: S entries per lane live in LDS (lane-strided so the 64 lanes of a
 // wave hit distinct banks); deeper entries spill to a global array laid out
@@ -45,83 +45,6 @@ struct Stack {
   }
 };
 
-struct LocalHit {
-  float t;        // local t of the best hit (valid when pos != kNoHit)
-  uint32_t pos;   // global Tri record index
-  float u, v;
-};
-
-// BLAS traversal from `root` (box `rb`).  Closest mode: finds the best hit
-// with t < tmax (strict), shrinking tmax.  Any mode: returns true at the
-// first hit whose world t (t / scale) is < world_tmax.
-template <bool ANY, bool NEAR>
-AR_HD bool blas_traverse(const WNode* __restrict__ nodes, const Tri* __restrict__ tris,
-                                              int root, const float* rb, f3 o, f3 d, float tmin, float tmax,
-                                              LocalHit& best, Stack& stk, float scale, float world_tmax) {
-  f3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-  float tn;
-  if (!slab(rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], o, inv, tmin, tmax, &tn)) return false;
-  const int base = stk.sp;
-  bool found = false;
-  int cur = root;
-  for (;;) {
-    if (cur >= 0) {
-      const WNode* n = nodes + cur;
-      float4 a = n->a, b = n->b, c = n->c;
-      int4 ch = n->d;
-      float tl, tr;
-      bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
-      bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
-      if (hl && hr) {
-        bool swap = NEAR && (tr < tl);
-        stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
-        cur = swap ? ch.y : ch.x;
-        continue;
-      }
-      if (hl) {
-        cur = ch.x;
-        continue;
-      }
-      if (hr) {
-        cur = ch.y;
-        continue;
-      }
-    } else {
-      int k = ~cur;
-      for (;;) {
-        const Tri* tp = tris + k;
-        float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
-        uint32_t tag = bits_u(v0.w);
-        float th, uh, vh;
-        if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tmax, &th,
-                    &uh, &vh)) {
-          if (ANY) {
-            if (th / scale < world_tmax) return true;
-          } else if (th < tmax || (NEAR && found && th == tmax && (uint32_t)k < best.pos)) {
-            tmax = th;
-            best.t = th;
-            best.pos = (uint32_t)k;
-            best.u = uh;
-            best.v = vh;
-            found = true;
-          }
-        }
-        if (tag & kLastBit) break;
-        ++k;
-      }
-    }
-    // pop the next deferred node still in range
-    for (;;) {
-      if (stk.sp == base) return found;
-      int2 e = stk.pop();
-      if (bits_f(e.y) <= tmax) {
-        cur = e.x;
-        break;
-      }
-    }
-  }
-}
-
 struct WorldHit {
   float t;
   uint32_t pos;   // Tri record (global leaf order), kNoHit on miss
@@ -129,119 +52,33 @@ struct WorldHit {
   float u, v;
 };
 
-// Instance leaf visit: transform, BLAS traverse, world acceptance.
-template <bool ANY, bool NEAR>
-AR_HD bool visit_instance(const SceneView& sv, uint32_t k, const Ray& r, WorldHit& best,
-                                               Stack& stk) {
-  const Inst* in = sv.insts + k;
-  f3 lo, ld;
-  if (in->identity) {
-    lo = r.o;
-    ld = r.d;
-  } else {
-    lo = xform_point(in->inv, r.o);
-    ld = xform_vector(in->inv, r.d);
-  }
-  float s = length(ld);
-  if (s < kF32Eps) return false;
-  f3 ldn = normalize(ld);  // Ray::new renormalizes
-  float wt = best.t;
-  LocalHit lh;
-  lh.pos = kNoHit;
-  lh.t = 0.0f;
-  lh.u = lh.v = 0.0f;
-  bool hit = blas_traverse<ANY, NEAR>(sv.nodes, sv.tris, in->root, in->box, lo, ldn, r.tmin * s, wt * s, lh, stk,
-                                      s, wt);
-  if (ANY) return hit;
-  if (!hit) return false;
-  float tw = lh.t / s;
-  if (tw < best.t || (NEAR && best.pos != kNoHit && tw == best.t && k < best.inst)) {
-    best.t = tw;
-    best.pos = lh.pos;
-    best.inst = k;
-    best.u = lh.u;
-    best.v = lh.v;
-    return true;
-  }
-  return false;
-}
-
-// Full query.  Closest: fills best (best.pos == kNoHit on miss).  Any:
-// returns whether anything is hit in [tmin, tmax).
-template <bool ANY, bool NEAR>
-AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack& stk) {
-  best.t = r.tmax;
-  best.pos = kNoHit;
-  best.inst = kNoHit;
-  best.u = best.v = 0.0f;
-  f3 inv = mk(1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z);
-  float tn;
-  const float* tb = sv.tlas_box;
-  if (!slab(tb[0], tb[1], tb[2], tb[3], tb[4], tb[5], r.o, inv, r.tmin, r.tmax, &tn)) return false;
-  const int base = stk.sp;
-  bool found = false;
-  int cur = sv.tlas_root;
-  for (;;) {
-    if (cur >= 0) {
-      const WNode* n = sv.nodes + cur;
-      float4 a = n->a, b = n->b, c = n->c;
-      int4 ch = n->d;
-      float tl, tr;
-      bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, r.o, inv, r.tmin, best.t, &tl);
-      bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, r.o, inv, r.tmin, best.t, &tr);
-      if (hl && hr) {
-        bool swap = NEAR && (tr < tl);
-        stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
-        cur = swap ? ch.y : ch.x;
-        continue;
-      }
-      if (hl) {
-        cur = ch.x;
-        continue;
-      }
-      if (hr) {
-        cur = ch.y;
-        continue;
-      }
-    } else {
-      uint32_t k = (uint32_t)~cur;
-      for (;;) {
-        uint32_t id = sv.insts[k].id;
-        if (visit_instance<ANY, NEAR>(sv, k, r, best, stk)) {
-          if (ANY) {
-            stk.sp = base;
-            return true;
-          }
-          found = true;
-        }
-        if (id & kLastBit) break;
-        ++k;
-      }
-    }
-    for (;;) {
-      if (stk.sp == base) return found;
-      int2 e = stk.pop();
-      if (bits_f(e.y) <= best.t) {
-        cur = e.x;
-        break;
-      }
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------
-// Resumable form of trace_scene.  The whole state of one query lives in the
-// Walker (registers), so a lane can finish its ray and start another while
-// the other lanes of its wave are still walking (walk_items below) instead
-// of idling until the wave's slowest ray ends.
+// Two-level traversal as a resumable walker.  The whole state of one query
+// lives in the Walker (registers), so a lane can finish its ray and start
+// another while the other lanes of its wave are still walking (walk_items
+// below) instead of idling until the wave's slowest ray ends; trace_scene
+// runs one walker to completion.
+//
+// REF order walks the binary trees left child first (bvh.rs:247-296) and
+// reproduces the reference bit for bit, ties included.  NEAR order walks the
+// 4-wide trees (QNode), nearest child first.  Their quantized child boxes
+// contain the exact ones, and with a finite 1/d the slab test is monotone in
+// the box planes, so every leaf the exact tree reaches is reached and the
+// closest hit is the same; equal-t hits are resolved by lower leaf position
+// / instance record, which is the reference's DFS winner.  A level whose
+// 1/d has an infinite component (the 0 * inf = NaN cases of the slab test)
+// is walked on the binary tree instead, so those rays see the reference's
+// exact box decisions.  What remains different is a hit that the exact
+// slab test loses to rounding at a box face (a ray grazing the box edge at
+// the hit point): the 4-wide boxes are larger and keep it.
 //
 // Work is split in two phases so that a wave's lanes run the same code at
 // the same time: descend() visits internal nodes until the lane holds a
 // leaf (or its level's stack is exhausted); leaf() then does the rarer work
 // -- a leaf's triangles, entering or leaving an instance, ending the query.
 //
-// Arithmetic and decisions are those of trace_scene / visit_instance /
-// blas_traverse, operation for operation, so results are bit-identical:
+// Per level (the reference's TLAS::intersect / Instance::intersect /
+// traverse_bvh arithmetic, operation for operation):
 //   TLAS level: ray (wo, 1/wd, wtmin, bt), pops accepted when tnear <= bt;
 //   BLAS level: local ray (M^-1 o, normalize(M^-1 d)), range (wtmin*s, bt*s)
 //   at entry, local acceptance by strict < (NEAR ties: lower leaf position),
@@ -266,6 +103,7 @@ struct Walker {
   int base;               // stack depth at instance entry
   int cur;                // node ref, leaf ref, or kLevelEnd
   bool in_blas;
+  bool wide;              // this level walks the 4-wide tree
 
   // Starts a query.  Returns true when it is already finished (the TLAS
   // root box is missed).
@@ -283,7 +121,8 @@ struct Walker {
     tmin = wtmin;
     tmax = bt;
     in_blas = false;
-    cur = sv.tlas_root;
+    wide = NEAR && finite_inv(inv);
+    cur = wide ? sv.qtlas_root : sv.tlas_root;
     float tn;
     const float* tb = sv.tlas_box;
     return !slab(tb[0], tb[1], tb[2], tb[3], tb[4], tb[5], o, inv, tmin, tmax, &tn);
@@ -308,23 +147,82 @@ struct Walker {
 
   // Visits internal nodes until the lane holds a leaf or kLevelEnd.
   AR_HD void descend(const SceneView& sv, Stack& stk) {
-    while (cur >= 0) {
-      const WNode* n = sv.nodes + cur;
-      float4 a = n->a, b = n->b, c = n->c;
-      int4 ch = n->d;
-      float tl, tr;
-      bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
-      bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
-      if (hl && hr) {
-        bool swap = NEAR && (tr < tl);
-        stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
-        cur = swap ? ch.y : ch.x;
-      } else if (hl) {
-        cur = ch.x;
-      } else if (hr) {
-        cur = ch.y;
-      } else {
-        pop_level(stk);
+    if (NEAR && wide) {
+      while (cur >= 0) {
+        const QNode* n = sv.qnodes + cur;
+        const float4 a = n->a, b = n->b, c = n->c;
+        const int4 rf = n->d;
+        const uint32_t lx = bits_u(b.z), ly = bits_u(b.w), lz = bits_u(c.x);
+        const uint32_t hx = bits_u(c.y), hy = bits_u(c.z), hz = bits_u(c.w);
+        float t[4];
+        int r[4] = {rf.x, rf.y, rf.z, rf.w};
+#ifdef __clang__
+#pragma unroll
+#endif
+        for (int k = 0; k < 4; ++k) {
+          const int sh = 8 * k;
+          float tn;
+          bool h = slab(a.x + (float)((lx >> sh) & 255u) * a.w, a.y + (float)((ly >> sh) & 255u) * b.x,
+                        a.z + (float)((lz >> sh) & 255u) * b.y, a.x + (float)((hx >> sh) & 255u) * a.w,
+                        a.y + (float)((hy >> sh) & 255u) * b.x, a.z + (float)((hz >> sh) & 255u) * b.y, o, inv,
+                        tmin, tmax, &tn);
+          h = h && r[k] != kEmptyRef;
+          t[k] = h ? tn : INFINITY;
+          r[k] = h ? r[k] : kEmptyRef;
+        }
+        // nearest first: sort the four (t, ref) pairs by t
+        auto cx = [&](int i, int j) {
+          if (t[j] < t[i]) {
+            float tt = t[i];
+            t[i] = t[j];
+            t[j] = tt;
+            int rr = r[i];
+            r[i] = r[j];
+            r[j] = rr;
+          }
+        };
+        cx(0, 1);
+        cx(2, 3);
+        cx(0, 2);
+        cx(1, 3);
+        cx(1, 2);
+        // hold the nearest, push the others farthest first
+        int hold = kEmptyRef;
+        float th = 0.0f;
+#ifdef __clang__
+#pragma unroll
+#endif
+        for (int k = 3; k >= 0; --k) {
+          if (r[k] != kEmptyRef) {
+            if (hold != kEmptyRef) stk.push(hold, th);
+            hold = r[k];
+            th = t[k];
+          }
+        }
+        if (hold != kEmptyRef)
+          cur = hold;
+        else
+          pop_level(stk);
+      }
+    } else {
+      while (cur >= 0) {
+        const WNode* n = sv.nodes + cur;
+        float4 a = n->a, b = n->b, c = n->c;
+        int4 ch = n->d;
+        float tl, tr;
+        bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
+        bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
+        if (hl && hr) {
+          bool swap = NEAR && (tr < tl);
+          stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
+          cur = swap ? ch.y : ch.x;
+        } else if (hl) {
+          cur = ch.x;
+        } else if (hr) {
+          cur = ch.y;
+        } else {
+          pop_level(stk);
+        }
       }
     }
   }
@@ -358,7 +256,8 @@ struct Walker {
           s = sc;
           ik = k;
           base = stk.sp;
-          cur = in->root;
+          wide = NEAR && finite_inv(li);
+          cur = wide ? in->qroot : in->root;
           in_blas = true;
           lpos = kNoHit;
           lu = lv = 0.0f;
@@ -390,6 +289,7 @@ struct Walker {
       o = wo;
       d = wd;
       inv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
+      wide = NEAR && finite_inv(inv);
       tmin = wtmin;
       tmax = bt;
       if (!(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
@@ -426,6 +326,25 @@ struct Walker {
     return false;
   }
 };
+
+// One query to completion.  Closest: fills best (best.pos == kNoHit on a
+// miss) and returns whether anything was hit.  Any: returns whether
+// anything is hit in [tmin, tmax).
+template <bool ANY, bool NEAR>
+AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack& stk) {
+  Walker<ANY, NEAR> w;
+  bool done = w.start(sv, r);
+  while (!done) {
+    w.descend(sv, stk);
+    done = w.leaf(sv, stk);
+  }
+  best.t = w.bt;
+  best.pos = w.bpos;
+  best.inst = w.binst;
+  best.u = w.bu;
+  best.v = w.bv;
+  return w.bpos != kNoHit;
+}
 
 // Items [0, n) are split in chunks of 64; wave `wave` of `waves` owns chunks
 // wave, wave + waves, ... and walks their items in order.  A lane that

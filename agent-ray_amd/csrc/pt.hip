@@ -320,7 +320,7 @@ static uint32_t persistent_grid(int block, int per_cu) {
 }
 
 uint32_t stack_spill_entries(const ar_scene* s) {
-  uint32_t need = s->info.max_depth + s->tlas_depth + 2;
+  uint32_t need = s->stack_depth;
   return need > (uint32_t)kTraceLdsStack ? need - kTraceLdsStack : 0;
 }
 
@@ -500,7 +500,7 @@ class PtJob : public JobRun {
     eg = std::min<uint32_t>((npix + 255) / 256, 4096);
     // Spill area: covers the megakernel's (128, 16) launch and the traversal
     // stages' tuned launch, whichever runs.
-    uint32_t need = scene->info.max_depth + scene->tlas_depth + 2;
+    uint32_t need = scene->stack_depth;
     const ar_trace_config& tc = scene->tcfg;
     uint64_t gt = (uint64_t)(device_cus() * tc.blocks_per_cu / 8 * 8) * tc.workgroup_size;
     uint64_t e_t = need > tc.lds_stack ? need - tc.lds_stack : 0;

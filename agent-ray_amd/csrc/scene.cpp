@@ -160,6 +160,124 @@ Flat flatten(const BinTree& t, std::vector<WNode>& out, uint32_t prim_base) {
   return f;
 }
 
+// ---- 4-wide trees for the near-first traversal ------------------------------
+static float box_area(const HBox& b) {
+  float dx = b.hi[0] - b.lo[0], dy = b.hi[1] - b.lo[1], dz = b.hi[2] - b.lo[2];
+  return 2.0f * (dx * dy + dy * dz + dz * dx);
+}
+
+// Smallest power of two >= x (x > 0, finite).
+static float pow2_at_least(float x) {
+  int e = 0;
+  float m = std::frexp(x, &e);  // x = m * 2^e, m in [0.5, 1)
+  return m == 0.5f ? std::ldexp(1.0f, e - 1) : std::ldexp(1.0f, e);
+}
+
+// The kernel's dequantization: q * s is exact (s a power of two, q < 256),
+// so this single rounding is what the device computes too.
+static float deq(float org, uint32_t q, float s) { return org + (float)q * s; }
+
+// Quantizes the child boxes of one 4-wide node relative to the node box,
+// rounding each plane outward until the dequantized box contains the exact
+// one; an axis whose planes cannot all be covered doubles its scale.
+static QNode quantize_node(const HBox& nb, const std::vector<const HBox*>& cb, const int* refs) {
+  uint32_t w[16];
+  float org[3], scl[3];
+  uint32_t qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+  for (int k = 0; k < 3; ++k) {
+    org[k] = nb.lo[k];
+    float ext = nb.hi[k] - nb.lo[k];
+    float s = ext > 0.0f ? pow2_at_least(ext / 255.0f) : 1.0f;
+    for (;;) {
+      bool ok = true;
+      uint32_t wl = 0, wh = 0;
+      for (size_t c = 0; c < cb.size() && ok; ++c) {
+        float lo = cb[c]->lo[k], hi = cb[c]->hi[k];
+        double fl = std::floor(((double)lo - (double)org[k]) / (double)s);
+        double fh = std::ceil(((double)hi - (double)org[k]) / (double)s);
+        uint32_t a = (uint32_t)std::min(std::max(fl, 0.0), 255.0);
+        uint32_t b = (uint32_t)std::min(std::max(fh, 0.0), 255.0);
+        while (a > 0 && deq(org[k], a, s) > lo) --a;
+        while (b < 255 && deq(org[k], b, s) < hi) ++b;
+        if (deq(org[k], a, s) > lo || deq(org[k], b, s) < hi) ok = false;
+        wl |= a << (8 * c);
+        wh |= b << (8 * c);
+      }
+      if (ok) {
+        scl[k] = s;
+        qlo[k] = wl;
+        qhi[k] = wh;
+        break;
+      }
+      s *= 2.0f;
+    }
+  }
+  memcpy(&w[0], org, 12);
+  memcpy(&w[3], &scl[0], 4);
+  memcpy(&w[4], &scl[1], 8);
+  w[6] = qlo[0];
+  w[7] = qlo[1];
+  w[8] = qlo[2];
+  w[9] = qhi[0];
+  w[10] = qhi[1];
+  w[11] = qhi[2];
+  memcpy(&w[12], refs, 16);
+  QNode q;
+  memcpy(&q, w, sizeof(q));
+  return q;
+}
+
+// Collapses a binary tree into 4-wide nodes: each node takes its binary
+// children and repeatedly opens the internal slot of largest surface area
+// (keeping left-to-right order) until it has four slots or only leaves.
+// Leaves keep their ranges (ref ~(prim_base + start)), so leaf positions and
+// the Tri records are shared with the binary form.  Nodes are stored in
+// preorder.
+struct QFlat {
+  int root_ref;
+  uint32_t depth;
+};
+static int qbvh_rec(const BinTree& t, uint32_t i, uint32_t depth, std::vector<QNode>& out, uint32_t prim_base,
+                    QFlat& f) {
+  const BinNode& n = t.nodes[i];
+  if (n.count != 0) return ~(int)(prim_base + n.left_or_start);
+  f.depth = std::max(f.depth, depth + 1);
+  std::vector<uint32_t> slots = {n.left_or_start, n.right_or_end};
+  while (slots.size() < 4) {
+    int best = -1;
+    float best_a = -1.0f;
+    for (size_t k = 0; k < slots.size(); ++k) {
+      const BinNode& c = t.nodes[slots[k]];
+      if (c.count != 0) continue;
+      float a = box_area(c.box);
+      if (a > best_a) {
+        best_a = a;
+        best = (int)k;
+      }
+    }
+    if (best < 0) break;
+    const BinNode& c = t.nodes[slots[best]];
+    uint32_t l = c.left_or_start, r = c.right_or_end;
+    slots[best] = l;
+    slots.insert(slots.begin() + best + 1, r);
+  }
+  uint32_t me = (uint32_t)out.size();
+  out.emplace_back();
+  int refs[4] = {kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef};
+  std::vector<const HBox*> boxes;
+  for (size_t k = 0; k < slots.size(); ++k) {
+    refs[k] = qbvh_rec(t, slots[k], depth + 1, out, prim_base, f);
+    boxes.push_back(&t.nodes[slots[k]].box);
+  }
+  out[me] = quantize_node(n.box, boxes, refs);
+  return (int)me;
+}
+static QFlat build_qbvh(const BinTree& t, std::vector<QNode>& out, uint32_t prim_base) {
+  QFlat f{0, 0};
+  f.root_ref = qbvh_rec(t, 0, 0, out, prim_base, f);
+  return f;
+}
+
 template <class T>
 hipError_t upload(DevBuf& b, const std::vector<T>& v) {
   size_t n = std::max<size_t>(v.size() * sizeof(T), 16);
@@ -328,6 +446,8 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     }
     for (uint32_t i = 0; i < nt; ++i) tri_obj.push_back((uint32_t)oi);
     Flat f = flatten(t, nodes, tri_base);
+    QFlat qf = build_qbvh(t, h.qnodes, tri_base);
+    h.qmax_depth = std::max(h.qmax_depth, qf.depth);
     Inst& in = inst_by_obj[oi];
     memset(&in, 0, sizeof(in));
     float inv[16];
@@ -339,6 +459,7 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
       in.box[3 + k] = f.root_box.hi[k];
     }
     in.root = f.root_ref;
+    in.qroot = qf.root_ref;
     in.id = (uint32_t)oi;
     in.identity = is_identity16(m.xf) ? 1u : 0u;
     in.tri_base = tri_base;
@@ -379,7 +500,10 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     }
   }
   Flat tf = flatten(tl, nodes, 0);
+  QFlat tq = build_qbvh(tl, h.qnodes, 0);
   h.tlas_root = tf.root_ref;
+  h.qtlas_root = tq.root_ref;
+  h.qtlas_depth = tq.depth;
   for (int k = 0; k < 3; ++k) {
     h.tlas_box[k] = tf.root_box.lo[k];
     h.tlas_box[3 + k] = tf.root_box.hi[k];
@@ -400,6 +524,9 @@ ar_status ar_scene_commit(ar_scene* s) {
   if (be != AR_OK) return be;
   s->tlas_depth = h.tlas_depth;
   const uint32_t max_depth = h.max_depth;
+  // Stack bound: the binary walk pushes at most one entry per level, the
+  // 4-wide walk at most three.
+  s->stack_depth = std::max(h.max_depth + h.tlas_depth + 2, 3 * (h.qmax_depth + h.qtlas_depth) + 2);
   // Materials / lights / distribution (lights.rs:115-235).
   std::vector<DLight> dl(s->lights.size());
   bool emit = false;
@@ -458,6 +585,7 @@ ar_status ar_scene_commit(ar_scene* s) {
     }
   }
   AR_HIP(upload(s->d_nodes, h.nodes));
+  AR_HIP(upload(s->d_qnodes, h.qnodes));
   AR_HIP(upload(s->d_tris, h.tris));
   AR_HIP(upload(s->d_insts, h.insts));
   AR_HIP(upload(s->d_objects, h.objs));
@@ -469,6 +597,8 @@ ar_status ar_scene_commit(ar_scene* s) {
   AR_HIP(upload(s->d_objmats, h.gmats));
   AR_HIP(upload(s->d_tri_obj, h.tri_obj));
   s->view.nodes = s->d_nodes.as<WNode>();
+  s->view.qnodes = s->d_qnodes.as<QNode>();
+  s->view.qtlas_root = h.qtlas_root;
   s->view.tris = s->d_tris.as<Tri>();
   s->view.insts = s->d_insts.as<Inst>();
   s->view.tlas_root = h.tlas_root;
@@ -483,7 +613,7 @@ ar_status ar_scene_commit(ar_scene* s) {
   s->info.n_nodes = (uint32_t)h.nodes.size();
   s->info.n_lights = nl;
   s->info.max_depth = max_depth;
-  s->info.device_bytes = s->d_nodes.bytes + s->d_tris.bytes + s->d_insts.bytes + s->d_objects.bytes +
+  s->info.device_bytes = s->d_nodes.bytes + s->d_qnodes.bytes + s->d_tris.bytes + s->d_insts.bytes + s->d_objects.bytes +
                          s->d_materials.bytes + s->d_lights.bytes + s->d_idx.bytes + s->d_nrm.bytes +
                          s->d_slots.bytes + s->d_objmats.bytes + s->d_tri_obj.bytes;
   AR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));

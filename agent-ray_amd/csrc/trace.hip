@@ -64,9 +64,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const ar_ray* __r
       });
 }
 
-// Stack sizing: a push happens only at an internal node with both children
-// hit, so the depth of the deepest BLAS plus the TLAS bounds the stack.
-static uint32_t stack_need(const ar_scene* s) { return s->info.max_depth + s->tlas_depth + 2; }
+// Stack entries one query can need (bounded by the tree depths at commit).
+static uint32_t stack_need(const ar_scene* s) { return s->stack_depth; }
 
 template <bool ANY, bool NEAR, int BLOCK, int S>
 static void launch_variant(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits,

@@ -90,6 +90,35 @@ def test_room_scene(ar, orc, wl):
     _check_scene(ar, orc, desc, np.concatenate([cr, rr]))
 
 
+def _plane_rays(wl, desc, m, seed):
+    """Rays that start on triangle vertices (so on leaf-box planes) with a
+    zero direction component: the slab test meets 0 * inf = NaN there and
+    the reference's SSE min/max order decides the box."""
+    rng = np.random.default_rng(seed)
+    P = np.concatenate([mm.positions.reshape(-1, 3) for mm in desc.meshes]).astype(np.float32)
+    o = P[rng.integers(0, P.shape[0], m)].copy()
+    for ax in range(3):
+        sel = rng.random(m) < 0.5
+        o[sel, ax] = P[rng.integers(0, P.shape[0], int(sel.sum())), ax]
+    d = np.zeros((m, 3), np.float32)
+    d[np.arange(m), rng.integers(0, 3, m)] = np.where(rng.random(m) < 0.5, -1.0, 1.0)
+    k = m // 3
+    d2 = rng.standard_normal((k, 3)).astype(np.float32)
+    d2[np.arange(k), rng.integers(0, 3, k)] = 0.0
+    d[:k] = wl.normalize_f32(d2)
+    r = np.zeros((m, 8), np.float32)
+    r[:, 0:3], r[:, 4:7], r[:, 7] = o, d, wl.F32_MAX
+    return r
+
+
+def test_axis_rays_on_box_planes(ar, orc, wl):
+    desc = wl.triangle_soup(1 << 12, seed=7)
+    rays = _plane_rays(wl, desc, 1 << 13, seed=8)
+    _check_scene(ar, orc, desc, rays, max_tie_frac=0.0)
+    desc = wl.cornell_box()
+    _check_scene(ar, orc, desc, _plane_rays(wl, desc, 1 << 12, seed=9), max_tie_frac=0.0)
+
+
 def test_empty_batch_and_device_pointers(ar, wl):
     import torch
     desc = wl.triangle_soup(4096, seed=51)
