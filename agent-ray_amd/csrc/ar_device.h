@@ -151,6 +151,20 @@ AR_HD bool slab(float lox, float loy, float loz, float hix, float hiy, float hiz
   return a <= b;
 }
 
+// The same decision with IEEE min / max, for an inv with no infinite
+// component: no NaN can arise then, and min / max of non-NaN values agree
+// with the select forms (a -0 / +0 difference compares equal).
+AR_HD bool slab_fast(float lox, float loy, float loz, float hix, float hiy, float hiz, f3 o, f3 inv,
+                     float tmin, float tmax, float* tn) {
+  float t0x = (lox - o.x) * inv.x, t1x = (hix - o.x) * inv.x;
+  float t0y = (loy - o.y) * inv.y, t1y = (hiy - o.y) * inv.y;
+  float t0z = (loz - o.z) * inv.z, t1z = (hiz - o.z) * inv.z;
+  float a = fmaxf(fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z)), tmin);
+  float b = fminf(fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)), tmax);
+  *tn = a;
+  return a <= b;
+}
+
 // True when no component of inv is infinite (no zero or subnormal direction
 // component): the slab test then never meets 0 * inf, and it is monotone in
 // the box planes.

@@ -8,6 +8,11 @@ constexpr int kTraceBlock = 128;       // 2 waves per workgroup
 constexpr int kTraceLdsStack = 16;     // LDS stack entries per lane (8 B each)
 constexpr int kTraceBlocksPerCU = 8;   // persistent grid: 16 waves per CU
 constexpr uint32_t kTraceRefillMin = 16;  // idle lanes a wave waits for before taking new rays
+// Traversal kernels with an 8-entry LDS stack are compiled for at least 6
+// waves per SIMD (<= 80 VGPRs): measured faster than the unconstrained 5
+// waves / 85 VGPRs; the few values the register limit moves to scratch sit
+// off the node loop.  A 16-entry stack is held to 5 waves by LDS anyway.
+constexpr int trace_min_waves(int lds_stack) { return lds_stack <= 8 ? 6 : 1; }
 
 uint32_t trace_grid(uint64_t n, int block);
 uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c);

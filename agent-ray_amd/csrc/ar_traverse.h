@@ -22,6 +22,15 @@
 #pragma once
 #include "ar_device.h"
 
+// Instrumentation hooks (empty in the library build).
+#ifndef AR_COUNT_NODE
+#define AR_COUNT_NODE()
+#endif
+#ifndef AR_COUNT_TRI
+#define AR_COUNT_TRI()
+#endif
+
+
 namespace ar {
 
 struct Stack {
@@ -102,12 +111,15 @@ struct Walker {
   uint32_t ik;            // instance record being walked
   int base;               // stack depth at instance entry
   int cur;                // node ref, leaf ref, or kLevelEnd
+  int pend;               // parked triangle leaf (kLevelEnd: none)
   bool in_blas;
-  bool wide;              // this level walks the 4-wide tree
+  bool fast;              // 1/d of this level is finite: IEEE min / max slab
+  bool wide;              // this level walks the 4-wide tree (near-first, fast)
 
-  // Starts a query.  Returns true when it is already finished (the TLAS
-  // root box is missed).
-  AR_HD bool start(const SceneView& sv, const Ray& r) {
+  // Starts a query on an empty stack.  Returns true when it is already
+  // finished (the TLAS root box is missed, or the TLAS is one leaf whose
+  // instances are all missed).
+  AR_HD bool start(const SceneView& sv, const Ray& r, const Stack& stk) {
     wo = r.o;
     wd = r.d;
     wtmin = r.tmin;
@@ -121,11 +133,16 @@ struct Walker {
     tmin = wtmin;
     tmax = bt;
     in_blas = false;
-    wide = NEAR && finite_inv(inv);
+    pend = kLevelEnd;
+    fast = finite_inv(inv);
+    wide = NEAR && fast;
     cur = wide ? sv.qtlas_root : sv.tlas_root;
     float tn;
     const float* tb = sv.tlas_box;
-    return !slab(tb[0], tb[1], tb[2], tb[3], tb[4], tb[5], o, inv, tmin, tmax, &tn);
+    if (!slab(tb[0], tb[1], tb[2], tb[3], tb[4], tb[5], o, inv, tmin, tmax, &tn)) return true;
+    // a TLAS that is a single leaf: enter its instances right away
+    if (cur < 0 && !enter(sv, (uint32_t)~cur, stk)) return true;
+    return false;
   }
 
   // Pops the deferred nodes of the current level until one is in range;
@@ -145,86 +162,130 @@ struct Walker {
     }
   }
 
-  // Visits internal nodes until the lane holds a leaf or kLevelEnd.
-  AR_HD void descend(const SceneView& sv, Stack& stk) {
-    if (NEAR && wide) {
-      while (cur >= 0) {
-        const QNode* n = sv.qnodes + cur;
-        const float4 a = n->a, b = n->b, c = n->c;
-        const int4 rf = n->d;
-        const uint32_t lx = bits_u(b.z), ly = bits_u(b.w), lz = bits_u(c.x);
-        const uint32_t hx = bits_u(c.y), hy = bits_u(c.z), hz = bits_u(c.w);
-        float t[4];
-        int r[4] = {rf.x, rf.y, rf.z, rf.w};
+  // One 4-wide node: box-test the four children, continue with the nearest
+  // hit, push the others farthest first.
+  AR_HD void visit_wide(const SceneView& sv, Stack& stk) {
+    AR_COUNT_NODE();
+    const QNode* n = sv.qnodes + cur;
+    const float4 a = n->a, b = n->b, c = n->c;
+    const int4 rf = n->d;
+    const uint32_t lx = bits_u(b.z), ly = bits_u(b.w), lz = bits_u(c.x);
+    const uint32_t hx = bits_u(c.y), hy = bits_u(c.z), hz = bits_u(c.w);
+    float t[4];
+    int r[4] = {rf.x, rf.y, rf.z, rf.w};
 #ifdef __clang__
 #pragma unroll
 #endif
-        for (int k = 0; k < 4; ++k) {
-          const int sh = 8 * k;
-          float tn;
-          bool h = slab(a.x + (float)((lx >> sh) & 255u) * a.w, a.y + (float)((ly >> sh) & 255u) * b.x,
-                        a.z + (float)((lz >> sh) & 255u) * b.y, a.x + (float)((hx >> sh) & 255u) * a.w,
-                        a.y + (float)((hy >> sh) & 255u) * b.x, a.z + (float)((hz >> sh) & 255u) * b.y, o, inv,
-                        tmin, tmax, &tn);
-          h = h && r[k] != kEmptyRef;
-          t[k] = h ? tn : INFINITY;
-          r[k] = h ? r[k] : kEmptyRef;
-        }
-        // nearest first: sort the four (t, ref) pairs by t
-        auto cx = [&](int i, int j) {
-          if (t[j] < t[i]) {
-            float tt = t[i];
-            t[i] = t[j];
-            t[j] = tt;
-            int rr = r[i];
-            r[i] = r[j];
-            r[j] = rr;
-          }
-        };
-        cx(0, 1);
-        cx(2, 3);
-        cx(0, 2);
-        cx(1, 3);
-        cx(1, 2);
-        // hold the nearest, push the others farthest first
-        int hold = kEmptyRef;
-        float th = 0.0f;
-#ifdef __clang__
-#pragma unroll
-#endif
-        for (int k = 3; k >= 0; --k) {
-          if (r[k] != kEmptyRef) {
-            if (hold != kEmptyRef) stk.push(hold, th);
-            hold = r[k];
-            th = t[k];
-          }
-        }
-        if (hold != kEmptyRef)
-          cur = hold;
-        else
-          pop_level(stk);
+    for (int k = 0; k < 4; ++k) {
+      const int sh = 8 * k;
+      // org + q * scl with q * scl exact: one rounding, as the builder computed
+      float tn;
+      bool h = slab_fast(fmaf((float)((lx >> sh) & 255u), a.w, a.x), fmaf((float)((ly >> sh) & 255u), b.x, a.y),
+                         fmaf((float)((lz >> sh) & 255u), b.y, a.z), fmaf((float)((hx >> sh) & 255u), a.w, a.x),
+                         fmaf((float)((hy >> sh) & 255u), b.x, a.y), fmaf((float)((hz >> sh) & 255u), b.y, a.z), o,
+                         inv, tmin, tmax, &tn);
+      h = h && r[k] != kEmptyRef;
+      t[k] = h ? tn : INFINITY;
+      r[k] = h ? r[k] : kEmptyRef;
+    }
+    // nearest first: sort the four (t, ref) pairs by t
+    auto cx = [&](int i, int j) {
+      if (t[j] < t[i]) {
+        float tt = t[i];
+        t[i] = t[j];
+        t[j] = tt;
+        int rr = r[i];
+        r[i] = r[j];
+        r[j] = rr;
       }
-    } else {
-      while (cur >= 0) {
-        const WNode* n = sv.nodes + cur;
-        float4 a = n->a, b = n->b, c = n->c;
-        int4 ch = n->d;
-        float tl, tr;
-        bool hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
-        bool hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
-        if (hl && hr) {
-          bool swap = NEAR && (tr < tl);
-          stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
-          cur = swap ? ch.y : ch.x;
-        } else if (hl) {
-          cur = ch.x;
-        } else if (hr) {
-          cur = ch.y;
-        } else {
-          pop_level(stk);
-        }
+    };
+    cx(0, 1);
+    cx(2, 3);
+    cx(0, 2);
+    cx(1, 3);
+    cx(1, 2);
+    int hold = kEmptyRef;
+    float th = 0.0f;
+#ifdef __clang__
+#pragma unroll
+#endif
+    for (int k = 3; k >= 0; --k) {
+      if (r[k] != kEmptyRef) {
+        if (hold != kEmptyRef) stk.push(hold, th);
+        hold = r[k];
+        th = t[k];
       }
     }
+    if (hold != kEmptyRef)
+      cur = hold;
+    else
+      pop_level(stk);
+  }
+
+  // One binary node (both child boxes in one WNode).  FAST: IEEE min / max
+  // slab (finite 1/d); otherwise the reference's select order.
+  template <bool FAST>
+  AR_HD void visit_binary(const SceneView& sv, Stack& stk) {
+    AR_COUNT_NODE();
+    const WNode* n = sv.nodes + cur;
+    float4 a = n->a, b = n->b, c = n->c;
+    int4 ch = n->d;
+    float tl, tr;
+    bool hl, hr;
+    if (FAST) {
+      hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
+      hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
+    } else {
+      hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
+      hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
+    }
+    if (hl && hr) {
+      bool swap = NEAR && (tr < tl);
+      stk.push(swap ? ch.x : ch.y, swap ? tl : tr);
+      cur = swap ? ch.y : ch.x;
+    } else if (hl) {
+      cur = ch.x;
+    } else if (hr) {
+      cur = ch.y;
+    } else {
+      pop_level(stk);
+    }
+  }
+
+  AR_HD void visit(const SceneView& sv, Stack& stk) {
+    if (NEAR && wide)
+      visit_wide(sv, stk);
+    else if (fast)
+      visit_binary<true>(sv, stk);
+    else
+      visit_binary<false>(sv, stk);
+  }
+
+  // Node phase: visits internal nodes until the lane holds a leaf or
+  // kLevelEnd.  A lane that reaches a triangle leaf first parks it in `pend`
+  // and keeps descending from its next deferred node for as long as
+  // others_searching(searching) reports a lane of its wave still looking
+  // for its first leaf (the wave's lanes then stay busy with node work); a
+  // second leaf or the level's end stops it.  The order in which leaves are
+  // tested does not change the closest hit (equal t: lower leaf position).
+  template <class Others>
+  AR_HD void descend(const SceneView& sv, Stack& stk, Others others_searching) {
+    for (;;) {
+      const bool searching = pend == kLevelEnd && cur >= 0;
+      const bool others = others_searching(searching);
+      if (cur >= 0) {
+        if (pend != kLevelEnd && !others) return;
+        visit(sv, stk);
+      } else if (cur != kLevelEnd && in_blas && pend == kLevelEnd) {
+        pend = cur;
+        pop_level(stk);
+      } else {
+        return;
+      }
+    }
+  }
+  AR_HD void descend(const SceneView& sv, Stack& stk) {
+    descend(sv, stk, [](bool) { return false; });
   }
 
   // Enters the first instance from record k on (same TLAS leaf) whose BLAS
@@ -256,7 +317,8 @@ struct Walker {
           s = sc;
           ik = k;
           base = stk.sp;
-          wide = NEAR && finite_inv(li);
+          fast = finite_inv(li);
+          wide = NEAR && fast;
           cur = wide ? in->qroot : in->root;
           in_blas = true;
           lpos = kNoHit;
@@ -269,10 +331,56 @@ struct Walker {
     }
   }
 
-  // Leaf phase: the lane holds a leaf ref or kLevelEnd.  Returns true when
-  // the query is done; otherwise cur is a node, a leaf or kLevelEnd again.
+  // Tests the triangles of BLAS leaf `ref`.  Any hit: true at the first hit
+  // in range (the query is then done).
+  AR_HD bool tri_leaf(const SceneView& sv, Stack& stk, int ref) {
+    uint32_t k = (uint32_t)~ref;
+    for (;;) {
+      AR_COUNT_TRI();
+      const Tri* tp = sv.tris + k;
+      float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
+      uint32_t tag = bits_u(v0.w);
+      float th, uh, vh;
+      if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tmax, &th, &uh,
+                  &vh)) {
+        if (ANY) {
+          if (th / s < bt) {
+            bpos = k;
+            binst = ik;
+            stk.sp = 0;
+            return true;
+          }
+        } else if (th < tmax || (NEAR && lpos != kNoHit && th == tmax && k < lpos)) {
+          tmax = th;
+          lpos = k;
+          lu = uh;
+          lv = vh;
+        }
+      }
+      if (tag & kLastBit) return false;
+      ++k;
+    }
+  }
+
+  // Leaf phase: a parked leaf if there is one, else the leaf ref or
+  // kLevelEnd the lane holds.  Returns true when the query is done;
+  // otherwise cur is a node or a leaf again (or kLevelEnd after a parked
+  // leaf, handled here too).  Finished instances and the end of the query
+  // are handled in the same call.
   AR_HD bool leaf(const SceneView& sv, Stack& stk) {
-    if (cur == kLevelEnd) {
+    if (pend != kLevelEnd) {
+      const int p = pend;
+      pend = kLevelEnd;
+      if (tri_leaf(sv, stk, p)) return true;
+    } else if (cur != kLevelEnd) {
+      if (in_blas) {
+        if (tri_leaf(sv, stk, cur)) return true;
+      } else if (enter(sv, (uint32_t)~cur, stk)) {
+        return false;
+      }
+      pop_level(stk);
+    }
+    while (cur == kLevelEnd) {
       if (!in_blas) return true;  // TLAS exhausted: query done
       // instance finished: world acceptance of its local best
       if (!ANY && lpos != kNoHit) {
@@ -289,40 +397,13 @@ struct Walker {
       o = wo;
       d = wd;
       inv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
-      wide = NEAR && finite_inv(inv);
+      fast = finite_inv(inv);
+      wide = NEAR && fast;
       tmin = wtmin;
       tmax = bt;
       if (!(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
-    } else if (in_blas) {
-      uint32_t k = (uint32_t)~cur;
-      for (;;) {
-        const Tri* tp = sv.tris + k;
-        float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
-        uint32_t tag = bits_u(v0.w);
-        float th, uh, vh;
-        if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tmax, &th,
-                    &uh, &vh)) {
-          if (ANY) {
-            if (th / s < bt) {
-              bpos = k;
-              binst = ik;
-              stk.sp = 0;
-              return true;
-            }
-          } else if (th < tmax || (NEAR && lpos != kNoHit && th == tmax && k < lpos)) {
-            tmax = th;
-            lpos = k;
-            lu = uh;
-            lv = vh;
-          }
-        }
-        if (tag & kLastBit) break;
-        ++k;
-      }
-    } else if (enter(sv, (uint32_t)~cur, stk)) {
-      return false;
+      pop_level(stk);
     }
-    pop_level(stk);
     return false;
   }
 };
@@ -333,7 +414,7 @@ struct Walker {
 template <bool ANY, bool NEAR>
 AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack& stk) {
   Walker<ANY, NEAR> w;
-  bool done = w.start(sv, r);
+  bool done = w.start(sv, r, stk);
   while (!done) {
     w.descend(sv, stk);
     done = w.leaf(sv, stk);
@@ -376,14 +457,14 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
         uint64_t it = (wave + (p >> 6) * waves) * 64 + (p & 63);
         if (it < n) {
           item = (uint32_t)it;
-          alive = !w.start(sv, src(it));
+          alive = !w.start(sv, src(it), stk);
           if (!alive) sink(it, w);
         }
       }
       next += nidle;
     }
     if (alive) {
-      w.descend(sv, stk);
+      w.descend(sv, stk, [](bool searching) { return __ballot(searching) != 0ull; });
       if (w.leaf(sv, stk)) {
         sink(item, w);
         alive = false;

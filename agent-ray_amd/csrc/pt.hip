@@ -162,7 +162,7 @@ __device__ __forceinline__ Ray make_ray4(float4 o, float4 d) {
 // Lanes take the queue's rays through walk_items, so a wave keeps all its
 // lanes busy until its share of the queue is done.
 template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill,
+__global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill,
                                                      uint32_t refill_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, PtParams P, Wav
 // Shadow rays of this bounce: any hit; an unoccluded ray adds its
 // contribution to its path's radiance (one shadow ray per path per bounce).
 template <int BLOCK, int S>
-__global__ __launch_bounds__(BLOCK) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill,
+__global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill,
                                                      uint32_t refill_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;

@@ -28,7 +28,7 @@ __device__ __forceinline__ Ray load_ray(const ar_ray* rays, uint64_t i) {
 }
 
 template <bool ANY, bool NEAR, int S, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace(SceneView sv, const ar_ray* __restrict__ rays, uint64_t n,
+__global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_trace(SceneView sv, const ar_ray* __restrict__ rays, uint64_t n,
                                                  ar_hit* __restrict__ hits, uint32_t* __restrict__ inst_ids,
                                                  uint32_t* __restrict__ occ, int2* __restrict__ spill,
                                                  uint32_t refill_min) {
