@@ -262,19 +262,16 @@ struct Walker {
   }
 
   // Node phase: visits internal nodes until the lane holds a leaf or
-  // kLevelEnd.  A lane that reaches a triangle leaf first parks it in `pend`
-  // and keeps descending from its next deferred node for as long as
-  // others_searching(searching) reports a lane of its wave still looking
-  // for its first leaf (the wave's lanes then stay busy with node work); a
-  // second leaf or the level's end stops it.  The order in which leaves are
-  // tested does not change the closest hit (equal t: lower leaf position).
-  template <class Others>
-  AR_HD void descend(const SceneView& sv, Stack& stk, Others others_searching) {
+  // kLevelEnd.  The first triangle leaf reached is parked in `pend` and the
+  // lane keeps descending from its next deferred node until a second leaf
+  // or the level's end, so a wave's lanes spend more of the phase on node
+  // work and less waiting for each other.  The rule depends on the ray
+  // alone, so every schedule and launch configuration visits the same
+  // sequence for a ray (equal-t ties resolve identically); the order in
+  // which leaves are tested does not change the closest hit.
+  AR_HD void descend(const SceneView& sv, Stack& stk) {
     for (;;) {
-      const bool searching = pend == kLevelEnd && cur >= 0;
-      const bool others = others_searching(searching);
       if (cur >= 0) {
-        if (pend != kLevelEnd && !others) return;
         visit(sv, stk);
       } else if (cur != kLevelEnd && in_blas && pend == kLevelEnd) {
         pend = cur;
@@ -283,9 +280,6 @@ struct Walker {
         return;
       }
     }
-  }
-  AR_HD void descend(const SceneView& sv, Stack& stk) {
-    descend(sv, stk, [](bool) { return false; });
   }
 
   // Enters the first instance from record k on (same TLAS leaf) whose BLAS
@@ -464,7 +458,7 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
       next += nidle;
     }
     if (alive) {
-      w.descend(sv, stk, [](bool searching) { return __ballot(searching) != 0ull; });
+      w.descend(sv, stk);
       if (w.leaf(sv, stk)) {
         sink(item, w);
         alive = false;

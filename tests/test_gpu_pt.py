@@ -137,3 +137,11 @@ def test_room_scene_schedules_and_oracle(ar, orc, wl):
     rel = np.abs(a - o) / np.maximum(1.0, np.abs(o))
     print(f"  room: frac(rel>1e-3) {(rel > 1e-3).mean():.4%}  gpu mean {a.mean():.5f} cpu mean {o.mean():.5f}")
     assert float((rel > 1e-3).mean()) < 0.03
+    # larger render: 2M paths, millions of extension and shadow rays through
+    # the wave-refill traversal vs one ray per lane; films bitwise equal
+    cam2 = ar.PinholeCamera.from_array(wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup,
+                                                        desc.camera.vfov, 480 / 270))
+    c, sc_ = render(ar, sc, cam2, 480, 270, 16, depth=16, schedule=0, seed=9)
+    d, sd_ = render(ar, sc, cam2, 480, 270, 16, depth=16, schedule=1, seed=9)
+    assert sc_.rays == sd_.rays
+    assert np.array_equal(c.view(np.uint32), d.view(np.uint32))

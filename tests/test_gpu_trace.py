@@ -137,6 +137,26 @@ def test_empty_batch_and_device_pointers(ar, wl):
     assert np.array_equal(occ.cpu().numpy().astype(bool), sc.occluded(rays))
 
 
+def test_room_launch_variants_same_answer(ar, wl):
+    """Multi-instance scene: the traversal order of a ray does not depend on
+    the launch (wave composition, refill threshold), so every variant returns
+    the same hits, equal-t ties across instances included."""
+    desc = wl.procedural_room(n_boxes=30)
+    rng = np.random.default_rng(71)
+    n = 1 << 16
+    o = rng.uniform((0.1, 0.1, 0.1), (7.9, 3.9, 7.9), (n, 3)).astype(np.float32)
+    d = wl.normalize_f32(rng.standard_normal((n, 3)).astype(np.float32))
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 3], rays[:, 4:7], rays[:, 7] = o, 1e-4, d, wl.F32_MAX
+    sc = ar.Scene.from_desc(desc)
+    ref_t, ref_p, ref_i = sc.intersect(rays)
+    for (b, s, pc, rf) in [(64, 8, 4, 1), (256, 8, 8, 64), (128, 16, 16, 8)]:
+        sc.set_trace_config(b, s, pc, rf)
+        t, p, i = sc.intersect(rays)
+        assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)), (b, s, pc, rf)
+        assert np.array_equal(p, ref_p) and np.array_equal(i, ref_i), (b, s, pc, rf)
+
+
 def test_every_launch_variant_same_answer(ar, wl):
     """All (workgroup, LDS stack, blocks/CU) variants, including stacks that
     spill to HBM, give identical hits; the autotuner picks one of them."""
