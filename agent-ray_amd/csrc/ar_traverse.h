@@ -96,6 +96,14 @@ struct WorldHit {
 // ---------------------------------------------------------------------------
 constexpr int kLevelEnd = (int)0x80000000;  // cur: the current level's stack is exhausted
 
+// Near-first closest hit culls boxes, pops and triangle ranges against
+// best * kCullSlack rather than best: a hit tied with the final best is then
+// reached whatever the visiting order (the slack covers the few-ulp gap
+// between a box's entry distance, a triangle's t and t * s / s), and ties
+// are broken by (instance record, leaf position).  The answer is therefore
+// independent of the order, which depends on the wave a ray shares.
+constexpr float kCullSlack = 1.0f + 0x1p-18f;
+
 template <bool ANY, bool NEAR>
 struct Walker {
   f3 wo, wd;              // world ray
@@ -115,6 +123,9 @@ struct Walker {
   bool in_blas;
   bool fast;              // 1/d of this level is finite: IEEE min / max slab
   bool wide;              // this level walks the 4-wide tree (near-first, fast)
+
+  // Bound for culling and for the triangle range (see kCullSlack).
+  AR_HD float tcull() const { return (NEAR && !ANY) ? tmax * kCullSlack : tmax; }
 
   // Starts a query on an empty stack.  Returns true when it is already
   // finished (the TLAS root box is missed, or the TLAS is one leaf whose
@@ -155,7 +166,7 @@ struct Walker {
         return;
       }
       int2 e = stk.pop();
-      if (bits_f(e.y) <= tmax) {
+      if (bits_f(e.y) <= tcull()) {
         cur = e.x;
         return;
       }
@@ -173,6 +184,7 @@ struct Walker {
     const uint32_t hx = bits_u(c.y), hy = bits_u(c.z), hz = bits_u(c.w);
     float t[4];
     int r[4] = {rf.x, rf.y, rf.z, rf.w};
+    const float tc = tcull();
 #ifdef __clang__
 #pragma unroll
 #endif
@@ -183,7 +195,7 @@ struct Walker {
       bool h = slab_fast(fmaf((float)((lx >> sh) & 255u), a.w, a.x), fmaf((float)((ly >> sh) & 255u), b.x, a.y),
                          fmaf((float)((lz >> sh) & 255u), b.y, a.z), fmaf((float)((hx >> sh) & 255u), a.w, a.x),
                          fmaf((float)((hy >> sh) & 255u), b.x, a.y), fmaf((float)((hz >> sh) & 255u), b.y, a.z), o,
-                         inv, tmin, tmax, &tn);
+                         inv, tmin, tc, &tn);
       h = h && r[k] != kEmptyRef;
       t[k] = h ? tn : INFINITY;
       r[k] = h ? r[k] : kEmptyRef;
@@ -232,12 +244,13 @@ struct Walker {
     int4 ch = n->d;
     float tl, tr;
     bool hl, hr;
+    const float tc = tcull();
     if (FAST) {
-      hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
-      hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
+      hl = slab_fast(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tc, &tl);
+      hr = slab_fast(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tc, &tr);
     } else {
-      hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tmax, &tl);
-      hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tmax, &tr);
+      hl = slab(a.x, a.y, a.z, a.w, b.x, b.y, o, inv, tmin, tc, &tl);
+      hr = slab(b.z, b.w, c.x, c.y, c.z, c.w, o, inv, tmin, tc, &tr);
     }
     if (hl && hr) {
       bool swap = NEAR && (tr < tl);
@@ -262,16 +275,19 @@ struct Walker {
   }
 
   // Node phase: visits internal nodes until the lane holds a leaf or
-  // kLevelEnd.  The first triangle leaf reached is parked in `pend` and the
-  // lane keeps descending from its next deferred node until a second leaf
-  // or the level's end, so a wave's lanes spend more of the phase on node
-  // work and less waiting for each other.  The rule depends on the ray
-  // alone, so every schedule and launch configuration visits the same
-  // sequence for a ray (equal-t ties resolve identically); the order in
-  // which leaves are tested does not change the closest hit.
-  AR_HD void descend(const SceneView& sv, Stack& stk) {
+  // kLevelEnd.  A lane that reaches a triangle leaf first parks it in `pend`
+  // and keeps descending from its next deferred node while
+  // others_searching(searching) reports a lane of its wave still looking
+  // for its first leaf, so the wave's lanes stay busy with node work; a
+  // second leaf or the level's end stops it.  The visiting order then
+  // depends on the wave, and the answer does not (kCullSlack).
+  template <class Others>
+  AR_HD void descend(const SceneView& sv, Stack& stk, Others others_searching) {
     for (;;) {
+      const bool searching = pend == kLevelEnd && cur >= 0;
+      const bool others = others_searching(searching);
       if (cur >= 0) {
+        if (pend != kLevelEnd && !others) return;
         visit(sv, stk);
       } else if (cur != kLevelEnd && in_blas && pend == kLevelEnd) {
         pend = cur;
@@ -280,6 +296,9 @@ struct Walker {
         return;
       }
     }
+  }
+  AR_HD void descend(const SceneView& sv, Stack& stk) {
+    descend(sv, stk, [](bool) { return false; });
   }
 
   // Enters the first instance from record k on (same TLAS leaf) whose BLAS
@@ -335,8 +354,8 @@ struct Walker {
       float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
       uint32_t tag = bits_u(v0.w);
       float th, uh, vh;
-      if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tmax, &th, &uh,
-                  &vh)) {
+      if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tcull(), &th,
+                  &uh, &vh)) {
         if (ANY) {
           if (th / s < bt) {
             bpos = k;
@@ -344,7 +363,18 @@ struct Walker {
             stk.sp = 0;
             return true;
           }
-        } else if (th < tmax || (NEAR && lpos != kNoHit && th == tmax && k < lpos)) {
+        } else if (NEAR) {
+          // world-space comparison right away, ties by (instance, leaf position)
+          float tw = th / s;
+          if (tw < bt || (bpos != kNoHit && tw == bt && (ik < binst || (ik == binst && k < bpos)))) {
+            bt = tw;
+            bpos = k;
+            binst = ik;
+            bu = uh;
+            bv = vh;
+            tmax = bt * s;
+          }
+        } else if (th < tmax) {
           tmax = th;
           lpos = k;
           lu = uh;
@@ -376,10 +406,11 @@ struct Walker {
     }
     while (cur == kLevelEnd) {
       if (!in_blas) return true;  // TLAS exhausted: query done
-      // instance finished: world acceptance of its local best
-      if (!ANY && lpos != kNoHit) {
+      // instance finished: world acceptance of its local best (reference
+      // order; near-first order has compared every hit in world space)
+      if (!ANY && !NEAR && lpos != kNoHit) {
         float tw = tmax / s;
-        if (tw < bt || (NEAR && bpos != kNoHit && tw == bt && ik < binst)) {
+        if (tw < bt) {
           bt = tw;
           bpos = lpos;
           binst = ik;
@@ -458,7 +489,7 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
       next += nidle;
     }
     if (alive) {
-      w.descend(sv, stk);
+      w.descend(sv, stk, [](bool searching) { return __ballot(searching) != 0ull; });
       if (w.leaf(sv, stk)) {
         sink(item, w);
         alive = false;

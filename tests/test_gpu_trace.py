@@ -119,6 +119,22 @@ def test_axis_rays_on_box_planes(ar, orc, wl):
     _check_scene(ar, orc, desc, _plane_rays(wl, desc, 1 << 12, seed=9), max_tie_frac=0.0)
 
 
+def test_duplicate_geometry_ties(ar, orc, wl):
+    """Every hit is a tie: each triangle appears twice in its mesh and the
+    mesh is instanced twice with the same transform.  The reference keeps the
+    first one its DFS meets (lowest instance record, then lowest leaf
+    position); near-first order must pick the same one although its visiting
+    order depends on the wave a ray shares."""
+    base = wl.triangle_soup(1 << 11, seed=12)
+    m = base.meshes[0]
+    idx = m.indices.reshape(-1, 3)
+    dup = wl.MeshDesc(positions=m.positions, indices=np.concatenate([idx, idx[::-1]]).astype(np.uint32),
+                      materials=m.materials, slots=m.slots, normals=None, transform=m.transform)
+    desc = wl.SceneDesc(materials=base.materials, meshes=[dup, dup], lights=base.lights, camera=base.camera)
+    rays = wl.soup_rays(1 << 15, seed=13, image=128)
+    _check_scene(ar, orc, desc, rays, max_tie_frac=0.0)
+
+
 def test_empty_batch_and_device_pointers(ar, wl):
     import torch
     desc = wl.triangle_soup(4096, seed=51)
