@@ -5,11 +5,14 @@
 // as soon as a slot is free and the HBM budget covers the job's footprint.
 // Every slot owns a HIP stream, a grow-only workspace (path state, queues,
 // film, stack spill) and a pinned output buffer.  One scheduler thread issues
-// the admitted jobs' chunks round-robin, keeping up to two chunks in flight
-// per job, so the chunks of different jobs run concurrently on the device
-// and new arrivals start while others are mid-render.  A job's film depends
-// only on its own description (per-job seed, deterministic accumulation),
-// never on what it shared the device with.
+// chunks of the admitted jobs, shortest remaining work first, with at most
+// kInflight chunks on the device (two per job): a job that arrives while a
+// long one renders waits only for the chunks already issued, then runs
+// ahead of it, and stream priorities are not needed.  Server jobs without an
+// explicit sample_chunk get chunks of about 2^20 path-equivalents so those
+// waits stay short.  A job's film depends only on its own description
+// (per-job seed, deterministic accumulation in sample order, independent of
+// chunking), never on what it shared the device with.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -21,6 +24,15 @@
 #include "ar_job.h"
 
 namespace ar {
+
+constexpr int kInflight = 3;                 // chunks on the device, all jobs together
+constexpr uint64_t kServerChunkPaths = 1ull << 20;
+
+// Relative cost of one path (or BDPT sample) of each integrator, used only
+// to order jobs by remaining work.
+static double path_weight(uint32_t integrator) {
+  return integrator == AR_INTEGRATOR_BDPT ? 4.0 : (integrator == AR_INTEGRATOR_GDPT ? 3.0 : 1.0);
+}
 
 struct Slot {
   hipStream_t stream = nullptr;
@@ -41,6 +53,8 @@ struct Job {
   int slot = -1;
   bool cancel = false;
   bool issued_all = false;   // every chunk + resolve + copy-out enqueued
+  double work_left = 0;      // weighted paths not yet issued
+  double work_chunk = 0;     // weighted paths per chunk
   std::unique_ptr<JobRun> run;
   std::deque<hipEvent_t> inflight;
   std::vector<float> film;
@@ -140,6 +154,15 @@ struct ar_server {
         }
         sl.pinned_bytes = out_bytes;
       }
+      if (j.spec.cfg.sample_chunk == 0) {
+        uint64_t per_sample = npix * (j.spec.cfg.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
+        uint64_t target = j.spec.cfg.integrator == AR_INTEGRATOR_BDPT ? kServerChunkPaths / 4 : kServerChunkPaths;
+        j.spec.cfg.sample_chunk = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(j.spec.cfg.spp, target / std::max<uint64_t>(per_sample, 1)));
+      }
+      const double w = path_weight(j.spec.cfg.integrator) * (double)npix;
+      j.work_left = w * j.spec.cfg.spp;
+      j.work_chunk = w * chunk_samples(j.spec);
       sl.busy = true;
       j.slot = free_slot;
       used_bytes += j.footprint;
@@ -157,7 +180,8 @@ struct ar_server {
     }
   }
 
-  // One pass over the active jobs: retire finished chunks, issue new ones.
+  // One pass: retire finished chunks and complete drained jobs, then issue
+  // chunks, shortest remaining work first, up to kInflight on the device.
   // Returns true if any progress was made.  Called without mu held; takes it
   // only for bookkeeping.
   bool pump() {
@@ -167,6 +191,7 @@ struct ar_server {
       act = active;
     }
     bool progress = false;
+    int inflight = 0;
     for (auto& jp : act) {
       Job& j = *jp;
       while (!j.inflight.empty() && hipEventQuery(j.inflight.front()) == hipSuccess) {
@@ -180,7 +205,10 @@ struct ar_server {
         cancel = j.cancel;
       }
       if (j.issued_all || cancel) {
-        if (!j.inflight.empty()) continue;
+        if (!j.inflight.empty()) {
+          inflight += (int)j.inflight.size();
+          continue;
+        }
         // drained: complete the job
         std::lock_guard<std::mutex> lk(mu);
         if (cancel && !j.issued_all) {
@@ -201,9 +229,23 @@ struct ar_server {
         progress = true;
         continue;
       }
-      if (j.inflight.size() >= 2) continue;
+      inflight += (int)j.inflight.size();
+    }
+    while (inflight < kInflight) {
+      std::shared_ptr<Job> pick;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& jp : active) {
+          Job& j = *jp;
+          if (j.issued_all || j.cancel || j.inflight.size() >= 2) continue;
+          if (!pick || j.work_left < pick->work_left) pick = jp;
+        }
+      }
+      if (!pick) break;
+      Job& j = *pick;
       bool done = false;
       ar_status e = j.run->step(&done);
+      j.work_left = std::max(0.0, j.work_left - j.work_chunk);
       if (e == AR_OK && done) {
         e = j.run->finish();
         if (e == AR_OK) {
@@ -218,16 +260,19 @@ struct ar_server {
         (void)hipStreamSynchronize(slots[j.slot]->stream);
         std::lock_guard<std::mutex> lk(mu);
         for (auto ev : j.inflight) event_pool.push_back(ev);
+        inflight -= (int)j.inflight.size();
         j.inflight.clear();
         fail_job(j, e, ar_last_error());
         release(j);
-        active.erase(std::find(active.begin(), active.end(), jp));
+        active.erase(std::find(active.begin(), active.end(), pick));
         cv_done.notify_all();
+        progress = true;
         continue;
       }
       hipEvent_t ev = get_event();
       (void)hipEventRecord(ev, slots[j.slot]->stream);
       j.inflight.push_back(ev);
+      ++inflight;
       progress = true;
     }
     return progress;
