@@ -349,14 +349,14 @@ def bench_serving(args, ar, torch):
     side U{64..1024 step 64}, spp round(exp(U[0, ln 256])), PT 60% / BDPT 25% /
     GDPT 15%, seed = job index.
 
-    1. Capacity: the solo render time of an evenly spaced sample of the job
-       mix at the specified rate gives the offered load rho = rate x mean solo
-       time.  Above 1 no server keeps up, so the queue and p99 grow without
-       bound over the run.
-    2. Serving: Poisson arrivals at min(rate, load / mean solo) for
-       `serve_seconds`; every job's latency (server clock, submit to film
-       ready) and its solo render time (rendered alone afterwards) give p50 /
-       p99 latency and p50 / p99 of latency / solo."""
+    1. Capacity: the first `serve_jobs` jobs of the mix are rendered alone;
+       their mean solo time gives the offered load at the specified rate,
+       rho = rate x mean solo time.  Above 1 no server keeps up and the
+       queue and p99 grow without bound over the run.
+    2. Serving: the same jobs arrive as a Poisson stream whose window makes
+       the offered load exactly `load` (sum of solo times / window); each
+       job's latency (server clock, submit to film ready) over its solo time
+       gives p50 / p99 of latency / solo."""
     wl = ar.workloads
     desc = wl.procedural_room()
     sc = ar.Scene.from_desc(desc)
@@ -381,27 +381,35 @@ def bench_serving(args, ar, torch):
 
     spec_jobs = wl.serving_jobs(rate=args.rate, duration=args.duration, seed=0)
     solo(spec_jobs[0])  # warm-up (workspace, code objects)
-    picks = spec_jobs[:: max(1, len(spec_jobs) // 32)]
-    solo_sample = [solo(j) for j in picks]
-    mean_solo = float(np.mean(solo_sample))
+    jobs = spec_jobs[: args.serve_jobs]
+    solos = []
+    for k, j in enumerate(jobs):
+        solos.append(solo(j))
+        if k % 20 == 19:
+            log(f"serving: solo {k + 1}/{len(jobs)}")
+    solos = np.asarray(solos)
+    mean_solo = float(solos.mean())
     rho_spec = args.rate * mean_solo
-    log(f"serving: {len(spec_jobs)} jobs at {args.rate}/s specified; mean solo {mean_solo:.3f}s over "
-        f"{len(picks)} jobs -> offered load {rho_spec:.2f}")
-    rate = min(args.rate, args.load / mean_solo)
-    jobs = wl.serving_jobs(rate=rate, duration=args.serve_seconds, seed=1)
+    # arrival window for the target load: sum of solo times / window = load
+    window = float(solos.sum()) / args.load
+    rate = len(jobs) / window
+    rng = np.random.default_rng(1)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
+    log(f"serving: {len(spec_jobs)} jobs at {args.rate}/s specified; mean solo {mean_solo:.3f}s -> offered load "
+        f"{rho_spec:.2f}; serving the first {len(jobs)} at {rate:.3f}/s (load {args.load})")
     srv = ar.RenderServer(sc, hbm_budget_bytes=200 << 30, max_active_jobs=args.slots)
     warm = [srv.submit(integ(j), cam(j), j.width, j.height) for j in jobs[: args.slots]]
     for w in warm:
         srv.wait(w)
     t0 = time.perf_counter()
     ids, last_log = [], 0.0
-    for j in jobs:
+    for j, at in zip(jobs, arrivals):
         now = time.perf_counter() - t0
-        if j.arrival > now:
-            time.sleep(j.arrival - now)
+        if at > now:
+            time.sleep(at - now)
         ids.append((j, srv.submit(integ(j), cam(j), j.width, j.height)))
         if now - last_log > 10.0:
-            log(f"serving: {len(ids)}/{len(jobs)} jobs submitted at t={now:.1f}s (rate {rate:.2f}/s)")
+            log(f"serving: {len(ids)}/{len(jobs)} jobs submitted at t={now:.1f}s")
             last_log = now
     lat, paths = [], 0
     for j, jid in ids:
@@ -410,18 +418,14 @@ def bench_serving(args, ar, torch):
         paths += st["paths"]
     wall = time.perf_counter() - t0
     srv.close()
-    log(f"serving: {len(ids)} jobs done in {wall:.1f}s; rendering each alone for the latency / solo ratios")
-    solos = []
-    for k, (j, _) in enumerate(ids):
-        solos.append(solo(j))
-        if k % 20 == 19:
-            log(f"serving: solo {k + 1}/{len(ids)}")
-    lat, solos = np.asarray(lat), np.asarray(solos)
+    lat = np.asarray(lat)
     ratio = lat / np.maximum(solos, 1e-6)
-    return {"jobs": len(jobs), "rate_per_s": round(rate, 3), "serve_seconds": args.serve_seconds,
+    return {"jobs": len(jobs), "rate_per_s": round(rate, 3), "arrival_window_s": round(window, 2),
             "slots": args.slots, "specified_rate_per_s": args.rate, "specified_jobs": len(spec_jobs),
             "mean_solo_s": round(mean_solo, 4), "offered_load_at_specified_rate": round(rho_spec, 2),
-            "offered_load": round(float(solos.sum()) / args.serve_seconds, 3),
+            "offered_load": args.load,
+            "job_mix": {"pt": sum(j.integrator == 0 for j in jobs), "bdpt": sum(j.integrator == 1 for j in jobs),
+                        "gdpt": sum(j.integrator == 2 for j in jobs)},
             "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
             "p50_latency_over_solo": round(float(np.percentile(ratio, 50)), 3),
@@ -447,7 +451,7 @@ def main():
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0, help="specified arrival window (s)")
     ap.add_argument("--load", type=float, default=0.7, help="serving: target offered load when the rate is above it")
-    ap.add_argument("--serve-seconds", type=float, default=45.0, help="serving: arrival window actually run (s)")
+    ap.add_argument("--serve-jobs", type=int, default=60, help="serving: jobs of the specified mix actually served")
     ap.add_argument("--slots", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
