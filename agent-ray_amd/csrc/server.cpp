@@ -9,8 +9,9 @@
 // kInflight chunks on the device (two per job): a job that arrives while a
 // long one renders waits only for the chunks already issued, then runs
 // ahead of it, and stream priorities are not needed.  Server jobs without an
-// explicit sample_chunk get chunks of about 2^20 path-equivalents so those
-// waits stay short.  A job's film depends only on its own description
+// explicit sample_chunk run in one chunk when small and in chunks of 2^20
+// path-equivalents (2^16 BDPT samples) otherwise, so those waits stay short.
+// A job's film depends only on its own description
 // (per-job seed, deterministic accumulation in sample order, independent of
 // chunking), never on what it shared the device with.
 #include <algorithm>
@@ -25,8 +26,9 @@
 
 namespace ar {
 
-constexpr int kInflight = 3;                 // chunks on the device, all jobs together
-constexpr uint64_t kServerChunkPaths = 1ull << 20;
+constexpr int kInflight = 2;                 // chunks on the device, all jobs together
+constexpr uint64_t kServerChunkPaths = 1ull << 20;   // PT / GDPT path-equivalents per chunk
+constexpr uint64_t kServerChunkBdpt = 1ull << 16;    // BDPT samples per chunk
 
 // Relative cost of one path (or BDPT sample) of each integrator, used only
 // to order jobs by remaining work.
@@ -155,10 +157,15 @@ struct ar_server {
         sl.pinned_bytes = out_bytes;
       }
       if (j.spec.cfg.sample_chunk == 0) {
+        // a job of up to two chunks' work runs as one chunk; larger jobs are
+        // split so that the device never holds much of them at a time
         uint64_t per_sample = npix * (j.spec.cfg.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
-        uint64_t target = j.spec.cfg.integrator == AR_INTEGRATOR_BDPT ? kServerChunkPaths / 4 : kServerChunkPaths;
-        j.spec.cfg.sample_chunk = (uint32_t)std::max<uint64_t>(
-            1, std::min<uint64_t>(j.spec.cfg.spp, target / std::max<uint64_t>(per_sample, 1)));
+        uint64_t target = j.spec.cfg.integrator == AR_INTEGRATOR_BDPT ? kServerChunkBdpt : kServerChunkPaths;
+        if (per_sample * j.spec.cfg.spp <= 2 * target)
+          j.spec.cfg.sample_chunk = j.spec.cfg.spp;
+        else
+          j.spec.cfg.sample_chunk = (uint32_t)std::max<uint64_t>(
+              1, std::min<uint64_t>(j.spec.cfg.spp, target / std::max<uint64_t>(per_sample, 1)));
       }
       const double w = path_weight(j.spec.cfg.integrator) * (double)npix;
       j.work_left = w * j.spec.cfg.spp;

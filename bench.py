@@ -420,6 +420,10 @@ def bench_serving(args, ar, torch):
     srv.close()
     lat = np.asarray(lat)
     ratio = lat / np.maximum(solos, 1e-6)
+    for k in np.argsort(-ratio)[:8]:
+        j = jobs[k]
+        log(f"serving: job {j.index} {['pt', 'bdpt', 'gdpt'][j.integrator]} {j.width}^2 x {j.spp} spp: solo "
+            f"{solos[k] * 1e3:.1f} ms, latency {lat[k] * 1e3:.1f} ms ({ratio[k]:.1f}x)")
     return {"jobs": len(jobs), "rate_per_s": round(rate, 3), "arrival_window_s": round(window, 2),
             "slots": args.slots, "specified_rate_per_s": args.rate, "specified_jobs": len(spec_jobs),
             "mean_solo_s": round(mean_solo, 4), "offered_load_at_specified_rate": round(rho_spec, 2),
