@@ -9,7 +9,11 @@ uint32_t chunk_samples(const JobSpec& spec) {
   const ar_render_config& c = spec.cfg;
   uint64_t npix = (uint64_t)spec.width * spec.height;
   uint64_t per_sample = npix * (c.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
-  uint64_t target = c.integrator == AR_INTEGRATOR_BDPT ? (1ull << 20) : (1ull << 22);
+  // Paths per chunk: large, because every chunk pays the tail of its last
+  // bounces (a handful of long paths whose traversal is one dependent chain
+  // of loads) and HBM has room: 2^25 paths of wavefront state is ~6.6 GB,
+  // 2^22 BDPT samples of vertices ~4 GB at depth 10.
+  uint64_t target = c.integrator == AR_INTEGRATOR_BDPT ? (1ull << 22) : (1ull << 25);
   uint32_t k = c.sample_chunk;
   if (k == 0) k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.spp, target / std::max<uint64_t>(per_sample, 1)));
   return std::min(k, c.spp);

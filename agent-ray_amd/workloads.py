@@ -95,6 +95,21 @@ class SceneDesc:
         n = sum(int(m.indices.shape[0]) for m in self.meshes)
         return n + 2 * sum(1 for l in self.lights if l.type == LIGHT_QUAD)
 
+    def bounds(self, shrink: float = 0.01):
+        """World AABB of all meshes (transforms applied), shrunk by `shrink`
+        of its extent on every side."""
+        lo = np.full(3, np.inf)
+        hi = np.full(3, -np.inf)
+        for m in self.meshes:
+            p = m.positions.astype(np.float64)
+            if m.transform is not None:
+                t = np.asarray(m.transform, np.float64).reshape(4, 4).T  # column-major storage
+                p = p @ t[:3, :3].T + t[:3, 3]
+            lo = np.minimum(lo, p.min(0))
+            hi = np.maximum(hi, p.max(0))
+        e = (hi - lo) * shrink
+        return (lo + e).astype(np.float32), (hi - e).astype(np.float32)
+
 
 # ---------------------------------------------------------------------------
 # fp32 helpers with the reference's operation order

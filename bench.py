@@ -323,6 +323,20 @@ def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
                                                        desc.width / desc.height))
     out = {"width": desc.width, "height": desc.height, "spp": desc.spp, "max_depth": desc.max_depth,
            "triangles": desc.n_triangles, "scene_build_s": round(build, 2)}
+    # per-scene traversal autotuning on 2^20 rays: camera rays and rays from
+    # random interior points (what the extension and shadow stages trace)
+    rng = np.random.default_rng(3)
+    m = 1 << 19
+    cam16 = wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, desc.width / desc.height)
+    cr = wl.camera_rays(cam16, rng.uniform(0, 1, m), rng.uniform(0, 1, m))
+    lo, hi = desc.bounds()
+    o = rng.uniform(lo, hi, (m, 3)).astype(np.float32)
+    d = wl.normalize_f32(rng.standard_normal((m, 3)).astype(np.float32))
+    rr = np.zeros((m, 8), np.float32)
+    rr[:, 0:3], rr[:, 3], rr[:, 4:7], rr[:, 7] = o, 1e-4, d, wl.F32_MAX
+    sample = torch.from_numpy(np.concatenate([cr, rr]).astype(np.float32)).cuda()
+    out["launch_config"] = sc.autotune(sample)
+    del sample
     films = {}
     for sched in schedules:
         pt = ar.PathTracer(ar.PathTracerConfig(spp=desc.spp, max_depth=desc.max_depth, rr_depth=3), seed=0,
@@ -442,7 +456,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "copy", "pmc-probe"],
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "pt", "copy", "pmc-probe"],
                     help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--trace-config", default=None,
@@ -468,6 +482,10 @@ def main():
         sys.exit(1)
     ar = load_pkg()
     ar.init(0)
+    if args.workload == "pt":
+        print(json.dumps({"pt_cornell_256x256x16spp": bench_pt(ar, torch, "cornell"),
+                          "pt_room_1920x1080x64spp": bench_pt(ar, torch, "room")}), flush=True)
+        return
     if args.workload == "copy":
         print(json.dumps(bench_copy(args, torch)), flush=True)
         return
