@@ -452,6 +452,7 @@ AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack&
   return w.bpos != kNoHit;
 }
 
+#ifdef __HIP__  // wave intrinsics: HIP translation units only
 // Items [0, n) are split in chunks of 64; wave `wave` of `waves` owns chunks
 // wave, wave + waves, ... and walks their items in order.  A lane that
 // finishes takes the next unassigned item of its wave; idle lanes are
@@ -497,5 +498,7 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
     }
   }
 }
+
+#endif  // __HIP__
 
 }  // namespace ar

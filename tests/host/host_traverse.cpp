@@ -1,0 +1,122 @@
+// Test helper (built by tests/test_host_traverse.py with g++, never shipped):
+// runs the library's traversal code -- ar_traverse.h, compiled for the host
+// -- over host copies of a scene's trees, so the CPU suite can compare it
+// with the oracle ray by ray.  Two drivers per query type:
+//   * trace_scene, one ray after another (no leaf parking across lanes);
+//   * an emulation of walk_items for 64-lane waves with wave-local refill
+//     and maximal leaf parking, which checks that every ray is started once
+//     and that the visiting order does not change the answer.
+#include <cstdio>
+#include <vector>
+
+#include "../../agent-ray_amd/csrc/ar_internal.h"
+#include "../../agent-ray_amd/csrc/ar_traverse.h"
+
+using namespace ar;
+
+namespace {
+
+Ray ray_at(const float* rays, uint64_t i) {
+  const float* q = rays + 8 * i;
+  Ray r;
+  r.o = mk(q[0], q[1], q[2]);
+  r.tmin = q[3];
+  r.d = mk(q[4], q[5], q[6]);
+  r.tmax = q[7];
+  return r;
+}
+
+template <bool ANY, bool NEAR>
+void record(const Walker<ANY, NEAR>& w, const SceneView& sv, uint64_t i, float* t, uint32_t* prim, uint32_t* inst) {
+  if (ANY) {
+    t[i] = w.bpos != kNoHit ? 1.0f : 0.0f;
+    prim[i] = inst[i] = 0;
+    return;
+  }
+  t[i] = w.bpos != kNoHit ? w.bt : INFINITY;
+  prim[i] = w.bpos != kNoHit ? (bits_u(sv.tris[w.bpos].v0.w) & ~kLastBit) : kNoHit;
+  inst[i] = w.bpos != kNoHit ? (sv.insts[w.binst].id & ~kLastBit) : kNoHit;
+}
+
+template <bool ANY, bool NEAR>
+int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, float* t0, uint32_t* p0, uint32_t* i0,
+        float* t1, uint32_t* p1, uint32_t* i1) {
+  std::vector<int2> lds(64 * 64), spill(64 * 512);
+  for (uint64_t i = 0; i < n; ++i) {
+    Stack stk{lds.data(), spill.data(), 1, 1, S, 0};
+    Walker<ANY, NEAR> w;
+    bool done = w.start(sv, ray_at(rays, i), stk);
+    while (!done) {
+      w.descend(sv, stk);
+      done = w.leaf(sv, stk);
+    }
+    if (stk.sp != 0) return -1;
+    record(w, sv, i, t0, p0, i0);
+  }
+  std::vector<uint8_t> seen(n, 0);
+  for (int wave = 0; wave < waves; ++wave) {
+    Walker<ANY, NEAR> w[64];
+    Stack stk[64];
+    bool alive[64] = {};
+    uint64_t item[64] = {};
+    for (int l = 0; l < 64; ++l) stk[l] = Stack{lds.data() + 64 * l, spill.data() + 512 * l, 1, 1, S, 0};
+    uint64_t next = 0;
+    const uint32_t refill_min = 16;
+    for (;;) {
+      uint64_t idle = 0;
+      for (int l = 0; l < 64; ++l)
+        if (!alive[l]) idle |= 1ull << l;
+      uint32_t nidle = (uint32_t)__builtin_popcountll(idle);
+      uint64_t first = ((uint64_t)wave + (next >> 6) * waves) * 64 + (next & 63);
+      bool more = first < n;
+      if (!more && nidle == 64) break;
+      if (more && nidle >= refill_min) {
+        for (int l = 0; l < 64; ++l) {
+          if (alive[l]) continue;
+          uint64_t below = l == 0 ? 0ull : ((~0ull) >> (64 - l));
+          uint64_t p = next + (uint64_t)__builtin_popcountll(idle & below);
+          uint64_t it = ((uint64_t)wave + (p >> 6) * waves) * 64 + (p & 63);
+          if (it >= n) continue;
+          if (seen[it]++) return -2;
+          item[l] = it;
+          alive[l] = !w[l].start(sv, ray_at(rays, it), stk[l]);
+          if (!alive[l]) record(w[l], sv, it, t1, p1, i1);
+        }
+        next += nidle;
+      }
+      for (int l = 0; l < 64; ++l) {
+        if (!alive[l]) continue;
+        w[l].descend(sv, stk[l], [](bool) { return true; });  // park whenever possible
+        if (w[l].leaf(sv, stk[l])) {
+          if (stk[l].sp != 0) return -3;
+          record(w[l], sv, item[l], t1, p1, i1);
+          alive[l] = false;
+        }
+      }
+    }
+  }
+  for (uint64_t i = 0; i < n; ++i)
+    if (seen[i] != 1) return -4;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any, int near, int lds_stack,
+                             float* t0, uint32_t* p0, uint32_t* i0, float* t1, uint32_t* p1, uint32_t* i1) {
+  HostAccel h;
+  if (build_host_accel(s, h) != AR_OK) return -10;
+  SceneView sv{};
+  sv.nodes = h.nodes.data();
+  sv.qnodes = h.qnodes.data();
+  sv.tris = h.tris.data();
+  sv.insts = h.insts.data();
+  sv.tlas_root = h.tlas_root;
+  sv.qtlas_root = h.qtlas_root;
+  for (int k = 0; k < 6; ++k) sv.tlas_box[k] = h.tlas_box[k];
+  sv.n_inst = (uint32_t)h.insts.size();
+  if (any && near) return run<true, true>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
+  if (any) return run<true, false>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
+  if (near) return run<false, true>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
+  return run<false, false>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
+}

@@ -1,0 +1,168 @@
+"""The library's traversal code (agent-ray_amd/csrc/ar_traverse.h, written
+for host and device) compiled for the host with g++ and compared with the
+oracle ray by ray, without a GPU.
+
+It walks the same trees the device gets (build_host_accel: binary trees for
+reference order, quantized 4-wide trees for near-first order) and runs each
+query two ways: one ray after another, and in an emulation of the kernels'
+64-lane waves with wave-local refill and maximal leaf parking.  Checks:
+* reference order reproduces the oracle's (t bits, prim, instance) on every
+  ray, closest and any hit;
+* near-first order reproduces them too, except near-ties (t within 4 ulps,
+  never farther) and at most a stated count;
+* both drivers agree bit for bit (the visiting order does not change the
+  answer) and the wave emulation starts every ray exactly once.
+Sets: random soups, Cornell instances (rotated / scaled), the room, rays
+starting on triangle vertices with a zero direction component (the slab
+test's 0 * inf cases), and duplicated geometry where every hit is a tie."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "host", "host_traverse.cpp")
+CSRC = os.path.join(ROOT, "agent-ray_amd", "csrc")
+OUT = os.path.join(ROOT, "tests", "host", "_build", "libhost_traverse.so")
+LIB = os.path.join(ROOT, "agent-ray_amd", "libagentray.so")
+
+
+def _helper():
+    deps = [SRC, LIB] + [os.path.join(CSRC, f) for f in ("ar_traverse.h", "ar_device.h", "ar_internal.h")]
+    if not os.path.exists(OUT) or os.path.getmtime(OUT) < max(os.path.getmtime(d) for d in deps):
+        if not os.path.exists("/opt/rocm/include/hip/hip_runtime.h"):
+            pytest.skip("HIP headers not installed")
+        os.makedirs(os.path.dirname(OUT), exist_ok=True)
+        subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
+                               SRC, "-o", OUT + ".tmp", "-L" + os.path.dirname(LIB), "-lagentray",
+                               "-Wl,-rpath," + os.path.dirname(LIB)])
+        os.replace(OUT + ".tmp", OUT)
+    h = C.CDLL(OUT)
+    h.host_traverse.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6
+    h.host_traverse.restype = C.c_int
+    return h
+
+
+def _scene(ar, desc):
+    """The scene as the library holds it before commit (no device needed)."""
+    lib = ar.lib()
+    s = C.c_void_p()
+    assert lib.ar_scene_create(C.byref(s)) == 0
+    mids = []
+    for m in desc.materials:
+        out = C.c_uint32()
+        assert lib.ar_scene_add_material(s, C.byref(ar._mat_from_desc(m).to_c()), C.byref(out)) == 0
+        mids.append(out.value)
+    p = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+    for m in desc.meshes:
+        pos = np.ascontiguousarray(m.positions, np.float32)
+        idx = np.ascontiguousarray(m.indices, np.uint32)
+        nrm = None if m.normals is None else np.ascontiguousarray(m.normals, np.float32)
+        slots = np.zeros(1, np.uint32) if m.slots is None else np.ascontiguousarray(m.slots, np.uint32).reshape(-1)
+        mats = np.asarray([mids[i] for i in m.materials], np.uint32)
+        xf = None if m.transform is None else np.ascontiguousarray(m.transform, np.float32).reshape(16)
+        assert lib.ar_scene_add_mesh(s, p(pos), pos.shape[0], p(idx), idx.shape[0], p(nrm), p(slots),
+                                     slots.shape[0], p(mats), mats.shape[0], p(xf), None) == 0
+    for l in desc.lights:
+        lt = ar.PointLight(l.position, l.intensity) if l.type == 0 else \
+            ar.QuadLight(l.position, l.edge_u, l.edge_v, l.intensity)
+        assert lib.ar_scene_add_light(s, C.byref(lt.to_c()), None) == 0
+    return s
+
+
+def _run(h, s, rays, any_hit, near, lds_stack=4):
+    n = rays.shape[0]
+    bufs = [np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)] * 2
+    rc = h.host_traverse(s, rays.ctypes.data, n, int(any_hit), int(near), lds_stack, *[b.ctypes.data for b in bufs])
+    assert rc == 0, rc
+    return bufs[:3], bufs[3:]
+
+
+def _near_ties(a, b):
+    ta, tb = a[0].view(np.uint32).astype(np.int64), b[0].view(np.uint32).astype(np.int64)
+    exact = (ta == tb) & (a[1] == b[1]) & (a[2] == b[2])
+    both = (a[1] != 0xFFFFFFFF) & (b[1] != 0xFFFFFFFF)
+    ties = (~exact) & both & (np.abs(ta - tb) <= 4) & (a[0] <= b[0])
+    return int((~exact).sum()), int(ties.sum())
+
+
+def _check(ar, orc, desc, rays, max_near_ties=0):
+    h = _helper()
+    rays = np.ascontiguousarray(rays, np.float32)
+    s = _scene(ar, desc)
+    try:
+        osc = orc.OracleScene(desc)
+        ot, op, oi, _, _ = osc.trace(rays)
+        occ, _, _ = osc.occluded(rays)
+        ora = (ot, op, oi)
+        for near in (False, True):
+            one, wave = _run(h, s, rays, False, near)
+            for k in range(3):  # the two drivers agree bit for bit
+                assert np.array_equal(one[k].view(np.uint32), wave[k].view(np.uint32)), (near, k)
+            bad, ties = _near_ties(one, ora)
+            if near:
+                assert bad == ties and ties <= max_near_ties, (bad, ties)
+            else:
+                assert bad == 0, bad
+            a1, a2 = _run(h, s, rays, True, near)
+            assert np.array_equal(a1[0].astype(bool), occ.astype(bool)), near
+            assert np.array_equal(a2[0].astype(bool), occ.astype(bool)), near
+    finally:
+        ar.lib().ar_scene_destroy(s)
+
+
+def _interior_rays(wl, lo, hi, n, seed, tmin=0.0):
+    rng = np.random.default_rng(seed)
+    r = np.zeros((n, 8), np.float32)
+    r[:, 0:3] = rng.uniform(lo, hi, (n, 3))
+    r[:, 3] = tmin
+    r[:, 4:7] = wl.normalize_f32(rng.standard_normal((n, 3)).astype(np.float32))
+    r[:, 7] = wl.F32_MAX
+    r[::5, 7] = rng.uniform(0.0, 0.5, r[::5].shape[0])  # short (shadow-like) rays
+    return r
+
+
+def test_soup(ar, orc, wl):
+    _check(ar, orc, wl.triangle_soup(1 << 14, seed=3), wl.soup_rays(1 << 14, seed=4, image=128))
+
+
+def test_cornell_instances(ar, orc, wl):
+    _check(ar, orc, wl.cornell_box(), _interior_rays(wl, 0.02, 0.98, 20000, 5), max_near_ties=2)
+
+
+def test_room(ar, orc, wl):
+    desc = wl.procedural_room(n_boxes=30)
+    _check(ar, orc, desc, _interior_rays(wl, (0.1, 0.1, 0.1), (7.9, 3.9, 7.9), 12000, 6, 1e-4), max_near_ties=2)
+
+
+def test_rays_on_box_planes(ar, orc, wl):
+    desc = wl.triangle_soup(1 << 12, seed=7)
+    rng = np.random.default_rng(8)
+    P = desc.meshes[0].positions.reshape(-1, 3).astype(np.float32)
+    m = 6000
+    o = P[rng.integers(0, P.shape[0], m)].copy()
+    for ax in range(3):
+        sel = rng.random(m) < 0.5
+        o[sel, ax] = P[rng.integers(0, P.shape[0], int(sel.sum())), ax]
+    d = np.zeros((m, 3), np.float32)
+    d[np.arange(m), rng.integers(0, 3, m)] = np.where(rng.random(m) < 0.5, -1.0, 1.0)
+    k = m // 3
+    d2 = rng.standard_normal((k, 3)).astype(np.float32)
+    d2[np.arange(k), rng.integers(0, 3, k)] = 0.0
+    d[:k] = wl.normalize_f32(d2)
+    r = np.zeros((m, 8), np.float32)
+    r[:, 0:3], r[:, 4:7], r[:, 7] = o, d, wl.F32_MAX
+    _check(ar, orc, desc, r)
+
+
+def test_duplicate_geometry_ties(ar, orc, wl):
+    base = wl.triangle_soup(1 << 11, seed=12)
+    m = base.meshes[0]
+    idx = m.indices.reshape(-1, 3)
+    dup = wl.MeshDesc(positions=m.positions, indices=np.concatenate([idx, idx[::-1]]).astype(np.uint32),
+                      materials=m.materials, slots=m.slots, normals=None, transform=m.transform)
+    desc = wl.SceneDesc(materials=base.materials, meshes=[dup, dup], lights=base.lights, camera=base.camera)
+    _check(ar, orc, desc, wl.soup_rays(1 << 14, seed=13, image=128))
