@@ -180,8 +180,17 @@ struct Walker {
     const QNode* n = sv.qnodes + cur;
     const float4 a = n->a, b = n->b, c = n->c;
     const int4 rf = n->d;
+    // With a finite 1/d (the only case the 4-wide trees are walked) the
+    // entry plane of every child on axis k is its low plane when 1/d_k >= 0
+    // and its high plane otherwise (dequantized planes are ordered and
+    // (x - o) * inv is monotone), so one select per axis per node replaces
+    // the per-child min / max of the slab test; the decisions and entry
+    // distances are those of slab_fast.
+    const bool px = inv.x >= 0.0f, py = inv.y >= 0.0f, pz = inv.z >= 0.0f;
     const uint32_t lx = bits_u(b.z), ly = bits_u(b.w), lz = bits_u(c.x);
     const uint32_t hx = bits_u(c.y), hy = bits_u(c.z), hz = bits_u(c.w);
+    const uint32_t nx = px ? lx : hx, ny = py ? ly : hy, nz = pz ? lz : hz;
+    const uint32_t fx = px ? hx : lx, fy = py ? hy : ly, fz = pz ? hz : lz;
     float t[4];
     int r[4] = {rf.x, rf.y, rf.z, rf.w};
     const float tc = tcull();
@@ -191,12 +200,15 @@ struct Walker {
     for (int k = 0; k < 4; ++k) {
       const int sh = 8 * k;
       // org + q * scl with q * scl exact: one rounding, as the builder computed
-      float tn;
-      bool h = slab_fast(fmaf((float)((lx >> sh) & 255u), a.w, a.x), fmaf((float)((ly >> sh) & 255u), b.x, a.y),
-                         fmaf((float)((lz >> sh) & 255u), b.y, a.z), fmaf((float)((hx >> sh) & 255u), a.w, a.x),
-                         fmaf((float)((hy >> sh) & 255u), b.x, a.y), fmaf((float)((hz >> sh) & 255u), b.y, a.z), o,
-                         inv, tmin, tc, &tn);
-      h = h && r[k] != kEmptyRef;
+      const float tnx = (fmaf((float)((nx >> sh) & 255u), a.w, a.x) - o.x) * inv.x;
+      const float tny = (fmaf((float)((ny >> sh) & 255u), b.x, a.y) - o.y) * inv.y;
+      const float tnz = (fmaf((float)((nz >> sh) & 255u), b.y, a.z) - o.z) * inv.z;
+      const float tfx = (fmaf((float)((fx >> sh) & 255u), a.w, a.x) - o.x) * inv.x;
+      const float tfy = (fmaf((float)((fy >> sh) & 255u), b.x, a.y) - o.y) * inv.y;
+      const float tfz = (fmaf((float)((fz >> sh) & 255u), b.y, a.z) - o.z) * inv.z;
+      const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
+      const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tc);
+      const bool h = tn <= tf && r[k] != kEmptyRef;
       t[k] = h ? tn : INFINITY;
       r[k] = h ? r[k] : kEmptyRef;
     }
