@@ -358,6 +358,72 @@ def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
     return out
 
 
+def bench_bdpt_gdpt(ar, torch):
+    """The room at 1280x720: BDPT 16 spp, depth 10, power heuristic beta 2;
+    GDPT 32 spp, depth 16, base + two shifted paths per sample, screened
+    Poisson alpha 0.2 with 50 CG iterations.  Each timed render follows a
+    warm-up render; the Poisson solve is also timed alone (device buffers,
+    HIP events, allocation of its work planes included) against its
+    algorithmic bytes: 120 B per pixel and iteration (two passes over fp32
+    RGB planes: 4 and 6 plane accesses of 12 B) plus 204 B per pixel of
+    set-up (de-interleave, right-hand side, first residual, output)."""
+    wl = ar.workloads
+    desc = wl.procedural_room(width=1280, height=720)
+    sc = ar.Scene.from_desc(desc)
+    cd = desc.camera
+    w, h = 1280, 720
+    cam = ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, w / h))
+    out = {}
+    bd = ar.BidirectionalPathTracer(ar.BdptConfig(spp=16, max_depth=10, rr_depth=3), seed=0)
+    bd.render(sc, cam, w, h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bd.render(sc, cam, w, h)
+    dt = time.perf_counter() - t0
+    st = bd.last_stats
+    out["bdpt_room_1280x720x16spp"] = {"msamples_s": round(w * h * 16 / dt / 1e6, 2),
+                                       "mrays_s": round(st.rays / dt / 1e6, 1), "seconds": round(dt, 4),
+                                       "rays": int(st.rays)}
+    log(f"bdpt room: {out['bdpt_room_1280x720x16spp']}")
+    gd = ar.GradientDomainPathTracer(ar.GdptConfig(spp=32, max_depth=16, rr_depth=3, alpha=0.2,
+                                                   poisson_iterations=50), seed=0)
+    gd.render(sc, cam, w, h)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gd.render(sc, cam, w, h)
+    dt = time.perf_counter() - t0
+    st = gd.last_stats
+    g = {"mpaths_s": round(st.paths / dt / 1e6, 2), "mrays_s": round(st.rays / dt / 1e6, 1),
+         "seconds": round(dt, 4), "paths": int(st.paths)}
+    # the Poisson solve alone on the render's own films
+    films = [torch.from_numpy(np.ascontiguousarray(gd.aux[k], np.float32)).cuda() for k in range(3)]
+    dst = torch.empty_like(films[0])
+    stream = torch.cuda.Stream()
+    res = __import__("ctypes").c_double()
+    lib = ar.lib()
+    call = lambda: ar._check(lib.ar_poisson_solve(C_void(films[0].data_ptr()), C_void(films[1].data_ptr()),
+                                                  C_void(films[2].data_ptr()), w, h, 0.2, 50, 1,
+                                                  C_void(dst.data_ptr()), __import__("ctypes").byref(res),
+                                                  ar.AR_ASYNC, C_void(stream.cuda_stream)))
+    with torch.cuda.stream(stream):
+        call()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(5):
+            call()
+        e1.record(stream)
+        torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    alg = (120 * 50 + 204) * w * h
+    g["poisson_cg50_ms"] = round(ms, 3)
+    g["poisson_achieved_gbs"] = round(alg / (ms / 1e3) / 1e9, 1)
+    g["poisson_frac_of_hbm"] = round(alg / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
+    out["gdpt_room_1280x720x32spp"] = g
+    log(f"gdpt room: {g}")
+    return out
+
+
 def bench_serving(args, ar, torch):
     """Continuous batching against the resident room scene.  Job mix: square
     side U{64..1024 step 64}, spp round(exp(U[0, ln 256])), PT 60% / BDPT 25% /
@@ -483,8 +549,10 @@ def main():
     ar = load_pkg()
     ar.init(0)
     if args.workload == "pt":
-        print(json.dumps({"pt_cornell_256x256x16spp": bench_pt(ar, torch, "cornell"),
-                          "pt_room_1920x1080x64spp": bench_pt(ar, torch, "room")}), flush=True)
+        res = {"pt_cornell_256x256x16spp": bench_pt(ar, torch, "cornell"),
+               "pt_room_1920x1080x64spp": bench_pt(ar, torch, "room")}
+        res.update(bench_bdpt_gdpt(ar, torch))
+        print(json.dumps(res), flush=True)
         return
     if args.workload == "copy":
         print(json.dumps(bench_copy(args, torch)), flush=True)
@@ -504,6 +572,7 @@ def main():
     if not args.no_extra:
         out["extra"] = {"pt_cornell_256x256x16spp": bench_pt(ar, torch, "cornell"),
                         "pt_room_1920x1080x64spp": bench_pt(ar, torch, "room")}
+        out["extra"].update(bench_bdpt_gdpt(ar, torch))
     print(json.dumps(out), flush=True)
 
 
