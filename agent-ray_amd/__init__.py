@@ -90,11 +90,12 @@ class ArJobDesc(C.Structure):
 
 class ArTraceConfig(C.Structure):
     _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("refill_min", C.c_uint32), ("ms", C.c_float)]
+                ("refill_min", C.c_uint32), ("search_min", C.c_uint32), ("ms", C.c_float)]
 
     def as_dict(self) -> dict:
         return {"workgroup_size": self.workgroup_size, "lds_stack": self.lds_stack,
-                "blocks_per_cu": self.blocks_per_cu, "refill_min": self.refill_min, "ms": self.ms}
+                "blocks_per_cu": self.blocks_per_cu, "refill_min": self.refill_min, "search_min": self.search_min,
+                "ms": self.ms}
 
 
 class ArJobStats(C.Structure):
@@ -345,8 +346,9 @@ class Scene:
         _check(_lib.ar_scene_get_trace_config(self._h, C.byref(c)))
         return c.as_dict()
 
-    def set_trace_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16):
-        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, 0.0)
+    def set_trace_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16,
+                         search_min: int = 8):
+        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, 0.0)
         _check(_lib.ar_scene_set_trace_config(self._h, C.byref(c)))
 
     def autotune(self, rays, stream=None) -> dict:

@@ -33,9 +33,23 @@
 
 namespace ar {
 
+// Stack pointers carry their address space in device code, so LDS entries
+// are read and written with LDS instructions and spilled ones with global
+// ones (a generic pointer would make the compiler merge both paths into
+// flat accesses, which wait on both memory counters).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define AR_LDS_AS __attribute__((address_space(3)))
+#define AR_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define AR_LDS_AS
+#define AR_GLOBAL_AS
+#endif
+using LdsEntryPtr = AR_LDS_AS int2*;
+using SpillEntryPtr = AR_GLOBAL_AS unsigned long long*;  // (ref, tnear bits) packed low, high
+
 struct Stack {
-  int2* lds;          // &lds_base[threadIdx.x]; entry i at lds[i * stride]
-  int2* spill;        // &spill_base[global thread]; entry i >= S at spill[(i - S) * spill_stride]
+  LdsEntryPtr lds;      // &lds_base[threadIdx.x]; entry i at lds[i * stride]
+  SpillEntryPtr spill;  // &spill_base[global thread]; entry i >= S at spill[(i - S) * spill_stride]
   uint32_t stride;
   uint32_t spill_stride;
   int S;
@@ -45,12 +59,19 @@ struct Stack {
     if (sp < S)
       lds[sp * stride] = e;
     else
-      spill[(size_t)(sp - S) * spill_stride] = e;
+      spill[(size_t)(sp - S) * spill_stride] = (unsigned long long)(uint32_t)e.x | ((unsigned long long)(uint32_t)e.y << 32);
     ++sp;
   }
   AR_HD int2 pop() {
     --sp;
-    return sp < S ? lds[sp * stride] : spill[(size_t)(sp - S) * spill_stride];
+    int2 e;
+    if (sp < S) {
+      e = lds[sp * stride];
+    } else {
+      const unsigned long long v = spill[(size_t)(sp - S) * spill_stride];
+      e = make_int2((int)(uint32_t)v, (int)(uint32_t)(v >> 32));
+    }
+    return e;
   }
 };
 
@@ -119,7 +140,7 @@ struct Walker {
   uint32_t ik;            // instance record being walked
   int base;               // stack depth at instance entry
   int cur;                // node ref, leaf ref, or kLevelEnd
-  int pend;               // parked triangle leaf (kLevelEnd: none)
+  int park0, park1;       // parked triangle leaves in the order reached (kLevelEnd: free)
   bool in_blas;
   bool fast;              // 1/d of this level is finite: IEEE min / max slab
   bool wide;              // this level walks the 4-wide tree (near-first, fast)
@@ -144,7 +165,7 @@ struct Walker {
     tmin = wtmin;
     tmax = bt;
     in_blas = false;
-    pend = kLevelEnd;
+    park0 = park1 = kLevelEnd;
     fast = finite_inv(inv);
     wide = NEAR && fast;
     cur = wide ? sv.qtlas_root : sv.tlas_root;
@@ -286,31 +307,44 @@ struct Walker {
       visit_binary<false>(sv, stk);
   }
 
-  // Node phase: visits internal nodes until the lane holds a leaf or
-  // kLevelEnd.  A lane that reaches a triangle leaf first parks it in `pend`
-  // and keeps descending from its next deferred node while
-  // others_searching(searching) reports a lane of its wave still looking
-  // for its first leaf, so the wave's lanes stay busy with node work; a
-  // second leaf or the level's end stops it.  The visiting order then
-  // depends on the wave, and the answer does not (kCullSlack).
-  template <class Others>
-  AR_HD void descend(const SceneView& sv, Stack& stk, Others others_searching) {
+  // Node phase.  A lane visits internal nodes and parks the triangle leaves
+  // it reaches (up to two, in the order reached) instead of
+  // testing them at once, so the lanes of a wave keep doing node work
+  // together; the leaf phase then tests the parked leaves.  A lane can act
+  // while it holds a node, or a triangle leaf and a free park slot; a lane
+  // that has parked nothing yet is "searching".  Every lane stays in the
+  // loop until go(can, searching) -- wave-uniform on the device, see
+  // walk_items -- ends it for all of them.  Alone, a lane stops at its first
+  // parked leaf or when it cannot act.  The visiting order then depends on
+  // the wave, and the answer does not (kCullSlack).
+  //
+  // The loop has one back-edge and one (wave-uniform) exit, and a lane that
+  // cannot act stays in it, masked: a `continue` for such lanes makes the
+  // compiler split the loop into an inner loop that lanes leave one by one,
+  // after which lane masks computed in the loop (e.g. park0 != kLevelEnd)
+  // only keep the lanes of its last iteration.
+  template <class Go>
+  AR_HD void descend(const SceneView& sv, Stack& stk, Go go) {
     for (;;) {
-      const bool searching = pend == kLevelEnd && cur >= 0;
-      const bool others = others_searching(searching);
-      if (cur >= 0) {
-        if (pend != kLevelEnd && !others) return;
-        visit(sv, stk);
-      } else if (cur != kLevelEnd && in_blas && pend == kLevelEnd) {
-        pend = cur;
-        pop_level(stk);
-      } else {
-        return;
+      const bool node = cur >= 0;
+      const bool can = park1 == kLevelEnd && (node || (cur != kLevelEnd && in_blas));
+      const bool searching = can && park0 == kLevelEnd;
+      if (!go(can, searching)) return;
+      if (can) {
+        if (node) {
+          visit(sv, stk);
+        } else {
+          if (park0 == kLevelEnd)
+            park0 = cur;
+          else
+            park1 = cur;
+          pop_level(stk);
+        }
       }
     }
   }
   AR_HD void descend(const SceneView& sv, Stack& stk) {
-    descend(sv, stk, [](bool) { return false; });
+    descend(sv, stk, [](bool can, bool searching) { return searching; });
   }
 
   // Enters the first instance from record k on (same TLAS leaf) whose BLAS
@@ -358,58 +392,87 @@ struct Walker {
 
   // Tests the triangles of BLAS leaf `ref`.  Any hit: true at the first hit
   // in range (the query is then done).
-  AR_HD bool tri_leaf(const SceneView& sv, Stack& stk, int ref) {
-    uint32_t k = (uint32_t)~ref;
-    for (;;) {
-      AR_COUNT_TRI();
-      const Tri* tp = sv.tris + k;
-      float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
-      uint32_t tag = bits_u(v0.w);
-      float th, uh, vh;
-      if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tcull(), &th,
-                  &uh, &vh)) {
-        if (ANY) {
-          if (th / s < bt) {
-            bpos = k;
-            binst = ik;
-            stk.sp = 0;
-            return true;
-          }
-        } else if (NEAR) {
-          // world-space comparison right away, ties by (instance, leaf position)
-          float tw = th / s;
-          if (tw < bt || (bpos != kNoHit && tw == bt && (ik < binst || (ik == binst && k < bpos)))) {
-            bt = tw;
-            bpos = k;
-            binst = ik;
-            bu = uh;
-            bv = vh;
-            tmax = bt * s;
-          }
-        } else if (th < tmax) {
-          tmax = th;
-          lpos = k;
-          lu = uh;
-          lv = vh;
+  // One triangle (Tri record k).  Returns true when an any-hit query is
+  // answered.
+  AR_HD bool tri(const SceneView& sv, Stack& stk, uint32_t k, uint32_t& tag) {
+    AR_COUNT_TRI();
+    const Tri* tp = sv.tris + k;
+    float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
+    tag = bits_u(v0.w);
+    float th, uh, vh;
+    if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tcull(), &th, &uh,
+                &vh)) {
+      if (ANY) {
+        if (th / s < bt) {
+          bpos = k;
+          binst = ik;
+          stk.sp = 0;
+          return true;
         }
+      } else if (NEAR) {
+        // world-space comparison right away, ties by (instance, leaf position)
+        float tw = th / s;
+        if (tw < bt || (bpos != kNoHit && tw == bt && (ik < binst || (ik == binst && k < bpos)))) {
+          bt = tw;
+          bpos = k;
+          binst = ik;
+          bu = uh;
+          bv = vh;
+          tmax = bt * s;
+        }
+      } else if (th < tmax) {
+        tmax = th;
+        lpos = k;
+        lu = uh;
+        lv = vh;
       }
+    }
+    return false;
+  }
+
+  // All triangles of one leaf, in leaf order.
+  AR_HD bool tri_leaf(const SceneView& sv, Stack& stk, int ref) {
+    for (uint32_t k = (uint32_t)~ref;; ++k) {
+      uint32_t tag;
+      if (tri(sv, stk, k, tag)) return true;
       if (tag & kLastBit) return false;
-      ++k;
     }
   }
 
-  // Leaf phase: a parked leaf if there is one, else the leaf ref or
-  // kLevelEnd the lane holds.  Returns true when the query is done;
-  // otherwise cur is a node or a leaf again (or kLevelEnd after a parked
-  // leaf, handled here too).  Finished instances and the end of the query
-  // are handled in the same call.
+  // The parked leaves' triangles as one loop (park0's, then park1's), so a
+  // lane moves on to its second leaf without waiting for the wave.
+  AR_HD bool tri_parked(const SceneView& sv, Stack& stk) {
+    uint32_t k = (uint32_t)~park0;
+    park0 = kLevelEnd;
+    for (;;) {
+      uint32_t tag;
+      if (tri(sv, stk, k, tag)) {
+        park1 = kLevelEnd;
+        return true;
+      }
+      if (!(tag & kLastBit)) {
+        ++k;
+      } else if (park1 != kLevelEnd) {
+        k = (uint32_t)~park1;
+        park1 = kLevelEnd;
+      } else {
+        return false;
+      }
+    }
+  }
+
+  // Leaf phase: the parked leaves, then what the lane holds -- a node (the
+  // wave ended the node phase first: nothing to do), a triangle leaf (tested
+  // here only if nothing was parked; otherwise the next node phase parks
+  // it), a TLAS leaf (enter its instances) or kLevelEnd (instance finished,
+  // or the query is done).  Returns true when the query is done.
   AR_HD bool leaf(const SceneView& sv, Stack& stk) {
-    if (pend != kLevelEnd) {
-      const int p = pend;
-      pend = kLevelEnd;
-      if (tri_leaf(sv, stk, p)) return true;
-    } else if (cur != kLevelEnd) {
+    const bool had = park0 != kLevelEnd;
+    if (had && tri_parked(sv, stk)) return true;
+    if (cur >= 0) return false;
+    if (cur != kLevelEnd) {
       if (in_blas) {
+        if (had) return false;
         if (tri_leaf(sv, stk, cur)) return true;
       } else if (enter(sv, (uint32_t)~cur, stk)) {
         return false;
@@ -476,7 +539,8 @@ AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack&
 //   sink(i, w)  consumes the finished walker of item i
 template <bool ANY, bool NEAR, class Src, class Sink>
 __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint64_t n, uint64_t wave,
-                                           uint64_t waves, uint32_t refill_min, Src src, Sink sink) {
+                                           uint64_t waves, uint32_t refill_min, uint32_t search_min, Src src,
+                                           Sink sink) {
   const uint32_t lane = __lane_id();
   const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
   uint32_t next = 0;  // wave-uniform: position of the next unassigned item
@@ -502,7 +566,12 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
       next += nidle;
     }
     if (alive) {
-      w.descend(sv, stk, [](bool searching) { return __ballot(searching) != 0ull; });
+      // the wave keeps walking nodes while more than search_min of its lanes
+      // have parked nothing, or while no lane has leaf-phase work
+      w.descend(sv, stk, [search_min](bool can, bool searching) {
+        const uint64_t sm = __ballot(searching);
+        return (uint32_t)__popcll(sm) > search_min || (sm != 0ull && __ballot(!searching) == 0ull);
+      });
       if (w.leaf(sv, stk)) {
         sink(item, w);
         alive = false;

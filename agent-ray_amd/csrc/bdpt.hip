@@ -241,9 +241,9 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_subpaths(ShadeView v, BdptParams
   const uint32_t gtid = lb * BLOCK + threadIdx.x;
   const uint32_t nthreads = G * BLOCK;
   Stack stk;
-  stk.lds = lds + threadIdx.x;
+  stk.lds = (LdsEntryPtr)(lds + threadIdx.x);
   stk.stride = BLOCK;
-  stk.spill = spill + gtid;
+  stk.spill = (SpillEntryPtr)(spill + gtid);
   stk.spill_stride = nthreads;
   stk.S = kTraceLdsStack;
   stk.sp = 0;
@@ -380,9 +380,9 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams 
   const uint32_t gtid = lb * BLOCK + threadIdx.x;
   const uint32_t nthreads = G * BLOCK;
   Stack stk;
-  stk.lds = lds + threadIdx.x;
+  stk.lds = (LdsEntryPtr)(lds + threadIdx.x);
   stk.stride = BLOCK;
-  stk.spill = spill + gtid;
+  stk.spill = (SpillEntryPtr)(spill + gtid);
   stk.spill_stride = nthreads;
   stk.S = kTraceLdsStack;
   stk.sp = 0;

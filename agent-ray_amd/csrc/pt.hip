@@ -47,9 +47,9 @@ __device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint32
 template <int BLOCK, int S = kTraceLdsStack>
 __device__ __forceinline__ Stack make_stack(int2* lds, int2* spill, uint32_t gtid, uint32_t nthreads) {
   Stack stk;
-  stk.lds = lds + threadIdx.x;
+  stk.lds = (LdsEntryPtr)(lds + threadIdx.x);
   stk.stride = BLOCK;
-  stk.spill = spill + gtid;
+  stk.spill = (SpillEntryPtr)(spill + gtid);
   stk.spill_stride = nthreads;
   stk.S = S;
   stk.sp = 0;
@@ -163,7 +163,7 @@ __device__ __forceinline__ Ray make_ray4(float4 o, float4 d) {
 // lanes busy until its share of the queue is done.
 template <int BLOCK, int S>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill,
-                                                     uint32_t refill_min) {
+                                                     uint32_t refill_min, uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneVi
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
   walk_items<false, true>(
-      sv, stk, n, gtid / 64, nthreads / 64, refill_min,
+      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
       [&](uint64_t i) {
         uint32_t p = q[i];
         return make_ray4(B.ray_o[p], B.ray_d[p]);
@@ -232,14 +232,14 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, PtParams P, Wav
 // contribution to its path's radiance (one shadow ray per path per bounce).
 template <int BLOCK, int S>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill,
-                                                     uint32_t refill_min) {
+                                                     uint32_t refill_min, uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
   Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = B.cnt[2];
   walk_items<true, true>(
-      sv, stk, n, gtid / 64, nthreads / 64, refill_min,
+      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
       [&](uint64_t i) { return make_ray4(B.sh_o[i], B.sh_d[i]); },
       [&](uint64_t i, const Walker<true, true>& w) {
         if (w.bpos == kNoHit) {
@@ -402,14 +402,14 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
 #define AR_EXT(BB, SS)                                                                                         \
   if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                           \
     hipLaunchKernelGGL((k_wf_extend<BB, SS>), dim3(clamp_resident(Gt, (const void*)&k_wf_extend<BB, SS>, BB)),  \
-                       dim3(BB), 0, st, v.sv, B, qi, ws.spill, tc.refill_min);
+                       dim3(BB), 0, st, v.sv, B, qi, ws.spill, tc.refill_min, tc.search_min);
     AR_TRACE_VARIANTS(AR_EXT)
 #undef AR_EXT
     hipLaunchKernelGGL((k_wf_shade<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, B, qi);
 #define AR_SHD(BB, SS)                                                                                         \
   if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                           \
     hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(clamp_resident(Gt, (const void*)&k_wf_shadow<BB, SS>, BB)),  \
-                       dim3(BB), 0, st, v.sv, B, ws.spill, tc.refill_min);
+                       dim3(BB), 0, st, v.sv, B, ws.spill, tc.refill_min, tc.search_min);
     AR_TRACE_VARIANTS(AR_SHD)
 #undef AR_SHD
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, B, qi);

@@ -31,23 +31,23 @@ template <bool ANY, bool NEAR, int S, int BLOCK>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_trace(SceneView sv, const ar_ray* __restrict__ rays, uint64_t n,
                                                  ar_hit* __restrict__ hits, uint32_t* __restrict__ inst_ids,
                                                  uint32_t* __restrict__ occ, int2* __restrict__ spill,
-                                                 uint32_t refill_min) {
+                                                 uint32_t refill_min, uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t G = gridDim.x;
   const uint32_t b = blockIdx.x;
   const uint32_t lb = (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
   const uint32_t gtid = lb * BLOCK + threadIdx.x;
   Stack stk;
-  stk.lds = lds + threadIdx.x;
+  stk.lds = (LdsEntryPtr)(lds + threadIdx.x);
   stk.stride = BLOCK;
-  stk.spill = spill + gtid;
+  stk.spill = (SpillEntryPtr)(spill + gtid);
   stk.spill_stride = G * BLOCK;
   stk.S = S;
   stk.sp = 0;
   const uint64_t waves = (uint64_t)G * (BLOCK / 64);
   const uint64_t wave = (uint64_t)lb * (BLOCK / 64) + threadIdx.x / 64;
   walk_items<ANY, NEAR>(
-      sv, stk, n, wave, waves, refill_min, [&](uint64_t i) { return load_ray(rays, i); },
+      sv, stk, n, wave, waves, refill_min, search_min, [&](uint64_t i) { return load_ray(rays, i); },
       [&](uint64_t i, const Walker<ANY, NEAR>& w) {
         if (ANY) {
           occ[i] = w.bpos != kNoHit ? 1u : 0u;
@@ -69,10 +69,10 @@ static uint32_t stack_need(const ar_scene* s) { return s->stack_depth; }
 
 template <bool ANY, bool NEAR, int BLOCK, int S>
 static void launch_variant(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits,
-                           uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st, uint32_t refill_min) {
+                           uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st, const ar_trace_config& c) {
   G = clamp_resident(G, reinterpret_cast<const void*>(&k_trace<ANY, NEAR, S, BLOCK>), BLOCK);
   hipLaunchKernelGGL((k_trace<ANY, NEAR, S, BLOCK>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits, inst, occ,
-                     spill, refill_min);
+                     spill, c.refill_min, c.search_min);
 }
 
 template <bool ANY, bool NEAR>
@@ -96,7 +96,7 @@ static ar_status launch_trace(const ar_scene* s, const ar_ray* rays, uint64_t n,
     uint32_t* oc = occ ? occ + off : nullptr;
 #define AR_LAUNCH(B, SS)                                                           \
   if (c.workgroup_size == B && c.lds_stack == SS) {                                \
-    launch_variant<ANY, NEAR, B, SS>(s, G, r, m, h, in, oc, spill, st, c.refill_min); \
+    launch_variant<ANY, NEAR, B, SS>(s, G, r, m, h, in, oc, spill, st, c);            \
     ok = true;                                                                     \
   }
     AR_TRACE_VARIANTS(AR_LAUNCH)
@@ -159,7 +159,7 @@ uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c) {
 }
 
 uint32_t trace_grid(uint64_t n, int block) {
-  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, kTraceRefillMin, 0.0f};
+  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, kTraceRefillMin, 8, 0.0f};
   return trace_grid_cfg(n, c);
 }
 
@@ -276,6 +276,7 @@ ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c) {
     return fail(AR_ERR_UNSUPPORTED_TYPE, "no traversal variant for this workgroup size / stack depth");
   if (c->blocks_per_cu == 0 || c->blocks_per_cu > 64) return fail(AR_ERR_INVALID_ARG, "blocks_per_cu out of range");
   if (c->refill_min == 0 || c->refill_min > 64) return fail(AR_ERR_INVALID_ARG, "refill_min out of range (1..64)");
+  if (c->search_min > 63) return fail(AR_ERR_INVALID_ARG, "search_min out of range (0..63)");
   std::lock_guard<std::mutex> lk(s->mu);
   s->tcfg = *c;
   s->tcfg.ms = 0.0f;
@@ -314,7 +315,7 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
   for (auto& vv : variants) {
     for (uint32_t pc : per_cu) {
       for (uint32_t rf : refills) {
-        ar_trace_config c{vv[0], vv[1], pc, rf, 0.0f};
+        ar_trace_config c{vv[0], vv[1], pc, rf, s->tcfg.search_min, 0.0f};
         err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
         if (err != AR_OK) break;
         (void)hipEventRecord(e0, st);

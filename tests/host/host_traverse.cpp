@@ -41,7 +41,8 @@ void record(const Walker<ANY, NEAR>& w, const SceneView& sv, uint64_t i, float* 
 template <bool ANY, bool NEAR>
 int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, float* t0, uint32_t* p0, uint32_t* i0,
         float* t1, uint32_t* p1, uint32_t* i1) {
-  std::vector<int2> lds(64 * 64), spill(64 * 512);
+  std::vector<int2> lds(64 * 64);
+  std::vector<unsigned long long> spill(64 * 512);
   for (uint64_t i = 0; i < n; ++i) {
     Stack stk{lds.data(), spill.data(), 1, 1, S, 0};
     Walker<ANY, NEAR> w;
@@ -86,7 +87,7 @@ int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, fl
       }
       for (int l = 0; l < 64; ++l) {
         if (!alive[l]) continue;
-        w[l].descend(sv, stk[l], [](bool) { return true; });  // park whenever possible
+        w[l].descend(sv, stk[l], [](bool can, bool) { return can; });  // park whenever possible
         if (w[l].leaf(sv, stk[l])) {
           if (stk[l].sp != 0) return -3;
           record(w[l], sv, item[l], t1, p1, i1);

@@ -162,8 +162,9 @@ def test_empty_batch_and_device_pointers(ar, wl):
 
 def test_room_launch_variants_same_answer(ar, wl):
     """Multi-instance scene: the traversal order of a ray does not depend on
-    the launch (wave composition, refill threshold), so every variant returns
-    the same hits, equal-t ties across instances included."""
+    the launch (wave composition, refill threshold, how many lanes may still
+    be searching when a wave switches to its leaf phase), so every variant
+    returns the same hits, equal-t ties across instances included."""
     desc = wl.procedural_room(n_boxes=30)
     rng = np.random.default_rng(71)
     n = 1 << 16
@@ -173,11 +174,11 @@ def test_room_launch_variants_same_answer(ar, wl):
     rays[:, 0:3], rays[:, 3], rays[:, 4:7], rays[:, 7] = o, 1e-4, d, wl.F32_MAX
     sc = ar.Scene.from_desc(desc)
     ref_t, ref_p, ref_i = sc.intersect(rays)
-    for (b, s, pc, rf) in [(64, 8, 4, 1), (256, 8, 8, 64), (128, 16, 16, 8)]:
-        sc.set_trace_config(b, s, pc, rf)
+    for (b, s, pc, rf, sm) in [(64, 8, 4, 1, 0), (256, 8, 8, 64, 63), (128, 16, 16, 8, 8), (128, 8, 8, 24, 32)]:
+        sc.set_trace_config(b, s, pc, rf, sm)
         t, p, i = sc.intersect(rays)
-        assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)), (b, s, pc, rf)
-        assert np.array_equal(p, ref_p) and np.array_equal(i, ref_i), (b, s, pc, rf)
+        assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)), (b, s, pc, rf, sm)
+        assert np.array_equal(p, ref_p) and np.array_equal(i, ref_i), (b, s, pc, rf, sm)
 
 
 def test_every_launch_variant_same_answer(ar, wl):
@@ -190,10 +191,10 @@ def test_every_launch_variant_same_answer(ar, wl):
     ref_t, ref_p, _ = sc.intersect(rays)
     ref_o = sc.occluded(rays)
     for (b, s) in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8)]:
-        for pc, rf in ((4, 1), (16, 64), (8, 24)):
-            sc.set_trace_config(b, s, pc, rf)
+        for pc, rf, sm in ((4, 1, 0), (16, 64, 63), (8, 24, 8)):
+            sc.set_trace_config(b, s, pc, rf, sm)
             t, p, _ = sc.intersect(rays)
-            assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p), (b, s, pc, rf)
+            assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p), (b, s, pc, rf, sm)
             assert np.array_equal(sc.occluded(rays), ref_o)
     cfg = sc.autotune(torch.from_numpy(rays).cuda())
     assert cfg["ms"] > 0 and cfg["workgroup_size"] in (64, 128, 256)
@@ -202,6 +203,8 @@ def test_every_launch_variant_same_answer(ar, wl):
         sc.set_trace_config(96, 8, 4)
     with pytest.raises(ar.ArError):
         sc.set_trace_config(64, 8, 4, 65)
+    with pytest.raises(ar.ArError):
+        sc.set_trace_config(64, 8, 4, 24, 64)
 
 
 @pytest.mark.slow
