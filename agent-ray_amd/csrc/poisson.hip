@@ -129,13 +129,25 @@ __global__ __launch_bounds__(kPoisBlock) void k_pois_residual(PoisBufs B) {
 
 // One block: fixed-order sum of per-block partials.  mode 0: bb and rr0
 // setup (called twice), mode 1: alpha = rr / pq, mode 2: beta = rr_new / rr.
-__global__ void k_pois_scalar(PoisBufs B, int nblocks, int mode) {
+// The partials are summed by kPoisBlock threads in a fixed order (thread j
+// takes partials j, j + kPoisBlock, ...; then a fixed shuffle tree per wave
+// and the waves in order), so the scalars are deterministic.
+__global__ __launch_bounds__(kPoisBlock) void k_pois_scalar(PoisBufs B, int nblocks, int mode) {
+  __shared__ double sh[3][kPoisBlock / 64];
   __shared__ double tot[3];
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int k = threadIdx.x; k < nblocks; k += kPoisBlock)
+    for (int c = 0; c < 3; ++c) v[c] += B.partial[c * kPoisMaxBlocks + k];
+  for (int c = 0; c < 3; ++c)
+    for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  if (lane == 0)
+    for (int c = 0; c < 3; ++c) sh[c][wave] = v[c];
+  __syncthreads();
   if (threadIdx.x < 3) {
-    double s = 0.0;
-    const double* pp = B.partial + threadIdx.x * kPoisMaxBlocks;
-    for (int k = 0; k < nblocks; ++k) s += pp[k];
-    tot[threadIdx.x] = s;
+    double t = 0.0;
+    for (int k = 0; k < kPoisBlock / 64; ++k) t += sh[threadIdx.x][k];
+    tot[threadIdx.x] = t;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -279,15 +291,15 @@ ar_status poisson_solve_device(const float* P_rgb, const float* gx_rgb, const fl
   hipLaunchKernelGGL(k_deinterleave, dim3(grid_e), dim3(256), 0, st, gy_rgb, planes[2], n);
   int nb = (int)std::min<uint32_t>((n + kPoisBlock - 1) / kPoisBlock, kPoisMaxBlocks);
   hipLaunchKernelGGL(k_pois_init, dim3(nb), dim3(kPoisBlock), 0, st, B);
-  hipLaunchKernelGGL(k_pois_scalar, dim3(1), dim3(64), 0, st, B, nb, 10);
+  hipLaunchKernelGGL(k_pois_scalar, dim3(1), dim3(kPoisBlock), 0, st, B, nb, 10);
   hipLaunchKernelGGL(k_pois_residual, dim3(nb), dim3(kPoisBlock), 0, st, B);
-  hipLaunchKernelGGL(k_pois_scalar, dim3(1), dim3(64), 0, st, B, nb, 11);
+  hipLaunchKernelGGL(k_pois_scalar, dim3(1), dim3(kPoisBlock), 0, st, B, nb, 11);
   int pin = 0;
   for (uint32_t it = 0; it < iters; ++it) {
     hipLaunchKernelGGL(k_pois_A, dim3(nb), dim3(kPoisBlock), 0, st, B, pin);
-    hipLaunchKernelGGL(k_pois_scalar, dim3(1), dim3(64), 0, st, B, nb, 1);
+    hipLaunchKernelGGL(k_pois_scalar, dim3(1), dim3(kPoisBlock), 0, st, B, nb, 1);
     hipLaunchKernelGGL(k_pois_B, dim3(nb), dim3(kPoisBlock), 0, st, B, pin ^ 1);
-    hipLaunchKernelGGL(k_pois_scalar, dim3(1), dim3(64), 0, st, B, nb, 2);
+    hipLaunchKernelGGL(k_pois_scalar, dim3(1), dim3(kPoisBlock), 0, st, B, nb, 2);
     pin ^= 1;
   }
   hipLaunchKernelGGL(k_pois_out, dim3(grid_e), dim3(256), 0, st, B, out_rgb);

@@ -5,12 +5,20 @@
 // as soon as a slot is free and the HBM budget covers the job's footprint.
 // Every slot owns a HIP stream, a grow-only workspace (path state, queues,
 // film, stack spill) and a pinned output buffer.  One scheduler thread issues
-// chunks of the admitted jobs, shortest remaining work first, with at most
-// kInflight chunks on the device (two per job): a job that arrives while a
-// long one renders waits only for the chunks already issued, then runs
-// ahead of it, and stream priorities are not needed.  Server jobs without an
-// explicit sample_chunk run in one chunk when small and in chunks of 2^20
-// path-equivalents (2^16 BDPT samples) otherwise, so those waits stay short.
+// chunks of the admitted jobs one at a time (kInflight), always of the job
+// whose projected slowdown -- (time since submission + estimated remaining
+// device time) / estimated solo time -- is highest.  A job that arrives
+// while a long one renders therefore waits for one chunk and then runs
+// ahead of it (its projected slowdown starts at 1 and grows fastest), and a
+// long job is not starved: its own projected slowdown grows while it waits.
+// Device time is estimated from the job's work (pixels x spp, weighted per
+// integrator) and a running measurement of the device's rate; stream
+// priorities are not used.  Server jobs without an explicit sample_chunk run
+// in one chunk when small and in chunks of 2^20 path-equivalents (2^16 BDPT
+// samples) otherwise, so the wait for a chunk stays short.  Several chunks
+// on different streams would not help a short job: the device interleaves
+// kernels of concurrent streams one for one, so each of the short job's
+// kernels would wait for one of the long job's.
 // A job's film depends only on its own description
 // (per-job seed, deterministic accumulation in sample order, independent of
 // chunking), never on what it shared the device with.
@@ -26,14 +34,16 @@
 
 namespace ar {
 
-constexpr int kInflight = 2;                 // chunks on the device, all jobs together
+constexpr int kInflight = 1;                 // chunks on the device, all jobs together
 constexpr uint64_t kServerChunkPaths = 1ull << 20;   // PT / GDPT path-equivalents per chunk
 constexpr uint64_t kServerChunkBdpt = 1ull << 16;    // BDPT samples per chunk
 
-// Relative cost of one path (or BDPT sample) of each integrator, used only
-// to order jobs by remaining work.
+// Relative device cost of one pixel-sample of each integrator, measured on
+// the room at 1280x720 (PT path 1; GDPT sample = base + two shifted paths;
+// BDPT sample = both subpaths and all connections), used to estimate device
+// time for scheduling.
 static double path_weight(uint32_t integrator) {
-  return integrator == AR_INTEGRATOR_BDPT ? 4.0 : (integrator == AR_INTEGRATOR_GDPT ? 3.0 : 1.0);
+  return integrator == AR_INTEGRATOR_BDPT ? 14.0 : (integrator == AR_INTEGRATOR_GDPT ? 2.8 : 1.0);
 }
 
 struct Slot {
@@ -57,8 +67,10 @@ struct Job {
   bool issued_all = false;   // every chunk + resolve + copy-out enqueued
   double work_left = 0;      // weighted paths not yet issued
   double work_chunk = 0;     // weighted paths per chunk
+  double work_total = 0;     // weighted paths of the whole job
   std::unique_ptr<JobRun> run;
   std::deque<hipEvent_t> inflight;
+  std::deque<double> inflight_issue, inflight_work;  // host issue time, weighted paths of each chunk
   std::vector<float> film;
   ar_render_stats stats{};
 };
@@ -85,6 +97,10 @@ struct ar_server {
   int device = 0;
 
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+  // device rate in weighted paths per second (running estimate from retired
+  // chunks) and when the device last finished a chunk
+  double rate = 150e6;
+  double last_done = 0;
 
   hipEvent_t get_event() {
     if (!event_pool.empty()) {
@@ -169,6 +185,7 @@ struct ar_server {
       }
       const double w = path_weight(j.spec.cfg.integrator) * (double)npix;
       j.work_left = w * j.spec.cfg.spp;
+      j.work_total = j.work_left;
       j.work_chunk = w * chunk_samples(j.spec);
       sl.busy = true;
       j.slot = free_slot;
@@ -204,6 +221,12 @@ struct ar_server {
       while (!j.inflight.empty() && hipEventQuery(j.inflight.front()) == hipSuccess) {
         event_pool.push_back(j.inflight.front());
         j.inflight.pop_front();
+        const double t = now();
+        const double dt = t - std::max(j.inflight_issue.front(), last_done);
+        if (dt > 1e-4) rate = 0.8 * rate + 0.2 * (j.inflight_work.front() / dt);
+        last_done = t;
+        j.inflight_issue.pop_front();
+        j.inflight_work.pop_front();
         progress = true;
       }
       bool cancel;
@@ -239,18 +262,27 @@ struct ar_server {
       inflight += (int)j.inflight.size();
     }
     while (inflight < kInflight) {
+      // highest projected slowdown first: (waited + remaining) / solo, both
+      // in device time at the measured rate; ties go to less remaining work
       std::shared_ptr<Job> pick;
+      double best = -1.0;
+      const double t = now();
       {
         std::lock_guard<std::mutex> lk(mu);
         for (auto& jp : active) {
           Job& j = *jp;
           if (j.issued_all || j.cancel || j.inflight.size() >= 2) continue;
-          if (!pick || j.work_left < pick->work_left) pick = jp;
+          const double sd = ((t - j.t_submit) * rate + j.work_left) / std::max(j.work_total, 1.0);
+          if (!pick || sd > best || (sd == best && j.work_left < pick->work_left)) {
+            pick = jp;
+            best = sd;
+          }
         }
       }
       if (!pick) break;
       Job& j = *pick;
       bool done = false;
+      const double chunk_work = std::min(j.work_left, j.work_chunk);
       ar_status e = j.run->step(&done);
       j.work_left = std::max(0.0, j.work_left - j.work_chunk);
       if (e == AR_OK && done) {
@@ -269,6 +301,8 @@ struct ar_server {
         for (auto ev : j.inflight) event_pool.push_back(ev);
         inflight -= (int)j.inflight.size();
         j.inflight.clear();
+        j.inflight_issue.clear();
+        j.inflight_work.clear();
         fail_job(j, e, ar_last_error());
         release(j);
         active.erase(std::find(active.begin(), active.end(), pick));
@@ -279,6 +313,8 @@ struct ar_server {
       hipEvent_t ev = get_event();
       (void)hipEventRecord(ev, slots[j.slot]->stream);
       j.inflight.push_back(ev);
+      j.inflight_issue.push_back(t);
+      j.inflight_work.push_back(chunk_work);
       ++inflight;
       progress = true;
     }

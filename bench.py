@@ -491,19 +491,26 @@ def bench_serving(args, ar, torch):
         if now - last_log > 10.0:
             log(f"serving: {len(ids)}/{len(jobs)} jobs submitted at t={now:.1f}s")
             last_log = now
-    lat, paths = [], 0
+    lat, waits, paths = [], [], 0
     for j, jid in ids:
         _, st = srv.wait(jid)
         lat.append(st["latency"])
+        waits.append(st["start"] - st["submit"])
         paths += st["paths"]
     wall = time.perf_counter() - t0
     srv.close()
     lat = np.asarray(lat)
     ratio = lat / np.maximum(solos, 1e-6)
+    worst = []
     for k in np.argsort(-ratio)[:8]:
         j = jobs[k]
-        log(f"serving: job {j.index} {['pt', 'bdpt', 'gdpt'][j.integrator]} {j.width}^2 x {j.spp} spp: solo "
-            f"{solos[k] * 1e3:.1f} ms, latency {lat[k] * 1e3:.1f} ms ({ratio[k]:.1f}x)")
+        worst.append({"job": int(j.index), "integrator": ['pt', 'bdpt', 'gdpt'][j.integrator],
+                      "size": [int(j.width), int(j.height)], "spp": int(j.spp), "solo_ms": round(solos[k] * 1e3, 2),
+                      "latency_ms": round(lat[k] * 1e3, 2), "admission_wait_ms": round(waits[k] * 1e3, 2),
+                      "ratio": round(float(ratio[k]), 2)})
+        log(f"serving: job {j.index} {['pt', 'bdpt', 'gdpt'][j.integrator]} {j.width}x{j.height} x {j.spp} spp: solo "
+            f"{solos[k] * 1e3:.1f} ms, latency {lat[k] * 1e3:.1f} ms (admission wait {waits[k] * 1e3:.1f} ms, "
+            f"{ratio[k]:.1f}x)")
     return {"jobs": len(jobs), "rate_per_s": round(rate, 3), "arrival_window_s": round(window, 2),
             "slots": args.slots, "specified_rate_per_s": args.rate, "specified_jobs": len(spec_jobs),
             "mean_solo_s": round(mean_solo, 4), "offered_load_at_specified_rate": round(rho_spec, 2),
@@ -514,6 +521,9 @@ def bench_serving(args, ar, torch):
             "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
             "p50_latency_over_solo": round(float(np.percentile(ratio, 50)), 3),
             "p99_latency_over_solo": round(float(np.percentile(ratio, 99)), 3),
+            "p99_solo_s": round(float(np.percentile(solos, 99)), 4),
+            "p99_latency_over_p99_solo": round(float(np.percentile(lat, 99) / np.percentile(solos, 99)), 3),
+            "worst_jobs_by_ratio": worst,
             "mpaths_s": round(paths / wall / 1e6, 2), "wall_s": round(wall, 2)}
 
 
