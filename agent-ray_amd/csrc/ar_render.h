@@ -34,9 +34,11 @@ struct Workspace {
   float4* acc = nullptr;  // per-pixel film accumulators (3 planes of npix)
   int2* spill = nullptr;  // traversal stack spill, [entry][thread]
   uint32_t capacity = 0;
+  void* params = nullptr;  // device copy of the chunk's PtParams (read by the generate / shade stages)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
+  uint64_t graph_key[8] = {};  // what the captured bounce loop was recorded for
   ar_status reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_entries, uint32_t grid_threads);
   ~Workspace();
 };

@@ -22,6 +22,8 @@
 #include "ar_pt.h"
 #include "ar_job.h"
 
+#include <cstring>
+
 namespace ar {
 
 __device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
@@ -114,7 +116,8 @@ __device__ __forceinline__ void store_state(const WaveBufs& B, uint32_t i, const
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_generate(PtParams P, WaveBufs B) {
+__global__ __launch_bounds__(BLOCK) void k_wf_generate(const PtParams* __restrict__ Pd, WaveBufs B) {
+  const PtParams P = *Pd;
   const uint32_t npaths = P.npix * P.k * P.variants;
   const uint32_t nthreads = gridDim.x * BLOCK;
   const uint32_t rounds = (npaths + nthreads - 1) / nthreads;
@@ -141,6 +144,12 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(PtParams P, WaveBufs B) {
     uint32_t qi = wave_push(&B.cnt[0], live);
     if (live) B.q[0][qi] = p;
   }
+}
+
+// The chunk's parameters, written to device memory ahead of the (possibly
+// graph-replayed) bounce loop, whose kernels read them there.
+__global__ void k_wf_params(PtParams P, PtParams* dst) {
+  if (threadIdx.x == 0) *dst = P;
 }
 
 // Queue counters start at zero for a chunk (a kernel, not a memset, so the
@@ -184,7 +193,8 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneVi
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, PtParams P, WaveBufs B, int qi) {
+__global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, const PtParams* __restrict__ Pd, WaveBufs B, int qi) {
+  const PtParams P = *Pd;
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
   uint32_t* qn = B.q[qi ^ 1];
@@ -351,6 +361,8 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
   add(o_st, 64);
   add(o_acc, (size_t)npix * f4 * 3);
   add(o_spill, (size_t)spill_entries * grid_threads * sizeof(int2));
+  size_t o_par;
+  add(o_par, sizeof(PtParams));
   if (buf.bytes < bytes) {
     AR_HIP(hipDeviceSynchronize());
     AR_HIP(buf.alloc(bytes));
@@ -375,6 +387,7 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
   B.stats = (unsigned long long*)(b + o_st);
   acc = (float4*)(b + o_acc);
   spill = (int2*)(b + o_spill);
+  params = b + o_par;
   capacity = cap;
   return AR_OK;
 }
@@ -384,38 +397,101 @@ Workspace::~Workspace() {
   if (graph) (void)hipGraphDestroy(graph);
 }
 
+// FNV-1a over the fields of the scene views a captured graph holds by value.
+static uint64_t view_hash(const ShadeView& v) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t x) {
+    for (int i = 0; i < 8; ++i) h = (h ^ ((x >> (8 * i)) & 255u)) * 1099511628211ull;
+  };
+  const SceneView& sv = v.sv;
+  for (const void* p : {(const void*)sv.nodes, (const void*)sv.qnodes, (const void*)sv.tris, (const void*)sv.insts,
+                        (const void*)v.objects, (const void*)v.materials, (const void*)v.lights, (const void*)v.idx,
+                        (const void*)v.nrm, (const void*)v.slots, (const void*)v.objmats})
+    mix((uint64_t)(uintptr_t)p);
+  mix(((uint64_t)(uint32_t)sv.tlas_root << 32) | (uint32_t)sv.qtlas_root);
+  for (int k = 0; k < 6; ++k) mix(bits_u(sv.tlas_box[k]));
+  mix(((uint64_t)sv.n_inst << 32) | v.n_lights);
+  mix(v.dist_mode);
+  return h;
+}
+
+// The bounce loop of one chunk: begin, generate, then per bounce extend ->
+// shade -> shadow -> reset.  Every kernel argument is fixed for a workspace
+// (buffers, scene views, launch configuration); the chunk's parameters are
+// written to device memory first (k_wf_params), so with use_graph the loop
+// is captured once in a HIP graph and replayed for every chunk with one
+// launch instead of 2 + 4 * max_depth.
 static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const ShadeView& v, const PtParams& P,
                                      hipStream_t st, bool use_graph) {
   constexpr int BLOCK = kTraceBlock;
-  uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU);
-  WaveBufs B = ws.B;
-  // Chunk parameters live in the kernel arguments, so a graph is captured
-  // per distinct (chunk size, sample offset) only when use_graph is set and
-  // replayed for every chunk with identical parameters.
+  const uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU);
+  const WaveBufs B = ws.B;
+  const PtParams* Pd = reinterpret_cast<const PtParams*>(ws.params);
   // Traversal stages use the scene's (autotuned) launch configuration.
-  const ar_trace_config& tc = s->tcfg;
-  uint32_t Gt = (uint32_t)(device_cus() * tc.blocks_per_cu / 8 * 8);
-  hipLaunchKernelGGL(k_wf_begin, dim3(1), dim3(64), 0, st, B);
-  hipLaunchKernelGGL((k_wf_generate<BLOCK>), dim3(G), dim3(BLOCK), 0, st, P, B);
-  for (uint32_t d = 0; d < P.max_depth; ++d) {
-    int qi = d & 1;
-#define AR_EXT(BB, SS)                                                                                         \
-  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                           \
-    hipLaunchKernelGGL((k_wf_extend<BB, SS>), dim3(clamp_resident(Gt, (const void*)&k_wf_extend<BB, SS>, BB)),  \
-                       dim3(BB), 0, st, v.sv, B, qi, ws.spill, tc.refill_min, tc.search_min);
-    AR_TRACE_VARIANTS(AR_EXT)
-#undef AR_EXT
-    hipLaunchKernelGGL((k_wf_shade<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, P, B, qi);
-#define AR_SHD(BB, SS)                                                                                         \
-  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                           \
-    hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(clamp_resident(Gt, (const void*)&k_wf_shadow<BB, SS>, BB)),  \
-                       dim3(BB), 0, st, v.sv, B, ws.spill, tc.refill_min, tc.search_min);
-    AR_TRACE_VARIANTS(AR_SHD)
-#undef AR_SHD
-    hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, B, qi);
+  const ar_trace_config tc = s->tcfg;
+  const uint32_t Gt = (uint32_t)(device_cus() * tc.blocks_per_cu / 8 * 8);
+  uint32_t Ge = 0, Gs = 0;  // resident-clamped grids, computed outside any capture
+#define AR_GRIDS(BB, SS)                                                      \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS) {                        \
+    Ge = clamp_resident(Gt, (const void*)&k_wf_extend<BB, SS>, BB);           \
+    Gs = clamp_resident(Gt, (const void*)&k_wf_shadow<BB, SS>, BB);           \
   }
-  (void)use_graph;
-  AR_HIP(hipGetLastError());
+  AR_TRACE_VARIANTS(AR_GRIDS)
+#undef AR_GRIDS
+  if (Ge == 0) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
+  hipLaunchKernelGGL(k_wf_params, dim3(1), dim3(64), 0, st, P, reinterpret_cast<PtParams*>(ws.params));
+  auto record = [&]() {
+    hipLaunchKernelGGL(k_wf_begin, dim3(1), dim3(64), 0, st, B);
+    hipLaunchKernelGGL((k_wf_generate<BLOCK>), dim3(G), dim3(BLOCK), 0, st, Pd, B);
+    for (uint32_t d = 0; d < P.max_depth; ++d) {
+      int qi = d & 1;
+#define AR_EXT(BB, SS)                                                                                   \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                     \
+    hipLaunchKernelGGL((k_wf_extend<BB, SS>), dim3(Ge), dim3(BB), 0, st, v.sv, B, qi, ws.spill, tc.refill_min, \
+                       tc.search_min);
+      AR_TRACE_VARIANTS(AR_EXT)
+#undef AR_EXT
+      hipLaunchKernelGGL((k_wf_shade<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, Pd, B, qi);
+#define AR_SHD(BB, SS)                                                                                   \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                     \
+    hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(Gs), dim3(BB), 0, st, v.sv, B, ws.spill, tc.refill_min,    \
+                       tc.search_min);
+      AR_TRACE_VARIANTS(AR_SHD)
+#undef AR_SHD
+      hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, B, qi);
+    }
+  };
+  if (!use_graph) {
+    record();
+    AR_HIP(hipGetLastError());
+    return AR_OK;
+  }
+  // what the graph depends on besides the chunk parameters
+  const uint64_t key[8] = {(uint64_t)(uintptr_t)s,
+                           (uint64_t)(uintptr_t)ws.buf.p,
+                           ((uint64_t)tc.workgroup_size << 32) | tc.lds_stack,
+                           ((uint64_t)tc.refill_min << 32) | tc.search_min,
+                           ((uint64_t)Ge << 32) | Gs,
+                           ((uint64_t)G << 32) | P.max_depth,
+                           view_hash(v),
+                           (uint64_t)ws.capacity};
+  if (!ws.graph_valid || std::memcmp(key, ws.graph_key, sizeof(key)) != 0) {
+    if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
+    if (ws.graph) (void)hipGraphDestroy(ws.graph);
+    ws.graph_exec = nullptr;
+    ws.graph = nullptr;
+    ws.graph_valid = false;
+    AR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    record();
+    hipError_t le = hipGetLastError();
+    hipError_t ce = hipStreamEndCapture(st, &ws.graph);
+    if (le != hipSuccess) return fail(AR_ERR_DEVICE, std::string("bounce loop capture: ") + hipGetErrorString(le));
+    AR_HIP(ce);
+    AR_HIP(hipGraphInstantiate(&ws.graph_exec, ws.graph, nullptr, nullptr, 0));
+    std::memcpy(ws.graph_key, key, sizeof(key));
+    ws.graph_valid = true;
+  }
+  AR_HIP(hipGraphLaunch(ws.graph_exec, st));
   return AR_OK;
 }
 
@@ -534,7 +610,7 @@ class PtJob : public JobRun {
         hipLaunchKernelGGL((k_pt_mega<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, ws->B.L, ws->spill,
                            ws->B.stats);
       } else {
-        ar_status e = run_chunk_wavefront(scene, *ws, v, P, stream, false);
+        ar_status e = run_chunk_wavefront(scene, *ws, v, P, stream, true);
         if (e != AR_OK) return e;
       }
       if (gdpt)
