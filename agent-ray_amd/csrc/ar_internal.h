@@ -142,7 +142,7 @@ struct ar_scene {
   uint32_t stack_depth = 0;  // traversal stack entries a query can need (either tree form)
   ar_trace_config tcfg{256, 8, 8, 24, 8, 0.0f};  // traversal launch configuration (autotuned)
   ar::DevBuf d_nodes, d_qnodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
-      d_tri_obj;
+      d_tri_obj, d_stri;
   ar::SceneView view{};
   uint32_t n_lights = 0;
   bool has_dist = false;
@@ -166,6 +166,11 @@ struct HostAccel {
   std::vector<DObject> objs;
   std::vector<uint32_t> gidx, gslots, gmats, tri_obj;
   std::vector<float> gnrm;
+  // per Tri record (leaf order), 3 float4: (n0.xyz, material id bits),
+  // (n1.xyz, 1 if vertex normals else 0), (n2.xyz, object id bits); object
+  // space vertex normals of the triangle, resolved from the object's index,
+  // normal, slot and material tables
+  std::vector<float4> gshade;
   int tlas_root = 0, qtlas_root = 0;
   float tlas_box[6] = {0, 0, 0, 0, 0, 0};
   uint32_t max_depth = 0, tlas_depth = 0;    // binary trees

@@ -18,6 +18,7 @@ struct ShadeView {
   const float* nrm;
   const uint32_t* slots;
   const uint32_t* objmats;
+  const float4* stri;  // per Tri record: vertex normals, material, object (3 float4)
   uint32_t n_lights;
   uint32_t dist_mode;  // 0 none, 1 uniform, 2 power
 };
@@ -45,23 +46,24 @@ struct ShadingPoint {
 // Scene::intersect's shading part (scene.rs:142-162) for a closest hit.
 __device__ __forceinline__ void make_shading(const ShadeView& v, const Ray& r, float t, uint32_t pos, uint32_t inst,
                                              float bu, float bv, ShadingPoint& sp) {
+  // The Tri record's shading record holds its vertex normals, material and
+  // object, resolved at commit from the object's tables, so those loads do
+  // not wait on each other (the instance record is not needed: a Tri record
+  // belongs to one object).
   const Tri* tr = v.sv.tris + pos;
-  float4 v0 = tr->v0, e1 = tr->e1, e2 = tr->e2;
-  uint32_t prim = __float_as_uint(v0.w) & ~kLastBit;
-  uint32_t oid = v.sv.insts[inst].id & ~kLastBit;
-  const DObject& ob = v.objects[oid];
+  const float4* srec = v.stri + 3 * (size_t)pos;
+  const float4 sa = srec[0], sb = srec[1], sc = srec[2];
+  const DObject& ob = v.objects[__float_as_uint(sc.w)];
   sp.p = r.o + r.d * t;
   f3 n_obj;
-  if (ob.nrm_base >= 0) {  // scene.rs:75-84
-    const uint32_t* ix = v.idx + ob.idx_base + 3 * prim;
-    const float* N = v.nrm + 3 * (size_t)ob.nrm_base;
-    uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
-    f3 n0 = mk(N[3 * i0], N[3 * i0 + 1], N[3 * i0 + 2]);
-    f3 n1 = mk(N[3 * i1], N[3 * i1 + 1], N[3 * i1 + 2]);
-    f3 n2 = mk(N[3 * i2], N[3 * i2 + 1], N[3 * i2 + 2]);
+  if (__float_as_uint(sb.w) != 0u) {  // scene.rs:75-84
+    f3 n0 = mk(sa.x, sa.y, sa.z);
+    f3 n1 = mk(sb.x, sb.y, sb.z);
+    f3 n2 = mk(sc.x, sc.y, sc.z);
     float w = 1.0f - bu - bv;
     n_obj = normalize((n0 * w + n1 * bu) + n2 * bv);
   } else {
+    float4 e1 = tr->e1, e2 = tr->e2;
     n_obj = normalize(cross(mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z)));  // geometry.rs:203
   }
   const float* M = ob.nmat;  // Mat3 * Vec3: (c0*x + c1*y) + c2*z
@@ -72,9 +74,7 @@ __device__ __forceinline__ void make_shading(const ShadeView& v, const Ray& r, f
   sp.ng = n;
   sp.n = dot(n, wo) < 0.0f ? -n : n;
   sp.onb = onb_from_normal(sp.n);
-  uint32_t slot = ob.n_slots == 1 ? v.slots[ob.slot_base] : v.slots[ob.slot_base + prim];  // scene.rs:41-48
-  uint32_t k = slot < ob.n_mats - 1 ? slot : ob.n_mats - 1;
-  sp.mat = v.objmats[ob.mat_base + k];
+  sp.mat = __float_as_uint(sa.w);  // scene.rs:41-48
 }
 
 // ---------------------------------------------------------------------------

@@ -26,18 +26,42 @@
 
 namespace ar {
 
-__device__ __forceinline__ uint32_t wave_push(uint32_t* counter, bool pred) {
-  // Wave-aggregated queue append: one atomic per wave, lane offsets by
-  // popcount of the ballot below the lane.
-  uint64_t mask = __ballot(pred);
-  if (mask == 0) return 0;
-  uint32_t lane = __lane_id();
-  uint32_t leader = __ffsll((unsigned long long)mask) - 1;
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-  base = __shfl(base, leader);
-  uint64_t below = lane == 0 ? 0ull : (mask & ((~0ull) >> (64 - lane)));
-  return base + (uint32_t)__popcll(below);
+// Workgroup-aggregated queue appends for NQ queues at once: every wave
+// ballots its flags, the waves' counts meet in LDS, and one lane per queue
+// takes the workgroup's range with a single atomic.  Appends from every wave
+// of the device to one counter serialise in L2 (about 11 ns each here), so
+// one atomic per workgroup instead of per wave is what keeps the generate
+// and shade stages from being bound by them.  Lanes get consecutive slots in
+// wave order.  Every thread of the workgroup must call it the same number of
+// times (it contains two barriers); `par` alternates between calls so one
+// call's LDS row is not overwritten while another wave still reads it.
+template <int BLOCK, int NQ>
+__device__ __forceinline__ void block_push(uint32_t (&sh)[2][NQ][BLOCK / 64], uint32_t par,
+                                           uint32_t* const (&counter)[NQ], const bool (&flag)[NQ],
+                                           uint32_t (&slot)[NQ]) {
+  const uint32_t wave = threadIdx.x / 64, lane = __lane_id();
+  uint64_t m[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    m[k] = __ballot(flag[k]);
+    if (lane == 0) sh[par][k][wave] = (uint32_t)__popcll(m[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    const int k = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      const uint32_t c = sh[par][k][w];
+      sh[par][k][w] = tot;
+      tot += c;
+    }
+    const uint32_t base = tot ? atomicAdd(counter[k], tot) : 0u;
+    for (int w = 0; w < BLOCK / 64; ++w) sh[par][k][w] += base;
+  }
+  __syncthreads();
+  const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) slot[k] = sh[par][k][wave] + (uint32_t)__popcll(m[k] & below);
 }
 
 __device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint32_t v) {
@@ -122,6 +146,8 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(const PtParams* __restric
   const uint32_t nthreads = gridDim.x * BLOCK;
   const uint32_t rounds = (npaths + nthreads - 1) / nthreads;
   const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+  __shared__ uint32_t qsh[2][1][BLOCK / 64];
+  uint32_t* const ctr[1] = {&B.cnt[0]};
   for (uint32_t r = 0; r < rounds; ++r) {
     uint32_t p = r * nthreads + gtid;
     bool live = false;
@@ -141,8 +167,10 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(const PtParams* __restric
       }
       store_state(B, p, s);
     }
-    uint32_t qi = wave_push(&B.cnt[0], live);
-    if (live) B.q[0][qi] = p;
+    const bool fl[1] = {live};
+    uint32_t slot[1];
+    block_push<BLOCK, 1>(qsh, r & 1, ctr, fl, slot);
+    if (live) B.q[0][slot[0]] = p;
   }
 }
 
@@ -202,6 +230,8 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, const PtParams*
   // Iterate whole waves together so the wave-aggregated pushes see every lane.
   uint32_t rounds = (n + nthreads - 1) / nthreads;
   uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  __shared__ uint32_t qsh[2][2][BLOCK / 64];
+  uint32_t* const ctr[2] = {&B.cnt[2], &B.cnt[qi ^ 1]};
   for (uint32_t r = 0; r < rounds; ++r) {
     uint32_t i = r * nthreads + gtid;
     bool active = i < n;
@@ -226,15 +256,17 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, const PtParams*
       s.dim = rng.dim;
       store_state(B, p, s);
     }
-    uint32_t si = wave_push(&B.cnt[2], active && sh.valid);
+    const bool fl[2] = {active && sh.valid, active && cont};
+    uint32_t slot[2];
+    block_push<BLOCK, 2>(qsh, r & 1, ctr, fl, slot);
+    const uint32_t si = slot[0];
     if (active && sh.valid) {
       B.sh_path[si] = p;
       B.sh_o[si] = make_float4(sh.ray.o.x, sh.ray.o.y, sh.ray.o.z, sh.ray.tmin);
       B.sh_d[si] = make_float4(sh.ray.d.x, sh.ray.d.y, sh.ray.d.z, sh.ray.tmax);
       B.sh_c[si] = make_float4(sh.c.x, sh.c.y, sh.c.z, 0.0f);
     }
-    uint32_t qi2 = wave_push(&B.cnt[qi ^ 1], active && cont);
-    if (active && cont) qn[qi2] = p;
+    if (active && cont) qn[slot[1]] = p;
   }
 }
 
@@ -312,6 +344,7 @@ ShadeView shade_view(const ar_scene* s) {
   v.nrm = s->d_nrm.as<float>();
   v.slots = s->d_slots.as<uint32_t>();
   v.objmats = s->d_objmats.as<uint32_t>();
+  v.stri = s->d_stri.as<float4>();
   v.n_lights = s->n_lights;
   v.dist_mode = s->has_dist ? s->dist_mode : AR_LIGHTDIST_NONE;
   return v;
@@ -406,7 +439,7 @@ static uint64_t view_hash(const ShadeView& v) {
   const SceneView& sv = v.sv;
   for (const void* p : {(const void*)sv.nodes, (const void*)sv.qnodes, (const void*)sv.tris, (const void*)sv.insts,
                         (const void*)v.objects, (const void*)v.materials, (const void*)v.lights, (const void*)v.idx,
-                        (const void*)v.nrm, (const void*)v.slots, (const void*)v.objmats})
+                        (const void*)v.nrm, (const void*)v.slots, (const void*)v.objmats, (const void*)v.stri})
     mix((uint64_t)(uintptr_t)p);
   mix(((uint64_t)(uint32_t)sv.tlas_root << 32) | (uint32_t)sv.qtlas_root);
   for (int k = 0; k < 6; ++k) mix(bits_u(sv.tlas_box[k]));
@@ -423,8 +456,10 @@ static uint64_t view_hash(const ShadeView& v) {
 // launch instead of 2 + 4 * max_depth.
 static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const ShadeView& v, const PtParams& P,
                                      hipStream_t st, bool use_graph) {
-  constexpr int BLOCK = kTraceBlock;
-  const uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU);
+  // generate / shade: 512-thread workgroups (8 waves share one queue atomic),
+  // as many threads per CU as the traversal stages' default
+  constexpr int BLOCK = 512;
+  const uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU * kTraceBlock / BLOCK);
   const WaveBufs B = ws.B;
   const PtParams* Pd = reinterpret_cast<const PtParams*>(ws.params);
   // Traversal stages use the scene's (autotuned) launch configuration.

@@ -483,6 +483,27 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     gmats.insert(gmats.end(), m.mats.begin(), m.mats.end());
     d.tri_base = tri_base;
     d.n_tris = nt;
+    // shading records of this object's Tri records (scene.rs:41-48, 75-84)
+    for (uint32_t r = 0; r < nt; ++r) {
+      uint32_t tag;
+      memcpy(&tag, &tris[tri_base + r].v0.w, 4);
+      const uint32_t prim = tag & ~kLastBit;
+      const uint32_t slot = m.slots.size() == 1 ? m.slots[0] : m.slots[prim];
+      const uint32_t mat = m.mats[slot < m.mats.size() - 1 ? slot : (uint32_t)m.mats.size() - 1];
+      float n[3][3] = {};
+      const uint32_t has = m.nrm.empty() ? 0u : 1u;
+      if (has)
+        for (int c = 0; c < 3; ++c)
+          for (int k = 0; k < 3; ++k) n[c][k] = m.nrm[3 * (size_t)m.idx[3 * prim + c] + k];
+      float mf, hf, of;
+      const uint32_t ou = (uint32_t)oi;
+      memcpy(&mf, &mat, 4);
+      memcpy(&hf, &has, 4);
+      memcpy(&of, &ou, 4);
+      h.gshade.push_back(make_float4(n[0][0], n[0][1], n[0][2], mf));
+      h.gshade.push_back(make_float4(n[1][0], n[1][1], n[1][2], hf));
+      h.gshade.push_back(make_float4(n[2][0], n[2][1], n[2][2], of));
+    }
   }
   // TLAS over instance world boxes (bvh.rs:395-410); instance records are
   // stored in TLAS leaf order.
@@ -596,6 +617,7 @@ ar_status ar_scene_commit(ar_scene* s) {
   AR_HIP(upload(s->d_slots, h.gslots));
   AR_HIP(upload(s->d_objmats, h.gmats));
   AR_HIP(upload(s->d_tri_obj, h.tri_obj));
+  AR_HIP(upload(s->d_stri, h.gshade));
   s->view.nodes = s->d_nodes.as<WNode>();
   s->view.qnodes = s->d_qnodes.as<QNode>();
   s->view.qtlas_root = h.qtlas_root;
@@ -615,7 +637,7 @@ ar_status ar_scene_commit(ar_scene* s) {
   s->info.max_depth = max_depth;
   s->info.device_bytes = s->d_nodes.bytes + s->d_qnodes.bytes + s->d_tris.bytes + s->d_insts.bytes + s->d_objects.bytes +
                          s->d_materials.bytes + s->d_lights.bytes + s->d_idx.bytes + s->d_nrm.bytes +
-                         s->d_slots.bytes + s->d_objmats.bytes + s->d_tri_obj.bytes;
+                         s->d_slots.bytes + s->d_objmats.bytes + s->d_tri_obj.bytes + s->d_stri.bytes;
   AR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   s->committed = true;
   return AR_OK;
