@@ -146,6 +146,8 @@ def _load():
         "ar_bvh_build_host": (i32, [P, u32, P, P, P, C.POINTER(u32), C.POINTER(u32)]),
         "ar_scene_set_trace_config": (i32, [P, C.POINTER(ArTraceConfig)]),
         "ar_scene_get_trace_config": (i32, [P, C.POINTER(ArTraceConfig)]),
+        "ar_scene_set_anyhit_config": (i32, [P, C.POINTER(ArTraceConfig)]),
+        "ar_scene_get_anyhit_config": (i32, [P, C.POINTER(ArTraceConfig)]),
         "ar_scene_autotune": (i32, [P, P, u64, u32, P, C.POINTER(ArTraceConfig)]),
     }
     for name, (res, args) in sig.items():
@@ -348,8 +350,20 @@ class Scene:
 
     def set_trace_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16,
                          search_min: int = 8):
+        """Launch configuration of closest-hit and any-hit traversal."""
         c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, 0.0)
         _check(_lib.ar_scene_set_trace_config(self._h, C.byref(c)))
+
+    def set_anyhit_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16,
+                          search_min: int = 8):
+        """Launch configuration of any-hit traversal only (occlusion, shadow rays)."""
+        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, 0.0)
+        _check(_lib.ar_scene_set_anyhit_config(self._h, C.byref(c)))
+
+    def anyhit_config(self) -> dict:
+        c = ArTraceConfig()
+        _check(_lib.ar_scene_get_anyhit_config(self._h, C.byref(c)))
+        return c.as_dict()
 
     def autotune(self, rays, stream=None) -> dict:
         """Times the traversal launch variants on a device ray sample (torch
@@ -357,7 +371,9 @@ class Scene:
         c = ArTraceConfig()
         sh = C.c_void_p(stream.cuda_stream) if stream is not None else None
         _check(_lib.ar_scene_autotune(self._h, C.c_void_p(rays.data_ptr()), rays.shape[0], 0, sh, C.byref(c)))
-        return c.as_dict()
+        d = c.as_dict()
+        d["anyhit"] = self.anyhit_config()
+        return d
 
     def occluded(self, rays, ref_order: bool = False):
         """Scene::occluded (scene.rs:166-168) as an any-hit query."""

@@ -113,6 +113,7 @@ struct SceneView {
   int qtlas_root;       // the same in the 4-wide node array
   float tlas_box[6];    // TLAS root box (world)
   uint32_t n_inst;
+  uint32_t wide_ok;     // every 4-wide node's origin and scale are below 2^62 in magnitude
 };
 
 struct Ray {

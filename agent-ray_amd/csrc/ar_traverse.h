@@ -125,6 +125,17 @@ constexpr int kLevelEnd = (int)0x80000000;  // cur: the current level's stack is
 // independent of the order, which depends on the wave a ray shares.
 constexpr float kCullSlack = 1.0f + 0x1p-18f;
 
+// Whether a level's ray walks the 4-wide trees (near-first order): finite
+// 1/d bounded by 2^32 and an origin bounded by 2^64, with node origins and
+// scales below 2^62 (SceneView::wide_ok), keep every term of the 4-wide box
+// test finite and far from underflow-only ranges; other rays walk the
+// binary trees with the exact slab test.
+AR_HD bool wide_ray(const SceneView& sv, f3 o, f3 inv) {
+  const float mi = fmaxf(fmaxf(fabsf(inv.x), fabsf(inv.y)), fabsf(inv.z));
+  const float mo = fmaxf(fmaxf(fabsf(o.x), fabsf(o.y)), fabsf(o.z));
+  return sv.wide_ok != 0u && mi <= 0x1p32f && mo <= 0x1p64f;
+}
+
 template <bool ANY, bool NEAR>
 struct Walker {
   f3 wo, wd;              // world ray
@@ -167,7 +178,7 @@ struct Walker {
     in_blas = false;
     park0 = park1 = kLevelEnd;
     fast = finite_inv(inv);
-    wide = NEAR && fast;
+    wide = NEAR && fast && wide_ray(sv, o, inv);
     cur = wide ? sv.qtlas_root : sv.tlas_root;
     float tn;
     const float* tb = sv.tlas_box;
@@ -201,48 +212,64 @@ struct Walker {
     const QNode* n = sv.qnodes + cur;
     const float4 a = n->a, b = n->b, c = n->c;
     const int4 rf = n->d;
-    // With a finite 1/d (the only case the 4-wide trees are walked) the
-    // entry plane of every child on axis k is its low plane when 1/d_k >= 0
-    // and its high plane otherwise (dequantized planes are ordered and
-    // (x - o) * inv is monotone), so one select per axis per node replaces
-    // the per-child min / max of the slab test; the decisions and entry
-    // distances are those of slab_fast.
+    // Plane t per child and axis as one fma, q * A + B -/+ E, with A = scl / d
+    // (exact: scl is a power of two) and B = (org - o) / d per node and axis.
+    // With u = 2^-24, fmaf(q, A, B) differs from the (fmaf(q, scl, org) - o)
+    // * (1 / d) of the exact-box test by at most about
+    // u (770 |A| + 4.03 |B| + 1.01 |org / d|) (rounding of org + q scl and of
+    // both formulas), and rounding B -/+ E and the fma adds at most
+    // u (255 |A| + 2 |B| + 2 E).  So with, per axis,
+    //   E = 2^-12 |A| + 2^-20 |B| + 2^-21 |org / d| + 2^-126
+    // (2^-126 covers underflow; wide_ray keeps every term finite) the near
+    // planes' q * A + (B - E) are not above, and the far planes'
+    // q * A + (B + E) not below, the exact-box test's values.  Every child it
+    // accepts is accepted here with an entry distance that is not larger:
+    // the walk is conservative and the triangle tests decide the hit.  E is
+    // per axis because an axis the ray is nearly parallel to has a huge A,
+    // and its bound must not loosen the other axes.
+    const float Ax = a.w * inv.x, Ay = b.x * inv.y, Az = b.y * inv.z;
+    const float Bx = (a.x - o.x) * inv.x, By = (a.y - o.y) * inv.y, Bz = (a.z - o.z) * inv.z;
+    const float Ex = fmaf(fabsf(Ax), 0x1p-12f, fmaf(fabsf(Bx), 0x1p-20f, fmaf(fabsf(a.x * inv.x), 0x1p-21f, 0x1p-126f)));
+    const float Ey = fmaf(fabsf(Ay), 0x1p-12f, fmaf(fabsf(By), 0x1p-20f, fmaf(fabsf(a.y * inv.y), 0x1p-21f, 0x1p-126f)));
+    const float Ez = fmaf(fabsf(Az), 0x1p-12f, fmaf(fabsf(Bz), 0x1p-20f, fmaf(fabsf(a.z * inv.z), 0x1p-21f, 0x1p-126f)));
+    const float Bnx = Bx - Ex, Bny = By - Ey, Bnz = Bz - Ez;
+    const float Bfx = Bx + Ex, Bfy = By + Ey, Bfz = Bz + Ez;
+    // With finite 1/d the entry plane of every child on axis k is its low
+    // plane when 1/d_k >= 0 and its high plane otherwise.
     const bool px = inv.x >= 0.0f, py = inv.y >= 0.0f, pz = inv.z >= 0.0f;
     const uint32_t lx = bits_u(b.z), ly = bits_u(b.w), lz = bits_u(c.x);
     const uint32_t hx = bits_u(c.y), hy = bits_u(c.z), hz = bits_u(c.w);
     const uint32_t nx = px ? lx : hx, ny = py ? ly : hy, nz = pz ? lz : hz;
     const uint32_t fx = px ? hx : lx, fy = py ? hy : ly, fz = pz ? hz : lz;
+    const float tc = tcull();
     float t[4];
     int r[4] = {rf.x, rf.y, rf.z, rf.w};
-    const float tc = tcull();
 #ifdef __clang__
 #pragma unroll
 #endif
     for (int k = 0; k < 4; ++k) {
       const int sh = 8 * k;
-      // org + q * scl with q * scl exact: one rounding, as the builder computed
-      const float tnx = (fmaf((float)((nx >> sh) & 255u), a.w, a.x) - o.x) * inv.x;
-      const float tny = (fmaf((float)((ny >> sh) & 255u), b.x, a.y) - o.y) * inv.y;
-      const float tnz = (fmaf((float)((nz >> sh) & 255u), b.y, a.z) - o.z) * inv.z;
-      const float tfx = (fmaf((float)((fx >> sh) & 255u), a.w, a.x) - o.x) * inv.x;
-      const float tfy = (fmaf((float)((fy >> sh) & 255u), b.x, a.y) - o.y) * inv.y;
-      const float tfz = (fmaf((float)((fz >> sh) & 255u), b.y, a.z) - o.z) * inv.z;
+      const float tnx = fmaf((float)((nx >> sh) & 255u), Ax, Bnx);
+      const float tny = fmaf((float)((ny >> sh) & 255u), Ay, Bny);
+      const float tnz = fmaf((float)((nz >> sh) & 255u), Az, Bnz);
+      const float tfx = fmaf((float)((fx >> sh) & 255u), Ax, Bfx);
+      const float tfy = fmaf((float)((fy >> sh) & 255u), Ay, Bfy);
+      const float tfz = fmaf((float)((fz >> sh) & 255u), Az, Bfz);
       const float tn = fmaxf(fmaxf(fmaxf(tnx, tny), tnz), tmin);
       const float tf = fminf(fminf(fminf(tfx, tfy), tfz), tc);
       const bool h = tn <= tf && r[k] != kEmptyRef;
       t[k] = h ? tn : INFINITY;
       r[k] = h ? r[k] : kEmptyRef;
     }
-    // nearest first: sort the four (t, ref) pairs by t
+    // nearest first: a sorting network of selects over the (t, ref) pairs
     auto cx = [&](int i, int j) {
-      if (t[j] < t[i]) {
-        float tt = t[i];
-        t[i] = t[j];
-        t[j] = tt;
-        int rr = r[i];
-        r[i] = r[j];
-        r[j] = rr;
-      }
+      const bool sw = t[j] < t[i];
+      const float ti = sw ? t[j] : t[i], tj = sw ? t[i] : t[j];
+      const int ri = sw ? r[j] : r[i], rj = sw ? r[i] : r[j];
+      t[i] = ti;
+      t[j] = tj;
+      r[i] = ri;
+      r[j] = rj;
     };
     cx(0, 1);
     cx(2, 3);
@@ -377,7 +404,7 @@ struct Walker {
           ik = k;
           base = stk.sp;
           fast = finite_inv(li);
-          wide = NEAR && fast;
+          wide = NEAR && fast && wide_ray(sv, lo, li);
           cur = wide ? in->qroot : in->root;
           in_blas = true;
           lpos = kNoHit;
@@ -498,7 +525,7 @@ struct Walker {
       d = wd;
       inv = mk(1.0f / wd.x, 1.0f / wd.y, 1.0f / wd.z);
       fast = finite_inv(inv);
-      wide = NEAR && fast;
+      wide = NEAR && fast && wide_ray(sv, o, inv);
       tmin = wtmin;
       tmax = bt;
       if (!(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;

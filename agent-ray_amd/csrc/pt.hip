@@ -462,18 +462,18 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   const uint32_t G = persistent_grid(BLOCK, kTraceBlocksPerCU * kTraceBlock / BLOCK);
   const WaveBufs B = ws.B;
   const PtParams* Pd = reinterpret_cast<const PtParams*>(ws.params);
-  // Traversal stages use the scene's (autotuned) launch configuration.
-  const ar_trace_config tc = s->tcfg;
+  // Traversal stages use the scene's (autotuned) launch configurations:
+  // closest hit for extension rays, any hit for shadow rays.
+  const ar_trace_config tc = s->tcfg, ta = s->tcfg_any;
   const uint32_t Gt = (uint32_t)(device_cus() * tc.blocks_per_cu / 8 * 8);
+  const uint32_t Ga = (uint32_t)(device_cus() * ta.blocks_per_cu / 8 * 8);
   uint32_t Ge = 0, Gs = 0;  // resident-clamped grids, computed outside any capture
-#define AR_GRIDS(BB, SS)                                                      \
-  if (tc.workgroup_size == BB && tc.lds_stack == SS) {                        \
-    Ge = clamp_resident(Gt, (const void*)&k_wf_extend<BB, SS>, BB);           \
-    Gs = clamp_resident(Gt, (const void*)&k_wf_shadow<BB, SS>, BB);           \
-  }
+#define AR_GRIDS(BB, SS)                                                                                   \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS) Ge = clamp_resident(Gt, (const void*)&k_wf_extend<BB, SS>, BB); \
+  if (ta.workgroup_size == BB && ta.lds_stack == SS) Gs = clamp_resident(Ga, (const void*)&k_wf_shadow<BB, SS>, BB);
   AR_TRACE_VARIANTS(AR_GRIDS)
 #undef AR_GRIDS
-  if (Ge == 0) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
+  if (Ge == 0 || Gs == 0) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
   hipLaunchKernelGGL(k_wf_params, dim3(1), dim3(64), 0, st, P, reinterpret_cast<PtParams*>(ws.params));
   auto record = [&]() {
     hipLaunchKernelGGL(k_wf_begin, dim3(1), dim3(64), 0, st, B);
@@ -488,9 +488,9 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
 #undef AR_EXT
       hipLaunchKernelGGL((k_wf_shade<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, Pd, B, qi);
 #define AR_SHD(BB, SS)                                                                                   \
-  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                     \
-    hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(Gs), dim3(BB), 0, st, v.sv, B, ws.spill, tc.refill_min,    \
-                       tc.search_min);
+  if (ta.workgroup_size == BB && ta.lds_stack == SS)                                                     \
+    hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(Gs), dim3(BB), 0, st, v.sv, B, ws.spill, ta.refill_min,    \
+                       ta.search_min);
       AR_TRACE_VARIANTS(AR_SHD)
 #undef AR_SHD
       hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, B, qi);
@@ -504,8 +504,10 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   // what the graph depends on besides the chunk parameters
   const uint64_t key[8] = {(uint64_t)(uintptr_t)s,
                            (uint64_t)(uintptr_t)ws.buf.p,
-                           ((uint64_t)tc.workgroup_size << 32) | tc.lds_stack,
-                           ((uint64_t)tc.refill_min << 32) | tc.search_min,
+                           ((uint64_t)tc.workgroup_size << 48) | ((uint64_t)tc.lds_stack << 32) |
+                               ((uint64_t)ta.workgroup_size << 16) | ta.lds_stack,
+                           ((uint64_t)tc.refill_min << 48) | ((uint64_t)tc.search_min << 32) |
+                               ((uint64_t)ta.refill_min << 16) | ta.search_min,
                            ((uint64_t)Ge << 32) | Gs,
                            ((uint64_t)G << 32) | P.max_depth,
                            view_hash(v),
@@ -610,13 +612,14 @@ class PtJob : public JobRun {
     G = persistent_grid(BLOCK, kTraceBlocksPerCU);
     eg = std::min<uint32_t>((npix + 255) / 256, 4096);
     // Spill area: covers the megakernel's (128, 16) launch and the traversal
-    // stages' tuned launch, whichever runs.
+    // stages' tuned launches (closest hit, any hit), whichever runs.
     uint32_t need = scene->stack_depth;
-    const ar_trace_config& tc = scene->tcfg;
-    uint64_t gt = (uint64_t)(device_cus() * tc.blocks_per_cu / 8 * 8) * tc.workgroup_size;
-    uint64_t e_t = need > tc.lds_stack ? need - tc.lds_stack : 0;
-    uint64_t e_m = stack_spill_entries(scene);
-    uint64_t spill_words = std::max<uint64_t>(e_t * gt, e_m * (uint64_t)G * BLOCK);
+    uint64_t spill_words = stack_spill_entries(scene) * (uint64_t)G * BLOCK;
+    for (const ar_trace_config* tc : {&scene->tcfg, &scene->tcfg_any}) {
+      uint64_t gt = (uint64_t)(device_cus() * tc->blocks_per_cu / 8 * 8) * tc->workgroup_size;
+      uint64_t e_t = need > tc->lds_stack ? need - tc->lds_stack : 0;
+      spill_words = std::max<uint64_t>(spill_words, e_t * gt);
+    }
     ar_status e = ws->reserve_paths(P.variants * npix * k, npix, (uint32_t)((spill_words + G * BLOCK - 1) / (G * BLOCK)),
                                     G * BLOCK);
     if (e != AR_OK) return e;

@@ -525,6 +525,14 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
   h.tlas_root = tf.root_ref;
   h.qtlas_root = tq.root_ref;
   h.qtlas_depth = tq.depth;
+  // the 4-wide box test evaluates q * (scl / d) + (org - o) / d per plane;
+  // bounded origins and scales keep every term finite for the rays it accepts
+  h.wide_ok = 1;
+  for (const QNode& q : h.qnodes) {
+    const float v[6] = {q.a.x, q.a.y, q.a.z, q.a.w, q.b.x, q.b.y};
+    for (float x : v)
+      if (!(std::fabs(x) < 0x1p62f)) h.wide_ok = 0;
+  }
   for (int k = 0; k < 3; ++k) {
     h.tlas_box[k] = tf.root_box.lo[k];
     h.tlas_box[3 + k] = tf.root_box.hi[k];
@@ -626,6 +634,7 @@ ar_status ar_scene_commit(ar_scene* s) {
   s->view.tlas_root = h.tlas_root;
   for (int k = 0; k < 6; ++k) s->view.tlas_box[k] = h.tlas_box[k];
   s->view.n_inst = (uint32_t)h.insts.size();
+  s->view.wide_ok = h.wide_ok;
   s->n_lights = nl;
   s->has_emitters = emit;
   s->h_lights = dl;

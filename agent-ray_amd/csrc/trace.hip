@@ -258,8 +258,8 @@ ar_status ar_occluded(const ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
     d_rays = sg.rays.as<ar_ray>();
     d_occ = sg.out.as<uint32_t>();
   }
-  ar_status e = (flags & AR_REF_ORDER) ? launch_trace<true, false>(s, d_rays, n, nullptr, nullptr, d_occ, st, s->tcfg)
-                                       : launch_trace<true, true>(s, d_rays, n, nullptr, nullptr, d_occ, st, s->tcfg);
+  ar_status e = (flags & AR_REF_ORDER) ? launch_trace<true, false>(s, d_rays, n, nullptr, nullptr, d_occ, st, s->tcfg_any)
+                                       : launch_trace<true, true>(s, d_rays, n, nullptr, nullptr, d_occ, st, s->tcfg_any);
   if (e != AR_OK) return e;
   if (host) AR_HIP(hipMemcpyAsync(occluded, d_occ, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   if (host || !(flags & AR_ASYNC)) AR_HIP(hipStreamSynchronize(st));
@@ -270,16 +270,32 @@ ar_status ar_occluded(const ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
 
 extern "C" {
 
-ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c) {
-  if (!s || !c) return fail(AR_ERR_INVALID_ARG, "ar_scene_set_trace_config: NULL argument");
+static ar_status check_trace_config(const ar_scene* s, const ar_trace_config* c, const char* who) {
+  if (!s || !c) return fail(AR_ERR_INVALID_ARG, std::string(who) + ": NULL argument");
   if (!trace_variant_supported(c->workgroup_size, c->lds_stack))
     return fail(AR_ERR_UNSUPPORTED_TYPE, "no traversal variant for this workgroup size / stack depth");
   if (c->blocks_per_cu == 0 || c->blocks_per_cu > 64) return fail(AR_ERR_INVALID_ARG, "blocks_per_cu out of range");
   if (c->refill_min == 0 || c->refill_min > 64) return fail(AR_ERR_INVALID_ARG, "refill_min out of range (1..64)");
   if (c->search_min > 63) return fail(AR_ERR_INVALID_ARG, "search_min out of range (0..63)");
+  return AR_OK;
+}
+
+ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c) {
+  ar_status e = check_trace_config(s, c, "ar_scene_set_trace_config");
+  if (e != AR_OK) return e;
   std::lock_guard<std::mutex> lk(s->mu);
   s->tcfg = *c;
   s->tcfg.ms = 0.0f;
+  s->tcfg_any = s->tcfg;
+  return AR_OK;
+}
+
+ar_status ar_scene_set_anyhit_config(ar_scene* s, const ar_trace_config* c) {
+  ar_status e = check_trace_config(s, c, "ar_scene_set_anyhit_config");
+  if (e != AR_OK) return e;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->tcfg_any = *c;
+  s->tcfg_any.ms = 0.0f;
   return AR_OK;
 }
 
@@ -289,21 +305,30 @@ ar_status ar_scene_get_trace_config(const ar_scene* s, ar_trace_config* c) {
   return AR_OK;
 }
 
-// Times every (variant, blocks per CU) candidate on the sample (closest hit,
-// near-first order, one warm-up launch each) and keeps the fastest.
+ar_status ar_scene_get_anyhit_config(const ar_scene* s, ar_trace_config* c) {
+  if (!s || !c) return fail(AR_ERR_INVALID_ARG, "ar_scene_get_anyhit_config: NULL argument");
+  *c = s->tcfg_any;
+  return AR_OK;
+}
+
+// Times every (variant, blocks per CU, refill threshold) candidate on the
+// sample, near-first order, one warm-up launch each, once for closest hit and
+// once for any hit (their best launches differ), and keeps the fastest of
+// each.
 ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t flags, void* stream,
                             ar_trace_config* best) {
   if (!s || !rays || n == 0) return fail(AR_ERR_INVALID_ARG, "ar_scene_autotune: NULL or empty sample");
   if (!s->committed) return fail(AR_ERR_STATE, "ar_scene_autotune: scene not committed");
   if (flags & AR_HOST_PTRS) return fail(AR_ERR_INVALID_ARG, "ar_scene_autotune: sample must be on the device");
   hipStream_t st = pick_stream(s, stream);
-  DevBuf hits;
+  DevBuf hits, occ;
   AR_HIP(hits.alloc(n * sizeof(ar_hit)));
+  AR_HIP(occ.alloc(n * sizeof(uint32_t)));
   hipEvent_t e0, e1;
   AR_HIP(hipEventCreate(&e0));
   AR_HIP(hipEventCreate(&e1));
-  ar_trace_config winner = s->tcfg;
-  float best_ms = INFINITY;
+  ar_trace_config winners[2] = {s->tcfg, s->tcfg_any};
+  float best_ms[2] = {INFINITY, INFINITY};
   const uint32_t per_cu[3] = {4, 8, 16};
   const uint32_t refills[3] = {8, 24, 48};
   const uint32_t variants[][2] = {
@@ -312,38 +337,46 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
 #undef AR_LIST
   };
   ar_status err = AR_OK;
-  for (auto& vv : variants) {
-    for (uint32_t pc : per_cu) {
-      for (uint32_t rf : refills) {
-        ar_trace_config c{vv[0], vv[1], pc, rf, s->tcfg.search_min, 0.0f};
-        err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
-        if (err != AR_OK) break;
-        (void)hipEventRecord(e0, st);
-        err = launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c);
-        if (err != AR_OK) break;
-        (void)hipEventRecord(e1, st);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < best_ms) {
-          best_ms = ms;
-          winner = c;
+  for (int q = 0; q < 2 && err == AR_OK; ++q) {
+    auto launch = [&](const ar_trace_config& c) {
+      return q == 0 ? launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c)
+                    : launch_trace<true, true>(s, rays, n, nullptr, nullptr, occ.as<uint32_t>(), st, c);
+    };
+    for (auto& vv : variants) {
+      for (uint32_t pc : per_cu) {
+        for (uint32_t rf : refills) {
+          ar_trace_config c{vv[0], vv[1], pc, rf, s->tcfg.search_min, 0.0f};
+          err = launch(c);
+          if (err != AR_OK) break;
+          (void)hipEventRecord(e0, st);
+          err = launch(c);
+          if (err != AR_OK) break;
+          (void)hipEventRecord(e1, st);
+          (void)hipEventSynchronize(e1);
+          float ms = 0.0f;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          if (ms < best_ms[q]) {
+            best_ms[q] = ms;
+            winners[q] = c;
+          }
         }
+        if (err != AR_OK) break;
       }
       if (err != AR_OK) break;
     }
-    if (err != AR_OK) break;
   }
   (void)hipStreamSynchronize(st);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (err != AR_OK) return err;
-  winner.ms = best_ms;
+  winners[0].ms = best_ms[0];
+  winners[1].ms = best_ms[1];
   {
     std::lock_guard<std::mutex> lk(s->mu);
-    s->tcfg = winner;
+    s->tcfg = winners[0];
+    s->tcfg_any = winners[1];
   }
-  if (best) *best = winner;
+  if (best) *best = winners[0];
   return AR_OK;
 }
 

@@ -172,7 +172,7 @@ def bench_traversal(args, ar, torch):
         "closest_kernel_ms": round(float(np.mean(t_closest)), 3),
         "anyhit_kernel_ms": round(float(np.mean(t_any)), 3),
         "scene_build_s": round(build_s, 2),
-        "launch_config": sc.trace_config(),
+        "launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()),
     }
     # per-half rates (coherent / incoherent), one extra timed call each
     for name, sl in (("coherent", slice(0, n_rays // 2)), ("incoherent", slice(n_rays // 2, n_rays))):
@@ -198,7 +198,7 @@ def bench_traversal(args, ar, torch):
                            "kernel_ms": round(float(np.mean(t_closest)), 3)}
         if not args.no_traffic:
             c = sc.trace_config()
-            cfg = f"{c['workgroup_size']},{c['lds_stack']},{c['blocks_per_cu']},{c['refill_min']}"
+            cfg = f"{c['workgroup_size']},{c['lds_stack']},{c['blocks_per_cu']},{c['refill_min']},{c['search_min']}"
             traffic, detail = measure_traffic(cfg)
             out["roofline"]["traffic"] = None if traffic is None else round(traffic)
             out["roofline"]["traffic_detail"] = detail
@@ -309,6 +309,27 @@ def C_void(p):
     return ctypes.c_void_p(p)
 
 
+def autotune_scene(ar, torch, sc, desc, aspect):
+    """Per-scene traversal autotuning (closest and any hit) on 2^20 rays:
+    camera rays and rays from random interior points (what the extension and
+    shadow stages trace)."""
+    wl = ar.workloads
+    cd = desc.camera
+    rng = np.random.default_rng(3)
+    m = 1 << 19
+    cam16 = wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, aspect)
+    cr = wl.camera_rays(cam16, rng.uniform(0, 1, m), rng.uniform(0, 1, m))
+    lo, hi = desc.bounds()
+    o = rng.uniform(lo, hi, (m, 3)).astype(np.float32)
+    d = wl.normalize_f32(rng.standard_normal((m, 3)).astype(np.float32))
+    rr = np.zeros((m, 8), np.float32)
+    rr[:, 0:3], rr[:, 3], rr[:, 4:7], rr[:, 7] = o, 1e-4, d, wl.F32_MAX
+    sample = torch.from_numpy(np.concatenate([cr, rr]).astype(np.float32)).cuda()
+    cfg = sc.autotune(sample)
+    del sample
+    return cfg
+
+
 def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
     """Path tracer throughput on the Cornell box (256^2, 16 spp, depth 8) or
     the procedural room (1920x1080, 64 spp, depth 16).  Mpaths/s = W*H*spp /
@@ -323,20 +344,7 @@ def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
                                                        desc.width / desc.height))
     out = {"width": desc.width, "height": desc.height, "spp": desc.spp, "max_depth": desc.max_depth,
            "triangles": desc.n_triangles, "scene_build_s": round(build, 2)}
-    # per-scene traversal autotuning on 2^20 rays: camera rays and rays from
-    # random interior points (what the extension and shadow stages trace)
-    rng = np.random.default_rng(3)
-    m = 1 << 19
-    cam16 = wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, desc.width / desc.height)
-    cr = wl.camera_rays(cam16, rng.uniform(0, 1, m), rng.uniform(0, 1, m))
-    lo, hi = desc.bounds()
-    o = rng.uniform(lo, hi, (m, 3)).astype(np.float32)
-    d = wl.normalize_f32(rng.standard_normal((m, 3)).astype(np.float32))
-    rr = np.zeros((m, 8), np.float32)
-    rr[:, 0:3], rr[:, 3], rr[:, 4:7], rr[:, 7] = o, 1e-4, d, wl.F32_MAX
-    sample = torch.from_numpy(np.concatenate([cr, rr]).astype(np.float32)).cuda()
-    out["launch_config"] = sc.autotune(sample)
-    del sample
+    out["launch_config"] = autotune_scene(ar, torch, sc, desc, desc.width / desc.height)
     films = {}
     for sched in schedules:
         pt = ar.PathTracer(ar.PathTracerConfig(spp=desc.spp, max_depth=desc.max_depth, rr_depth=3), seed=0,
@@ -374,6 +382,7 @@ def bench_bdpt_gdpt(ar, torch):
     w, h = 1280, 720
     cam = ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, w / h))
     out = {}
+    tuned = autotune_scene(ar, torch, sc, desc, w / h)
     bd = ar.BidirectionalPathTracer(ar.BdptConfig(spp=16, max_depth=10, rr_depth=3), seed=0)
     bd.render(sc, cam, w, h)
     torch.cuda.synchronize()
@@ -381,7 +390,7 @@ def bench_bdpt_gdpt(ar, torch):
     bd.render(sc, cam, w, h)
     dt = time.perf_counter() - t0
     st = bd.last_stats
-    out["bdpt_room_1280x720x16spp"] = {"msamples_s": round(w * h * 16 / dt / 1e6, 2),
+    out["bdpt_room_1280x720x16spp"] = {"launch_config": tuned, "msamples_s": round(w * h * 16 / dt / 1e6, 2),
                                        "mrays_s": round(st.rays / dt / 1e6, 1), "seconds": round(dt, 4),
                                        "rays": int(st.rays)}
     log(f"bdpt room: {out['bdpt_room_1280x720x16spp']}")

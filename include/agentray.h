@@ -195,8 +195,12 @@ typedef struct ar_trace_config {
   uint32_t search_min; /* 0..63: a wave's node phase goes on while more lanes than this have no leaf */
   float ms; /* measured time of the winning candidate (autotune only) */
 } ar_trace_config;
-ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c);
-ar_status ar_scene_get_trace_config(const ar_scene* s, ar_trace_config* c);
+ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c);  /* closest and any hit */
+ar_status ar_scene_get_trace_config(const ar_scene* s, ar_trace_config* c);  /* closest hit */
+/* Any-hit launches (ar_occluded, the path tracers' shadow stage) keep their
+ * own configuration: ar_scene_autotune tunes it separately. */
+ar_status ar_scene_set_anyhit_config(ar_scene* s, const ar_trace_config* c);
+ar_status ar_scene_get_anyhit_config(const ar_scene* s, ar_trace_config* c);
 ar_status ar_scene_autotune(ar_scene* s, const ar_ray* sample_rays, uint64_t n, uint32_t flags, void* stream,
                             ar_trace_config* best);
 

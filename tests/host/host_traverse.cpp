@@ -116,6 +116,7 @@ extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any
   sv.qtlas_root = h.qtlas_root;
   for (int k = 0; k < 6; ++k) sv.tlas_box[k] = h.tlas_box[k];
   sv.n_inst = (uint32_t)h.insts.size();
+  sv.wide_ok = h.wide_ok;
   if (any && near) return run<true, true>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
   if (any) return run<true, false>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
   if (near) return run<false, true>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);

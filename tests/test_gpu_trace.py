@@ -199,6 +199,16 @@ def test_every_launch_variant_same_answer(ar, wl):
     cfg = sc.autotune(torch.from_numpy(rays).cuda())
     assert cfg["ms"] > 0 and cfg["workgroup_size"] in (64, 128, 256)
     assert cfg["refill_min"] in (8, 24, 48)
+    # any hit keeps its own (separately tuned) launch configuration
+    assert cfg["anyhit"]["ms"] > 0 and cfg["anyhit"]["workgroup_size"] in (64, 128, 256)
+    closest_cfg = sc.trace_config()
+    sc.set_anyhit_config(64, 16, 4, 8, 0)
+    assert sc.anyhit_config()["lds_stack"] == 16 and sc.trace_config() == closest_cfg
+    assert np.array_equal(sc.occluded(rays), ref_o)
+    t, p, _ = sc.intersect(rays)
+    assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p)
+    with pytest.raises(ar.ArError):
+        sc.set_anyhit_config(64, 8, 4, 24, 64)
     with pytest.raises(ar.ArError):
         sc.set_trace_config(96, 8, 4)
     with pytest.raises(ar.ArError):
