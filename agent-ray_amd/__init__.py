@@ -36,6 +36,8 @@ AR_OK = 0
 AR_HOST_PTRS = 0x1
 AR_ASYNC = 0x2
 AR_REF_ORDER = 0x4
+AR_TUNE_CLOSEST = 0x10
+AR_TUNE_ANYHIT = 0x20
 NO_HIT = 0xFFFFFFFF
 
 
@@ -365,12 +367,13 @@ class Scene:
         _check(_lib.ar_scene_get_anyhit_config(self._h, C.byref(c)))
         return c.as_dict()
 
-    def autotune(self, rays, stream=None) -> dict:
+    def autotune(self, rays, stream=None, which: str = "both") -> dict:
         """Times the traversal launch variants on a device ray sample (torch
         CUDA tensor, (n, 8) float32) and keeps the fastest for this scene."""
         c = ArTraceConfig()
         sh = C.c_void_p(stream.cuda_stream) if stream is not None else None
-        _check(_lib.ar_scene_autotune(self._h, C.c_void_p(rays.data_ptr()), rays.shape[0], 0, sh, C.byref(c)))
+        flags = {"both": 0, "closest": AR_TUNE_CLOSEST, "anyhit": AR_TUNE_ANYHIT}[which]
+        _check(_lib.ar_scene_autotune(self._h, C.c_void_p(rays.data_ptr()), rays.shape[0], flags, sh, C.byref(c)))
         d = c.as_dict()
         d["anyhit"] = self.anyhit_config()
         return d
@@ -532,6 +535,50 @@ class _Integrator:
         c.poisson_sign = int(getattr(self.config, "poisson_sign", 1))
         c.gdpt_outputs = 1 if self.integrator_id == 2 else 0
         return c
+
+    def tune_launch(self, scene: Scene, camera: PinholeCamera, width: int, height: int, spp: int = 16,
+                    candidates=None) -> dict:
+        """Refines the scene's launch configurations on the job itself: times
+        a render of `spp` samples per pixel (one chunk of the real workload,
+        wavefront schedule) for each candidate (workgroup, LDS stack, blocks
+        per CU, refill threshold), the scene's current one first; first for
+        the closest-hit launch (extension stage), then, with the winner, for
+        the any-hit launch (shadow stage).  Keeps the fastest of each and
+        returns them with the timings.  Results do not depend on the launch
+        configuration, so this only changes speed."""
+        if candidates is None:
+            candidates = [(256, 8, 8, 8), (256, 8, 8, 24), (128, 8, 16, 8), (128, 8, 16, 24), (64, 16, 16, 8),
+                          (128, 16, 8, 8), (128, 8, 8, 8)]
+        key = lambda c: (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"])
+        cur, anyc = scene.trace_config(), scene.anyhit_config()
+        sm_c, sm_a = cur["search_min"], anyc["search_min"]
+        best_c, best_a = key(cur), key(anyc)
+        saved = (self.config.spp, self.schedule)
+        self.config.spp, self.schedule = spp, Schedule.WAVEFRONT
+        tried = {}
+
+        def timed(c, a):
+            scene.set_trace_config(*c, sm_c)
+            scene.set_anyhit_config(*a, sm_a)
+            self.render(scene, camera, width, height)
+            return self.last_stats.seconds
+
+        try:
+            for stage in ("closest", "anyhit"):
+                base = best_c if stage == "closest" else best_a
+                cands = [base] + [tuple(c) for c in candidates if tuple(c) != base]
+                times = [timed(c, best_a) if stage == "closest" else timed(best_c, c) for c in cands]
+                tried[stage] = {"x".join(map(str, c)): round(t * 1e3, 2) for c, t in zip(cands, times)}
+                win = cands[int(np.argmin(times))]
+                if stage == "closest":
+                    best_c = win
+                else:
+                    best_a = win
+        finally:
+            self.config.spp, self.schedule = saved
+            scene.set_trace_config(*best_c, sm_c)
+            scene.set_anyhit_config(*best_a, sm_a)
+        return {"closest": list(best_c), "anyhit": list(best_a), "tried": tried}
 
     def render(self, scene: Scene, camera: PinholeCamera, width: int, height: int) -> Film:
         """Integrator::render (integrators/mod.rs:19-27)."""

@@ -310,22 +310,53 @@ def C_void(p):
 
 
 def autotune_scene(ar, torch, sc, desc, aspect):
-    """Per-scene traversal autotuning (closest and any hit) on 2^20 rays:
-    camera rays and rays from random interior points (what the extension and
-    shadow stages trace)."""
+    """Per-scene traversal autotuning on ray samples shaped like what the path
+    tracer's stages trace.  Closest hit (extension stage): 2^18 camera rays
+    and ~2^19 bounce rays leaving the surface points that camera rays hit, in
+    uniform directions.  Any hit (shadow stage): segments from those points to
+    random points on the lights (quad lights) or to the light position."""
     wl = ar.workloads
     cd = desc.camera
     rng = np.random.default_rng(3)
-    m = 1 << 19
+    m = 1 << 18
     cam16 = wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, aspect)
-    cr = wl.camera_rays(cam16, rng.uniform(0, 1, m), rng.uniform(0, 1, m))
-    lo, hi = desc.bounds()
-    o = rng.uniform(lo, hi, (m, 3)).astype(np.float32)
-    d = wl.normalize_f32(rng.standard_normal((m, 3)).astype(np.float32))
-    rr = np.zeros((m, 8), np.float32)
-    rr[:, 0:3], rr[:, 3], rr[:, 4:7], rr[:, 7] = o, 1e-4, d, wl.F32_MAX
-    sample = torch.from_numpy(np.concatenate([cr, rr]).astype(np.float32)).cuda()
-    cfg = sc.autotune(sample)
+    cr = wl.camera_rays(cam16, rng.uniform(0, 1, 2 * m), rng.uniform(0, 1, 2 * m)).astype(np.float32)
+    t, prim, _ = sc.intersect(cr)
+    hit = prim != 0xFFFFFFFF
+    crh = cr[hit]
+    # surface points, pulled back a little along the camera ray
+    p = crh[:, 0:3] + crh[:, 4:7] * (t[hit][:, None] - 1e-3).astype(np.float32)
+    k = p.shape[0]
+    d = wl.normalize_f32(rng.standard_normal((k, 3)).astype(np.float32))
+    br = np.zeros((k, 8), np.float32)
+    br[:, 0:3], br[:, 3], br[:, 4:7], br[:, 7] = p, 1e-4, d, wl.F32_MAX
+    parts = [cr[:m], br]
+    shadow = None
+    if desc.lights and k:
+        sel = rng.integers(0, len(desc.lights), k)
+        tgt = np.zeros((k, 3), np.float32)
+        for li, L in enumerate(desc.lights):
+            mask = sel == li
+            u, v = rng.uniform(0, 1, (2, int(mask.sum()), 1))
+            tgt[mask] = (np.asarray(L.position, np.float32) + u * np.asarray(L.edge_u, np.float32)
+                         + v * np.asarray(L.edge_v, np.float32))
+        dv = tgt - p
+        dist = np.linalg.norm(dv, axis=1).astype(np.float32)
+        sr = np.zeros((k, 8), np.float32)
+        sr[:, 0:3], sr[:, 3], sr[:, 4:7] = p, 1e-4, wl.normalize_f32(dv.astype(np.float32))
+        sr[:, 7] = dist * np.float32(1 - 1e-4)
+        shadow = sr
+    # closest hit on camera + bounce rays (the extension stage), any hit on
+    # the shadow segments (the shadow stage) when the scene has lights
+    sample = torch.from_numpy(np.concatenate(parts).astype(np.float32)).cuda()
+    if shadow is None:
+        cfg = sc.autotune(sample)
+    else:
+        cfg = sc.autotune(sample, which="closest")
+        ssample = torch.from_numpy(shadow).cuda()
+        sc.autotune(ssample, which="anyhit")
+        cfg["anyhit"] = sc.anyhit_config()
+        del ssample
     del sample
     return cfg
 
@@ -345,6 +376,10 @@ def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
     out = {"width": desc.width, "height": desc.height, "spp": desc.spp, "max_depth": desc.max_depth,
            "triangles": desc.n_triangles, "scene_build_s": round(build, 2)}
     out["launch_config"] = autotune_scene(ar, torch, sc, desc, desc.width / desc.height)
+    # refine the closest-hit launch on one chunk of the real job (16 spp)
+    tuner = ar.PathTracer(ar.PathTracerConfig(spp=desc.spp, max_depth=desc.max_depth, rr_depth=3), seed=0)
+    out["launch_refined"] = tuner.tune_launch(sc, cam, desc.width, desc.height, spp=min(16, desc.spp))
+    out["launch_config"] = dict(sc.trace_config(), anyhit=sc.anyhit_config())
     films = {}
     for sched in schedules:
         pt = ar.PathTracer(ar.PathTracerConfig(spp=desc.spp, max_depth=desc.max_depth, rr_depth=3), seed=0,

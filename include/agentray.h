@@ -42,6 +42,8 @@ typedef int32_t ar_status;
 #define AR_HOST_PTRS 0x1u    /* buffers are host memory */
 #define AR_ASYNC 0x2u        /* do not synchronize the stream before returning */
 #define AR_REF_ORDER 0x4u    /* traverse left child first (reference DFS order) */
+#define AR_TUNE_CLOSEST 0x10u /* ar_scene_autotune: tune the closest-hit launch only */
+#define AR_TUNE_ANYHIT 0x20u  /* ar_scene_autotune: tune the any-hit launch only */
 
 /* ---- value types ------------------------------------------------------ */
 

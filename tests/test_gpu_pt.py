@@ -33,6 +33,26 @@ def cornell(ar, orc, wl):
     return desc, ar.Scene.from_desc(desc), ar.PinholeCamera.from_array(cam16), cam16, orc.OracleScene(desc)
 
 
+def test_launch_refinement_keeps_the_film(ar, wl):
+    """PathTracer.tune_launch times launch candidates on the job and keeps the
+    fastest; the film does not depend on which one is used."""
+    desc = wl.cornell_box()
+    sc = ar.Scene.from_desc(desc)
+    cam = ar.PinholeCamera.from_array(
+        wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 1.0))
+    before, _ = render(ar, sc, cam, 96, 64, 4)
+    pt = ar.PathTracer(ar.PathTracerConfig(spp=4, max_depth=8, rr_depth=3))
+    res = pt.tune_launch(sc, cam, 96, 64, spp=2)
+    assert len(res["tried"]["closest"]) >= 6 and all(v > 0 for v in res["tried"]["closest"].values())
+    assert len(res["tried"]["anyhit"]) >= 6
+    c, a = sc.trace_config(), sc.anyhit_config()
+    assert [c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"]] == res["closest"]
+    assert [a["workgroup_size"], a["lds_stack"], a["blocks_per_cu"], a["refill_min"]] == res["anyhit"]
+    assert pt.config.spp == 4
+    after, _ = render(ar, sc, cam, 96, 64, 4)
+    assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
+
+
 def test_schedules_and_chunking_bitwise_identical(ar, cornell):
     desc, sc, cam, _, _ = cornell
     a, sa = render(ar, sc, cam, 64, 48, 8, schedule=1)
