@@ -121,6 +121,8 @@ def bench_traversal(args, ar, torch):
     tuned = None
     if args.trace_config:
         sc.set_trace_config(*[int(v) for v in args.trace_config.split(",")])
+        if args.anyhit_config:
+            sc.set_anyhit_config(*[int(v) for v in args.anyhit_config.split(",")])
     elif not args.no_autotune:
         # per-scene autotuning of the traversal launch on 2^20 rays from both halves
         idx = torch.cat([torch.arange(0, 1 << 19), (n_rays // 2) + torch.arange(0, 1 << 19)]).cuda()
@@ -135,6 +137,44 @@ def bench_traversal(args, ar, torch):
 
     def anyhit():
         ar._check(lib.ar_occluded(sc.handle, C_void(dr.data_ptr()), n_rays, C_void(occ.data_ptr()), ar.AR_ASYNC, sh))
+
+    if tuned is not None:
+        # refinement on the workload itself (untimed): the sample winner and
+        # its neighbours, best of two launches each, closest and any hit
+        def launch_ms(fn):
+            best = float("inf")
+            for _ in range(2):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                fn()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            return best
+        key = lambda c: (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"])
+        cands = [(256, 8, 8, 8), (256, 8, 8, 24), (128, 8, 16, 8), (128, 8, 16, 24), (64, 16, 16, 8),
+                 (128, 16, 8, 8), (128, 8, 8, 8)]
+        sm = tuned["search_min"]
+        best_c, best_a = key(sc.trace_config()), key(sc.anyhit_config())
+        times_c = {}
+        for c in [best_c] + [c for c in cands if c != best_c]:
+            sc.set_trace_config(*c, sm)
+            sc.set_anyhit_config(*best_a, sm)
+            closest()
+            times_c[c] = launch_ms(closest)
+        best_c = min(times_c, key=times_c.get)
+        best_t = times_c[best_c]
+        sc.set_trace_config(*best_c, sm)
+        times_a = {}
+        for a in [best_a] + [a for a in cands if a != best_a]:
+            sc.set_anyhit_config(*a, sm)
+            anyhit()
+            times_a[a] = launch_ms(anyhit)
+        best_a = min(times_a, key=times_a.get)
+        best_ta = times_a[best_a]
+        sc.set_trace_config(*best_c, sm)
+        sc.set_anyhit_config(*best_a, sm)
+        log(f"refined on the workload: closest {best_c} ({best_t:.2f} ms), any hit {best_a} ({best_ta:.2f} ms)")
 
     for _ in range(args.warmup):
         closest()
@@ -582,6 +622,8 @@ def main():
     ap.add_argument("--trace-config", default=None,
                     help="WORKGROUP,LDS_STACK,BLOCKS_PER_CU[,REFILL_MIN]: fixed traversal launch instead of "
                          "autotuning")
+    ap.add_argument("--anyhit-config", default=None,
+                    help="W,S,P[,R[,SM]]: any-hit launch (with --trace-config; default: the same as closest hit)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the path-tracer throughput fields")
