@@ -90,6 +90,10 @@ struct alignas(16) Tri {
   float4 v0, e1, e2;
 };
 constexpr uint32_t kLastBit = 0x80000000u;
+// A 4-wide tree leaf ref ~(k | kSingleTri) stands for Tri record k alone (its
+// own quantized box in the parent); ~k without it for the records from k to
+// the one tagged kLastBit (a binary leaf's range).  Records stay below 2^30.
+constexpr uint32_t kSingleTri = 0x40000000u;
 
 // One instance (TLAS leaf record), in TLAS leaf order.
 struct alignas(16) Inst {

@@ -457,12 +457,15 @@ struct Walker {
     return false;
   }
 
-  // All triangles of one leaf, in leaf order.
+  // All triangles of one leaf, in leaf order (one for a single-triangle
+  // ref of the 4-wide trees).
   AR_HD bool tri_leaf(const SceneView& sv, Stack& stk, int ref) {
-    for (uint32_t k = (uint32_t)~ref;; ++k) {
+    const uint32_t r = (uint32_t)~ref;
+    const uint32_t last = (r & kSingleTri) ? 0u : kLastBit;  // tag bit that ends the leaf
+    for (uint32_t k = r & ~kSingleTri;; ++k) {
       uint32_t tag;
       if (tri(sv, stk, k, tag)) return true;
-      if (tag & kLastBit) return false;
+      if ((tag & last) || !last) return false;
     }
   }
 
@@ -472,12 +475,14 @@ struct Walker {
     uint32_t k = (uint32_t)~park0;
     park0 = kLevelEnd;
     for (;;) {
+      const bool single = (k & kSingleTri) != 0u;
+      k &= ~kSingleTri;
       uint32_t tag;
       if (tri(sv, stk, k, tag)) {
         park1 = kLevelEnd;
         return true;
       }
-      if (!(tag & kLastBit)) {
+      if (!single && !(tag & kLastBit)) {
         ++k;
       } else if (park1 != kLevelEnd) {
         k = (uint32_t)~park1;

@@ -231,16 +231,32 @@ static QNode quantize_node(const HBox& nb, const std::vector<const HBox*>& cb, c
 // children and repeatedly opens the internal slot of largest surface area
 // (keeping left-to-right order) until it has four slots or only leaves.
 // Leaves keep their ranges (ref ~(prim_base + start)), so leaf positions and
-// the Tri records are shared with the binary form.  Nodes are stored in
-// preorder.
+// the Tri records are shared with the binary form.  With prim_boxes (bottom
+// level trees), a leaf of two or more triangles becomes a 4-wide node whose
+// children are its triangles, each with its own quantized box (ref
+// ~(record | kSingleTri)): a ray then runs the full triangle test only for the
+// triangles whose box it enters.  Nodes are stored in preorder.
 struct QFlat {
   int root_ref;
   uint32_t depth;
 };
 static int qbvh_rec(const BinTree& t, uint32_t i, uint32_t depth, std::vector<QNode>& out, uint32_t prim_base,
-                    QFlat& f) {
+                    QFlat& f, const std::vector<HBox>* prim_boxes) {
   const BinNode& n = t.nodes[i];
-  if (n.count != 0) return ~(int)(prim_base + n.left_or_start);
+  if (n.count == 1 || (n.count != 0 && !prim_boxes)) return ~(int)(prim_base + n.left_or_start);
+  if (n.count != 0) {  // a leaf of 2..4 triangles: one node with a slot per triangle
+    f.depth = std::max(f.depth, depth + 1);
+    uint32_t me = (uint32_t)out.size();
+    out.emplace_back();
+    int refs[4] = {kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef};
+    std::vector<const HBox*> boxes;
+    for (uint32_t j = 0; j < n.count; ++j) {
+      refs[j] = ~(int)((prim_base + n.left_or_start + j) | kSingleTri);
+      boxes.push_back(&(*prim_boxes)[t.prims[n.left_or_start + j]]);
+    }
+    out[me] = quantize_node(n.box, boxes, refs);
+    return (int)me;
+  }
   f.depth = std::max(f.depth, depth + 1);
   std::vector<uint32_t> slots = {n.left_or_start, n.right_or_end};
   while (slots.size() < 4) {
@@ -266,15 +282,16 @@ static int qbvh_rec(const BinTree& t, uint32_t i, uint32_t depth, std::vector<QN
   int refs[4] = {kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef};
   std::vector<const HBox*> boxes;
   for (size_t k = 0; k < slots.size(); ++k) {
-    refs[k] = qbvh_rec(t, slots[k], depth + 1, out, prim_base, f);
+    refs[k] = qbvh_rec(t, slots[k], depth + 1, out, prim_base, f, prim_boxes);
     boxes.push_back(&t.nodes[slots[k]].box);
   }
   out[me] = quantize_node(n.box, boxes, refs);
   return (int)me;
 }
-static QFlat build_qbvh(const BinTree& t, std::vector<QNode>& out, uint32_t prim_base) {
+static QFlat build_qbvh(const BinTree& t, std::vector<QNode>& out, uint32_t prim_base,
+                        const std::vector<HBox>* prim_boxes) {
   QFlat f{0, 0};
-  f.root_ref = qbvh_rec(t, 0, 0, out, prim_base, f);
+  f.root_ref = qbvh_rec(t, 0, 0, out, prim_base, f, prim_boxes);
   return f;
 }
 
@@ -446,7 +463,7 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     }
     for (uint32_t i = 0; i < nt; ++i) tri_obj.push_back((uint32_t)oi);
     Flat f = flatten(t, nodes, tri_base);
-    QFlat qf = build_qbvh(t, h.qnodes, tri_base);
+    QFlat qf = build_qbvh(t, h.qnodes, tri_base, &boxes);
     h.qmax_depth = std::max(h.qmax_depth, qf.depth);
     Inst& in = inst_by_obj[oi];
     memset(&in, 0, sizeof(in));
@@ -521,8 +538,9 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     }
   }
   Flat tf = flatten(tl, nodes, 0);
-  QFlat tq = build_qbvh(tl, h.qnodes, 0);
+  QFlat tq = build_qbvh(tl, h.qnodes, 0, nullptr);
   h.tlas_root = tf.root_ref;
+  if (tris.size() >= kSingleTri) return fail(AR_ERR_INVALID_ARG, "scene has 2^30 or more triangles");
   h.qtlas_root = tq.root_ref;
   h.qtlas_depth = tq.depth;
   // the 4-wide box test evaluates q * (scl / d) + (org - o) / d per plane;
