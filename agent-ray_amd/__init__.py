@@ -508,6 +508,13 @@ class GdptConfig:
     poisson_sign: int = 1
 
 
+# Launch candidates the on-the-job refinements try around the ray-sample
+# autotuner's pick: (workgroup, LDS stack, blocks per CU, refill threshold,
+# search quorum).
+TUNE_CANDIDATES = [(256, 8, 8, 8, 16), (256, 8, 8, 8, 8), (256, 8, 8, 24, 16), (128, 8, 16, 8, 16),
+                   (128, 16, 8, 8, 16), (128, 16, 8, 8, 8), (64, 16, 16, 8, 16)]
+
+
 class _Integrator:
     integrator_id = 0
 
@@ -541,25 +548,22 @@ class _Integrator:
         """Refines the scene's launch configurations on the job itself: times
         a render of `spp` samples per pixel (one chunk of the real workload,
         wavefront schedule) for each candidate (workgroup, LDS stack, blocks
-        per CU, refill threshold), the scene's current one first; first for
-        the closest-hit launch (extension stage), then, with the winner, for
-        the any-hit launch (shadow stage).  Keeps the fastest of each and
-        returns them with the timings.  Results do not depend on the launch
-        configuration, so this only changes speed."""
+        per CU, refill threshold, search quorum), the scene's current one
+        first; first for the closest-hit launch (extension stage), then, with
+        the winner, for the any-hit launch (shadow stage).  Keeps the fastest
+        of each and returns them with the timings.  Results do not depend on
+        the launch configuration, so this only changes speed."""
         if candidates is None:
-            candidates = [(256, 8, 8, 8), (256, 8, 8, 24), (128, 8, 16, 8), (128, 8, 16, 24), (64, 16, 16, 8),
-                          (128, 16, 8, 8), (128, 8, 8, 8)]
-        key = lambda c: (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"])
-        cur, anyc = scene.trace_config(), scene.anyhit_config()
-        sm_c, sm_a = cur["search_min"], anyc["search_min"]
-        best_c, best_a = key(cur), key(anyc)
+            candidates = TUNE_CANDIDATES
+        key = lambda c: (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"], c["search_min"])
+        best_c, best_a = key(scene.trace_config()), key(scene.anyhit_config())
         saved = (self.config.spp, self.schedule)
         self.config.spp, self.schedule = spp, Schedule.WAVEFRONT
         tried = {}
 
         def timed(c, a):
-            scene.set_trace_config(*c, sm_c)
-            scene.set_anyhit_config(*a, sm_a)
+            scene.set_trace_config(*c)
+            scene.set_anyhit_config(*a)
             self.render(scene, camera, width, height)
             return self.last_stats.seconds
 
@@ -576,8 +580,8 @@ class _Integrator:
                     best_a = win
         finally:
             self.config.spp, self.schedule = saved
-            scene.set_trace_config(*best_c, sm_c)
-            scene.set_anyhit_config(*best_a, sm_a)
+            scene.set_trace_config(*best_c)
+            scene.set_anyhit_config(*best_a)
         return {"closest": list(best_c), "anyhit": list(best_a), "tried": tried}
 
     def render(self, scene: Scene, camera: PinholeCamera, width: int, height: int) -> Film:

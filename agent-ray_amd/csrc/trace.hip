@@ -311,7 +311,7 @@ ar_status ar_scene_get_anyhit_config(const ar_scene* s, ar_trace_config* c) {
   return AR_OK;
 }
 
-// Times every (variant, blocks per CU, refill threshold) candidate on the
+// Times every (variant, blocks per CU, refill threshold, search quorum) candidate on the
 // sample, near-first order, one warm-up launch each, once for closest hit and
 // once for any hit (their best launches differ), and keeps the fastest of
 // each.
@@ -331,6 +331,7 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
   float best_ms[2] = {INFINITY, INFINITY};
   const uint32_t per_cu[3] = {4, 8, 16};
   const uint32_t refills[3] = {8, 24, 48};
+  const uint32_t searches[2] = {8, 16};
   const uint32_t variants[][2] = {
 #define AR_LIST(B, SS) {B, SS},
       AR_TRACE_VARIANTS(AR_LIST)
@@ -347,8 +348,9 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
     };
     for (auto& vv : variants) {
       for (uint32_t pc : per_cu) {
-        for (uint32_t rf : refills) {
-          ar_trace_config c{vv[0], vv[1], pc, rf, s->tcfg.search_min, 0.0f};
+        for (uint32_t rf : refills)
+        for (uint32_t sm : searches) {
+          ar_trace_config c{vv[0], vv[1], pc, rf, sm, 0.0f};
           err = launch(c);
           if (err != AR_OK) break;
           float ms = INFINITY;  // best of two timed launches

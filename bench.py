@@ -151,29 +151,27 @@ def bench_traversal(args, ar, torch):
                 torch.cuda.synchronize()
                 best = min(best, e0.elapsed_time(e1))
             return best
-        key = lambda c: (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"])
-        cands = [(256, 8, 8, 8), (256, 8, 8, 24), (128, 8, 16, 8), (128, 8, 16, 24), (64, 16, 16, 8),
-                 (128, 16, 8, 8), (128, 8, 8, 8)]
-        sm = tuned["search_min"]
+        key = lambda c: (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"], c["search_min"])
+        cands = ar.TUNE_CANDIDATES
         best_c, best_a = key(sc.trace_config()), key(sc.anyhit_config())
         times_c = {}
         for c in [best_c] + [c for c in cands if c != best_c]:
-            sc.set_trace_config(*c, sm)
-            sc.set_anyhit_config(*best_a, sm)
+            sc.set_trace_config(*c)
+            sc.set_anyhit_config(*best_a)
             closest()
             times_c[c] = launch_ms(closest)
         best_c = min(times_c, key=times_c.get)
         best_t = times_c[best_c]
-        sc.set_trace_config(*best_c, sm)
+        sc.set_trace_config(*best_c)
         times_a = {}
         for a in [best_a] + [a for a in cands if a != best_a]:
-            sc.set_anyhit_config(*a, sm)
+            sc.set_anyhit_config(*a)
             anyhit()
             times_a[a] = launch_ms(anyhit)
         best_a = min(times_a, key=times_a.get)
         best_ta = times_a[best_a]
-        sc.set_trace_config(*best_c, sm)
-        sc.set_anyhit_config(*best_a, sm)
+        sc.set_trace_config(*best_c)
+        sc.set_anyhit_config(*best_a)
         log(f"refined on the workload: closest {best_c} ({best_t:.2f} ms), any hit {best_a} ({best_ta:.2f} ms)")
 
     for _ in range(args.warmup):

@@ -46,8 +46,8 @@ def test_launch_refinement_keeps_the_film(ar, wl):
     assert len(res["tried"]["closest"]) >= 6 and all(v > 0 for v in res["tried"]["closest"].values())
     assert len(res["tried"]["anyhit"]) >= 6
     c, a = sc.trace_config(), sc.anyhit_config()
-    assert [c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"]] == res["closest"]
-    assert [a["workgroup_size"], a["lds_stack"], a["blocks_per_cu"], a["refill_min"]] == res["anyhit"]
+    key = lambda x: [x["workgroup_size"], x["lds_stack"], x["blocks_per_cu"], x["refill_min"], x["search_min"]]
+    assert key(c) == res["closest"] and key(a) == res["anyhit"]
     assert pt.config.spp == 4
     after, _ = render(ar, sc, cam, 96, 64, 4)
     assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
