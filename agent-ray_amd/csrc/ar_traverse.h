@@ -151,7 +151,7 @@ struct Walker {
   uint32_t ik;            // instance record being walked
   int base;               // stack depth at instance entry
   int cur;                // node ref, leaf ref, or kLevelEnd
-  int park0, park1;       // parked triangle leaves in the order reached (kLevelEnd: free)
+  int park0, park1, park2;  // parked triangle leaves in the order reached (kLevelEnd: free)
   bool in_blas;
   bool fast;              // 1/d of this level is finite: IEEE min / max slab
   bool wide;              // this level walks the 4-wide tree (near-first, fast)
@@ -176,7 +176,7 @@ struct Walker {
     tmin = wtmin;
     tmax = bt;
     in_blas = false;
-    park0 = park1 = kLevelEnd;
+    park0 = park1 = park2 = kLevelEnd;
     fast = finite_inv(inv);
     wide = NEAR && fast && wide_ray(sv, o, inv);
     cur = wide ? sv.qtlas_root : sv.tlas_root;
@@ -335,7 +335,7 @@ struct Walker {
   }
 
   // Node phase.  A lane visits internal nodes and parks the triangle leaves
-  // it reaches (up to two, in the order reached) instead of
+  // it reaches (up to three, in the order reached) instead of
   // testing them at once, so the lanes of a wave keep doing node work
   // together; the leaf phase then tests the parked leaves.  A lane can act
   // while it holds a node, or a triangle leaf and a free park slot; a lane
@@ -354,7 +354,7 @@ struct Walker {
   AR_HD void descend(const SceneView& sv, Stack& stk, Go go) {
     for (;;) {
       const bool node = cur >= 0;
-      const bool can = park1 == kLevelEnd && (node || (cur != kLevelEnd && in_blas));
+      const bool can = park2 == kLevelEnd && (node || (cur != kLevelEnd && in_blas));
       const bool searching = can && park0 == kLevelEnd;
       if (!go(can, searching)) return;
       if (can) {
@@ -363,8 +363,10 @@ struct Walker {
         } else {
           if (park0 == kLevelEnd)
             park0 = cur;
-          else
+          else if (park1 == kLevelEnd)
             park1 = cur;
+          else
+            park2 = cur;
           pop_level(stk);
         }
       }
@@ -469,8 +471,9 @@ struct Walker {
     }
   }
 
-  // The parked leaves' triangles as one loop (park0's, then park1's), so a
-  // lane moves on to its second leaf without waiting for the wave.
+  // The parked leaves' triangles as one loop (park0's, then park1's, then
+  // park2's), so a lane moves on to its next leaf without waiting for the
+  // wave.
   AR_HD bool tri_parked(const SceneView& sv, Stack& stk) {
     uint32_t k = (uint32_t)~park0;
     park0 = kLevelEnd;
@@ -479,14 +482,15 @@ struct Walker {
       k &= ~kSingleTri;
       uint32_t tag;
       if (tri(sv, stk, k, tag)) {
-        park1 = kLevelEnd;
+        park1 = park2 = kLevelEnd;
         return true;
       }
       if (!single && !(tag & kLastBit)) {
         ++k;
       } else if (park1 != kLevelEnd) {
         k = (uint32_t)~park1;
-        park1 = kLevelEnd;
+        park1 = park2;
+        park2 = kLevelEnd;
       } else {
         return false;
       }
