@@ -92,12 +92,20 @@ class ArJobDesc(C.Structure):
 
 class ArTraceConfig(C.Structure):
     _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("refill_min", C.c_uint32), ("search_min", C.c_uint32), ("ms", C.c_float)]
+                ("refill_min", C.c_uint32), ("search_min", C.c_uint32), ("park_slots", C.c_uint32),
+                ("ms", C.c_float)]
 
     def as_dict(self) -> dict:
         return {"workgroup_size": self.workgroup_size, "lds_stack": self.lds_stack,
                 "blocks_per_cu": self.blocks_per_cu, "refill_min": self.refill_min, "search_min": self.search_min,
-                "ms": self.ms}
+                "park_slots": self.park_slots, "ms": self.ms}
+
+
+def config_key(c: dict) -> tuple:
+    """A launch configuration dict as the argument tuple of
+    Scene.set_trace_config / set_anyhit_config."""
+    return (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"], c["search_min"],
+            c["park_slots"])
 
 
 class ArJobStats(C.Structure):
@@ -351,15 +359,15 @@ class Scene:
         return c.as_dict()
 
     def set_trace_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16,
-                         search_min: int = 8):
+                         search_min: int = 8, park_slots: int = 3):
         """Launch configuration of closest-hit and any-hit traversal."""
-        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, 0.0)
+        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, park_slots, 0.0)
         _check(_lib.ar_scene_set_trace_config(self._h, C.byref(c)))
 
     def set_anyhit_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16,
-                          search_min: int = 8):
+                          search_min: int = 8, park_slots: int = 3):
         """Launch configuration of any-hit traversal only (occlusion, shadow rays)."""
-        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, 0.0)
+        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, park_slots, 0.0)
         _check(_lib.ar_scene_set_anyhit_config(self._h, C.byref(c)))
 
     def anyhit_config(self) -> dict:
@@ -510,9 +518,16 @@ class GdptConfig:
 
 # Launch candidates the on-the-job refinements try around the ray-sample
 # autotuner's pick: (workgroup, LDS stack, blocks per CU, refill threshold,
-# search quorum).
-TUNE_CANDIDATES = [(256, 8, 8, 8, 16), (256, 8, 8, 8, 8), (256, 8, 8, 24, 16), (128, 8, 16, 8, 16),
-                   (128, 16, 8, 8, 16), (128, 16, 8, 8, 8), (64, 16, 16, 8, 16)]
+# search quorum, park slots).  The winner is then tried with the other park
+# slot count (3 <-> 4).
+TUNE_CANDIDATES = [(256, 8, 8, 8, 16, 3), (256, 8, 8, 8, 8, 3), (256, 8, 8, 8, 16, 4), (256, 8, 8, 8, 8, 4),
+                   (256, 8, 8, 24, 16, 3), (128, 8, 16, 8, 16, 3), (128, 16, 8, 8, 16, 3), (128, 16, 8, 8, 8, 3),
+                   (64, 16, 16, 8, 16, 3)]
+
+
+def other_park_slots(c: tuple) -> tuple:
+    """The launch configuration tuple `c` with the other park slot count."""
+    return tuple(c[:5]) + (7 - c[5],)
 
 
 class _Integrator:
@@ -548,15 +563,15 @@ class _Integrator:
         """Refines the scene's launch configurations on the job itself: times
         a render of `spp` samples per pixel (one chunk of the real workload,
         wavefront schedule) for each candidate (workgroup, LDS stack, blocks
-        per CU, refill threshold, search quorum), the scene's current one
+        per CU, refill threshold, search quorum, park slots), the scene's current one
         first; first for the closest-hit launch (extension stage), then, with
-        the winner, for the any-hit launch (shadow stage).  Keeps the fastest
-        of each and returns them with the timings.  Results do not depend on
+        the winner, for the any-hit launch (shadow stage); each stage's
+        winner is then timed with the other park slot count.  Keeps the
+        fastest of each and returns them with the timings.  Results do not depend on
         the launch configuration, so this only changes speed."""
         if candidates is None:
             candidates = TUNE_CANDIDATES
-        key = lambda c: (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"], c["search_min"])
-        best_c, best_a = key(scene.trace_config()), key(scene.anyhit_config())
+        best_c, best_a = config_key(scene.trace_config()), config_key(scene.anyhit_config())
         saved = (self.config.spp, self.schedule)
         self.config.spp, self.schedule = spp, Schedule.WAVEFRONT
         tried = {}
@@ -572,8 +587,13 @@ class _Integrator:
                 base = best_c if stage == "closest" else best_a
                 cands = [base] + [tuple(c) for c in candidates if tuple(c) != base]
                 times = [timed(c, best_a) if stage == "closest" else timed(best_c, c) for c in cands]
-                tried[stage] = {"x".join(map(str, c)): round(t * 1e3, 2) for c, t in zip(cands, times)}
                 win = cands[int(np.argmin(times))]
+                flip = other_park_slots(win)
+                if flip not in cands:
+                    cands.append(flip)
+                    times.append(timed(flip, best_a) if stage == "closest" else timed(best_c, flip))
+                    win = cands[int(np.argmin(times))]
+                tried[stage] = {"x".join(map(str, c)): round(t * 1e3, 2) for c, t in zip(cands, times)}
                 if stage == "closest":
                     best_c = win
                 else:

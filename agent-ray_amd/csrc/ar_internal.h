@@ -140,8 +140,8 @@ struct ar_scene {
   hipEvent_t scratch_event = nullptr;
   uint32_t tlas_depth = 0;
   uint32_t stack_depth = 0;  // traversal stack entries a query can need (either tree form)
-  ar_trace_config tcfg{256, 8, 8, 24, 8, 0.0f};      // closest-hit launch configuration (autotuned)
-  ar_trace_config tcfg_any{256, 8, 8, 24, 8, 0.0f};  // any-hit (occlusion, shadow rays), tuned separately
+  ar_trace_config tcfg{256, 8, 8, 24, 8, 3, 0.0f};      // closest-hit launch configuration (autotuned)
+  ar_trace_config tcfg_any{256, 8, 8, 24, 8, 3, 0.0f};  // any-hit (occlusion, shadow rays), tuned separately
   ar::DevBuf d_nodes, d_qnodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
       d_tri_obj, d_stri;
   ar::SceneView view{};

@@ -151,7 +151,7 @@ struct Walker {
   uint32_t ik;            // instance record being walked
   int base;               // stack depth at instance entry
   int cur;                // node ref, leaf ref, or kLevelEnd
-  int park0, park1, park2;  // parked triangle leaves in the order reached (kLevelEnd: free)
+  int park0, park1, park2, park3;  // parked triangle leaves in the order reached (kLevelEnd: free)
   bool in_blas;
   bool fast;              // 1/d of this level is finite: IEEE min / max slab
   bool wide;              // this level walks the 4-wide tree (near-first, fast)
@@ -176,7 +176,7 @@ struct Walker {
     tmin = wtmin;
     tmax = bt;
     in_blas = false;
-    park0 = park1 = park2 = kLevelEnd;
+    park0 = park1 = park2 = park3 = kLevelEnd;
     fast = finite_inv(inv);
     wide = NEAR && fast && wide_ray(sv, o, inv);
     cur = wide ? sv.qtlas_root : sv.tlas_root;
@@ -335,7 +335,7 @@ struct Walker {
   }
 
   // Node phase.  A lane visits internal nodes and parks the triangle leaves
-  // it reaches (up to three, in the order reached) instead of
+  // it reaches (up to SLOTS, 3 or 4, in the order reached) instead of
   // testing them at once, so the lanes of a wave keep doing node work
   // together; the leaf phase then tests the parked leaves.  A lane can act
   // while it holds a node, or a triangle leaf and a free park slot; a lane
@@ -350,11 +350,13 @@ struct Walker {
   // compiler split the loop into an inner loop that lanes leave one by one,
   // after which lane masks computed in the loop (e.g. park0 != kLevelEnd)
   // only keep the lanes of its last iteration.
-  template <class Go>
+  template <uint32_t SLOTS = 3, class Go>
   AR_HD void descend(const SceneView& sv, Stack& stk, Go go) {
+    static_assert(SLOTS == 3 || SLOTS == 4, "three or four park slots");
     for (;;) {
       const bool node = cur >= 0;
-      const bool can = park2 == kLevelEnd && (node || (cur != kLevelEnd && in_blas));
+      const int last_slot = SLOTS == 4 ? park3 : park2;
+      const bool can = last_slot == kLevelEnd && (node || (cur != kLevelEnd && in_blas));
       const bool searching = can && park0 == kLevelEnd;
       if (!go(can, searching)) return;
       if (can) {
@@ -365,8 +367,10 @@ struct Walker {
             park0 = cur;
           else if (park1 == kLevelEnd)
             park1 = cur;
-          else
+          else if (SLOTS == 3 || park2 == kLevelEnd)
             park2 = cur;
+          else
+            park3 = cur;
           pop_level(stk);
         }
       }
@@ -471,9 +475,8 @@ struct Walker {
     }
   }
 
-  // The parked leaves' triangles as one loop (park0's, then park1's, then
-  // park2's), so a lane moves on to its next leaf without waiting for the
-  // wave.
+  // The parked leaves' triangles as one loop (park0's, then park1's, ...),
+  // so a lane moves on to its next leaf without waiting for the wave.
   AR_HD bool tri_parked(const SceneView& sv, Stack& stk) {
     uint32_t k = (uint32_t)~park0;
     park0 = kLevelEnd;
@@ -482,7 +485,7 @@ struct Walker {
       k &= ~kSingleTri;
       uint32_t tag;
       if (tri(sv, stk, k, tag)) {
-        park1 = park2 = kLevelEnd;
+        park1 = park2 = park3 = kLevelEnd;
         return true;
       }
       if (!single && !(tag & kLastBit)) {
@@ -490,7 +493,8 @@ struct Walker {
       } else if (park1 != kLevelEnd) {
         k = (uint32_t)~park1;
         park1 = park2;
-        park2 = kLevelEnd;
+        park2 = park3;
+        park3 = kLevelEnd;
       } else {
         return false;
       }
@@ -570,10 +574,11 @@ AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack&
 // refilled together once at least `refill_min` (1..64) of them wait, so
 // items fetched together are neighbours (coherent rays stay together).
 // Every item is started exactly once, and the loop ends when the wave's
-// items are exhausted and every lane is idle.  n < 2^32.
+// items are exhausted and every lane is idle.  In its node phase a lane
+// parks up to SLOTS (3 or 4) triangle leaves.  n < 2^32.
 //   src(i)      -> Ray for item i
 //   sink(i, w)  consumes the finished walker of item i
-template <bool ANY, bool NEAR, class Src, class Sink>
+template <bool ANY, bool NEAR, uint32_t SLOTS, class Src, class Sink>
 __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint64_t n, uint64_t wave,
                                            uint64_t waves, uint32_t refill_min, uint32_t search_min, Src src,
                                            Sink sink) {
@@ -604,7 +609,7 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
     if (alive) {
       // the wave keeps walking nodes while more than search_min of its lanes
       // have parked nothing, or while no lane has leaf-phase work
-      w.descend(sv, stk, [search_min](bool can, bool searching) {
+      w.template descend<SLOTS>(sv, stk, [search_min](bool can, bool searching) {
         const uint64_t sm = __ballot(searching);
         return (uint32_t)__popcll(sm) > search_min || (sm != 0ull && __ballot(!searching) == 0ull);
       });

@@ -198,7 +198,7 @@ __device__ __forceinline__ Ray make_ray4(float4 o, float4 d) {
 // Extension rays of the live paths (queue qi): closest hit, near-first order.
 // Lanes take the queue's rays through walk_items, so a wave keeps all its
 // lanes busy until its share of the queue is done.
-template <int BLOCK, int S>
+template <int BLOCK, int S, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill,
                                                      uint32_t refill_min, uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneVi
   Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
-  walk_items<false, true>(
+  walk_items<false, true, SLOTS>(
       sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
       [&](uint64_t i) {
         uint32_t p = q[i];
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, const PtParams*
 
 // Shadow rays of this bounce: any hit; an unoccluded ray adds its
 // contribution to its path's radiance (one shadow ray per path per bounce).
-template <int BLOCK, int S>
+template <int BLOCK, int S, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill,
                                                      uint32_t refill_min, uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_shadow(SceneVi
   const uint32_t nthreads = gridDim.x * BLOCK;
   Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = B.cnt[2];
-  walk_items<true, true>(
+  walk_items<true, true, SLOTS>(
       sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
       [&](uint64_t i) { return make_ray4(B.sh_o[i], B.sh_d[i]); },
       [&](uint64_t i, const Walker<true, true>& w) {
@@ -468,9 +468,17 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   const uint32_t Gt = (uint32_t)(device_cus() * tc.blocks_per_cu / 8 * 8);
   const uint32_t Ga = (uint32_t)(device_cus() * ta.blocks_per_cu / 8 * 8);
   uint32_t Ge = 0, Gs = 0;  // resident-clamped grids, computed outside any capture
-#define AR_GRIDS(BB, SS)                                                                                   \
-  if (tc.workgroup_size == BB && tc.lds_stack == SS) Ge = clamp_resident(Gt, (const void*)&k_wf_extend<BB, SS>, BB); \
-  if (ta.workgroup_size == BB && ta.lds_stack == SS) Gs = clamp_resident(Ga, (const void*)&k_wf_shadow<BB, SS>, BB);
+#define AR_GRIDS(BB, SS)                                                                                  \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                      \
+    Ge = clamp_resident(Gt,                                                                               \
+                        tc.park_slots == 4 ? (const void*)&k_wf_extend<BB, SS, 4>                         \
+                                           : (const void*)&k_wf_extend<BB, SS, 3>,                        \
+                        BB);                                                                              \
+  if (ta.workgroup_size == BB && ta.lds_stack == SS)                                                      \
+    Gs = clamp_resident(Ga,                                                                               \
+                        ta.park_slots == 4 ? (const void*)&k_wf_shadow<BB, SS, 4>                         \
+                                           : (const void*)&k_wf_shadow<BB, SS, 3>,                        \
+                        BB);
   AR_TRACE_VARIANTS(AR_GRIDS)
 #undef AR_GRIDS
   if (Ge == 0 || Gs == 0) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
@@ -480,17 +488,27 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
     hipLaunchKernelGGL((k_wf_generate<BLOCK>), dim3(G), dim3(BLOCK), 0, st, Pd, B);
     for (uint32_t d = 0; d < P.max_depth; ++d) {
       int qi = d & 1;
-#define AR_EXT(BB, SS)                                                                                   \
-  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                     \
-    hipLaunchKernelGGL((k_wf_extend<BB, SS>), dim3(Ge), dim3(BB), 0, st, v.sv, B, qi, ws.spill, tc.refill_min, \
-                       tc.search_min);
+#define AR_EXT(BB, SS)                                                                                       \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS) {                                                       \
+    if (tc.park_slots == 4)                                                                                  \
+      hipLaunchKernelGGL((k_wf_extend<BB, SS, 4>), dim3(Ge), dim3(BB), 0, st, v.sv, B, qi, ws.spill,         \
+                         tc.refill_min, tc.search_min);                                                      \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_wf_extend<BB, SS, 3>), dim3(Ge), dim3(BB), 0, st, v.sv, B, qi, ws.spill,         \
+                         tc.refill_min, tc.search_min);                                                      \
+  }
       AR_TRACE_VARIANTS(AR_EXT)
 #undef AR_EXT
       hipLaunchKernelGGL((k_wf_shade<BLOCK>), dim3(G), dim3(BLOCK), 0, st, v, Pd, B, qi);
-#define AR_SHD(BB, SS)                                                                                   \
-  if (ta.workgroup_size == BB && ta.lds_stack == SS)                                                     \
-    hipLaunchKernelGGL((k_wf_shadow<BB, SS>), dim3(Gs), dim3(BB), 0, st, v.sv, B, ws.spill, ta.refill_min,    \
-                       ta.search_min);
+#define AR_SHD(BB, SS)                                                                                       \
+  if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                       \
+    if (ta.park_slots == 4)                                                                                  \
+      hipLaunchKernelGGL((k_wf_shadow<BB, SS, 4>), dim3(Gs), dim3(BB), 0, st, v.sv, B, ws.spill,             \
+                         ta.refill_min, ta.search_min);                                                      \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_wf_shadow<BB, SS, 3>), dim3(Gs), dim3(BB), 0, st, v.sv, B, ws.spill,             \
+                         ta.refill_min, ta.search_min);                                                      \
+  }
       AR_TRACE_VARIANTS(AR_SHD)
 #undef AR_SHD
       hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, B, qi);
@@ -504,8 +522,9 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   // what the graph depends on besides the chunk parameters
   const uint64_t key[8] = {(uint64_t)(uintptr_t)s,
                            (uint64_t)(uintptr_t)ws.buf.p,
-                           ((uint64_t)tc.workgroup_size << 48) | ((uint64_t)tc.lds_stack << 32) |
-                               ((uint64_t)ta.workgroup_size << 16) | ta.lds_stack,
+                           ((uint64_t)tc.workgroup_size << 48) | ((uint64_t)tc.lds_stack << 40) |
+                               ((uint64_t)tc.park_slots << 32) | ((uint64_t)ta.workgroup_size << 16) |
+                               ((uint64_t)ta.lds_stack << 8) | ta.park_slots,
                            ((uint64_t)tc.refill_min << 48) | ((uint64_t)tc.search_min << 32) |
                                ((uint64_t)ta.refill_min << 16) | ta.search_min,
                            ((uint64_t)Ge << 32) | Gs,

@@ -27,7 +27,7 @@ __device__ __forceinline__ Ray load_ray(const ar_ray* rays, uint64_t i) {
   return r;
 }
 
-template <bool ANY, bool NEAR, int S, int BLOCK>
+template <bool ANY, bool NEAR, int S, int BLOCK, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_trace(SceneView sv, const ar_ray* __restrict__ rays, uint64_t n,
                                                  ar_hit* __restrict__ hits, uint32_t* __restrict__ inst_ids,
                                                  uint32_t* __restrict__ occ, int2* __restrict__ spill,
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_trace(SceneView s
   stk.sp = 0;
   const uint64_t waves = (uint64_t)G * (BLOCK / 64);
   const uint64_t wave = (uint64_t)lb * (BLOCK / 64) + threadIdx.x / 64;
-  walk_items<ANY, NEAR>(
+  walk_items<ANY, NEAR, SLOTS>(
       sv, stk, n, wave, waves, refill_min, search_min, [&](uint64_t i) { return load_ray(rays, i); },
       [&](uint64_t i, const Walker<ANY, NEAR>& w) {
         if (ANY) {
@@ -67,12 +67,21 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_trace(SceneView s
 // Stack entries one query can need (bounded by the tree depths at commit).
 static uint32_t stack_need(const ar_scene* s) { return s->stack_depth; }
 
+template <bool ANY, bool NEAR, int BLOCK, int S, uint32_t SLOTS>
+static void launch_slots(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits,
+                         uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st, const ar_trace_config& c) {
+  G = clamp_resident(G, reinterpret_cast<const void*>(&k_trace<ANY, NEAR, S, BLOCK, SLOTS>), BLOCK);
+  hipLaunchKernelGGL((k_trace<ANY, NEAR, S, BLOCK, SLOTS>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits,
+                     inst, occ, spill, c.refill_min, c.search_min);
+}
+
 template <bool ANY, bool NEAR, int BLOCK, int S>
 static void launch_variant(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits,
                            uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st, const ar_trace_config& c) {
-  G = clamp_resident(G, reinterpret_cast<const void*>(&k_trace<ANY, NEAR, S, BLOCK>), BLOCK);
-  hipLaunchKernelGGL((k_trace<ANY, NEAR, S, BLOCK>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits, inst, occ,
-                     spill, c.refill_min, c.search_min);
+  if (c.park_slots == 4)
+    launch_slots<ANY, NEAR, BLOCK, S, 4>(s, G, rays, n, hits, inst, occ, spill, st, c);
+  else
+    launch_slots<ANY, NEAR, BLOCK, S, 3>(s, G, rays, n, hits, inst, occ, spill, st, c);
 }
 
 template <bool ANY, bool NEAR>
@@ -159,7 +168,7 @@ uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c) {
 }
 
 uint32_t trace_grid(uint64_t n, int block) {
-  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, kTraceRefillMin, 8, 0.0f};
+  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, kTraceRefillMin, 8, 3, 0.0f};
   return trace_grid_cfg(n, c);
 }
 
@@ -277,6 +286,7 @@ static ar_status check_trace_config(const ar_scene* s, const ar_trace_config* c,
   if (c->blocks_per_cu == 0 || c->blocks_per_cu > 64) return fail(AR_ERR_INVALID_ARG, "blocks_per_cu out of range");
   if (c->refill_min == 0 || c->refill_min > 64) return fail(AR_ERR_INVALID_ARG, "refill_min out of range (1..64)");
   if (c->search_min > 63) return fail(AR_ERR_INVALID_ARG, "search_min out of range (0..63)");
+  if (c->park_slots != 3 && c->park_slots != 4) return fail(AR_ERR_INVALID_ARG, "park_slots must be 3 or 4");
   return AR_OK;
 }
 
@@ -311,10 +321,10 @@ ar_status ar_scene_get_anyhit_config(const ar_scene* s, ar_trace_config* c) {
   return AR_OK;
 }
 
-// Times every (variant, blocks per CU, refill threshold, search quorum) candidate on the
-// sample, near-first order, one warm-up launch each, once for closest hit and
-// once for any hit (their best launches differ), and keeps the fastest of
-// each.
+// Times every (variant, blocks per CU, refill threshold, search quorum, park
+// slots) candidate on the sample, near-first order, one warm-up launch each,
+// once for closest hit and once for any hit (their best launches differ), and
+// keeps the fastest of each.
 ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t flags, void* stream,
                             ar_trace_config* best) {
   if (!s || !rays || n == 0) return fail(AR_ERR_INVALID_ARG, "ar_scene_autotune: NULL or empty sample");
@@ -332,6 +342,7 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
   const uint32_t per_cu[3] = {4, 8, 16};
   const uint32_t refills[3] = {8, 24, 48};
   const uint32_t searches[2] = {8, 16};
+  const uint32_t slot_counts[2] = {3, 4};
   const uint32_t variants[][2] = {
 #define AR_LIST(B, SS) {B, SS},
       AR_TRACE_VARIANTS(AR_LIST)
@@ -346,33 +357,29 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
       return q == 0 ? launch_trace<false, true>(s, rays, n, hits.as<ar_hit>(), nullptr, nullptr, st, c)
                     : launch_trace<true, true>(s, rays, n, nullptr, nullptr, occ.as<uint32_t>(), st, c);
     };
-    for (auto& vv : variants) {
-      for (uint32_t pc : per_cu) {
-        for (uint32_t rf : refills)
-        for (uint32_t sm : searches) {
-          ar_trace_config c{vv[0], vv[1], pc, rf, sm, 0.0f};
-          err = launch(c);
-          if (err != AR_OK) break;
-          float ms = INFINITY;  // best of two timed launches
-          for (int rep = 0; rep < 2 && err == AR_OK; ++rep) {
-            (void)hipEventRecord(e0, st);
-            err = launch(c);
-            (void)hipEventRecord(e1, st);
-            (void)hipEventSynchronize(e1);
-            float m1 = 0.0f;
-            (void)hipEventElapsedTime(&m1, e0, e1);
-            ms = std::min(ms, m1);
-          }
-          if (err != AR_OK) break;
-          if (ms < best_ms[q]) {
-            best_ms[q] = ms;
-            winners[q] = c;
-          }
-        }
-        if (err != AR_OK) break;
+    auto consider = [&](const ar_trace_config& c) {
+      err = launch(c);
+      float ms = INFINITY;  // best of two timed launches after a warm-up
+      for (int rep = 0; rep < 2 && err == AR_OK; ++rep) {
+        (void)hipEventRecord(e0, st);
+        err = launch(c);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float m1 = 0.0f;
+        (void)hipEventElapsedTime(&m1, e0, e1);
+        ms = std::min(ms, m1);
       }
-      if (err != AR_OK) break;
-    }
+      if (err == AR_OK && ms < best_ms[q]) {
+        best_ms[q] = ms;
+        winners[q] = c;
+      }
+    };
+    for (auto& vv : variants)
+      for (uint32_t pc : per_cu)
+        for (uint32_t rf : refills)
+          for (uint32_t sm : searches)
+            for (uint32_t ps : slot_counts)
+              if (err == AR_OK) consider(ar_trace_config{vv[0], vv[1], pc, rf, sm, ps, 0.0f});
   }
   (void)hipStreamSynchronize(st);
   (void)hipEventDestroy(e0);

@@ -151,23 +151,34 @@ def bench_traversal(args, ar, torch):
                 torch.cuda.synchronize()
                 best = min(best, e0.elapsed_time(e1))
             return best
-        key = lambda c: (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"], c["search_min"])
         cands = ar.TUNE_CANDIDATES
-        best_c, best_a = key(sc.trace_config()), key(sc.anyhit_config())
+        best_c, best_a = ar.config_key(sc.trace_config()), ar.config_key(sc.anyhit_config())
         times_c = {}
-        for c in [best_c] + [c for c in cands if c != best_c]:
+
+        def try_c(c):
             sc.set_trace_config(*c)
             sc.set_anyhit_config(*best_a)
             closest()
             times_c[c] = launch_ms(closest)
+        for c in [best_c] + [c for c in cands if c != best_c]:
+            try_c(c)
+        flip = ar.other_park_slots(min(times_c, key=times_c.get))
+        if flip not in times_c:
+            try_c(flip)
         best_c = min(times_c, key=times_c.get)
         best_t = times_c[best_c]
         sc.set_trace_config(*best_c)
         times_a = {}
-        for a in [best_a] + [a for a in cands if a != best_a]:
+
+        def try_a(a):
             sc.set_anyhit_config(*a)
             anyhit()
             times_a[a] = launch_ms(anyhit)
+        for a in [best_a] + [a for a in cands if a != best_a]:
+            try_a(a)
+        flip = ar.other_park_slots(min(times_a, key=times_a.get))
+        if flip not in times_a:
+            try_a(flip)
         best_a = min(times_a, key=times_a.get)
         best_ta = times_a[best_a]
         sc.set_trace_config(*best_c)
@@ -236,7 +247,7 @@ def bench_traversal(args, ar, torch):
                            "kernel_ms": round(float(np.mean(t_closest)), 3)}
         if not args.no_traffic:
             c = sc.trace_config()
-            cfg = f"{c['workgroup_size']},{c['lds_stack']},{c['blocks_per_cu']},{c['refill_min']},{c['search_min']}"
+            cfg = ",".join(str(v) for v in ar.config_key(c))
             traffic, detail = measure_traffic(cfg)
             out["roofline"]["traffic"] = None if traffic is None else round(traffic)
             out["roofline"]["traffic_detail"] = detail
@@ -618,10 +629,10 @@ def main():
                     help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--trace-config", default=None,
-                    help="WORKGROUP,LDS_STACK,BLOCKS_PER_CU[,REFILL_MIN]: fixed traversal launch instead of "
-                         "autotuning")
+                    help="WORKGROUP,LDS_STACK,BLOCKS_PER_CU[,REFILL_MIN[,SEARCH_MIN[,PARK_SLOTS]]]: fixed "
+                         "traversal launch instead of autotuning")
     ap.add_argument("--anyhit-config", default=None,
-                    help="W,S,P[,R[,SM]]: any-hit launch (with --trace-config; default: the same as closest hit)")
+                    help="W,S,P[,R[,SM[,PS]]]: any-hit launch (with --trace-config; default: the same as closest hit)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the path-tracer throughput fields")

@@ -195,6 +195,7 @@ typedef struct ar_trace_config {
   uint32_t blocks_per_cu;
   uint32_t refill_min; /* 1..64: idle lanes a wave gathers before it takes new rays */
   uint32_t search_min; /* 0..63: a wave's node phase goes on while more lanes than this have no leaf */
+  uint32_t park_slots; /* 3 or 4: triangle leaves a lane parks in one node phase */
   float ms; /* measured time of the winning candidate (autotune only) */
 } ar_trace_config;
 ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c);  /* closest and any hit */

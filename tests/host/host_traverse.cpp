@@ -4,7 +4,7 @@
 // with the oracle ray by ray.  Two drivers per query type:
 //   * trace_scene, one ray after another (no leaf parking across lanes);
 //   * an emulation of walk_items for 64-lane waves with wave-local refill
-//     and maximal leaf parking, which checks that every ray is started once
+//     and maximal leaf parking (3 or 4 park slots), which checks that every ray is started once
 //     and that the visiting order does not change the answer.
 #include <cstdio>
 #include <vector>
@@ -39,7 +39,7 @@ void record(const Walker<ANY, NEAR>& w, const SceneView& sv, uint64_t i, float* 
 }
 
 template <bool ANY, bool NEAR>
-int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, float* t0, uint32_t* p0, uint32_t* i0,
+int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, uint32_t slots, float* t0, uint32_t* p0, uint32_t* i0,
         float* t1, uint32_t* p1, uint32_t* i1) {
   std::vector<int2> lds(64 * 64);
   std::vector<unsigned long long> spill(64 * 512);
@@ -87,7 +87,11 @@ int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, fl
       }
       for (int l = 0; l < 64; ++l) {
         if (!alive[l]) continue;
-        w[l].descend(sv, stk[l], [](bool can, bool) { return can; });  // park whenever possible
+        auto any_time = [](bool can, bool) { return can; };  // park whenever possible
+        if (slots == 4)
+          w[l].template descend<4>(sv, stk[l], any_time);
+        else
+          w[l].template descend<3>(sv, stk[l], any_time);
         if (w[l].leaf(sv, stk[l])) {
           if (stk[l].sp != 0) return -3;
           record(w[l], sv, item[l], t1, p1, i1);
@@ -103,7 +107,7 @@ int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, fl
 
 }  // namespace
 
-extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any, int near, int lds_stack,
+extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any, int near, int lds_stack, int slots,
                              float* t0, uint32_t* p0, uint32_t* i0, float* t1, uint32_t* p1, uint32_t* i1) {
   HostAccel h;
   if (build_host_accel(s, h) != AR_OK) return -10;
@@ -117,8 +121,9 @@ extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any
   for (int k = 0; k < 6; ++k) sv.tlas_box[k] = h.tlas_box[k];
   sv.n_inst = (uint32_t)h.insts.size();
   sv.wide_ok = h.wide_ok;
-  if (any && near) return run<true, true>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
-  if (any) return run<true, false>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
-  if (near) return run<false, true>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
-  return run<false, false>(sv, rays, n, lds_stack, 3, t0, p0, i0, t1, p1, i1);
+  if (slots != 3 && slots != 4) return -11;
+  if (any && near) return run<true, true>(sv, rays, n, lds_stack, 3, (uint32_t)slots, t0, p0, i0, t1, p1, i1);
+  if (any) return run<true, false>(sv, rays, n, lds_stack, 3, (uint32_t)slots, t0, p0, i0, t1, p1, i1);
+  if (near) return run<false, true>(sv, rays, n, lds_stack, 3, (uint32_t)slots, t0, p0, i0, t1, p1, i1);
+  return run<false, false>(sv, rays, n, lds_stack, 3, (uint32_t)slots, t0, p0, i0, t1, p1, i1);
 }

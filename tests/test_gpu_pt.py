@@ -46,8 +46,10 @@ def test_launch_refinement_keeps_the_film(ar, wl):
     assert len(res["tried"]["closest"]) >= 6 and all(v > 0 for v in res["tried"]["closest"].values())
     assert len(res["tried"]["anyhit"]) >= 6
     c, a = sc.trace_config(), sc.anyhit_config()
-    key = lambda x: [x["workgroup_size"], x["lds_stack"], x["blocks_per_cu"], x["refill_min"], x["search_min"]]
-    assert key(c) == res["closest"] and key(a) == res["anyhit"]
+    assert list(ar.config_key(c)) == res["closest"] and list(ar.config_key(a)) == res["anyhit"]
+    # the winners were also timed with the other park slot count
+    for stage, win in (("closest", res["closest"]), ("anyhit", res["anyhit"])):
+        assert "x".join(map(str, ar.other_park_slots(tuple(win)))) in res["tried"][stage]
     assert pt.config.spp == 4
     after, _ = render(ar, sc, cam, 96, 64, 4)
     assert np.array_equal(before.view(np.uint32), after.view(np.uint32))

@@ -174,11 +174,11 @@ def test_room_launch_variants_same_answer(ar, wl):
     rays[:, 0:3], rays[:, 3], rays[:, 4:7], rays[:, 7] = o, 1e-4, d, wl.F32_MAX
     sc = ar.Scene.from_desc(desc)
     ref_t, ref_p, ref_i = sc.intersect(rays)
-    for (b, s, pc, rf, sm) in [(64, 8, 4, 1, 0), (256, 8, 8, 64, 63), (128, 16, 16, 8, 8), (128, 8, 8, 24, 32)]:
-        sc.set_trace_config(b, s, pc, rf, sm)
+    for cfg in [(64, 8, 4, 1, 0, 3), (256, 8, 8, 64, 63, 4), (128, 16, 16, 8, 8, 4), (128, 8, 8, 24, 32, 3)]:
+        sc.set_trace_config(*cfg)
         t, p, i = sc.intersect(rays)
-        assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)), (b, s, pc, rf, sm)
-        assert np.array_equal(p, ref_p) and np.array_equal(i, ref_i), (b, s, pc, rf, sm)
+        assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)), cfg
+        assert np.array_equal(p, ref_p) and np.array_equal(i, ref_i), cfg
 
 
 def test_every_launch_variant_same_answer(ar, wl):
@@ -191,14 +191,15 @@ def test_every_launch_variant_same_answer(ar, wl):
     ref_t, ref_p, _ = sc.intersect(rays)
     ref_o = sc.occluded(rays)
     for (b, s) in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8)]:
-        for pc, rf, sm in ((4, 1, 0), (16, 64, 63), (8, 24, 8)):
-            sc.set_trace_config(b, s, pc, rf, sm)
+        for pc, rf, sm, ps in ((4, 1, 0, 3), (16, 64, 63, 4), (8, 24, 8, 4)):
+            sc.set_trace_config(b, s, pc, rf, sm, ps)
             t, p, _ = sc.intersect(rays)
-            assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p), (b, s, pc, rf, sm)
+            assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p), \
+                (b, s, pc, rf, sm, ps)
             assert np.array_equal(sc.occluded(rays), ref_o)
     cfg = sc.autotune(torch.from_numpy(rays).cuda())
     assert cfg["ms"] > 0 and cfg["workgroup_size"] in (64, 128, 256)
-    assert cfg["refill_min"] in (8, 24, 48)
+    assert cfg["refill_min"] in (8, 24, 48) and cfg["park_slots"] in (3, 4)
     # any hit keeps its own (separately tuned) launch configuration
     assert cfg["anyhit"]["ms"] > 0 and cfg["anyhit"]["workgroup_size"] in (64, 128, 256)
     closest_cfg = sc.trace_config()
@@ -215,6 +216,9 @@ def test_every_launch_variant_same_answer(ar, wl):
         sc.set_trace_config(64, 8, 4, 65)
     with pytest.raises(ar.ArError):
         sc.set_trace_config(64, 8, 4, 24, 64)
+    for ps in (0, 2, 5):
+        with pytest.raises(ar.ArError):
+            sc.set_trace_config(64, 8, 4, 24, 8, ps)
 
 
 @pytest.mark.slow

@@ -5,7 +5,8 @@ oracle ray by ray, without a GPU.
 It walks the same trees the device gets (build_host_accel: binary trees for
 reference order, quantized 4-wide trees for near-first order) and runs each
 query two ways: one ray after another, and in an emulation of the kernels'
-64-lane waves with wave-local refill and maximal leaf parking.  Checks:
+64-lane waves with wave-local refill and maximal leaf parking (three and
+four park slots).  Checks:
 * reference order reproduces the oracle's (t bits, prim, instance) on every
   ray, closest and any hit;
 * near-first order reproduces them too, except near-ties (t within 4 ulps,
@@ -41,7 +42,7 @@ def _helper():
                                "-Wl,-rpath," + os.path.dirname(LIB)])
         os.replace(OUT + ".tmp", OUT)
     h = C.CDLL(OUT)
-    h.host_traverse.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 6
+    h.host_traverse.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64] + [C.c_int] * 4 + [C.c_void_p] * 6
     h.host_traverse.restype = C.c_int
     return h
 
@@ -73,10 +74,11 @@ def _scene(ar, desc):
     return s
 
 
-def _run(h, s, rays, any_hit, near, lds_stack=4):
+def _run(h, s, rays, any_hit, near, lds_stack=4, slots=3):
     n = rays.shape[0]
-    bufs = [np.zeros(n, np.float32), np.zeros(n, np.uint32), np.zeros(n, np.uint32)] * 2
-    rc = h.host_traverse(s, rays.ctypes.data, n, int(any_hit), int(near), lds_stack, *[b.ctypes.data for b in bufs])
+    bufs = [np.zeros(n, np.float32) if k % 3 == 0 else np.zeros(n, np.uint32) for k in range(6)]
+    rc = h.host_traverse(s, rays.ctypes.data, n, int(any_hit), int(near), lds_stack, slots,
+                         *[b.ctypes.data for b in bufs])
     assert rc == 0, rc
     return bufs[:3], bufs[3:]
 
@@ -98,18 +100,18 @@ def _check(ar, orc, desc, rays, max_near_ties=0):
         ot, op, oi, _, _ = osc.trace(rays)
         occ, _, _ = osc.occluded(rays)
         ora = (ot, op, oi)
-        for near in (False, True):
-            one, wave = _run(h, s, rays, False, near)
+        for near, slots in ((False, 3), (True, 3), (True, 4)):
+            one, wave = _run(h, s, rays, False, near, slots=slots)
             for k in range(3):  # the two drivers agree bit for bit
-                assert np.array_equal(one[k].view(np.uint32), wave[k].view(np.uint32)), (near, k)
+                assert np.array_equal(one[k].view(np.uint32), wave[k].view(np.uint32)), (near, slots, k)
             bad, ties = _near_ties(one, ora)
             if near:
                 assert bad == ties and ties <= max_near_ties, (bad, ties)
             else:
                 assert bad == 0, bad
-            a1, a2 = _run(h, s, rays, True, near)
-            assert np.array_equal(a1[0].astype(bool), occ.astype(bool)), near
-            assert np.array_equal(a2[0].astype(bool), occ.astype(bool)), near
+            a1, a2 = _run(h, s, rays, True, near, slots=slots)
+            assert np.array_equal(a1[0].astype(bool), occ.astype(bool)), (near, slots)
+            assert np.array_equal(a2[0].astype(bool), occ.astype(bool)), (near, slots)
     finally:
         ar.lib().ar_scene_destroy(s)
 
