@@ -568,6 +568,37 @@ AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack&
 }
 
 #ifdef __HIP__  // wave intrinsics: HIP translation units only
+// Node-phase rule of a wave whose lanes walk together: the wave keeps
+// walking nodes while more than search_min of its active lanes have parked
+// nothing, or while no active lane has leaf-phase work.  Wave-uniform.
+__device__ __forceinline__ bool wave_quorum(uint32_t search_min, bool searching) {
+  const uint64_t sm = __ballot(searching);
+  return (uint32_t)__popcll(sm) > search_min || (sm != 0ull && __ballot(!searching) == 0ull);
+}
+
+// trace_scene for kernels whose lanes each carry one query to completion
+// (no refill): the active lanes run their node phases together under
+// wave_quorum, as walk_items does, instead of each stopping at its first
+// parked leaf.  Same answer as trace_scene.
+template <bool ANY, bool NEAR, uint32_t SLOTS = 3>
+__device__ __forceinline__ bool trace_scene_wave(const SceneView& sv, const Ray& r, WorldHit& best, Stack& stk,
+                                                 uint32_t search_min = 8) {
+  Walker<ANY, NEAR> w;
+  bool done = w.start(sv, r, stk);
+  while (!done) {
+    w.template descend<SLOTS>(sv, stk, [search_min](bool, bool searching) {
+      return wave_quorum(search_min, searching);
+    });
+    done = w.leaf(sv, stk);
+  }
+  best.t = w.bt;
+  best.pos = w.bpos;
+  best.inst = w.binst;
+  best.u = w.bu;
+  best.v = w.bv;
+  return w.bpos != kNoHit;
+}
+
 // Items [0, n) are split in chunks of 64; wave `wave` of `waves` owns chunks
 // wave, wave + waves, ... and walks their items in order.  A lane that
 // finishes takes the next unassigned item of its wave; idle lanes are
@@ -607,11 +638,8 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
       next += nidle;
     }
     if (alive) {
-      // the wave keeps walking nodes while more than search_min of its lanes
-      // have parked nothing, or while no lane has leaf-phase work
-      w.template descend<SLOTS>(sv, stk, [search_min](bool can, bool searching) {
-        const uint64_t sm = __ballot(searching);
-        return (uint32_t)__popcll(sm) > search_min || (sm != 0ull && __ballot(!searching) == 0ull);
+      w.template descend<SLOTS>(sv, stk, [search_min](bool, bool searching) {
+        return wave_quorum(search_min, searching);
       });
       if (w.leaf(sv, stk)) {
         sink(item, w);

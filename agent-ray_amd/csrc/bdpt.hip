@@ -187,7 +187,7 @@ __device__ __forceinline__ uint32_t extend_subpath(const ShadeView& v, const Bdp
   PV before = prev;  // vertex nv-2 (for the retroactive pdf_rev write)
   for (uint32_t depth = 0; depth < P.pt.max_depth; ++depth) {
     WorldHit h;
-    trace_scene<false, NEAR>(v.sv, ray, h, stk);
+    trace_scene_wave<false, NEAR>(v.sv, ray, h, stk);
     nrays++;
     if (h.pos == kNoHit) break;
     ShadingPoint sp;
@@ -406,7 +406,7 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams 
           Ray sr = make_ray(y.p + off, to_cam, 0.0f, cs.dist * (1.0f - 1e-4f));
           WorldHit hh;
           nrays++;
-          if (trace_scene<true, true>(v.sv, sr, hh, stk)) continue;
+          if (trace_scene_wave<true, true>(v.sv, sr, hh, stk)) continue;
           PV yp = load_v(Vt, 1, t - 2, p);
           Onb on = onb_from_normal(y.n);
           f3 wo_l = to_local(on, normalize(yp.p - y.p));
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams 
         Ray sr = make_ray(so, sd / sdist, 0.0f, sdist * (1.0f - 1e-4f));
         WorldHit hh;
         nrays++;
-        if (trace_scene<true, true>(v.sv, sr, hh, stk)) continue;
+        if (trace_scene_wave<true, true>(v.sv, sr, hh, stk)) continue;
         PV zp = load_v(Vt, 0, s - 2, p);
         Onb onz = onb_from_normal(z.n);
         f3 wo_z_l = to_local(onz, normalize(zp.p - z.p));
