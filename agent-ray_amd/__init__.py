@@ -159,6 +159,8 @@ def _load():
         "ar_scene_set_anyhit_config": (i32, [P, C.POINTER(ArTraceConfig)]),
         "ar_scene_get_anyhit_config": (i32, [P, C.POINTER(ArTraceConfig)]),
         "ar_scene_autotune": (i32, [P, P, u64, u32, P, C.POINTER(ArTraceConfig)]),
+        "ar_film_to_rgb8": (i32, [P, u32, u32, u32, f, f, f, P, u32, P]),
+        "ar_film_save_exr": (i32, [P, u32, u32, f, C.c_char_p, u32, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -460,6 +462,62 @@ def camera_for(desc: SceneDesc) -> PinholeCamera:
 # ---------------------------------------------------------------------------
 # Film and integrators
 # ---------------------------------------------------------------------------
+AR_TONEMAP_CLAMP, AR_TONEMAP_REINHARD, AR_TONEMAP_REINHARD_EXTENDED = 0, 1, 2
+
+
+@dataclass(frozen=True)
+class ToneMapper:
+    """film.rs:138-151 (ToneMapper): Clamp, Reinhard or ReinhardExtended
+    with a white point.  Use ToneMapper.Clamp, ToneMapper.Reinhard and
+    ToneMapper.ReinhardExtended(white_point)."""
+    kind: int
+    white_point: float = 1.0
+
+    @staticmethod
+    def ReinhardExtended(white_point: float) -> "ToneMapper":
+        return ToneMapper(AR_TONEMAP_REINHARD_EXTENDED, float(white_point))
+
+
+ToneMapper.Clamp = ToneMapper(AR_TONEMAP_CLAMP)
+ToneMapper.Reinhard = ToneMapper(AR_TONEMAP_REINHARD)
+
+
+def film_to_rgb8(pixels, tone_mapper: ToneMapper = ToneMapper.Reinhard, gamma: float = 2.2, scale: float = 1.0):
+    """Film::to_rgb_image (film.rs:283-303) on the GPU for a (h, w, 3)
+    float32 film: a numpy array (returns numpy uint8) or a torch CUDA tensor
+    (returns a torch uint8 CUDA tensor, nothing leaves the device)."""
+    if _is_torch_cuda(pixels):
+        import torch
+        src = pixels.contiguous()
+        h, w = src.shape[0], src.shape[1]
+        out = torch.empty((h, w, 3), dtype=torch.uint8, device=src.device)
+        st = torch.cuda.current_stream(src.device)
+        _check(_lib.ar_film_to_rgb8(C.c_void_p(src.data_ptr()), w, h, tone_mapper.kind, tone_mapper.white_point,
+                                    gamma, scale, C.c_void_p(out.data_ptr()), 0, C.c_void_p(st.cuda_stream)))
+        return out
+    src = np.ascontiguousarray(pixels, np.float32)
+    h, w = src.shape[0], src.shape[1]
+    out = np.empty((h, w, 3), np.uint8)
+    _check(_lib.ar_film_to_rgb8(_ptr(src), w, h, tone_mapper.kind, tone_mapper.white_point, gamma, scale,
+                                _ptr(out), AR_HOST_PTRS, None))
+    return out
+
+
+def film_save_exr(pixels, path, scale: float = 1.0):
+    """Film::save_exr (film.rs:310-346): half-float OpenEXR (B, G, R; ZIP, 16
+    lines per block) of a (h, w, 3) float32 film, numpy or torch CUDA."""
+    p = os.fsencode(os.fspath(path))
+    if _is_torch_cuda(pixels):
+        import torch
+        src = pixels.contiguous()
+        st = torch.cuda.current_stream(src.device)
+        _check(_lib.ar_film_save_exr(C.c_void_p(src.data_ptr()), src.shape[1], src.shape[0], scale, p, 0,
+                                     C.c_void_p(st.cuda_stream)))
+        return
+    src = np.ascontiguousarray(pixels, np.float32)
+    _check(_lib.ar_film_save_exr(_ptr(src), src.shape[1], src.shape[0], scale, p, AR_HOST_PTRS, None))
+
+
 class Film:
     """film.rs:190-274: linear HDR RGB, row-major, top row first."""
 
@@ -472,6 +530,16 @@ class Film:
 
     def to_hdr_vec(self):
         return self.pixels.reshape(-1, 3)
+
+    def to_rgb_image(self, tone_mapper: ToneMapper = ToneMapper.Reinhard, gamma: float = 2.2,
+                     scale: float = 1.0) -> np.ndarray:
+        """Film::to_rgb_image (film.rs:283-303): (h, w, 3) uint8, computed on
+        the GPU."""
+        return film_to_rgb8(self.pixels, tone_mapper, gamma, scale)
+
+    def save_exr(self, path, scale: float = 1.0):
+        """Film::save_exr (film.rs:310-346)."""
+        film_save_exr(self.pixels, path, scale)
 
 
 class Schedule(IntEnum):

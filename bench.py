@@ -447,7 +447,42 @@ def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
         a, b = films.values()
         out["schedules_bitwise_equal"] = bool(np.array_equal(a.view(np.uint32), b.view(np.uint32)))
         out["wavefront_speedup"] = round(out["megakernel"]["seconds"] / out["wavefront"]["seconds"], 3)
+    if which == "room" and films:
+        out["film_output"] = bench_film_output(ar, torch, next(iter(films.values())), desc.spp)
     return out
+
+
+def bench_film_output(ar, torch, pixels, spp):
+    """Film output stage on the rendered film (device resident): Reinhard +
+    gamma 2.2 + 8-bit encode (Film::to_rgb_image; 12 B read + 3 B written
+    per pixel), timed with HIP events over 20 launches, and the half-float
+    EXR file (Film::save_exr) timed on the wall clock, device film to file."""
+    import tempfile
+    h, w = pixels.shape[:2]
+    dev = torch.from_numpy(np.ascontiguousarray(pixels)).cuda()
+    st = torch.cuda.current_stream()
+    ar.film_to_rgb8(dev, ar.ToneMapper.Reinhard, 2.2, 1.0 / spp)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(st)
+    for _ in range(reps):
+        ar.film_to_rgb8(dev, ar.ToneMapper.Reinhard, 2.2, 1.0 / spp)
+    e1.record(st)
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) / reps * 1e3
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "film.exr")
+        ar.film_save_exr(dev, path, 1.0 / spp)
+        t0 = time.perf_counter()
+        ar.film_save_exr(dev, path, 1.0 / spp)
+        exr_ms = (time.perf_counter() - t0) * 1e3
+        exr_bytes = os.path.getsize(path)
+    res = {"rgb8_us": round(us, 1), "rgb8_gbs": round(15.0 * w * h / (us * 1e-6) / 1e9, 1),
+           "exr_ms": round(exr_ms, 2), "exr_bytes": exr_bytes, "d2h_bytes_rgb8": 3 * w * h,
+           "d2h_bytes_exr": 6 * w * h, "d2h_bytes_f32": 12 * w * h}
+    log(f"film output {w}x{h}: {res}")
+    return res
 
 
 def bench_bdpt_gdpt(ar, torch):

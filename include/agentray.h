@@ -238,6 +238,24 @@ ar_status ar_poisson_solve(const float* primal, const float* dx, const float* dy
                            uint32_t height, float alpha, uint32_t iterations, int32_t sign, float* out,
                            double* final_residual, uint32_t flags, void* stream);
 
+/* ---- film output (film.rs:133-346) -------------------------------------- */
+/* Films are width*height*3 floats (row-major, top row first), each pixel
+ * multiplied by `scale` first (1/spp normalisation in the reference). */
+#define AR_TONEMAP_CLAMP 0u             /* ToneMapper::Clamp (film.rs:141-142)            */
+#define AR_TONEMAP_REINHARD 1u          /* ToneMapper::Reinhard (film.rs:143-144)         */
+#define AR_TONEMAP_REINHARD_EXTENDED 2u /* ToneMapper::ReinhardExtended (film.rs:145-150) */
+/* Film::to_rgb_image (film.rs:283-303): tone map (tonemap_pixel,
+ * film.rs:153-180), encode with 1/gamma, quantise to 8 bits; out receives
+ * width*height*3 bytes (RGB).  white_point is read by
+ * AR_TONEMAP_REINHARD_EXTENDED only. */
+ar_status ar_film_to_rgb8(const float* film, uint32_t width, uint32_t height, uint32_t tonemapper,
+                          float white_point, float gamma, float scale, uint8_t* out, uint32_t flags, void* stream);
+/* Film::save_exr (film.rs:310-346): linear half-float OpenEXR file, channels
+ * B, G, R, ZIP compression in blocks of 16 scan lines.  Synchronises the
+ * stream (the file is written on return). */
+ar_status ar_film_save_exr(const float* film, uint32_t width, uint32_t height, float scale, const char* path,
+                           uint32_t flags, void* stream);
+
 /* ---- serving (no reference counterpart: the request loop above
  * Integrator::render) ------------------------------------------------------ */
 typedef struct ar_server ar_server;

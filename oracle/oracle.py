@@ -51,6 +51,9 @@ def lib():
             "orc_render_gdpt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, u64, C.c_int, P, P, P]),
             "orc_poisson_sor": (d, [P, P, P, u32, u32, f, u32, f, C.c_int, P]),
             "orc_render_bdpt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, f, C.c_int, u64, C.c_int, P, P]),
+            "orc_film_rgb8": (None, [P, u64, u32, f, f, f, P]),
+            "orc_f32_to_f16": (None, [P, u64, P]),
+            "orc_exr_blocks": (None, [P, u32, u32, f, P]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name, None)
@@ -181,6 +184,32 @@ def poisson_sor(primal, dx, dy, alpha, iterations=200, omega=1.6, sign=1):
     out = np.zeros_like(p)
     secs = lib().orc_poisson_sor(_p(p), _p(gx), _p(gy), w, h, alpha, iterations, omega, sign, _p(out))
     return out, secs
+
+
+def film_rgb8(film, tonemapper=1, white_point=1.0, gamma=2.2, scale=1.0):
+    """Film::to_rgb_image (film.rs:283-303): (h, w, 3) float32 -> (h, w, 3) uint8."""
+    f = np.ascontiguousarray(film, np.float32)
+    out = np.zeros(f.shape, np.uint8)
+    lib().orc_film_rgb8(_p(f), f.size // 3, tonemapper, white_point, gamma, scale, _p(out))
+    return out
+
+
+def f32_to_f16(x):
+    """half::f16::from_f32 bit patterns (uint16) of a float32 array."""
+    a = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(a.shape, np.uint16)
+    lib().orc_f32_to_f16(_p(a), a.size, _p(out))
+    return out
+
+
+def exr_blocks(film, scale=1.0):
+    """The pre-deflate bytes of each 16-line ZIP block of the B, G, R half
+    image of `film` (h, w, 3), blocks back to back."""
+    f = np.ascontiguousarray(film, np.float32)
+    h, w = f.shape[:2]
+    out = np.zeros(h * w * 6, np.uint8)
+    lib().orc_exr_blocks(_p(f), w, h, scale, _p(out))
+    return out
 
 
 def ray_new(o, d, tmin, tmax):
