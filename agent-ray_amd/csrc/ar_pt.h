@@ -9,6 +9,7 @@
 // and Le * w_bsdf (power heuristic against light sampling) on BSDF-sampled
 // hits; the reference's non-delta MIS branch is otherwise followed as is.
 #pragma once
+#include "ar_kernels.h"
 #include "ar_shade.h"
 #include "ar_traverse.h"
 
@@ -38,6 +39,33 @@ struct ShadowReq {
   f3 c;
   bool valid;
 };
+
+__device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint32_t v) {
+  uint32_t s = v;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if (__lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
+}
+
+// A lane's traversal stack: S entries in LDS (lane-strided), the rest
+// spilled to spill[entry * nthreads + gtid].
+template <int BLOCK, int S = kTraceLdsStack>
+__device__ __forceinline__ Stack make_stack(int2* lds, int2* spill, uint32_t gtid, uint32_t nthreads) {
+  Stack stk;
+  stk.lds = (LdsEntryPtr)(lds + threadIdx.x);
+  stk.stride = BLOCK;
+  stk.spill = (SpillEntryPtr)(spill + gtid);
+  stk.spill_stride = nthreads;
+  stk.S = S;
+  stk.sp = 0;
+  return stk;
+}
+
+// Block index remapped so that each of the 8 XCDs (which take workgroups
+// round-robin) sweeps a contiguous range of blocks.
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t G = gridDim.x, b = blockIdx.x;
+  return (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
+}
 
 // Camera ray for (x, y) with the path's first two dimensions as jitter
 // (path_tracer.rs:176-182).

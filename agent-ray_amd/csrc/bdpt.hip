@@ -6,13 +6,19 @@
 // subpath (:338-437) and the light subpath (:443-559) into a structure-of-
 // arrays vertex buffer in HBM, [side][vertex slot][path], so lanes of a wave
 // touch consecutive addresses for the same slot.
-// Phase 2 (k_bdpt_connect): one lane per path enumerates the strategies in
-// the reference order (t outer, s inner; :1281-1334), traces one any-hit
-// shadow ray per connection and evaluates the MIS weight (:594-770, fp64
-// sums).  Non-splat contributions go to the path's radiance slot (reduced
-// per pixel in sample order like the path tracer); light-tracing (s = 1)
-// splats are added as 64-bit fixed-point integers, so the splat film is the
-// same whatever order the atomics land in.
+// Phase 2, connection: one lane per path enumerates the strategies in the
+// reference order (t outer, s inner; :1281-1334) and evaluates each
+// contribution with its MIS weight (:594-770, fp64 sums); a contribution
+// counts if its any-hit shadow ray is unoccluded.  Wavefront schedule
+// (default): k_bdpt_conn_prep stores every positive contribution and its
+// shadow ray in the path's slots and queues the slot, k_bdpt_conn_vis traces
+// the whole queue in one any-hit stage (walk_items, the scene's any-hit
+// launch), k_bdpt_conn_gather adds the unoccluded ones in enumeration order.
+// Megakernel schedule: k_bdpt_connect traces each shadow ray where it is
+// enumerated.  Both give the same film bit for bit.  Non-splat contributions
+// go to the path's radiance slot (reduced per pixel in sample order like the
+// path tracer); light-tracing (s = 1) splats are added as 64-bit fixed-point
+// integers, so the splat film is the same whatever order the atomics land in.
 //
 // Reference behaviour kept (bug-compatible): t = 0 / s = 0 not implemented;
 // point-light t = 1 omits 1/sel_pmf; the camera vertex normal is the first
@@ -370,137 +376,246 @@ __device__ __forceinline__ void splat_add(unsigned long long* film, uint32_t idx
   }
 }
 
+constexpr uint32_t kOwnPixel = 0xffffffffu;  // target of a contribution to the path's own pixel
+
+// Every connection strategy of path p in the reference order (t outer, s
+// inner; :1281-1334) whose unweighted contribution is positive, as
+// emit(shadow ray, weighted contribution, target): the light-tracing
+// strategies (s = 1, :1051-1155) target the pixel the light vertex projects
+// to (a splat), the others (:844-1043) kOwnPixel.  The contribution counts
+// only if the shadow ray is unoccluded; the caller decides when to trace it.
+template <class Emit>
+__device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptParams& P, const Verts& Vt, uint32_t p,
+                                                Emit emit) {
+  const Cam& cam = P.pt.cam;
+  const uint32_t W = P.pt.width, H = P.pt.height;
+  uint2 cnt = Vt.count[p];
+  uint32_t smax = cnt.x, tmax = cnt.y, limit = P.pt.max_depth + 2;
+  for (uint32_t t = 0; t <= tmax; ++t) {
+    for (uint32_t s = 0; s <= smax; ++s) {
+      if (s + t < 2 || s + t > limit || s == 0) continue;
+      if (s == 1) {  // light tracing to the camera (:1051-1155)
+        if (t < 2) continue;
+        PV y = load_v(Vt, 1, t - 1, p);
+        WeSample cs;
+        if (!cam_sample_we(cam, y.p, W, H, cs)) continue;
+        f3 to_cam = cs.wi;
+        f3 off = y.n * 1e-4f * signum(dot(to_cam, y.n));
+        Ray sr = make_ray(y.p + off, to_cam, 0.0f, cs.dist * (1.0f - 1e-4f));
+        PV yp = load_v(Vt, 1, t - 2, p);
+        Onb on = onb_from_normal(y.n);
+        f3 wo_l = to_local(on, normalize(yp.p - y.p));
+        f3 wi_l = to_local(on, to_cam);
+        const DMaterial& m = v.materials[y.id];
+        f3 f = bsdf_eval(m, wi_l, wo_l);
+        if (is_zero(f)) continue;
+        float cos_y = fabsf(dot(y.n, to_cam));
+        float g = cos_y / (cs.dist * cs.dist);
+        f3 c = y.beta * f * g * cs.we;
+        if (maxc(c) <= 0.0f) continue;
+        float pd = bsdf_pdf(m, wi_l, wo_l);
+        PV c0 = load_v(Vt, 0, 0, p);
+        float pdf_cl = pdf_sa_to_area(pd, y.p, cam.origin, c0.n);
+        float w = mis_weight(P, Vt, p, smax, tmax, 1, t, 0.0f, pdf_cl);
+        c = c * w;
+        uint32_t ix = (uint32_t)cs.px, iy = (uint32_t)cs.py;
+        if (ix > W - 1) ix = W - 1;
+        if (iy > H - 1) iy = H - 1;
+        emit(sr, c, iy * W + ix);
+        continue;
+      }
+      if (t == 0) continue;  // (:810-818)
+      // general / NEE (:844-1043)
+      PV z = load_v(Vt, 0, s - 1, p);
+      PV y = load_v(Vt, 1, t - 1, p);
+      f3 d = y.p - z.p;
+      float d2 = dot(d, d);
+      if (d2 < 1e-10f) continue;
+      float dist = sqrtf(d2);
+      f3 wi_z = d / dist;
+      f3 wi_y = -wi_z;
+      f3 oz = z.n * 1e-4f * signum(dot(wi_z, z.n));
+      f3 oy = y.n * 1e-4f * signum(dot(wi_y, y.n));
+      f3 so = z.p + oz, st = y.p + oy;
+      f3 sd = st - so;
+      float sdist = length(sd);
+      if (sdist < 1e-6f) continue;
+      Ray sr = make_ray(so, sd / sdist, 0.0f, sdist * (1.0f - 1e-4f));
+      PV zp = load_v(Vt, 0, s - 2, p);
+      Onb onz = onb_from_normal(z.n);
+      f3 wo_z_l = to_local(onz, normalize(zp.p - z.p));
+      const DMaterial& mz = v.materials[z.id];
+      f3 f_z = bsdf_eval(mz, to_local(onz, wi_z), wo_z_l);
+      if (is_zero(f_z)) continue;
+      f3 f_y = mk(1.0f, 1.0f, 1.0f);
+      const DLight* Lt = nullptr;
+      Onb ony = onb_from_normal(y.n);
+      f3 wo_y_l = splat(0.0f);
+      if (t == 1) {
+        Lt = &v.lights[y.id];
+        if (Lt->type == AR_LIGHT_QUAD && !(dot(y.n, wi_y) > 0.0f)) f_y = splat(0.0f);
+      } else {
+        PV yp = load_v(Vt, 1, t - 2, p);
+        wo_y_l = to_local(ony, normalize(yp.p - y.p));
+        f_y = bsdf_eval(v.materials[y.id], to_local(ony, wi_y), wo_y_l);
+      }
+      if (is_zero(f_y)) continue;
+      float g = geometry_term(z.p, z.n, y.p, y.n);
+      f3 c;
+      if (t == 1 && Lt->type == AR_LIGHT_POINT) {  // :947-967
+        LightSample ls;
+        if (!light_sample(*Lt, z.p, 0.0f, 0.0f, ls)) continue;
+        f3 fd = bsdf_eval(mz, to_local(onz, ls.wi), wo_z_l);
+        float cos_z = fabsf(dot(z.n, ls.wi));
+        c = z.beta * fd * cos_z * ls.li;
+      } else {
+        c = z.beta * f_z * g * f_y * y.beta;
+      }
+      if (maxc(c) <= 0.0f) continue;
+      float pdf_cc;
+      if (t == 1) {
+        f3 dyz = normalize(z.p - y.p);
+        pdf_cc = pdf_sa_to_area(light_pdf_emission_dir(*Lt, dyz), y.p, z.p, z.n);
+      } else {
+        f3 wi_y2 = normalize(z.p - y.p);
+        float pd = bsdf_pdf(v.materials[y.id], to_local(ony, wi_y2), wo_y_l);
+        pdf_cc = pdf_sa_to_area(pd, y.p, z.p, z.n);
+      }
+      f3 wi_z2 = normalize(y.p - z.p);
+      float pdz = bsdf_pdf(mz, to_local(onz, wi_z2), wo_z_l);
+      float pdf_cl = pdf_sa_to_area(pdz, z.p, y.p, y.n);
+      float w = mis_weight(P, Vt, p, smax, tmax, s, t, pdf_cc, pdf_cl);
+      emit(sr, c * w, kOwnPixel);
+    }
+  }
+}
+
+// Connection strategies traced where they are enumerated, one lane per path
+// (the megakernel schedule).
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams P, Verts Vt, float4* __restrict__ Lout,
                                                         unsigned long long* __restrict__ splat_film, int2* __restrict__ spill,
                                                         unsigned long long* __restrict__ stats) {
   __shared__ int2 lds[kTraceLdsStack * BLOCK];
-  const uint32_t G = gridDim.x, b = blockIdx.x;
-  const uint32_t lb = (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
-  const uint32_t gtid = lb * BLOCK + threadIdx.x;
-  const uint32_t nthreads = G * BLOCK;
-  Stack stk;
-  stk.lds = (LdsEntryPtr)(lds + threadIdx.x);
-  stk.stride = BLOCK;
-  stk.spill = (SpillEntryPtr)(spill + gtid);
-  stk.spill_stride = nthreads;
-  stk.S = kTraceLdsStack;
-  stk.sp = 0;
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  Stack stk = make_stack<BLOCK>(lds, spill, gtid, nthreads);
   uint32_t nrays = 0;
-  const Cam& cam = P.pt.cam;
-  const uint32_t W = P.pt.width, H = P.pt.height;
   for (uint32_t p = gtid; p < Vt.npaths; p += nthreads) {
-    uint2 cnt = Vt.count[p];
-    uint32_t smax = cnt.x, tmax = cnt.y, limit = P.pt.max_depth + 2;
     f3 acc = splat(0.0f);
-    for (uint32_t t = 0; t <= tmax; ++t) {
-      for (uint32_t s = 0; s <= smax; ++s) {
-        if (s + t < 2 || s + t > limit || s == 0) continue;
-        if (s == 1) {  // light tracing to the camera (:1051-1155)
-          if (t < 2) continue;
-          PV y = load_v(Vt, 1, t - 1, p);
-          WeSample cs;
-          if (!cam_sample_we(cam, y.p, W, H, cs)) continue;
-          f3 to_cam = cs.wi;
-          f3 off = y.n * 1e-4f * signum(dot(to_cam, y.n));
-          Ray sr = make_ray(y.p + off, to_cam, 0.0f, cs.dist * (1.0f - 1e-4f));
-          WorldHit hh;
-          nrays++;
-          if (trace_scene_wave<true, true>(v.sv, sr, hh, stk)) continue;
-          PV yp = load_v(Vt, 1, t - 2, p);
-          Onb on = onb_from_normal(y.n);
-          f3 wo_l = to_local(on, normalize(yp.p - y.p));
-          f3 wi_l = to_local(on, to_cam);
-          const DMaterial& m = v.materials[y.id];
-          f3 f = bsdf_eval(m, wi_l, wo_l);
-          if (is_zero(f)) continue;
-          float cos_y = fabsf(dot(y.n, to_cam));
-          float g = cos_y / (cs.dist * cs.dist);
-          f3 c = y.beta * f * g * cs.we;
-          if (maxc(c) <= 0.0f) continue;
-          float pd = bsdf_pdf(m, wi_l, wo_l);
-          PV c0 = load_v(Vt, 0, 0, p);
-          float pdf_cl = pdf_sa_to_area(pd, y.p, cam.origin, c0.n);
-          float w = mis_weight(P, Vt, p, smax, tmax, 1, t, 0.0f, pdf_cl);
-          c = c * w;
-          uint32_t ix = (uint32_t)cs.px, iy = (uint32_t)cs.py;
-          if (ix > W - 1) ix = W - 1;
-          if (iy > H - 1) iy = H - 1;
-          splat_add(splat_film, iy * W + ix, c);
-          continue;
-        }
-        if (t == 0) continue;  // (:810-818)
-        // general / NEE (:844-1043)
-        PV z = load_v(Vt, 0, s - 1, p);
-        PV y = load_v(Vt, 1, t - 1, p);
-        f3 d = y.p - z.p;
-        float d2 = dot(d, d);
-        if (d2 < 1e-10f) continue;
-        float dist = sqrtf(d2);
-        f3 wi_z = d / dist;
-        f3 wi_y = -wi_z;
-        f3 oz = z.n * 1e-4f * signum(dot(wi_z, z.n));
-        f3 oy = y.n * 1e-4f * signum(dot(wi_y, y.n));
-        f3 so = z.p + oz, st = y.p + oy;
-        f3 sd = st - so;
-        float sdist = length(sd);
-        if (sdist < 1e-6f) continue;
-        Ray sr = make_ray(so, sd / sdist, 0.0f, sdist * (1.0f - 1e-4f));
-        WorldHit hh;
-        nrays++;
-        if (trace_scene_wave<true, true>(v.sv, sr, hh, stk)) continue;
-        PV zp = load_v(Vt, 0, s - 2, p);
-        Onb onz = onb_from_normal(z.n);
-        f3 wo_z_l = to_local(onz, normalize(zp.p - z.p));
-        const DMaterial& mz = v.materials[z.id];
-        f3 f_z = bsdf_eval(mz, to_local(onz, wi_z), wo_z_l);
-        if (is_zero(f_z)) continue;
-        f3 f_y = mk(1.0f, 1.0f, 1.0f);
-        const DLight* Lt = nullptr;
-        Onb ony = onb_from_normal(y.n);
-        f3 wo_y_l = splat(0.0f);
-        if (t == 1) {
-          Lt = &v.lights[y.id];
-          if (Lt->type == AR_LIGHT_QUAD && !(dot(y.n, wi_y) > 0.0f)) f_y = splat(0.0f);
-        } else {
-          PV yp = load_v(Vt, 1, t - 2, p);
-          wo_y_l = to_local(ony, normalize(yp.p - y.p));
-          f_y = bsdf_eval(v.materials[y.id], to_local(ony, wi_y), wo_y_l);
-        }
-        if (is_zero(f_y)) continue;
-        float g = geometry_term(z.p, z.n, y.p, y.n);
-        f3 c;
-        if (t == 1 && Lt->type == AR_LIGHT_POINT) {  // :947-967
-          LightSample ls;
-          if (!light_sample(*Lt, z.p, 0.0f, 0.0f, ls)) continue;
-          f3 fd = bsdf_eval(mz, to_local(onz, ls.wi), wo_z_l);
-          float cos_z = fabsf(dot(z.n, ls.wi));
-          c = z.beta * fd * cos_z * ls.li;
-        } else {
-          c = z.beta * f_z * g * f_y * y.beta;
-        }
-        if (maxc(c) <= 0.0f) continue;
-        float pdf_cc;
-        if (t == 1) {
-          f3 dyz = normalize(z.p - y.p);
-          pdf_cc = pdf_sa_to_area(light_pdf_emission_dir(*Lt, dyz), y.p, z.p, z.n);
-        } else {
-          f3 wi_y2 = normalize(z.p - y.p);
-          float pd = bsdf_pdf(v.materials[y.id], to_local(ony, wi_y2), wo_y_l);
-          pdf_cc = pdf_sa_to_area(pd, y.p, z.p, z.n);
-        }
-        f3 wi_z2 = normalize(y.p - z.p);
-        float pdz = bsdf_pdf(mz, to_local(onz, wi_z2), wo_z_l);
-        float pdf_cl = pdf_sa_to_area(pdz, z.p, y.p, y.n);
-        float w = mis_weight(P, Vt, p, smax, tmax, s, t, pdf_cc, pdf_cl);
-        acc = acc + c * w;
-      }
-    }
+    bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target) {
+      WorldHit hh;
+      nrays++;
+      if (trace_scene_wave<true, true>(v.sv, sr, hh, stk)) return;
+      if (target == kOwnPixel)
+        acc = acc + c;
+      else
+        splat_add(splat_film, target, c);
+    });
     Lout[p] = make_float4(acc.x, acc.y, acc.z, 0.0f);
   }
-  uint32_t s = nrays;
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (__lane_id() == 0 && s) atomicAdd(&stats[1], (unsigned long long)s);
+  wave_add_u64(&stats[1], nrays);
+}
+
+// ---- wavefront connection: enumerate, trace the shadow rays together, gather
+// Paths [p0, p0 + m) of a chunk per pass.  Path p's strategies occupy slots
+// (p - p0) * K + j, j < cnt, in enumeration order; the queue lists the used
+// slots (appended per wave) for the any-hit stage, which writes occl[slot].
+struct ConnBufs {
+  float4* o;        // shadow ray origin, t_min           [m * K]
+  float4* d;        // direction, t_max
+  float4* val;      // weighted contribution, target bits
+  uint8_t* occl;    // 1: occluded
+  uint32_t* queue;  // used slots                          [m * K]
+  uint32_t* cnt;    // strategies per path                 [m]
+  uint32_t* qn;     // queue length
+  uint32_t K, p0, m;
+};
+
+__global__ void k_bdpt_conn_reset(ConnBufs C) {
+  if (threadIdx.x == 0) *C.qn = 0u;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, BdptParams P, Verts Vt, ConnBufs C,
+                                                          unsigned long long* __restrict__ stats) {
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  const uint32_t lane = __lane_id();
+  uint32_t nrays = 0;
+  // whole waves iterate together so the wave-aggregated append sees every lane
+  const uint32_t rounds = (C.m + nthreads - 1) / nthreads;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t i = r * nthreads + gtid;
+    uint32_t j = 0;
+    if (i < C.m) {
+      const uint32_t p = C.p0 + i;
+      const size_t base = (size_t)i * C.K;
+      bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target) {
+        const size_t slot = base + j++;
+        C.o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmin);
+        C.d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.tmax);
+        C.val[slot] = make_float4(c.x, c.y, c.z, __uint_as_float(target));
+      });
+      C.cnt[i] = j;
+    }
+    nrays += j;
+    uint32_t x = j;  // inclusive scan of the wave's counts
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= (uint32_t)o) x += y;
+    }
+    const uint32_t total = __shfl(x, 63);
+    uint32_t start = 0;
+    if (lane == 63 && total) start = atomicAdd(C.qn, total);
+    start = __shfl(start, 63) + x - j;
+    for (uint32_t k = 0; k < j; ++k) C.queue[start + k] = i * C.K + k;
+  }
+  wave_add_u64(&stats[1], nrays);
+}
+
+template <int BLOCK, int S, uint32_t SLOTS>
+__global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_conn_vis(SceneView sv, ConnBufs C,
+                                                                             int2* __restrict__ spill,
+                                                                             uint32_t refill_min, uint32_t search_min) {
+  __shared__ int2 lds[S * BLOCK];
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
+  const uint32_t n = *C.qn;
+  walk_items<true, true, SLOTS>(
+      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
+      [&](uint64_t i) {
+        const uint32_t slot = C.queue[i];
+        const float4 o = C.o[slot], d = C.d[slot];
+        Ray r;
+        r.o = mk(o.x, o.y, o.z);
+        r.tmin = o.w;
+        r.d = mk(d.x, d.y, d.z);
+        r.tmax = d.w;
+        return r;
+      },
+      [&](uint64_t i, const Walker<true, true>& w) { C.occl[C.queue[i]] = w.bpos != kNoHit ? 1 : 0; });
+}
+
+// Unoccluded contributions in enumeration order: the path's own radiance is
+// summed exactly as the megakernel schedule sums it; splats are fixed-point.
+__global__ void k_bdpt_conn_gather(ConnBufs C, float4* __restrict__ Lout, unsigned long long* __restrict__ splat_film) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < C.m; i += gridDim.x * blockDim.x) {
+    const uint32_t n = C.cnt[i];
+    f3 acc = splat(0.0f);
+    for (uint32_t k = 0; k < n; ++k) {
+      const size_t slot = (size_t)i * C.K + k;
+      if (C.occl[slot]) continue;
+      const float4 c = C.val[slot];
+      const uint32_t target = __float_as_uint(c.w);
+      if (target == kOwnPixel)
+        acc = acc + mk(c.x, c.y, c.z);
+      else
+        splat_add(splat_film, target, mk(c.x, c.y, c.z));
+    }
+    Lout[C.p0 + i] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+  }
 }
 
 // film = acc / spp, then + splat * (1/spp) where the splat is non-zero (:1341-1348)
@@ -554,13 +669,29 @@ __global__ void k_bdpt_zero_stats(unsigned long long* st) {
   if (threadIdx.x < 2) st[threadIdx.x] = 0ull;
 }
 
+// Slots per path for the wavefront connection: the (s, t) strategies that
+// can emit a shadow ray with at most V vertices per subpath (s, t >= 1,
+// s + t <= limit, s = 1 only with t >= 2).
+static uint32_t bdpt_conn_slots(uint32_t V, uint32_t limit) {
+  uint32_t K = 0;
+  for (uint32_t t = 1; t <= V; ++t)
+    for (uint32_t s = 1; s <= V; ++s)
+      if (s + t <= limit && !(s == 1 && t < 2)) ++K;
+  return K;
+}
+
+constexpr uint64_t kConnBudget = 3ull << 30;  // bytes of connection slots per pass
+
 class BdptJob : public JobRun {
  public:
   BdptParams P{};
   ShadeView v{};
   Verts Vt{};
+  ConnBufs C{};
+  ar_trace_config ta{};
   unsigned long long* splat = nullptr;
-  uint32_t k = 1, s0 = 0, G = 0, eg = 0;
+  uint32_t k = 1, s0 = 0, G = 0, eg = 0, Gs = 0, pass = 0;
+  bool wavefront = true;
 
   ar_status begin() override {
     constexpr int BLOCK = kTraceBlock;
@@ -582,22 +713,63 @@ class BdptJob : public JobRun {
     k = chunk_samples(spec);
     G = bdpt_grid(BLOCK);
     uint32_t cap = npix * k;
-    ar_status e = ws->reserve_paths(cap, npix, stack_spill_entries(scene), G * BLOCK);
+    wavefront = cfg->schedule != AR_SCHED_MEGAKERNEL;
+    // any-hit stage of the wavefront connection: the scene's any-hit launch
+    ta = scene->tcfg_any;
+    uint64_t spill_words = stack_spill_entries(scene) * (uint64_t)G * BLOCK;
+    if (wavefront) {
+      const uint32_t Ga = (uint32_t)(device_cus() * ta.blocks_per_cu / 8 * 8);
+      Gs = 0;
+#define AR_VIS_GRID(BB, SS)                                                                         \
+  if (ta.workgroup_size == BB && ta.lds_stack == SS)                                                \
+    Gs = clamp_resident(Ga,                                                                         \
+                        ta.park_slots == 4 ? (const void*)&k_bdpt_conn_vis<BB, SS, 4>               \
+                                           : (const void*)&k_bdpt_conn_vis<BB, SS, 3>,              \
+                        BB);
+      AR_TRACE_VARIANTS(AR_VIS_GRID)
+#undef AR_VIS_GRID
+      if (Gs == 0) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
+      const uint32_t need = scene->stack_depth;
+      const uint64_t e_t = need > ta.lds_stack ? need - ta.lds_stack : 0;
+      spill_words = std::max<uint64_t>(spill_words, e_t * (uint64_t)Gs * ta.workgroup_size);
+    }
+    ar_status e = ws->reserve_paths(cap, npix, (uint32_t)((spill_words + G * BLOCK - 1) / (G * BLOCK)), G * BLOCK);
     if (e != AR_OK) return e;
     size_t nvert = 2ull * P.V * cap;
-    size_t need =
-        nvert * 3 * sizeof(float4) + (size_t)cap * sizeof(uint2) + (size_t)npix * 3 * sizeof(long long) + 1024;
+    C.K = bdpt_conn_slots(P.V, cfg->max_depth + 2);
+    const uint64_t per_path = (uint64_t)C.K * (3 * sizeof(float4) + 1 + sizeof(uint32_t)) + sizeof(uint32_t);
+    pass = wavefront ? (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(cap, kConnBudget / per_path)) : 0;
+    const size_t slots = (size_t)pass * C.K;
+    auto al = [](size_t n) { return (n + 255) / 256 * 256; };
+    size_t need = al(nvert * 3 * sizeof(float4)) + al((size_t)cap * sizeof(uint2)) +
+                  al((size_t)npix * 3 * sizeof(long long));
+    if (wavefront)
+      need += 3 * al(slots * sizeof(float4)) + al(slots) + al(slots * sizeof(uint32_t)) +
+              al((size_t)pass * sizeof(uint32_t)) + 256;
     if (ws->aux.bytes < need) {
       AR_HIP(hipStreamSynchronize(stream));
       AR_HIP(ws->aux.alloc(need));
     }
-    char* base = (char*)ws->aux.p;
-    Vt.pf = (float4*)base;
+    char* cur = (char*)ws->aux.p;
+    auto take = [&](size_t n) {
+      char* q = cur;
+      cur += al(n);
+      return q;
+    };
+    Vt.pf = (float4*)take(nvert * 3 * sizeof(float4));
     Vt.nr = Vt.pf + nvert;
     Vt.bm = Vt.nr + nvert;
-    Vt.count = (uint2*)(Vt.bm + nvert);
-    uintptr_t tail = (uintptr_t)(Vt.count + cap);
-    splat = (unsigned long long*)(tail + 256 - tail % 256);
+    Vt.count = (uint2*)take((size_t)cap * sizeof(uint2));
+    splat = (unsigned long long*)take((size_t)npix * 3 * sizeof(long long));
+    if (wavefront) {
+      C.o = (float4*)take(slots * sizeof(float4));
+      C.d = (float4*)take(slots * sizeof(float4));
+      C.val = (float4*)take(slots * sizeof(float4));
+      C.occl = (uint8_t*)take(slots);
+      C.queue = (uint32_t*)take(slots * sizeof(uint32_t));
+      C.cnt = (uint32_t*)take((size_t)pass * sizeof(uint32_t));
+      C.qn = (uint32_t*)take(256);
+    }
     Vt.V = P.V;
     eg = std::min<uint32_t>((npix + 255) / 256, 4096);
     hipLaunchKernelGGL(k_bdpt_zero_stats, dim3(1), dim3(64), 0, stream, ws->B.stats);
@@ -614,8 +786,29 @@ class BdptJob : public JobRun {
       Vt.npaths = P.pt.npix * P.pt.k;
       hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->spill,
                          ws->B.stats);
-      hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->B.L, splat,
-                         ws->spill, ws->B.stats);
+      if (!wavefront) {
+        hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->B.L, splat,
+                           ws->spill, ws->B.stats);
+      }
+      for (uint32_t p0 = 0; wavefront && p0 < Vt.npaths; p0 += pass) {
+        C.p0 = p0;
+        C.m = std::min(pass, Vt.npaths - p0);
+        hipLaunchKernelGGL(k_bdpt_conn_reset, dim3(1), dim3(64), 0, stream, C);
+        hipLaunchKernelGGL((k_bdpt_conn_prep<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, C, ws->B.stats);
+#define AR_VIS(BB, SS)                                                                                      \
+  if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                      \
+    if (ta.park_slots == 4)                                                                                 \
+      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 4>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, ws->spill,   \
+                         ta.refill_min, ta.search_min);                                                     \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 3>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, ws->spill,   \
+                         ta.refill_min, ta.search_min);                                                     \
+  }
+        AR_TRACE_VARIANTS(AR_VIS)
+#undef AR_VIS
+        hipLaunchKernelGGL(k_bdpt_conn_gather, dim3(std::min<uint32_t>((C.m + 255) / 256, 4096)), dim3(256), 0,
+                           stream, C, ws->B.L, splat);
+      }
       hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, P.pt.npix, P.pt.k,
                          s0 == 0 ? 1u : 0u);
       AR_HIP(hipGetLastError());

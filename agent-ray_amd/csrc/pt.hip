@@ -64,28 +64,6 @@ __device__ __forceinline__ void block_push(uint32_t (&sh)[2][NQ][BLOCK / 64], ui
   for (int k = 0; k < NQ; ++k) slot[k] = sh[par][k][wave] + (uint32_t)__popcll(m[k] & below);
 }
 
-__device__ __forceinline__ void wave_add_u64(unsigned long long* counter, uint32_t v) {
-  uint32_t s = v;
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (__lane_id() == 0 && s) atomicAdd(counter, (unsigned long long)s);
-}
-
-template <int BLOCK, int S = kTraceLdsStack>
-__device__ __forceinline__ Stack make_stack(int2* lds, int2* spill, uint32_t gtid, uint32_t nthreads) {
-  Stack stk;
-  stk.lds = (LdsEntryPtr)(lds + threadIdx.x);
-  stk.stride = BLOCK;
-  stk.spill = (SpillEntryPtr)(spill + gtid);
-  stk.spill_stride = nthreads;
-  stk.S = S;
-  stk.sp = 0;
-  return stk;
-}
-
-__device__ __forceinline__ uint32_t xcd_block() {
-  const uint32_t G = gridDim.x, b = blockIdx.x;
-  return (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
-}
 
 // ---------------------------------------------------------------------------
 // Megakernel

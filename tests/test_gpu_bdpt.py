@@ -9,7 +9,11 @@ therefore check parity with the oracle, not unbiasedness:
 * shared counter RNG: films agree to rounding (fixed-point splats vs fp64
   splats, rare branch flips allowed);
 * both light types, power and uniform MIS modes, beta = 1 and 2;
-* determinism: identical films across runs and sample chunkings.
+* determinism: identical films across runs and sample chunkings;
+* the wavefront connection (enumerate, trace every shadow ray in one any-hit
+  stage, gather in enumeration order) gives the film of the megakernel
+  schedule (shadow rays traced as enumerated) bit for bit, with one and with
+  several connection passes per chunk and for any any-hit launch.
 """
 import numpy as np
 import pytest
@@ -17,9 +21,10 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def bdpt(ar, sc, cam, w, h, spp, seed=0, depth=8, mode=0, beta=2.0, chunk=0):
+def bdpt(ar, sc, cam, w, h, spp, seed=0, depth=8, mode=0, beta=2.0, chunk=0, schedule=None):
     b = ar.BidirectionalPathTracer(ar.BdptConfig(spp=spp, max_depth=depth, rr_depth=3, mis_mode=ar.MisMode(mode),
-                                                 mis_beta=beta), seed=seed, sample_chunk=chunk)
+                                                 mis_beta=beta), seed=seed, sample_chunk=chunk,
+                                   schedule=ar.Schedule.WAVEFRONT if schedule is None else schedule)
     film = b.render(sc, cam, w, h)
     return film.pixels.copy(), b.last_stats
 
@@ -60,3 +65,31 @@ def test_bdpt_room_runs(ar, wl):
     cam16 = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 64 / 36)
     g, st = bdpt(ar, sc, ar.PinholeCamera.from_array(cam16), 64, 36, 2, depth=10)
     assert np.all(np.isfinite(g)) and g.mean() > 0
+
+
+@pytest.mark.parametrize("twin,mode", [(True, 0), (False, 0), (False, 1)])
+def test_bdpt_wavefront_connection_bitwise(ar, wl, twin, mode):
+    desc = wl.cornell_box(point_light_twin=twin)
+    sc = ar.Scene.from_desc(desc)
+    cam16 = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 1.0)
+    cam = ar.PinholeCamera.from_array(cam16)
+    # 256 x 256 x 16 at depth 10: 2^20 paths per chunk, more than one 3 GiB
+    # connection pass holds (65 slots of 53 B per path), so two passes
+    mega, sm = bdpt(ar, sc, cam, 256, 256, 16, seed=3, depth=10, mode=mode, schedule=ar.Schedule.MEGAKERNEL)
+    wave, sw = bdpt(ar, sc, cam, 256, 256, 16, seed=3, depth=10, mode=mode)
+    assert np.array_equal(mega.view(np.uint32), wave.view(np.uint32))
+    assert sm.shadow_rays == sw.shadow_rays > 0
+    chunked, _ = bdpt(ar, sc, cam, 256, 256, 16, seed=3, depth=10, mode=mode, chunk=5)
+    assert np.array_equal(chunked.view(np.uint32), wave.view(np.uint32))
+
+
+def test_bdpt_wavefront_connection_launches(ar, wl):
+    desc = wl.procedural_room(n_boxes=20, width=96, height=54)
+    sc = ar.Scene.from_desc(desc)
+    cam16 = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 96 / 54)
+    cam = ar.PinholeCamera.from_array(cam16)
+    ref, _ = bdpt(ar, sc, cam, 96, 54, 4, depth=10, schedule=ar.Schedule.MEGAKERNEL)
+    for cfg in [(64, 16, 16, 8, 16, 3), (256, 8, 8, 24, 8, 4), (128, 8, 4, 1, 0, 3)]:
+        sc.set_anyhit_config(*cfg)
+        g, _ = bdpt(ar, sc, cam, 96, 54, 4, depth=10)
+        assert np.array_equal(g.view(np.uint32), ref.view(np.uint32)), cfg
