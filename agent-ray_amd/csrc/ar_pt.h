@@ -67,6 +67,53 @@ __device__ __forceinline__ uint32_t xcd_block() {
   return (G % 8 == 0) ? (b % 8) * (G / 8) + (b / 8) : b;
 }
 
+// Workgroup-aggregated queue appends for NQ queues at once: every wave
+// ballots its flags, the waves' counts meet in LDS, and one lane per queue
+// takes the workgroup's range with a single atomic.  Appends from every wave
+// of the device to one counter serialise in L2 (about 11 ns each here), so
+// one atomic per workgroup instead of per wave is what keeps the generate
+// and shade stages from being bound by them.  Lanes get consecutive slots in
+// wave order.  Every thread of the workgroup must call it the same number of
+// times (it contains two barriers); `par` alternates between calls so one
+// call's LDS row is not overwritten while another wave still reads it.
+template <int BLOCK, int NQ>
+__device__ __forceinline__ void block_push(uint32_t (&sh)[2][NQ][BLOCK / 64], uint32_t par,
+                                           uint32_t* const (&counter)[NQ], const bool (&flag)[NQ],
+                                           uint32_t (&slot)[NQ]) {
+  const uint32_t wave = threadIdx.x / 64, lane = __lane_id();
+  uint64_t m[NQ];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    m[k] = __ballot(flag[k]);
+    if (lane == 0) sh[par][k][wave] = (uint32_t)__popcll(m[k]);
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    const int k = threadIdx.x;
+    uint32_t tot = 0;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+      const uint32_t c = sh[par][k][w];
+      sh[par][k][w] = tot;
+      tot += c;
+    }
+    const uint32_t base = tot ? atomicAdd(counter[k], tot) : 0u;
+    for (int w = 0; w < BLOCK / 64; ++w) sh[par][k][w] += base;
+  }
+  __syncthreads();
+  const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) slot[k] = sh[par][k][wave] + (uint32_t)__popcll(m[k] & below);
+}
+
+__device__ __forceinline__ Ray make_ray4(float4 o, float4 d) {
+  Ray r;
+  r.o = mk(o.x, o.y, o.z);
+  r.tmin = o.w;
+  r.d = mk(d.x, d.y, d.z);
+  r.tmax = d.w;
+  return r;
+}
+
 // Camera ray for (x, y) with the path's first two dimensions as jitter
 // (path_tracer.rs:176-182).
 __device__ __forceinline__ void pt_begin(const PtParams& P, uint32_t x, uint32_t y, Rng& rng, PathState& s) {

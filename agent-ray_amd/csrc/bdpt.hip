@@ -2,10 +2,14 @@
 // (BidirectionalPathTracer::render, agent-ray
 // src/integrators/bidirectional_path_tracer.rs:1221-1357).
 //
-// Phase 1 (k_bdpt_subpaths): one lane per (pixel, sample) builds the camera
-// subpath (:338-437) and the light subpath (:443-559) into a structure-of-
-// arrays vertex buffer in HBM, [side][vertex slot][path], so lanes of a wave
-// touch consecutive addresses for the same slot.
+// Phase 1, subpaths: the camera subpath (:338-437) and the light subpath
+// (:443-559) of every (pixel, sample) go into a structure-of-arrays vertex
+// buffer in HBM, [side][vertex slot][path], so lanes of a wave touch
+// consecutive addresses for the same slot.  Wavefront schedule (default):
+// all camera subpaths, then all light subpaths, as bounce loops of an extend
+// stage (closest hit over the live queue, walk_items, the scene's
+// closest-hit launch) and a shade stage.  Megakernel schedule
+// (k_bdpt_subpaths): one lane builds both subpaths of its path.
 // Phase 2, connection: one lane per path enumerates the strategies in the
 // reference order (t outer, s inner; :1281-1334) and evaluates each
 // contribution with its MIS weight (:594-770, fp64 sums); a contribution
@@ -313,6 +317,233 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_subpaths(ShadeView v, BdptParams
   if (__lane_id() == 0 && s) atomicAdd(&stats[0], (unsigned long long)s);
 }
 
+// ---- wavefront subpaths: the camera subpaths of every path, then the light
+// subpaths (a path's light subpath draws its random numbers after its camera
+// subpath's, so the two sides run one after the other), each as a bounce
+// loop of extend (closest hit over the live queue) and shade (one iteration
+// of extend_subpath's loop) stages.  Live subpath state: next ray in
+// B.ray_o / B.ray_d, (beta, pdf of the sampled direction) in B.beta,
+// (vertices stored, depth, random-number dimension, -) in B.meta.  The
+// vertices and counts equal those of k_bdpt_subpaths bit for bit.
+
+__device__ __forceinline__ void sub_state(const WaveBufs& B, uint32_t p, const Ray& r, f3 beta, float pdf_dir,
+                                          uint32_t nv, uint32_t depth, uint32_t dim) {
+  B.ray_o[p] = make_float4(r.o.x, r.o.y, r.o.z, r.tmin);
+  B.ray_d[p] = make_float4(r.d.x, r.d.y, r.d.z, r.tmax);
+  B.beta[p] = make_float4(beta.x, beta.y, beta.z, pdf_dir);
+  B.meta[p] = make_uint4(nv, depth, dim, 0u);
+}
+
+// Camera vertex and first ray of every path (:338-361); queue 0 = all paths.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bdpt_cam_gen(BdptParams P, Verts Vt, WaveBufs B) {
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  for (uint32_t p = xcd_block() * BLOCK + threadIdx.x; p < Vt.npaths; p += nthreads) {
+    const uint32_t pixel = p % P.pt.npix, smp = P.pt.s0 + p / P.pt.npix;
+    Rng rng;
+    rng.init(P.pt.seed, pixel, smp, 0);
+    float jx = rng.next(), jy = rng.next();
+    float u = ((float)(pixel % P.pt.width) + jx) / (float)P.pt.width;
+    float vv = 1.0f - ((float)(pixel / P.pt.width) + jy) / (float)P.pt.height;
+    Ray ray = camera_ray(P.pt.cam, u, vv);
+    PV cv;
+    cv.type = VT_CAMERA;
+    cv.delta = 1;
+    cv.id = 0;
+    cv.p = P.pt.cam.origin;
+    cv.n = ray.d;
+    cv.beta = mk(1.0f, 1.0f, 1.0f);
+    cv.pdf_fwd = 1.0f;
+    cv.pdf_rev = 0.0f;
+    store_v(Vt, 0, 0, p, cv);
+    sub_state(B, p, ray, mk(1.0f, 1.0f, 1.0f), cam_pdf_we(P.pt.cam, ray.d), 1u, 0u, rng.dim);
+    Vt.count[p] = make_uint2(1u, 0u);
+    B.q[0][p] = p;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    B.cnt[0] = P.pt.max_depth > 0 ? Vt.npaths : 0u;
+    B.cnt[1] = 0u;
+  }
+}
+
+// Light vertex and first ray of every path (:451-496), its random numbers
+// continuing at the camera subpath's last dimension; emitting lights go to
+// queue 0.
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bdpt_light_gen(ShadeView v, BdptParams P, Verts Vt, WaveBufs B) {
+  __shared__ uint32_t qsh[2][1][BLOCK / 64];
+  uint32_t* const ctr[1] = {&B.cnt[0]};
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t rounds = (Vt.npaths + nthreads - 1) / nthreads;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t p = r * nthreads + gtid;
+    bool push = false;
+    if (p < Vt.npaths && v.dist_mode != AR_LIGHTDIST_NONE && v.n_lights > 0) {
+      const uint32_t pixel = p % P.pt.npix, smp = P.pt.s0 + p / P.pt.npix;
+      Rng rng;
+      rng.init(P.pt.seed, pixel, smp, 0, B.meta[p].z);
+      float u_sel = rng.next();
+      uint32_t li;
+      float sel_pmf;
+      light_select(v, u_sel, &li, &sel_pmf);
+      const DLight& L = v.lights[li];
+      float upx = rng.next(), upy = rng.next(), udx = rng.next(), udy = rng.next();
+      Emission em;
+      if (light_emission(L, upx, upy, udx, udy, em)) {
+        PV lv;
+        lv.type = VT_LIGHT;
+        lv.delta = L.type == AR_LIGHT_POINT ? 1u : 0u;
+        lv.id = li;
+        lv.p = em.ray.o;
+        lv.n = em.n_light;
+        lv.pdf_fwd = em.pdf_pos * sel_pmf;
+        lv.pdf_rev = 0.0f;
+        lv.beta = em.le / (em.pdf_pos * sel_pmf);
+        store_v(Vt, 1, 0, p, lv);
+        Vt.count[p].y = 1u;
+        if (P.pt.max_depth > 0) {
+          f3 thr = em.le / (em.pdf_pos * sel_pmf * em.pdf_dir);
+          float cos_emit = fabsf(dot(em.n_light, em.ray.d));
+          thr = thr * cos_emit;
+          sub_state(B, p, em.ray, thr, em.pdf_dir, 1u, 0u, rng.dim);
+          push = true;
+        }
+      }
+    }
+    const bool fl[1] = {push};
+    uint32_t slot[1];
+    block_push<BLOCK, 1>(qsh, r & 1, ctr, fl, slot);
+    if (push) B.q[0][slot[0]] = p;
+  }
+}
+
+// Closest hit of the live subpaths of queue qi (near-first order).
+template <int BLOCK, int S, uint32_t SLOTS>
+__global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_sub_extend(SceneView sv, WaveBufs B, int qi,
+                                                                               int2* __restrict__ spill,
+                                                                               uint32_t refill_min,
+                                                                               uint32_t search_min) {
+  __shared__ int2 lds[S * BLOCK];
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
+  const uint32_t n = B.cnt[qi];
+  const uint32_t* q = B.q[qi];
+  walk_items<false, true, SLOTS>(
+      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
+      [&](uint64_t i) {
+        const uint32_t p = q[i];
+        return make_ray4(B.ray_o[p], B.ray_d[p]);
+      },
+      [&](uint64_t i, const Walker<false, true>& w) {
+        B.hit[i] = make_float4(w.bt, __uint_as_float(w.bpos), w.bu, w.bv);
+        B.hit_inst[i] = w.binst;
+      });
+  if (gtid == 0) atomicAdd(&B.stats[0], (unsigned long long)n);
+}
+
+// One iteration of extend_subpath's loop (:368-434, :498-556) for the live
+// subpaths of side `side` in queue qi; survivors go to queue qi ^ 1, the
+// others record their vertex count (and, on the camera side, the dimension
+// where the light subpath's random numbers start).
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_bdpt_sub_shade(ShadeView v, BdptParams P, Verts Vt, WaveBufs B, int qi,
+                                                          uint32_t side) {
+  __shared__ uint32_t qsh[2][1][BLOCK / 64];
+  uint32_t* const ctr[1] = {&B.cnt[qi ^ 1]};
+  const uint32_t n = B.cnt[qi];
+  const uint32_t* q = B.q[qi];
+  uint32_t* qn = B.q[qi ^ 1];
+  const uint32_t nthreads = gridDim.x * BLOCK;
+  const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
+  const uint32_t rounds = (n + nthreads - 1) / nthreads;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t i = r * nthreads + gtid;
+    bool cont = false;
+    uint32_t p = 0;
+    if (i < n) {
+      p = q[i];
+      const float4 hv = B.hit[i], st = B.beta[p];
+      const uint4 mt = B.meta[p];
+      uint32_t nv = mt.x, depth = mt.y;
+      f3 beta = mk(st.x, st.y, st.z);
+      Ray ray = make_ray4(B.ray_o[p], B.ray_d[p]);
+      Rng rng;
+      rng.init(P.pt.seed, p % P.pt.npix, P.pt.s0 + p / P.pt.npix, 0, mt.z);
+      const uint32_t pos = __float_as_uint(hv.y);
+      if (pos != kNoHit) {
+        ShadingPoint sp;
+        make_shading(v, ray, hv.x, pos, B.hit_inst[i], hv.z, hv.w, sp);
+        PV prev = load_v(Vt, side, nv - 1, p);
+        PV sv;
+        sv.type = VT_SURFACE;
+        sv.delta = 0;
+        sv.id = sp.mat;
+        sv.p = sp.p;
+        sv.n = sp.n;
+        sv.beta = beta;
+        sv.pdf_fwd = pdf_sa_to_area(st.w, prev.p, sp.p, sp.n);
+        sv.pdf_rev = 0.0f;
+        store_v(Vt, side, nv, p, sv);
+        nv++;
+        const DMaterial& m = v.materials[sp.mat];
+        f3 wo_l = to_local(sp.onb, -ray.d);
+        float u_sel = rng.next();
+        float ux = rng.next(), uy = rng.next();
+        BsdfSample bs;
+        if (bsdf_sample(m, wo_l, u_sel, ux, uy, bs) && !(bs.pdf == 0.0f || is_zero(bs.f))) {
+          float cos_wi = fabsf(bs.wi.z);
+          beta = beta * (bs.f * cos_wi / bs.pdf);
+          float pdf_rev_dir = bsdf_pdf(m, wo_l, bs.wi);
+          const float pdf_rev = pdf_sa_to_area(pdf_rev_dir, sp.p, prev.p, prev.n);
+          size_t bi = Vt.at(side, nv - 2, p);
+          float4 nr = Vt.nr[bi];
+          Vt.nr[bi] = make_float4(nr.x, nr.y, nr.z, pdf_rev);
+          bool alive = true;
+          if (depth + 1 >= P.pt.rr_depth) {
+            float surv = fminf(maxc(beta), 0.95f);
+            if (rng.next() > surv || surv <= 0.0f)
+              alive = false;
+            else
+              beta = beta / surv;
+          }
+          if (alive && depth + 1 < P.pt.max_depth) {
+            f3 wi_w = to_world(sp.onb, bs.wi);
+            f3 off = sp.n * 1e-4f * signum(dot(wi_w, sp.n));
+            sub_state(B, p, make_ray(sp.p + off, wi_w, 1e-4f, kF32Max), beta, bs.pdf, nv, depth + 1, rng.dim);
+            cont = true;
+          }
+        }
+      }
+      if (!cont) {
+        if (side == 0) {
+          Vt.count[p].x = nv;
+          B.meta[p] = make_uint4(nv, depth, rng.dim, 0u);
+        } else {
+          Vt.count[p].y = nv;
+        }
+      }
+    }
+    const bool fl[1] = {cont};
+    uint32_t slot[1];
+    block_push<BLOCK, 1>(qsh, r & 1, ctr, fl, slot);
+    if (cont) qn[slot[0]] = p;
+  }
+}
+
+// End of a bounce: the consumed queue becomes empty.  With qi < 0 both.
+__global__ void k_bdpt_sub_reset(WaveBufs B, int qi) {
+  if (threadIdx.x == 0) {
+    if (qi < 0) {
+      B.cnt[0] = 0u;
+      B.cnt[1] = 0u;
+    } else {
+      B.cnt[qi] = 0u;
+    }
+  }
+}
+
 __device__ __forceinline__ double powb(double x, float beta) {
   return beta == 2.0f ? x * x : pow(x, (double)beta);
 }
@@ -587,13 +818,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_conn_vis(Sce
       sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
       [&](uint64_t i) {
         const uint32_t slot = C.queue[i];
-        const float4 o = C.o[slot], d = C.d[slot];
-        Ray r;
-        r.o = mk(o.x, o.y, o.z);
-        r.tmin = o.w;
-        r.d = mk(d.x, d.y, d.z);
-        r.tmax = d.w;
-        return r;
+        return make_ray4(C.o[slot], C.d[slot]);
       },
       [&](uint64_t i, const Walker<true, true>& w) { C.occl[C.queue[i]] = w.bpos != kNoHit ? 1 : 0; });
 }
@@ -688,10 +913,11 @@ class BdptJob : public JobRun {
   ShadeView v{};
   Verts Vt{};
   ConnBufs C{};
-  ar_trace_config ta{};
+  ar_trace_config tc{}, ta{};
   unsigned long long* splat = nullptr;
-  uint32_t k = 1, s0 = 0, G = 0, eg = 0, Gs = 0, pass = 0;
+  uint32_t k = 1, s0 = 0, G = 0, eg = 0, Gs = 0, Ge = 0, Gw = 0, pass = 0;
   bool wavefront = true;
+  static constexpr int kSubBlock = 512;  // subpath generate / shade stages
 
   ar_status begin() override {
     constexpr int BLOCK = kTraceBlock;
@@ -714,10 +940,29 @@ class BdptJob : public JobRun {
     G = bdpt_grid(BLOCK);
     uint32_t cap = npix * k;
     wavefront = cfg->schedule != AR_SCHED_MEGAKERNEL;
-    // any-hit stage of the wavefront connection: the scene's any-hit launch
+    // subpath extension stages: the scene's closest-hit launch; any-hit
+    // stage of the wavefront connection: the scene's any-hit launch
+    tc = scene->tcfg;
     ta = scene->tcfg_any;
     uint64_t spill_words = stack_spill_entries(scene) * (uint64_t)G * BLOCK;
     if (wavefront) {
+      Gw = (uint32_t)(device_cus() * (kTraceBlocksPerCU * kTraceBlock / kSubBlock) / 8 * 8);
+      const uint32_t Gt = (uint32_t)(device_cus() * tc.blocks_per_cu / 8 * 8);
+      Ge = 0;
+#define AR_EXT_GRID(BB, SS)                                                                         \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS)                                                \
+    Ge = clamp_resident(Gt,                                                                         \
+                        tc.park_slots == 4 ? (const void*)&k_bdpt_sub_extend<BB, SS, 4>             \
+                                           : (const void*)&k_bdpt_sub_extend<BB, SS, 3>,            \
+                        BB);
+      AR_TRACE_VARIANTS(AR_EXT_GRID)
+#undef AR_EXT_GRID
+      if (Ge == 0) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
+      {
+        const uint32_t need = scene->stack_depth;
+        const uint64_t e_t = need > tc.lds_stack ? need - tc.lds_stack : 0;
+        spill_words = std::max<uint64_t>(spill_words, e_t * (uint64_t)Ge * tc.workgroup_size);
+      }
       const uint32_t Ga = (uint32_t)(device_cus() * ta.blocks_per_cu / 8 * 8);
       Gs = 0;
 #define AR_VIS_GRID(BB, SS)                                                                         \
@@ -784,8 +1029,36 @@ class BdptJob : public JobRun {
       P.pt.s0 = s0;
       P.pt.k = std::min(k, P.pt.spp - s0);
       Vt.npaths = P.pt.npix * P.pt.k;
-      hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->spill,
-                         ws->B.stats);
+      if (!wavefront) {
+        hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->spill,
+                           ws->B.stats);
+      } else {
+        const WaveBufs& B = ws->B;
+        for (uint32_t side = 0; side < 2; ++side) {
+          if (side == 0)
+            hipLaunchKernelGGL((k_bdpt_cam_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, P, Vt, B);
+          else
+            hipLaunchKernelGGL((k_bdpt_light_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, P, Vt, B);
+          for (uint32_t d = 0; d < P.pt.max_depth; ++d) {
+            const int qi = d & 1;
+#define AR_SUB_EXT(BB, SS)                                                                                   \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS) {                                                       \
+    if (tc.park_slots == 4)                                                                                  \
+      hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 4>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi,         \
+                         ws->spill, tc.refill_min, tc.search_min);                                           \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 3>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi,         \
+                         ws->spill, tc.refill_min, tc.search_min);                                           \
+  }
+            AR_TRACE_VARIANTS(AR_SUB_EXT)
+#undef AR_SUB_EXT
+            hipLaunchKernelGGL((k_bdpt_sub_shade<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, P, Vt, B, qi,
+                               side);
+            hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, qi);
+          }
+          hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, -1);
+        }
+      }
       if (!wavefront) {
         hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->B.L, splat,
                            ws->spill, ws->B.stats);

@@ -43,7 +43,7 @@ constexpr uint64_t kServerChunkBdpt = 1ull << 16;    // BDPT samples per chunk
 // BDPT sample = both subpaths and all connections), used to estimate device
 // time for scheduling.
 static double path_weight(uint32_t integrator) {
-  return integrator == AR_INTEGRATOR_BDPT ? 7.0 : (integrator == AR_INTEGRATOR_GDPT ? 2.8 : 1.0);
+  return integrator == AR_INTEGRATOR_BDPT ? 5.0 : (integrator == AR_INTEGRATOR_GDPT ? 2.8 : 1.0);
 }
 
 struct Slot {

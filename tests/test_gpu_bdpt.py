@@ -10,10 +10,11 @@ therefore check parity with the oracle, not unbiasedness:
   splats, rare branch flips allowed);
 * both light types, power and uniform MIS modes, beta = 1 and 2;
 * determinism: identical films across runs and sample chunkings;
-* the wavefront connection (enumerate, trace every shadow ray in one any-hit
-  stage, gather in enumeration order) gives the film of the megakernel
-  schedule (shadow rays traced as enumerated) bit for bit, with one and with
-  several connection passes per chunk and for any any-hit launch.
+* the wavefront schedule (subpaths as bounce loops of extend and shade
+  stages; connection enumerated, every shadow ray traced in one any-hit
+  stage, gathered in enumeration order) gives the film of the megakernel
+  schedule (one lane per path, rays traced as they come) bit for bit, with
+  one and with several connection passes per chunk and for any launch.
 """
 import numpy as np
 import pytest
