@@ -905,7 +905,7 @@ static uint32_t bdpt_conn_slots(uint32_t V, uint32_t limit) {
   return K;
 }
 
-constexpr uint64_t kConnBudget = 3ull << 30;  // bytes of connection slots per pass
+constexpr uint64_t kConnBudget = 12ull << 30;  // bytes of connection slots per pass
 
 class BdptJob : public JobRun {
  public:

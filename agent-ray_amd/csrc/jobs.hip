@@ -12,8 +12,8 @@ uint32_t chunk_samples(const JobSpec& spec) {
   // Paths per chunk: large, because every chunk pays the tail of its last
   // bounces (a handful of long paths whose traversal is one dependent chain
   // of loads) and HBM has room: 2^25 paths of wavefront state is ~6.6 GB,
-  // 2^22 BDPT samples of vertices ~4 GB at depth 10.
-  uint64_t target = c.integrator == AR_INTEGRATOR_BDPT ? (1ull << 22) : (1ull << 25);
+  // 2^24 BDPT samples of vertices ~17 GB at depth 10.
+  uint64_t target = c.integrator == AR_INTEGRATOR_BDPT ? (1ull << 24) : (1ull << 25);
   uint32_t k = c.sample_chunk;
   if (k == 0) k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.spp, target / std::max<uint64_t>(per_sample, 1)));
   return std::min(k, c.spp);

@@ -74,13 +74,13 @@ def test_bdpt_wavefront_connection_bitwise(ar, wl, twin, mode):
     sc = ar.Scene.from_desc(desc)
     cam16 = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 1.0)
     cam = ar.PinholeCamera.from_array(cam16)
-    # 256 x 256 x 16 at depth 10: 2^20 paths per chunk, more than one 3 GiB
-    # connection pass holds (65 slots of 53 B per path), so two passes
-    mega, sm = bdpt(ar, sc, cam, 256, 256, 16, seed=3, depth=10, mode=mode, schedule=ar.Schedule.MEGAKERNEL)
-    wave, sw = bdpt(ar, sc, cam, 256, 256, 16, seed=3, depth=10, mode=mode)
+    # 256 x 256 x 32 at depth 16: 2^21 paths in one chunk, more than one
+    # 12 GiB connection pass holds (152 slots of 53 B per path), so two passes
+    mega, sm = bdpt(ar, sc, cam, 256, 256, 32, seed=3, depth=16, mode=mode, schedule=ar.Schedule.MEGAKERNEL)
+    wave, sw = bdpt(ar, sc, cam, 256, 256, 32, seed=3, depth=16, mode=mode)
     assert np.array_equal(mega.view(np.uint32), wave.view(np.uint32))
     assert sm.shadow_rays == sw.shadow_rays > 0
-    chunked, _ = bdpt(ar, sc, cam, 256, 256, 16, seed=3, depth=10, mode=mode, chunk=5)
+    chunked, _ = bdpt(ar, sc, cam, 256, 256, 32, seed=3, depth=16, mode=mode, chunk=5)
     assert np.array_equal(chunked.view(np.uint32), wave.view(np.uint32))
 
 
