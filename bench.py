@@ -488,7 +488,9 @@ def bench_film_output(ar, torch, pixels, spp):
 def bench_bdpt_gdpt(ar, torch):
     """The room at 1280x720: BDPT 16 spp, depth 10, power heuristic beta 2;
     GDPT 32 spp, depth 16, base + two shifted paths per sample, screened
-    Poisson alpha 0.2 with 50 CG iterations.  Each timed render follows a
+    Poisson alpha 0.2 with 50 CG iterations.  The scene is autotuned on
+    ray samples, then each integrator refines the launches on a short render
+    of its own job (PathTracer.tune_launch).  Each timed render follows a
     warm-up render; the Poisson solve is also timed alone (device buffers,
     HIP events, allocation of its work planes included) against its
     algorithmic bytes: 120 B per pixel and iteration (two passes over fp32
@@ -502,19 +504,29 @@ def bench_bdpt_gdpt(ar, torch):
     cam = ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, w / h))
     out = {}
     tuned = autotune_scene(ar, torch, sc, desc, w / h)
+    sample_c, sample_a = ar.config_key(sc.trace_config()), ar.config_key(sc.anyhit_config())
     bd = ar.BidirectionalPathTracer(ar.BdptConfig(spp=16, max_depth=10, rr_depth=3), seed=0)
+    # refine both launches on one 4-spp chunk of the BDPT job itself (subpath
+    # extension = closest hit, connection = any hit)
+    bd_refined = bd.tune_launch(sc, cam, w, h, spp=4)
     bd.render(sc, cam, w, h)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bd.render(sc, cam, w, h)
     dt = time.perf_counter() - t0
     st = bd.last_stats
-    out["bdpt_room_1280x720x16spp"] = {"launch_config": tuned, "msamples_s": round(w * h * 16 / dt / 1e6, 2),
+    out["bdpt_room_1280x720x16spp"] = {"launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()),
+                                       "launch_refined": bd_refined, "msamples_s": round(w * h * 16 / dt / 1e6, 2),
                                        "mrays_s": round(st.rays / dt / 1e6, 1), "seconds": round(dt, 4),
                                        "rays": int(st.rays)}
     log(f"bdpt room: {out['bdpt_room_1280x720x16spp']}")
     gd = ar.GradientDomainPathTracer(ar.GdptConfig(spp=32, max_depth=16, rr_depth=3, alpha=0.2,
                                                    poisson_iterations=50), seed=0)
+    # GDPT traces path-tracer stages: start again from the sample-tuned
+    # launches and refine them on one 8-spp chunk of its own job
+    sc.set_trace_config(*sample_c)
+    sc.set_anyhit_config(*sample_a)
+    gd_refined = gd.tune_launch(sc, cam, w, h, spp=8)
     gd.render(sc, cam, w, h)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -522,7 +534,8 @@ def bench_bdpt_gdpt(ar, torch):
     dt = time.perf_counter() - t0
     st = gd.last_stats
     g = {"mpaths_s": round(st.paths / dt / 1e6, 2), "mrays_s": round(st.rays / dt / 1e6, 1),
-         "seconds": round(dt, 4), "paths": int(st.paths)}
+         "seconds": round(dt, 4), "paths": int(st.paths),
+         "launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()), "launch_refined": gd_refined}
     # the Poisson solve alone on the render's own films
     films = [torch.from_numpy(np.ascontiguousarray(gd.aux[k], np.float32)).cuda() for k in range(3)]
     dst = torch.empty_like(films[0])
