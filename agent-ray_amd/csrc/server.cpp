@@ -14,8 +14,11 @@
 // Device time is estimated from the job's work (pixels x spp, weighted per
 // integrator) and a running measurement of the device's rate; stream
 // priorities are not used.  Server jobs without an explicit sample_chunk run
-// in one chunk when small and in chunks of 2^20 path-equivalents (2^16 BDPT
-// samples) otherwise, so the wait for a chunk stays short.  Several chunks
+// in one chunk when small and in chunks of 2^23 path-equivalents (2^19 BDPT
+// samples, both ~20-40 ms of device time) otherwise: smaller chunks keep the
+// wait for the running chunk short but pay their bounce tails and launches
+// more often (at 2^20 / 2^16 the device spent about half its time on them),
+// larger ones make small jobs wait longer.  Several chunks
 // on different streams would not help a short job: the device interleaves
 // kernels of concurrent streams one for one, so each of the short job's
 // kernels would wait for one of the long job's.
@@ -35,8 +38,8 @@
 namespace ar {
 
 constexpr int kInflight = 1;                 // chunks on the device, all jobs together
-constexpr uint64_t kServerChunkPaths = 1ull << 20;   // PT / GDPT path-equivalents per chunk
-constexpr uint64_t kServerChunkBdpt = 1ull << 16;    // BDPT samples per chunk
+constexpr uint64_t kServerChunkPaths = 1ull << 23;   // PT / GDPT path-equivalents per chunk
+constexpr uint64_t kServerChunkBdpt = 1ull << 19;    // BDPT samples per chunk
 
 // Relative device cost of one pixel-sample of each integrator, measured on
 // the room at 1280x720 (PT path 1; GDPT sample = base + two shifted paths;
