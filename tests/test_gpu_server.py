@@ -52,7 +52,8 @@ def test_mix_equals_alone(ar, wl, room):
 def test_short_job_overtakes_long_job(ar, wl, room):
     desc, sc = room
     srv = ar.RenderServer(sc, max_active_jobs=4)
-    long_id = srv.submit(_integ(ar, 0, 256, 1), _cam(ar, wl, desc, 256, 256), 256, 256)
+    # 512 x 512 x 256 = 2^26 paths: eight of the server's 2^23-path chunks
+    long_id = srv.submit(_integ(ar, 0, 256, 1), _cam(ar, wl, desc, 512, 512), 512, 512)
     time.sleep(0.02)
     short_id = srv.submit(_integ(ar, 0, 1, 2), _cam(ar, wl, desc, 16, 16), 16, 16)
     _, s_short = srv.wait(short_id)
