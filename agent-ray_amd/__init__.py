@@ -51,7 +51,8 @@ class ArError(RuntimeError):
 # ctypes mirrors of include/agentray.h
 # ---------------------------------------------------------------------------
 class ArMaterial(C.Structure):
-    _fields_ = [("type", C.c_uint32), ("albedo", C.c_float * 3), ("roughness", C.c_float), ("eta", C.c_float)]
+    _fields_ = [("type", C.c_uint32), ("albedo", C.c_float * 3), ("roughness", C.c_float), ("eta", C.c_float),
+                ("albedo_texture", C.c_uint32)]
 
 
 class ArLight(C.Structure):
@@ -160,6 +161,11 @@ def _load():
         "ar_scene_get_anyhit_config": (i32, [P, C.POINTER(ArTraceConfig)]),
         "ar_scene_autotune": (i32, [P, P, u64, u32, P, C.POINTER(ArTraceConfig)]),
         "ar_film_to_rgb8": (i32, [P, u32, u32, u32, f, f, f, P, u32, P]),
+        "ar_scene_add_texture": (i32, [P, P, u32, u32, C.POINTER(u32)]),
+        "ar_scene_set_mesh_texcoords": (i32, [P, u32, P, u32]),
+        "ar_image_load_png": (i32, [C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(u32), C.POINTER(u32)]),
+        "ar_image_free": (None, [P]),
+        "ar_scene_load_obj": (i32, [P, C.c_char_p, P, C.POINTER(u32)]),
         "ar_film_save_exr": (i32, [P, u32, u32, f, C.c_char_p, u32, P]),
     }
     for name, (res, args) in sig.items():
@@ -196,13 +202,44 @@ def _ptr(a: np.ndarray):
 # ---------------------------------------------------------------------------
 # Scene description mirrors of the reference types
 # ---------------------------------------------------------------------------
+def load_png(path) -> np.ndarray:
+    """image::open(path).into_rgb8() for a PNG file: (h, w, 3) uint8, top row
+    first (decoded by the library)."""
+    buf, w, h = C.c_void_p(), C.c_uint32(), C.c_uint32()
+    _check(_lib.ar_image_load_png(os.fsencode(os.fspath(path)), C.byref(buf), C.byref(w), C.byref(h)))
+    try:
+        out = np.ctypeslib.as_array(C.cast(buf, C.POINTER(C.c_uint8)), shape=(h.value, w.value, 3)).copy()
+    finally:
+        _lib.ar_image_free(buf)
+    return out
+
+
+class ImageTexture:
+    """texture.rs:65-135: an 8-bit sRGB image, linear after loading, sampled
+    bilinearly with repeat wrap.  `rgb`: (h, w, 3) uint8, top row first."""
+
+    def __init__(self, rgb: np.ndarray):
+        self.rgb = np.ascontiguousarray(rgb, np.uint8).reshape(np.shape(rgb)[0], np.shape(rgb)[1], 3)
+
+    @classmethod
+    def load(cls, path) -> "ImageTexture":
+        """ImageTexture::load (texture.rs:74-93)."""
+        return cls(load_png(path))
+
+
 @dataclass
 class Lambertian:
-    """surfaces/lambertian.rs:17-65 (constant albedo texture)."""
-    albedo: Sequence[float]
+    """surfaces/lambertian.rs:17-65: a constant albedo, or an ImageTexture
+    (Lambertian::with_texture)."""
+    albedo: object
 
-    def to_c(self):
-        return ArMaterial(0, (C.c_float * 3)(*self.albedo), 1.0, 1.0)
+    @classmethod
+    def with_texture(cls, tex: ImageTexture) -> "Lambertian":
+        return cls(tex)
+
+    def to_c(self, texture_id: int = 0):
+        a = (0.5, 0.5, 0.5) if isinstance(self.albedo, ImageTexture) else self.albedo
+        return ArMaterial(0, (C.c_float * 3)(*a), 1.0, 1.0, texture_id)
 
 
 @dataclass
@@ -225,8 +262,10 @@ class DielectricBsdf:
         return ArMaterial(2, (C.c_float * 3)(1.0, 1.0, 1.0), self.roughness, self.eta)
 
 
-def _mat_from_desc(m):
+def _mat_from_desc(m, textures=()):
     if m.type == 0:
+        if getattr(m, "texture", -1) >= 0:
+            return Lambertian.with_texture(textures[m.texture])
         return Lambertian(m.albedo)
     if m.type == 1:
         return ConductorBsdf(m.albedo, m.roughness)
@@ -265,6 +304,7 @@ class TriangleMesh:
     indices: np.ndarray
     normals: Optional[np.ndarray] = None
     material_slots: Optional[np.ndarray] = None
+    tex_coords: Optional[np.ndarray] = None  # (nv, 2) float32
 
 
 @dataclass
@@ -282,22 +322,41 @@ class Scene:
     """scene.rs:113-168.  Builds the BLASes and TLAS on the host with the
     reference's SAH, uploads the flattened scene to HBM and keeps it there."""
 
-    def __init__(self, objects: List[SceneObject], lights: List[object], light_dist: str = "power"):
+    def __init__(self, objects: List[SceneObject], lights: List[object], light_dist: str = "power",
+                 obj_files: Sequence = ()):
+        """`obj_files`: (path, transform or None) pairs loaded with
+        load_obj_scene (importer.rs:167-241) after `objects`."""
         init()
         self._h = C.c_void_p()
         _check(_lib.ar_scene_create(C.byref(self._h)))
         self._keep = []
-        mat_ids = {}
+        mat_ids, tex_ids = {}, {}
         for obj in objects:
             ids = []
             for m in obj.materials:
                 key = id(m)
                 if key not in mat_ids:
+                    tid = 0
+                    if isinstance(getattr(m, "albedo", None), ImageTexture):
+                        t = m.albedo
+                        if id(t) not in tex_ids:
+                            out = C.c_uint32()
+                            _check(_lib.ar_scene_add_texture(self._h, _ptr(t.rgb), t.rgb.shape[1], t.rgb.shape[0],
+                                                             C.byref(out)))
+                            tex_ids[id(t)] = out.value
+                        tid = tex_ids[id(t)]
                     out = C.c_uint32()
-                    _check(_lib.ar_scene_add_material(self._h, C.byref(m.to_c()), C.byref(out)))
+                    c = m.to_c(tid) if tid else m.to_c()
+                    _check(_lib.ar_scene_add_material(self._h, C.byref(c), C.byref(out)))
                     mat_ids[key] = out.value
                 ids.append(mat_ids[key])
             self._add_mesh(obj.mesh, ids, obj.transform)
+        self.obj_objects = 0
+        for path, xf in obj_files:
+            n = C.c_uint32()
+            xfa = None if xf is None else np.ascontiguousarray(xf, np.float32).reshape(16)
+            _check(_lib.ar_scene_load_obj(self._h, os.fsencode(os.fspath(path)), _ptr(xfa), C.byref(n)))
+            self.obj_objects += n.value
         for l in lights:
             _check(_lib.ar_scene_add_light(self._h, C.byref(l.to_c()), None))
         _check(_lib.ar_scene_set_light_distribution(self._h, LIGHTDIST[light_dist]))
@@ -313,16 +372,21 @@ class Scene:
             np.ascontiguousarray(mesh.material_slots, np.uint32).reshape(-1)
         mats = np.asarray(mat_ids, np.uint32)
         xf = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+        oid = C.c_uint32()
         _check(_lib.ar_scene_add_mesh(self._h, _ptr(pos), pos.shape[0], _ptr(idx), idx.shape[0], _ptr(nrm),
-                                      _ptr(slots), slots.shape[0], _ptr(mats), mats.shape[0], _ptr(xf), None))
+                                      _ptr(slots), slots.shape[0], _ptr(mats), mats.shape[0], _ptr(xf), C.byref(oid)))
+        if mesh.tex_coords is not None:
+            uv = np.ascontiguousarray(mesh.tex_coords, np.float32).reshape(-1, 2)
+            _check(_lib.ar_scene_set_mesh_texcoords(self._h, oid.value, _ptr(uv), uv.shape[0]))
 
     @classmethod
     def from_desc(cls, desc: SceneDesc) -> "Scene":
-        mats = [_mat_from_desc(m) for m in desc.materials]
+        textures = [ImageTexture(t) for t in getattr(desc, "textures", [])]
+        mats = [_mat_from_desc(m, textures) for m in desc.materials]
         objs = []
         for m in desc.meshes:
-            objs.append(SceneObject(TriangleMesh(m.positions, m.indices, m.normals, m.slots), m.transform,
-                                    [mats[i] for i in m.materials]))
+            objs.append(SceneObject(TriangleMesh(m.positions, m.indices, m.normals, m.slots, getattr(m, "uvs", None)),
+                                    m.transform, [mats[i] for i in m.materials]))
         lights = []
         for l in desc.lights:
             if l.type == 0:

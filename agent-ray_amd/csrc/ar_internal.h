@@ -75,7 +75,7 @@ struct alignas(16) DMaterial {
   int32_t light_id;  // >= 0 for an emitter (quad light geometry)
   float roughness, eta;
   float albedo[3];
-  float pad0;
+  int32_t tex = -1;  // albedo texture (index into the texture table), -1: constant albedo
   float emission[3];
   float pad1;
 };
@@ -115,7 +115,7 @@ struct alignas(16) DObject {
 static_assert(sizeof(DObject) == 80, "object layout");
 
 struct HostMesh {
-  std::vector<float> pos, nrm;
+  std::vector<float> pos, nrm, uv;  // uv: tex_coords, empty when absent
   std::vector<uint32_t> idx, slots, mats;
   float xf[16];
 };
@@ -130,6 +130,8 @@ struct ar_scene {
   ar::Workspace* ws = nullptr;  // render workspace used by ar_render
   std::vector<ar::DMaterial> materials;
   std::vector<ar::HostMesh> meshes;
+  std::vector<float4> texels;    // linear texels of every texture, back to back
+  std::vector<uint4> tex_info;   // per texture: (first texel, width, height, -)
   std::vector<ar_light> lights;
   uint32_t dist_mode = AR_LIGHTDIST_POWER;
   bool committed = false;
@@ -143,11 +145,12 @@ struct ar_scene {
   ar_trace_config tcfg{256, 8, 8, 24, 8, 3, 0.0f};      // closest-hit launch configuration (autotuned)
   ar_trace_config tcfg_any{256, 8, 8, 24, 8, 3, 0.0f};  // any-hit (occlusion, shadow rays), tuned separately
   ar::DevBuf d_nodes, d_qnodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
-      d_tri_obj, d_stri;
+      d_tri_obj, d_stri, d_suv, d_tex, d_texinfo;
   ar::SceneView view{};
   uint32_t n_lights = 0;
   bool has_dist = false;
   bool has_emitters = false;
+  bool has_uv = false;  // some mesh has tex_coords (d_suv holds the per-Tri-record uvs)
   std::vector<ar::DLight> h_lights;
   std::vector<ar::DObject> h_objects;
   hipStream_t stream = nullptr;
@@ -172,6 +175,7 @@ struct HostAccel {
   // space vertex normals of the triangle, resolved from the object's index,
   // normal, slot and material tables
   std::vector<float4> gshade;
+  std::vector<float4> guv;  // per Tri record: (u0, v0, u1, v1), (u2, v2, has_uv, -); empty if no mesh has uvs
   int tlas_root = 0, qtlas_root = 0;
   float tlas_box[6] = {0, 0, 0, 0, 0, 0};
   uint32_t max_depth = 0, tlas_depth = 0;    // binary trees

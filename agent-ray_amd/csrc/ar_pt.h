@@ -137,7 +137,7 @@ __device__ __forceinline__ bool pt_shade(const ShadeView& v, const PtParams& P, 
   if (h.pos == kNoHit) return false;  // miss: black environment
   ShadingPoint sp;
   make_shading(v, s.ray, h.t, h.pos, h.inst, h.u, h.v, sp);
-  const DMaterial& m = v.materials[sp.mat];
+  const DMaterial m = material_at(v, sp.mat, sp.tu, sp.tv);
   f3 wo = -s.ray.d;
   if (m.light_id >= 0) {  // emitter (black surface): add Le and stop
     if (dot(sp.ng, wo) > 0.0f) {

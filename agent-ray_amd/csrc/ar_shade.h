@@ -19,6 +19,10 @@ struct ShadeView {
   const uint32_t* slots;
   const uint32_t* objmats;
   const float4* stri;  // per Tri record: vertex normals, material, object (3 float4)
+  const float4* suv;   // per Tri record: vertex tex_coords (2 float4), or NULL when no mesh has them
+  const float4* texels;     // linear texels of every texture
+  const uint4* tex_info;    // per texture: (first texel, width, height, -)
+  uint32_t n_tex;           // textures in the table
   uint32_t n_lights;
   uint32_t dist_mode;  // 0 none, 1 uniform, 2 power
 };
@@ -41,7 +45,48 @@ struct ShadingPoint {
   f3 p, n, ng;  // n: shading normal flipped toward wo; ng: unflipped world normal
   Onb onb;
   uint32_t mat;
+  float tu, tv;  // tex_uv (scene.rs:55-64), set when the material is textured
 };
+
+// f32::fract(x).rem_euclid(1.0): the fractional part, moved into [0, 1).
+__device__ __forceinline__ float wrap01(float x) {
+  const float r = fmodf(x - truncf(x), 1.0f);
+  return r < 0.0f ? r + 1.0f : r;
+}
+
+// ImageTexture::sample (texture.rs:97-135): bilinear over the four nearest
+// texels, repeat wrap, in the reference's operation order.
+__device__ __forceinline__ f3 texture_sample(const ShadeView& v, int32_t tex, float uvx, float uvy) {
+  const uint4 ti = v.tex_info[tex];
+  const int w = (int)ti.y, h = (int)ti.z;
+  const float u = wrap01(uvx) * (float)w - 0.5f;
+  const float vv = wrap01(uvy) * (float)h - 0.5f;
+  const float fu = floorf(u), fv = floorf(vv);
+  const int x0 = (int)fu, y0 = (int)fv;
+  const float tx = u - fu, ty = vv - fv;
+  auto fetch = [&](int x, int y) {
+    int xx = x % w, yy = y % h;  // rem_euclid
+    xx = xx < 0 ? xx + w : xx;
+    yy = yy < 0 ? yy + h : yy;
+    const float4 c = v.texels[ti.x + (uint32_t)yy * (uint32_t)w + (uint32_t)xx];
+    return mk(c.x, c.y, c.z);
+  };
+  const f3 c00 = fetch(x0, y0), c10 = fetch(x0 + 1, y0), c01 = fetch(x0, y0 + 1), c11 = fetch(x0 + 1, y0 + 1);
+  return ((c00 * ((1.0f - tx) * (1.0f - ty)) + c10 * (tx * (1.0f - ty))) + c01 * ((1.0f - tx) * ty)) + c11 * (tx * ty);
+}
+
+// The material of a surface point: a textured Lambertian's albedo sampled
+// at tex_uv (lambertian.rs:35-56), every other material as stored.
+__device__ __forceinline__ DMaterial material_at(const ShadeView& v, uint32_t mat, float tu, float tv) {
+  DMaterial m = v.materials[mat];
+  if (m.tex >= 0 && (uint32_t)m.tex < v.n_tex) {
+    const f3 a = texture_sample(v, m.tex, tu, tv);
+    m.albedo[0] = a.x;
+    m.albedo[1] = a.y;
+    m.albedo[2] = a.z;
+  }
+  return m;
+}
 
 // Scene::intersect's shading part (scene.rs:142-162) for a closest hit.
 __device__ __forceinline__ void make_shading(const ShadeView& v, const Ray& r, float t, uint32_t pos, uint32_t inst,
@@ -75,6 +120,16 @@ __device__ __forceinline__ void make_shading(const ShadeView& v, const Ray& r, f
   sp.n = dot(n, wo) < 0.0f ? -n : n;
   sp.onb = onb_from_normal(sp.n);
   sp.mat = __float_as_uint(sa.w);  // scene.rs:41-48
+  sp.tu = bu;
+  sp.tv = bv;
+  if (v.suv && v.materials[sp.mat].tex >= 0 && (uint32_t)v.materials[sp.mat].tex < v.n_tex) {  // scene.rs:55-64
+    const float4 ua = v.suv[2 * (size_t)pos], ub = v.suv[2 * (size_t)pos + 1];
+    if (ub.z != 0.0f) {
+      const float w = 1.0f - bu - bv;
+      sp.tu = (ua.x * w + ua.z * bu) + ub.x * bv;
+      sp.tv = (ua.y * w + ua.w * bu) + ub.y * bv;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -49,6 +49,7 @@ struct Verts {
   float4* pf;    // (p.xyz, pdf_fwd)          [2][V][npaths]
   float4* nr;    // (n.xyz, pdf_rev)
   float4* bm;    // (beta.xyz, meta bits: type | delta << 2 | id << 4)
+  float2* uv;    // tex_uv of surface vertices (textured albedo)
   uint2* count;  // (camera vertices, light vertices) per path
   uint32_t npaths;
   uint32_t V;
@@ -61,6 +62,7 @@ struct PV {
   f3 p, n, beta;
   float pdf_fwd, pdf_rev;
   uint32_t type, delta, id;  // id: material (surface) or light index
+  float tu = 0.0f, tv = 0.0f;  // tex_uv (surface vertices)
 };
 
 __device__ __forceinline__ PV load_v(const Verts& Vt, uint32_t side, uint32_t k, uint32_t p) {
@@ -76,6 +78,9 @@ __device__ __forceinline__ PV load_v(const Verts& Vt, uint32_t side, uint32_t k,
   v.type = m & 3u;
   v.delta = (m >> 2) & 1u;
   v.id = m >> 4;
+  const float2 t = Vt.uv[i];
+  v.tu = t.x;
+  v.tv = t.y;
   return v;
 }
 __device__ __forceinline__ void store_v(const Verts& Vt, uint32_t side, uint32_t k, uint32_t p, const PV& v) {
@@ -84,6 +89,7 @@ __device__ __forceinline__ void store_v(const Verts& Vt, uint32_t side, uint32_t
   Vt.pf[i] = make_float4(v.p.x, v.p.y, v.p.z, v.pdf_fwd);
   Vt.nr[i] = make_float4(v.n.x, v.n.y, v.n.z, v.pdf_rev);
   Vt.bm[i] = make_float4(v.beta.x, v.beta.y, v.beta.z, __uint_as_float(m));
+  Vt.uv[i] = make_float2(v.tu, v.tv);
 }
 
 // :305-314
@@ -206,6 +212,8 @@ __device__ __forceinline__ uint32_t extend_subpath(const ShadeView& v, const Bdp
     sv.type = VT_SURFACE;
     sv.delta = 0;
     sv.id = sp.mat;
+    sv.tu = sp.tu;
+    sv.tv = sp.tv;
     sv.p = sp.p;
     sv.n = sp.n;
     sv.beta = beta;
@@ -215,7 +223,7 @@ __device__ __forceinline__ uint32_t extend_subpath(const ShadeView& v, const Bdp
     before = prev;
     prev = sv;
     nv++;
-    const DMaterial& m = v.materials[sp.mat];
+    const DMaterial m = material_at(v, sp.mat, sp.tu, sp.tv);
     f3 wo_l = to_local(sp.onb, -ray.d);
     float u_sel = rng.next();
     float ux = rng.next(), uy = rng.next();
@@ -480,6 +488,8 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_sub_shade(ShadeView v, BdptParam
         sv.type = VT_SURFACE;
         sv.delta = 0;
         sv.id = sp.mat;
+        sv.tu = sp.tu;
+        sv.tv = sp.tv;
         sv.p = sp.p;
         sv.n = sp.n;
         sv.beta = beta;
@@ -487,7 +497,7 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_sub_shade(ShadeView v, BdptParam
         sv.pdf_rev = 0.0f;
         store_v(Vt, side, nv, p, sv);
         nv++;
-        const DMaterial& m = v.materials[sp.mat];
+        const DMaterial m = material_at(v, sp.mat, sp.tu, sp.tv);
         f3 wo_l = to_local(sp.onb, -ray.d);
         float u_sel = rng.next();
         float ux = rng.next(), uy = rng.next();
@@ -637,7 +647,7 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
         Onb on = onb_from_normal(y.n);
         f3 wo_l = to_local(on, normalize(yp.p - y.p));
         f3 wi_l = to_local(on, to_cam);
-        const DMaterial& m = v.materials[y.id];
+        const DMaterial m = material_at(v, y.id, y.tu, y.tv);
         f3 f = bsdf_eval(m, wi_l, wo_l);
         if (is_zero(f)) continue;
         float cos_y = fabsf(dot(y.n, to_cam));
@@ -675,7 +685,7 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
       PV zp = load_v(Vt, 0, s - 2, p);
       Onb onz = onb_from_normal(z.n);
       f3 wo_z_l = to_local(onz, normalize(zp.p - z.p));
-      const DMaterial& mz = v.materials[z.id];
+      const DMaterial mz = material_at(v, z.id, z.tu, z.tv);
       f3 f_z = bsdf_eval(mz, to_local(onz, wi_z), wo_z_l);
       if (is_zero(f_z)) continue;
       f3 f_y = mk(1.0f, 1.0f, 1.0f);
@@ -688,7 +698,7 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
       } else {
         PV yp = load_v(Vt, 1, t - 2, p);
         wo_y_l = to_local(ony, normalize(yp.p - y.p));
-        f_y = bsdf_eval(v.materials[y.id], to_local(ony, wi_y), wo_y_l);
+        f_y = bsdf_eval(material_at(v, y.id, y.tu, y.tv), to_local(ony, wi_y), wo_y_l);
       }
       if (is_zero(f_y)) continue;
       float g = geometry_term(z.p, z.n, y.p, y.n);
@@ -709,7 +719,7 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
         pdf_cc = pdf_sa_to_area(light_pdf_emission_dir(*Lt, dyz), y.p, z.p, z.n);
       } else {
         f3 wi_y2 = normalize(z.p - y.p);
-        float pd = bsdf_pdf(v.materials[y.id], to_local(ony, wi_y2), wo_y_l);
+        float pd = bsdf_pdf(material_at(v, y.id, y.tu, y.tv), to_local(ony, wi_y2), wo_y_l);
         pdf_cc = pdf_sa_to_area(pd, y.p, z.p, z.n);
       }
       f3 wi_z2 = normalize(y.p - z.p);
@@ -986,7 +996,7 @@ class BdptJob : public JobRun {
     pass = wavefront ? (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(cap, kConnBudget / per_path)) : 0;
     const size_t slots = (size_t)pass * C.K;
     auto al = [](size_t n) { return (n + 255) / 256 * 256; };
-    size_t need = al(nvert * 3 * sizeof(float4)) + al((size_t)cap * sizeof(uint2)) +
+    size_t need = al(nvert * 3 * sizeof(float4)) + al(nvert * sizeof(float2)) + al((size_t)cap * sizeof(uint2)) +
                   al((size_t)npix * 3 * sizeof(long long));
     if (wavefront)
       need += 3 * al(slots * sizeof(float4)) + al(slots) + al(slots * sizeof(uint32_t)) +
@@ -1004,6 +1014,7 @@ class BdptJob : public JobRun {
     Vt.pf = (float4*)take(nvert * 3 * sizeof(float4));
     Vt.nr = Vt.pf + nvert;
     Vt.bm = Vt.nr + nvert;
+    Vt.uv = (float2*)take(nvert * sizeof(float2));
     Vt.count = (uint2*)take((size_t)cap * sizeof(uint2));
     splat = (unsigned long long*)take((size_t)npix * 3 * sizeof(long long));
     if (wavefront) {

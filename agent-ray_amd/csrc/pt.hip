@@ -278,6 +278,10 @@ ShadeView shade_view(const ar_scene* s) {
   v.slots = s->d_slots.as<uint32_t>();
   v.objmats = s->d_objmats.as<uint32_t>();
   v.stri = s->d_stri.as<float4>();
+  v.suv = s->has_uv ? s->d_suv.as<float4>() : nullptr;
+  v.texels = s->d_tex.as<float4>();
+  v.tex_info = s->d_texinfo.as<uint4>();
+  v.n_tex = (uint32_t)s->tex_info.size();
   v.n_lights = s->n_lights;
   v.dist_mode = s->has_dist ? s->dist_mode : AR_LIGHTDIST_NONE;
   return v;

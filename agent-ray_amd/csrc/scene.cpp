@@ -335,6 +335,12 @@ ar_status ar_scene_add_material(ar_scene* s, const ar_material* m, uint32_t* out
   d.roughness = std::min(std::max(m->roughness, 1e-3f), 1.0f);
   d.eta = m->eta;
   for (int k = 0; k < 3; ++k) d.albedo[k] = m->albedo[k];
+  d.tex = -1;
+  if (m->albedo_texture != 0) {
+    if (m->type != AR_MAT_LAMBERTIAN) return fail(AR_ERR_UNSUPPORTED_TYPE, "only a Lambertian albedo can be textured");
+    if (m->albedo_texture > s->tex_info.size()) return fail(AR_ERR_INVALID_ARG, "unknown texture id");
+    d.tex = (int32_t)m->albedo_texture - 1;
+  }
   s->materials.push_back(d);
   if (out_id) *out_id = (uint32_t)s->materials.size() - 1;
   return AR_OK;
@@ -373,6 +379,45 @@ ar_status ar_scene_add_mesh(ar_scene* s, const float* positions, uint32_t nv, co
   return AR_OK;
 }
 
+ar_status ar_scene_add_texture(ar_scene* s, const uint8_t* rgb, uint32_t width, uint32_t height, uint32_t* out_id) {
+  if (!s || !rgb) return fail(AR_ERR_INVALID_ARG, "ar_scene_add_texture: NULL argument");
+  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
+  if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 28))
+    return fail(AR_ERR_INVALID_ARG, "ar_scene_add_texture: empty or oversized image");
+  // srgb_to_linear of every 8-bit level (texture.rs:144-151), as
+  // ImageTexture::load applies it to p / 255 (texture.rs:80-89)
+  static float lut[256];
+  static bool lut_ready = false;
+  if (!lut_ready) {
+    for (int k = 0; k < 256; ++k) {
+      const float x = (float)k / 255.0f;
+      lut[k] = x <= 0.04045f ? x / 12.92f : std::pow((x + 0.055f) / 1.055f, 2.4f);
+    }
+    lut_ready = true;
+  }
+  const uint64_t n = (uint64_t)width * height;
+  const uint32_t first = (uint32_t)s->texels.size();
+  if ((uint64_t)first + n > (1ull << 31)) return fail(AR_ERR_INVALID_ARG, "ar_scene_add_texture: too many texels");
+  s->texels.reserve(first + n);
+  for (uint64_t i = 0; i < n; ++i)
+    s->texels.push_back(make_float4(lut[rgb[3 * i]], lut[rgb[3 * i + 1]], lut[rgb[3 * i + 2]], 0.0f));
+  s->tex_info.push_back(make_uint4(first, width, height, 0u));
+  if (out_id) *out_id = (uint32_t)s->tex_info.size();  // ids start at 1; 0 means no texture
+  return AR_OK;
+}
+
+ar_status ar_scene_set_mesh_texcoords(ar_scene* s, uint32_t object_id, const float* uv, uint32_t nv) {
+  if (!s || !uv) return fail(AR_ERR_INVALID_ARG, "ar_scene_set_mesh_texcoords: NULL argument");
+  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
+  if (object_id >= s->meshes.size()) return fail(AR_ERR_INVALID_ARG, "unknown object id");
+  HostMesh& m = s->meshes[object_id];
+  if ((uint64_t)nv * 3 != m.pos.size()) return fail(AR_ERR_INVALID_ARG, "tex_coords need one (u, v) per vertex");
+  for (uint64_t i = 0; i < 2ull * nv; ++i)
+    if (!std::isfinite(uv[i])) return fail(AR_ERR_INVALID_ARG, "non-finite texture coordinate");
+  m.uv.assign(uv, uv + 2ull * nv);
+  return AR_OK;
+}
+
 ar_status ar_scene_add_light(ar_scene* s, const ar_light* l, uint32_t* out_id) {
   if (!s || !l) return fail(AR_ERR_INVALID_ARG, "ar_scene_add_light: NULL argument");
   if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
@@ -382,6 +427,7 @@ ar_status ar_scene_add_light(ar_scene* s, const ar_light* l, uint32_t* out_id) {
     f3 eu = mk(l->edge_u[0], l->edge_u[1], l->edge_u[2]), ev = mk(l->edge_v[0], l->edge_v[1], l->edge_v[2]);
     if (length(cross(eu, ev)) <= 0.0f) return fail(AR_ERR_INVALID_ARG, "degenerate quad light");
     DMaterial d{};
+    d.tex = -1;  // emitters are never textured
     d.type = AR_MAT_LAMBERTIAN;
     d.light_id = (int32_t)id;
     d.roughness = 1.0f;
@@ -426,6 +472,8 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
   std::vector<uint32_t>&gidx = h.gidx, &gslots = h.gslots, &gmats = h.gmats, &tri_obj = h.tri_obj;
   std::vector<float>& gnrm = h.gnrm;
   uint32_t max_depth = 0;
+  bool any_uv = false;
+  for (const HostMesh& m : s->meshes) any_uv = any_uv || !m.uv.empty();
   for (size_t oi = 0; oi < s->meshes.size(); ++oi) {
     const HostMesh& m = s->meshes[oi];
     uint32_t nt = (uint32_t)(m.idx.size() / 3);
@@ -520,6 +568,15 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
       h.gshade.push_back(make_float4(n[0][0], n[0][1], n[0][2], mf));
       h.gshade.push_back(make_float4(n[1][0], n[1][1], n[1][2], hf));
       h.gshade.push_back(make_float4(n[2][0], n[2][1], n[2][2], of));
+      if (any_uv) {  // scene.rs:55-64
+        float t[3][2] = {};
+        const float has_uv = m.uv.empty() ? 0.0f : 1.0f;
+        if (!m.uv.empty())
+          for (int c = 0; c < 3; ++c)
+            for (int k = 0; k < 2; ++k) t[c][k] = m.uv[2 * (size_t)m.idx[3 * prim + c] + k];
+        h.guv.push_back(make_float4(t[0][0], t[0][1], t[1][0], t[1][1]));
+        h.guv.push_back(make_float4(t[2][0], t[2][1], has_uv, 0.0f));
+      }
     }
   }
   // TLAS over instance world boxes (bvh.rs:395-410); instance records are
@@ -644,6 +701,10 @@ ar_status ar_scene_commit(ar_scene* s) {
   AR_HIP(upload(s->d_objmats, h.gmats));
   AR_HIP(upload(s->d_tri_obj, h.tri_obj));
   AR_HIP(upload(s->d_stri, h.gshade));
+  AR_HIP(upload(s->d_suv, h.guv));
+  s->has_uv = !h.guv.empty();
+  AR_HIP(upload(s->d_tex, s->texels));
+  AR_HIP(upload(s->d_texinfo, s->tex_info));
   s->view.nodes = s->d_nodes.as<WNode>();
   s->view.qnodes = s->d_qnodes.as<QNode>();
   s->view.qtlas_root = h.qtlas_root;

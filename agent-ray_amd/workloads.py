@@ -37,6 +37,7 @@ class MaterialDesc:
     albedo: tuple = (0.5, 0.5, 0.5)  # Lambertian albedo or conductor F0
     roughness: float = 0.5
     eta: float = 1.5
+    texture: int = -1                # Lambertian albedo texture (index into SceneDesc.textures), -1: none
 
 
 @dataclass
@@ -47,6 +48,7 @@ class MeshDesc:
     slots: np.ndarray = None         # (1,) or (nt,) uint32
     normals: Optional[np.ndarray] = None  # (nv, 3) float32 or None
     transform: Optional[np.ndarray] = None  # (16,) float32 column-major, None = identity
+    uvs: Optional[np.ndarray] = None  # (nv, 2) float32 tex_coords or None
 
     def __post_init__(self):
         self.positions = np.ascontiguousarray(self.positions, dtype=np.float32).reshape(-1, 3)
@@ -54,6 +56,8 @@ class MeshDesc:
         if self.slots is None:
             self.slots = np.zeros(1, np.uint32)
         self.slots = np.ascontiguousarray(self.slots, dtype=np.uint32).reshape(-1)
+        if self.uvs is not None:
+            self.uvs = np.ascontiguousarray(self.uvs, dtype=np.float32).reshape(-1, 2)
         if self.normals is not None:
             self.normals = np.ascontiguousarray(self.normals, dtype=np.float32).reshape(-1, 3)
         if self.transform is not None:
@@ -81,6 +85,7 @@ class CameraDesc:
 class SceneDesc:
     materials: List[MaterialDesc] = field(default_factory=list)
     meshes: List[MeshDesc] = field(default_factory=list)
+    textures: List[np.ndarray] = field(default_factory=list)  # (h, w, 3) uint8 sRGB images, top row first
     lights: List[LightDesc] = field(default_factory=list)
     light_dist: int = 2  # 0 none, 1 uniform, 2 power (the reference binaries use power)
     camera: Optional[CameraDesc] = None
@@ -199,6 +204,24 @@ def unit_cube_mesh():
 # ---------------------------------------------------------------------------
 # Cornell box
 # ---------------------------------------------------------------------------
+def textured_cornell_box(seed: int = 0, point_light_twin: bool = True) -> SceneDesc:
+    """cornell_box with image textures (texture.rs:65-135): the floor and one
+    wall take an 8x6 texture through per-vertex tex_coords that run past
+    [0, 1] and below 0 (repeat wrap), and the tall box a 3x5 texture
+    addressed by the hit's barycentrics (it has no tex_coords; scene.rs:55-64).
+    Texels are random 8-bit sRGB values (seeded)."""
+    desc = cornell_box(seed=seed, point_light_twin=point_light_twin)
+    rng = np.random.default_rng(seed + 1000)
+    desc.textures = [rng.integers(0, 256, (6, 8, 3), dtype=np.uint8), rng.integers(0, 256, (5, 3, 3), dtype=np.uint8)]
+    room = desc.meshes[0]
+    p = room.positions
+    room.uvs = np.stack([p[:, 0] * 2.0 + p[:, 2] * 1.5 - 0.3, p[:, 1] * 2.0 - 0.25], -1).astype(np.float32)
+    desc.materials[0].texture = 0
+    desc.materials[2].texture = 0
+    desc.materials[desc.meshes[1].materials[0]].texture = 1
+    return desc
+
+
 def cornell_box(seed: int = 0, width: int = 256, height: int = 256, spp: int = 16, max_depth: int = 8,
                 point_light_twin: bool = False) -> SceneDesc:
     """Procedural Cornell box: unit-cube room (5 faces, 10 triangles, front

@@ -73,6 +73,8 @@ typedef struct ar_material {
   float albedo[3]; /* Lambertian albedo, or conductor F0 */
   float roughness; /* GGX alpha, clamped to [1e-3, 1] as the reference does */
   float eta;       /* dielectric interior IOR */
+  uint32_t albedo_texture; /* Lambertian::with_texture (lambertian.rs:28-33): a texture id from
+                              ar_scene_add_texture, 0 for the constant albedo */
 } ar_material;
 
 /* Light types: PointLight (lights/point.rs:22-100) and a quad area light
@@ -237,6 +239,30 @@ ar_status ar_render(const ar_scene* s, const ar_camera* cam, uint32_t width, uin
 ar_status ar_poisson_solve(const float* primal, const float* dx, const float* dy, uint32_t width,
                            uint32_t height, float alpha, uint32_t iterations, int32_t sign, float* out,
                            double* final_residual, uint32_t flags, void* stream);
+
+/* ---- textures and OBJ import (texture.rs, importer.rs) --------------------- */
+/* ImageTexture::load (texture.rs:74-93) from decoded 8-bit RGB (host memory,
+ * row-major, top row first): texels are converted from sRGB to linear
+ * (texture.rs:144-151) and sampled bilinearly with repeat wrap
+ * (texture.rs:97-135).  Returns an id >= 1 for ar_material.albedo_texture. */
+ar_status ar_scene_add_texture(ar_scene* s, const uint8_t* rgb, uint32_t width, uint32_t height, uint32_t* out_id);
+/* TriangleMesh.tex_coords (primitives/mesh.rs:10) of an object added by
+ * ar_scene_add_mesh: nv (u, v) pairs, one per vertex.  Without them a
+ * texture is addressed by the hit's barycentrics (scene.rs:55-64). */
+ar_status ar_scene_set_mesh_texcoords(ar_scene* s, uint32_t object_id, const float* uv, uint32_t nv);
+/* image::open(path).into_rgb8() for PNG files (8- and 16-bit; gray, RGB,
+ * palette, gray + alpha, RGBA; non-interlaced and Adam7): *rgb receives
+ * width*height*3 bytes, freed with ar_image_free. */
+ar_status ar_image_load_png(const char* path, uint8_t** rgb, uint32_t* width, uint32_t* height);
+void ar_image_free(uint8_t* rgb);
+/* load_obj_scene (importer.rs:167-241): the OBJ's models become objects of
+ * the scene (one per model, split by object/group and material as tobj's
+ * GPU_LOAD_OPTIONS do: triangulated, one index per vertex), MTL materials
+ * are converted by convert_material (importer.rs:87-135; map_Kd textures
+ * loaded relative to the OBJ's directory), `transform` (16 floats,
+ * column-major, NULL = identity) applies to every object.  *n_objects
+ * receives the number of objects added. */
+ar_status ar_scene_load_obj(ar_scene* s, const char* path, const float* transform, uint32_t* n_objects);
 
 /* ---- film output (film.rs:133-346) -------------------------------------- */
 /* Films are width*height*3 floats (row-major, top row first), each pixel

@@ -52,6 +52,11 @@ def lib():
             "orc_poisson_sor": (d, [P, P, P, u32, u32, f, u32, f, C.c_int, P]),
             "orc_render_bdpt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, f, C.c_int, u64, C.c_int, P, P]),
             "orc_film_rgb8": (None, [P, u64, u32, f, f, f, P]),
+            "orc_scene_add_texture": (C.c_int, [P, P, u32, u32]),
+            "orc_scene_set_material_texture": (None, [P, u32, C.c_int]),
+            "orc_scene_set_mesh_uv": (None, [P, u32, P, u32]),
+            "orc_texture_sample": (None, [P, u32, P, u32, P]),
+            "orc_srgb_to_linear": (f, [f]),
             "orc_f32_to_f16": (None, [P, u64, P]),
             "orc_exr_blocks": (None, [P, u32, u32, f, P]),
         }
@@ -81,10 +86,15 @@ class OracleScene:
         self.h = C.c_void_p(L.orc_scene_new())
         self._ntris = [int(m.indices.shape[0]) for m in desc.meshes] + \
             [2 for l in desc.lights if l.type == 1]
-        for m in desc.materials:
+        for t in getattr(desc, "textures", []):
+            t = np.ascontiguousarray(t, np.uint8)
+            L.orc_scene_add_texture(self.h, _p(t), t.shape[1], t.shape[0])
+        for k, m in enumerate(desc.materials):
             p = np.array(list(m.albedo) + [m.roughness, m.eta, 0, 0, 0], np.float32)
             L.orc_scene_add_material(self.h, m.type, _p(p))
-        for m in desc.meshes:
+            if getattr(m, "texture", -1) >= 0:
+                L.orc_scene_set_material_texture(self.h, k, m.texture)
+        for k, m in enumerate(desc.meshes):
             pos = np.ascontiguousarray(m.positions, np.float32)
             idx = np.ascontiguousarray(m.indices, np.uint32)
             nrm = None if m.normals is None else np.ascontiguousarray(m.normals, np.float32)
@@ -93,10 +103,20 @@ class OracleScene:
             xf = None if m.transform is None else np.ascontiguousarray(m.transform, np.float32)
             L.orc_scene_add_mesh(self.h, _p(pos), pos.shape[0], _p(idx), idx.shape[0], _p(nrm), _p(slots),
                                  slots.shape[0], _p(mats), mats.shape[0], _p(xf))
+            if getattr(m, "uvs", None) is not None:
+                uv = np.ascontiguousarray(m.uvs, np.float32)
+                L.orc_scene_set_mesh_uv(self.h, k, _p(uv), uv.shape[0])
         for l in desc.lights:
             p = np.array(list(l.position) + list(l.intensity) + list(l.edge_u) + list(l.edge_v), np.float32)
             L.orc_scene_add_light(self.h, l.type, _p(p))
         L.orc_scene_commit(self.h, desc.light_dist)
+
+    def texture_sample(self, tex, uv):
+        """ImageTexture::sample (texture.rs:97-122) at (n, 2) uv -> (n, 3)."""
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((uv.shape[0], 3), np.float32)
+        lib().orc_texture_sample(self.h, tex, _p(uv), uv.shape[0], _p(out))
+        return out
 
     def __del__(self):
         try:

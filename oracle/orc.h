@@ -251,6 +251,7 @@ struct TraceCounters {
 struct Mesh {  // primitives/mesh.rs:7-16
   std::vector<V3> pos;
   std::vector<V3> nrm;  // empty when absent
+  std::vector<float> uv;  // tex_coords, (u, v) per vertex; empty when absent
   std::vector<uint32_t> idx;
   std::vector<uint32_t> slots;
   uint32_t ntri() const { return (uint32_t)(idx.size() / 3); }

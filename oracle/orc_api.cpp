@@ -47,6 +47,33 @@ int orc_scene_add_material(void* s, uint32_t type, const float* p) {
   return (int)sc->materials.size() - 1;
 }
 
+// ImageTexture::load from decoded 8-bit RGB (texture.rs:74-93).
+int orc_scene_add_texture(void* s, const uint8_t* rgb, uint32_t w, uint32_t h) {
+  Scene* sc = (Scene*)s;
+  sc->textures.emplace_back();
+  sc->textures.back().load_rgb8(rgb, w, h);
+  return (int)sc->textures.size() - 1;
+}
+
+// Lambertian::with_texture (lambertian.rs:28-33): albedo from texture `tex`.
+void orc_scene_set_material_texture(void* s, uint32_t mat, int tex) { ((Scene*)s)->materials[mat].tex = tex; }
+
+// TriangleMesh.tex_coords of mesh object `obj` (mesh.rs:10).
+void orc_scene_set_mesh_uv(void* s, uint32_t obj, const float* uv, uint32_t nv) {
+  ((Scene*)s)->objects[obj]->mesh->uv.assign(uv, uv + 2 * (size_t)nv);
+}
+
+// Texture sampling for the tests: n (u, v) pairs -> n linear RGB.
+void orc_texture_sample(void* s, uint32_t tex, const float* uv, uint32_t n, float* out) {
+  const Texture& t = ((Scene*)s)->textures[tex];
+  for (uint32_t i = 0; i < n; ++i) {
+    V3 c = t.sample(uv[2 * i], uv[2 * i + 1]);
+    out[3 * i] = c.x, out[3 * i + 1] = c.y, out[3 * i + 2] = c.z;
+  }
+}
+
+float orc_srgb_to_linear(float x) { return srgb_to_linear(x); }
+
 int orc_scene_add_mesh(void* s, const float* pos, uint32_t nv, const uint32_t* idx, uint32_t nt,
                        const float* nrm, const uint32_t* slots, uint32_t nslots, const uint32_t* mats,
                        uint32_t nmats, const float* xf16) {

@@ -24,7 +24,7 @@ struct PVertex {
   VType type;
   V3 p, n;
   Onb onb;
-  const Material* mat = nullptr;
+  Material mat;  // the surface's material, textured albedo resolved
   V3 beta;
   float pdf_fwd = 0, pdf_rev = 0;
   bool is_delta = false;
@@ -71,7 +71,7 @@ static void extend_subpath(const Scene& sc, Ray ray, V3 throughput, float prev_p
     sv.p = hit.p;
     sv.n = hit.n;
     sv.onb = hit.onb;
-    sv.mat = hit.mat;
+    sv.mat = *hit.mat;
     sv.beta = throughput;
     sv.pdf_fwd = pdf_sa_to_area(prev_pdf_dir, v.back().p, hit.p, hit.n);
     v.push_back(sv);
@@ -224,7 +224,7 @@ static bool connect(const Scene& sc, const std::vector<PVertex>& cv, const std::
   if (sc.occluded(shadow, c)) return false;
   V3 wo_z = normalize(cv[s - 2].p - z.p);
   V3 wo_z_l = z.onb.to_local(wo_z);
-  V3 f_z = bsdf_eval(*z.mat, z.onb.to_local(wi_z), wo_z_l);
+  V3 f_z = bsdf_eval(z.mat, z.onb.to_local(wi_z), wo_z_l);
   if (is_zero(f_z)) return false;
   const Light* Lt = (t == 1) ? &sc.lights[y.light] : nullptr;
   V3 f_y(1, 1, 1);
@@ -232,7 +232,7 @@ static bool connect(const Scene& sc, const std::vector<PVertex>& cv, const std::
     if (Lt->type == LIGHT_QUAD && !(dot(y.n, wi_y) > 0.0f)) f_y = V3();  // one-sided emitter (extension)
   } else {
     V3 wo_y = normalize(lv[t - 2].p - y.p);
-    f_y = bsdf_eval(*y.mat, y.onb.to_local(wi_y), y.onb.to_local(wo_y));
+    f_y = bsdf_eval(y.mat, y.onb.to_local(wi_y), y.onb.to_local(wo_y));
   }
   if (is_zero(f_y)) return false;
   float g = geometry_term(z.p, z.n, y.p, y.n);
@@ -240,7 +240,7 @@ static bool connect(const Scene& sc, const std::vector<PVertex>& cv, const std::
   if (t == 1 && Lt->type == LIGHT_POINT) {  // :947-967
     LightSample ls;
     if (!Lt->sample(z.p, 0.0f, 0.0f, &ls)) return false;
-    V3 f_direct = bsdf_eval(*z.mat, z.onb.to_local(ls.wi), wo_z_l);
+    V3 f_direct = bsdf_eval(z.mat, z.onb.to_local(ls.wi), wo_z_l);
     float cos_z = std::fabs(dot(z.n, ls.wi));
     contrib = z.beta * f_direct * cos_z * ls.li;
   } else {
@@ -254,11 +254,11 @@ static bool connect(const Scene& sc, const std::vector<PVertex>& cv, const std::
   } else {  // :995-1006
     V3 wo_y = normalize(lv[t - 2].p - y.p);
     V3 wi_y2 = normalize(z.p - y.p);
-    float pd = bsdf_pdf(*y.mat, y.onb.to_local(wi_y2), y.onb.to_local(wo_y));
+    float pd = bsdf_pdf(y.mat, y.onb.to_local(wi_y2), y.onb.to_local(wo_y));
     pdf_cc = pdf_sa_to_area(pd, y.p, z.p, z.n);
   }
   V3 wi_z2 = normalize(y.p - z.p);  // :1011-1021
-  float pdz = bsdf_pdf(*z.mat, z.onb.to_local(wi_z2), wo_z_l);
+  float pdz = bsdf_pdf(z.mat, z.onb.to_local(wi_z2), wo_z_l);
   float pdf_cl = pdf_sa_to_area(pdz, z.p, y.p, y.n);
   float w = mis_weight(cv, lv, s, t, pdf_cc, pdf_cl, cfg);
   *out = contrib * w;
@@ -282,13 +282,13 @@ static bool connect_s1(const Scene& sc, const Camera& cam, const std::vector<PVe
   V3 wo_y = normalize(lv[t - 2].p - y.p);
   V3 wo_y_l = y.onb.to_local(wo_y);
   V3 wi_y_l = y.onb.to_local(to_cam);
-  V3 f_y = bsdf_eval(*y.mat, wi_y_l, wo_y_l);
+  V3 f_y = bsdf_eval(y.mat, wi_y_l, wo_y_l);
   if (is_zero(f_y)) return false;
   float cos_y = std::fabs(dot(y.n, to_cam));
   float g = cos_y / (dist * dist);
   V3 contrib = y.beta * f_y * g * cs.we;
   if (max_element(contrib) <= 0.0f) return false;
-  float pd = bsdf_pdf(*y.mat, wi_y_l, wo_y_l);
+  float pd = bsdf_pdf(y.mat, wi_y_l, wo_y_l);
   float pdf_cl = pdf_sa_to_area(pd, y.p, cam.origin, cv[0].n);
   float wgt = mis_weight(cv, lv, 1, t, 0.0f, pdf_cl, cfg);
   *out = contrib * wgt;

@@ -396,6 +396,47 @@ void Scene::commit(int dist_mode) {
 }
 
 // scene.rs:142-162
+float srgb_to_linear(float x) {
+  if (x <= 0.04045f) return x / 12.92f;
+  return std::pow((x + 0.055f) / 1.055f, 2.4f);
+}
+
+void Texture::load_rgb8(const uint8_t* rgb, uint32_t width, uint32_t height) {
+  w = width;
+  h = height;
+  data.resize((size_t)w * h);
+  for (size_t i = 0; i < data.size(); ++i)
+    data[i] = V3(srgb_to_linear((float)rgb[3 * i] / 255.0f), srgb_to_linear((float)rgb[3 * i + 1] / 255.0f),
+                 srgb_to_linear((float)rgb[3 * i + 2] / 255.0f));
+}
+
+static int rem_euclid(int a, int b) {
+  int r = a % b;
+  return r < 0 ? r + b : r;
+}
+
+V3 Texture::fetch(int x, int y) const {
+  uint32_t xx = (uint32_t)rem_euclid(x, (int)w), yy = (uint32_t)rem_euclid(y, (int)h);
+  return data[(size_t)yy * w + xx];
+}
+
+// f32::fract (x - trunc(x)), then rem_euclid(1.0): a negative fraction + 1.
+static float wrap01(float x) {
+  float f = x - std::trunc(x);
+  float r = std::fmod(f, 1.0f);
+  return r < 0.0f ? r + 1.0f : r;
+}
+
+V3 Texture::sample(float uvx, float uvy) const {
+  float fw = (float)w, fh = (float)h;
+  float u = wrap01(uvx) * fw - 0.5f;
+  float v = wrap01(uvy) * fh - 0.5f;
+  int x0 = (int)std::floor(u), y0 = (int)std::floor(v);
+  float tx = u - std::floor(u), ty = v - std::floor(v);
+  V3 c00 = fetch(x0, y0), c10 = fetch(x0 + 1, y0), c01 = fetch(x0, y0 + 1), c11 = fetch(x0 + 1, y0 + 1);
+  return ((c00 * ((1.0f - tx) * (1.0f - ty)) + c10 * (tx * (1.0f - ty))) + c01 * ((1.0f - tx) * ty)) + c11 * (tx * ty);
+}
+
 bool Scene::intersect(const Ray& r, float* tout, ShadingPoint* sp, TraceCounters* c) const {
   RayHit h;
   if (!tlas.intersect(r, &h, c)) return false;
@@ -419,7 +460,18 @@ bool Scene::intersect(const Ray& r, float* tout, ShadingPoint* sp, TraceCounters
   sp->onb = Onb::from_normal(nf);
   uint32_t slot = m.slots.size() == 1 ? m.slots[0] : m.slots[h.prim];  // scene.rs:41-48
   size_t nm = obj.mats.size();
-  sp->mat = &materials[obj.mats[std::min<size_t>(slot, nm - 1)]];
+  if (!m.uv.empty()) {  // scene.rs:55-64: interpolated tex_coords, else the barycentrics
+    uint32_t i0 = m.idx[3 * h.prim], i1 = m.idx[3 * h.prim + 1], i2 = m.idx[3 * h.prim + 2];
+    float w = 1.0f - h.u - h.v;
+    sp->tu = (m.uv[2 * i0] * w + m.uv[2 * i1] * h.u) + m.uv[2 * i2] * h.v;
+    sp->tv = (m.uv[2 * i0 + 1] * w + m.uv[2 * i1 + 1] * h.u) + m.uv[2 * i2 + 1] * h.v;
+  } else {
+    sp->tu = h.u;
+    sp->tv = h.v;
+  }
+  sp->mtl = materials[obj.mats[std::min<size_t>(slot, nm - 1)]];
+  if (sp->mtl.tex >= 0) sp->mtl.albedo = textures[sp->mtl.tex].sample(sp->tu, sp->tv);  // lambertian.rs:35-56
+  sp->mat = &sp->mtl;
   sp->inst = h.inst;
   sp->prim = h.prim;
   *tout = h.t;

@@ -19,11 +19,25 @@ enum MatType : uint32_t { MAT_LAMBERT = 0, MAT_CONDUCTOR = 1, MAT_DIELECTRIC = 2
 struct Material {
   uint32_t type = MAT_LAMBERT;
   V3 albedo;       // Lambertian albedo, or conductor F0
+  int tex = -1;    // Lambertian albedo texture (Lambertian::with_texture), -1: constant albedo
   float roughness = 0.5f;
   float eta = 1.5f;
   V3 emission;     // radiance emitted on the front side (extension)
   int light_id = -1;
   bool emissive() const { return !is_zero(emission); }
+};
+
+// IEC 61966-2-1 sRGB -> linear (texture.rs:144-151).
+float srgb_to_linear(float x);
+
+// ImageTexture (texture.rs:65-135): linear texels, row-major, top row
+// first, converted from 8-bit sRGB at load; bilinear, repeat wrap.
+struct Texture {
+  uint32_t w = 0, h = 0;
+  std::vector<V3> data;
+  void load_rgb8(const uint8_t* rgb, uint32_t width, uint32_t height);  // texture.rs:74-93
+  V3 fetch(int x, int y) const;                                            // texture.rs:126-134
+  V3 sample(float u, float v) const;                                       // texture.rs:97-122
 };
 
 struct BsdfSample {
@@ -94,12 +108,15 @@ struct ShadingPoint {
   V3 p, n;       // n: shading normal flipped toward wo
   V3 ng_world;   // unflipped world normal (front-face test for emitters)
   Onb onb;
-  const Material* mat;
+  float tu, tv;  // tex_uv (scene.rs:55-64)
+  Material mtl;  // the material with a textured albedo resolved at tex_uv
+  const Material* mat;  // &mtl
   uint32_t inst, prim;
 };
 
 struct Scene {
   std::vector<Material> materials;
+  std::vector<Texture> textures;
   std::vector<std::unique_ptr<Object>> objects;
   std::vector<Light> lights;
   LightDist dist;
