@@ -155,6 +155,8 @@ def _load():
         "ar_job_cancel": (i32, [P, u64]),
         "ar_server_clock": (C.c_double, [P]),
         "ar_bvh_build_host": (i32, [P, u32, P, P, P, C.POINTER(u32), C.POINTER(u32)]),
+        "ar_bvh_build_device": (i32, [P, u32, P, P, P, C.POINTER(u32), C.POINTER(u32)]),
+        "ar_scene_set_builder": (i32, [P, u32]),
         "ar_scene_set_trace_config": (i32, [P, C.POINTER(ArTraceConfig)]),
         "ar_scene_get_trace_config": (i32, [P, C.POINTER(ArTraceConfig)]),
         "ar_scene_set_anyhit_config": (i32, [P, C.POINTER(ArTraceConfig)]),
@@ -323,12 +325,15 @@ class Scene:
     reference's SAH, uploads the flattened scene to HBM and keeps it there."""
 
     def __init__(self, objects: List[SceneObject], lights: List[object], light_dist: str = "power",
-                 obj_files: Sequence = ()):
+                 obj_files: Sequence = (), builder: str = "sah"):
         """`obj_files`: (path, transform or None) pairs loaded with
-        load_obj_scene (importer.rs:167-241) after `objects`."""
+        load_obj_scene (importer.rs:167-241) after `objects`.  `builder`:
+        "sah" (the reference's host SAH) or "lbvh" (device linear BVH) for the
+        meshes' trees."""
         init()
         self._h = C.c_void_p()
         _check(_lib.ar_scene_create(C.byref(self._h)))
+        _check(_lib.ar_scene_set_builder(self._h, {"sah": 0, "lbvh": 1}[builder]))
         self._keep = []
         mat_ids, tex_ids = {}, {}
         for obj in objects:
@@ -380,7 +385,7 @@ class Scene:
             _check(_lib.ar_scene_set_mesh_texcoords(self._h, oid.value, _ptr(uv), uv.shape[0]))
 
     @classmethod
-    def from_desc(cls, desc: SceneDesc) -> "Scene":
+    def from_desc(cls, desc: SceneDesc, builder: str = "sah") -> "Scene":
         textures = [ImageTexture(t) for t in getattr(desc, "textures", [])]
         mats = [_mat_from_desc(m, textures) for m in desc.materials]
         objs = []
@@ -393,7 +398,7 @@ class Scene:
                 lights.append(PointLight(l.position, l.intensity))
             else:
                 lights.append(QuadLight(l.position, l.edge_u, l.edge_v, l.intensity))
-        return cls(objs, lights, {0: "none", 1: "uniform", 2: "power"}[desc.light_dist])
+        return cls(objs, lights, {0: "none", 1: "uniform", 2: "power"}[desc.light_dist], builder=builder)
 
     @property
     def handle(self):
@@ -776,9 +781,10 @@ def screened_poisson_reconstruct(primal, dx, dy, alpha: float, iterations: int =
     return out, res.value
 
 
-def bvh_build_host(boxes: np.ndarray):
-    """Host SAH build (the library's builder) for topology tests: returns
-    (boxes (m,6), meta (m,3) [left_or_start, right_or_end, count], prims)."""
+def bvh_build_host(boxes: np.ndarray, device: bool = False):
+    """Host SAH build (the library's builder) for topology tests, or with
+    device=True the device LBVH builder: returns (boxes (m,6), meta (m,3)
+    [left_or_start, right_or_end, count], prims, depth)."""
     b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
     n = b.shape[0]
     cap = max(1, 2 * n)
@@ -787,7 +793,8 @@ def bvh_build_host(boxes: np.ndarray):
     op = np.zeros(max(1, n), np.uint32)
     nn = C.c_uint32()
     dep = C.c_uint32()
-    _check(_lib.ar_bvh_build_host(_ptr(b), n, _ptr(ob), _ptr(om), _ptr(op), C.byref(nn), C.byref(dep)))
+    fn = _lib.ar_bvh_build_device if device else _lib.ar_bvh_build_host
+    _check(fn(_ptr(b), n, _ptr(ob), _ptr(om), _ptr(op), C.byref(nn), C.byref(dep)))
     m = nn.value
     return ob[:m], om[:m], op[:n], dep.value
 

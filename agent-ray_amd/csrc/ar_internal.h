@@ -68,6 +68,9 @@ struct BinTree {
 // Exact full-sweep SAH (same split decisions as bvh.rs:71-150) computed with
 // per-axis presorted lists and stable partitioning: O(n log n) per level.
 BinTree build_sah(const std::vector<HBox>& boxes, int nthreads);
+// Linear BVH built on the device (Morton order, Karras hierarchy), the same
+// form as build_sah's output with leaves of up to four primitives.
+ar_status build_lbvh(const std::vector<HBox>& boxes, BinTree& out);
 
 // ---- materials / lights as stored on the device -----------------------------
 struct alignas(16) DMaterial {
@@ -134,6 +137,7 @@ struct ar_scene {
   std::vector<uint4> tex_info;   // per texture: (first texel, width, height, -)
   std::vector<ar_light> lights;
   uint32_t dist_mode = AR_LIGHTDIST_POWER;
+  uint32_t builder = AR_BUILDER_SAH;  // bottom-level tree builder
   bool committed = false;
   // host copies of what was uploaded (used by tests through the info call)
   ar_scene_info info{};

@@ -379,6 +379,14 @@ ar_status ar_scene_add_mesh(ar_scene* s, const float* positions, uint32_t nv, co
   return AR_OK;
 }
 
+ar_status ar_scene_set_builder(ar_scene* s, uint32_t builder) {
+  if (!s) return fail(AR_ERR_INVALID_ARG, "ar_scene_set_builder: NULL scene");
+  if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
+  if (builder > AR_BUILDER_LBVH) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown tree builder");
+  s->builder = builder;
+  return AR_OK;
+}
+
 ar_status ar_scene_add_texture(ar_scene* s, const uint8_t* rgb, uint32_t width, uint32_t height, uint32_t* out_id) {
   if (!s || !rgb) return fail(AR_ERR_INVALID_ARG, "ar_scene_add_texture: NULL argument");
   if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
@@ -487,7 +495,13 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
         boxes[i].hi[k] = smax(smax(v0[k], v1[k]), v2[k]);
       }
     }
-    BinTree t = build_sah(boxes, 0);
+    BinTree t;
+    if (s->builder == AR_BUILDER_LBVH) {
+      ar_status be = build_lbvh(boxes, t);
+      if (be != AR_OK) return be;
+    } else {
+      t = build_sah(boxes, 0);
+    }
     max_depth = std::max(max_depth, t.depth);
     uint32_t tri_base = (uint32_t)tris.size();
     for (size_t i = 0; i < t.nodes.size(); ++i) {

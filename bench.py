@@ -452,6 +452,51 @@ def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
     return out
 
 
+def bench_builders(ar, torch):
+    """The 2^20-triangle soup committed with the reference's host SAH and with
+    the device LBVH builder (commit wall time: tree build, flattening,
+    upload), then the 2^24 bench rays traced through each with the same
+    launches (closest hit 256 x 8 x 8 x 8 x 16 x 3, any hit 128 x 16 x 8 x 8 x
+    16 x 4), best of three launches each."""
+    wl = ar.workloads
+    desc = wl.triangle_soup(1 << 20, seed=0)
+    rays = wl.soup_rays(1 << 24, seed=1)
+    dr = torch.from_numpy(rays).cuda()
+    n = rays.shape[0]
+    hits = torch.empty((n, 2), dtype=torch.float32, device="cuda")
+    occ = torch.empty((n,), dtype=torch.int32, device="cuda")
+    stream = torch.cuda.Stream()
+    sh = C_void(stream.cuda_stream)
+    lib = ar.lib()
+    out = {}
+    for builder in ("sah", "lbvh"):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sc = ar.Scene.from_desc(desc, builder=builder)
+        build_s = time.perf_counter() - t0
+        sc.set_trace_config(256, 8, 8, 8, 16, 3)
+        sc.set_anyhit_config(128, 16, 8, 8, 16, 4)
+        res = {"commit_s": round(build_s, 3), "wide_nodes": int(sc.info.n_nodes), "max_depth": int(sc.info.max_depth)}
+        for name, fn in (("closest", lambda: ar._check(lib.ar_trace(sc.handle, C_void(dr.data_ptr()), n,
+                                                                      C_void(hits.data_ptr()), None, ar.AR_ASYNC, sh))),
+                         ("anyhit", lambda: ar._check(lib.ar_occluded(sc.handle, C_void(dr.data_ptr()), n,
+                                                                       C_void(occ.data_ptr()), ar.AR_ASYNC, sh)))):
+            fn()
+            best = float("inf")
+            for _ in range(3):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                fn()
+                e1.record(stream)
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            res[f"{name}_mrays_s"] = round(n / best / 1e3, 1)
+        out[builder] = res
+        del sc
+    log(f"tree builders (soup 1M): {out}")
+    return out
+
+
 def bench_film_output(ar, torch, pixels, spp):
     """Film output stage on the rendered film (device resident): Reinhard +
     gamma 2.2 + 8-bit encode (Film::to_rgb_image; 12 B read + 3 B written
@@ -673,7 +718,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "pt", "copy", "pmc-probe"],
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "pt", "copy", "pmc-probe", "builders"],
                     help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--trace-config", default=None,
@@ -707,6 +752,9 @@ def main():
         res.update(bench_bdpt_gdpt(ar, torch))
         print(json.dumps(res), flush=True)
         return
+    if args.workload == "builders":
+        print(json.dumps(bench_builders(ar, torch)), flush=True)
+        return
     if args.workload == "copy":
         print(json.dumps(bench_copy(args, torch)), flush=True)
         return
@@ -726,6 +774,7 @@ def main():
         out["extra"] = {"pt_cornell_256x256x16spp": bench_pt(ar, torch, "cornell"),
                         "pt_room_1920x1080x64spp": bench_pt(ar, torch, "room")}
         out["extra"].update(bench_bdpt_gdpt(ar, torch))
+        out["extra"]["tree_builders_soup1M"] = bench_builders(ar, torch)
     print(json.dumps(out), flush=True)
 
 
