@@ -140,10 +140,10 @@ def bench_traversal(args, ar, torch):
 
     if tuned is not None:
         # refinement on the workload itself (untimed): the sample winner and
-        # its neighbours, best of two launches each, closest and any hit
+        # its neighbours, best of three launches each, closest and any hit
         def launch_ms(fn):
             best = float("inf")
-            for _ in range(2):
+            for _ in range(3):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record(stream)
                 fn()
