@@ -476,7 +476,7 @@ def bench_builders(ar, torch):
         build_s = time.perf_counter() - t0
         sc.set_trace_config(256, 8, 8, 8, 16, 3)
         sc.set_anyhit_config(128, 16, 8, 8, 16, 4)
-        res = {"commit_s": round(build_s, 3), "wide_nodes": int(sc.info.n_nodes), "max_depth": int(sc.info.max_depth)}
+        res = {"commit_s": round(build_s, 3), "binary_nodes": int(sc.info.n_nodes), "max_depth": int(sc.info.max_depth)}
         for name, fn in (("closest", lambda: ar._check(lib.ar_trace(sc.handle, C_void(dr.data_ptr()), n,
                                                                       C_void(hits.data_ptr()), None, ar.AR_ASYNC, sh))),
                          ("anyhit", lambda: ar._check(lib.ar_occluded(sc.handle, C_void(dr.data_ptr()), n,

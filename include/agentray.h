@@ -144,7 +144,7 @@ typedef struct ar_render_stats {
 typedef struct ar_scene_info {
   uint32_t n_objects;
   uint32_t n_triangles;
-  uint32_t n_nodes;      /* wide nodes over all BLAS + TLAS */
+  uint32_t n_nodes;      /* binary nodes (two child boxes each) over all BLAS + TLAS */
   uint32_t n_lights;
   uint32_t max_depth;    /* deepest BLAS */
   uint64_t device_bytes; /* scene bytes resident in HBM */
