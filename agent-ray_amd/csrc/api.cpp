@@ -1,4 +1,5 @@
 // Library-level entry points: errors, device init, camera construction.
+#include <new>
 #include <cmath>
 
 #include "ar_internal.h"
@@ -12,6 +13,18 @@ void set_error(const std::string& msg) { g_error = msg; }
 ar_status fail(ar_status code, const std::string& msg) {
   g_error = msg;
   return code;
+}
+
+ar_status fail_exception(const char* fn) {
+  try {
+    throw;
+  } catch (const std::bad_alloc&) {
+    return fail(AR_ERR_OUT_OF_BUDGET, std::string(fn) + ": out of host memory");
+  } catch (const std::exception& e) {
+    return fail(AR_ERR_INVALID_ARG, std::string(fn) + ": " + e.what());
+  } catch (...) {
+    return fail(AR_ERR_INVALID_ARG, std::string(fn) + ": unknown exception");
+  }
 }
 
 hipStream_t pick_stream(const ar_scene* s, void* stream) {

@@ -15,6 +15,11 @@ namespace ar {
 // ---- errors -----------------------------------------------------------------
 void set_error(const std::string& msg);
 ar_status fail(ar_status code, const std::string& msg);
+// The status for the exception in flight (host memory -> AR_ERR_OUT_OF_BUDGET,
+// anything else -> AR_ERR_INVALID_ARG).  Entry points that allocate on the
+// host are function-try-blocks ending in it, so nothing unwinds through the
+// C ABI.
+ar_status fail_exception(const char* fn);
 #define AR_HIP(call)                                                                     \
   do {                                                                                   \
     hipError_t e_ = (call);                                                              \

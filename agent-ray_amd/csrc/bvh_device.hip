@@ -199,7 +199,7 @@ ar_status build_lbvh(const std::vector<HBox>& boxes, BinTree& out) {
 // The device builder on an explicit list of boxes, in ar_bvh_build_host's
 // output format (for the tests).
 extern "C" ar_status ar_bvh_build_device(const float* boxes, uint32_t n, float* out_boxes, uint32_t* out_meta,
-                                         uint32_t* out_prims, uint32_t* out_nodes, uint32_t* out_depth) {
+                                         uint32_t* out_prims, uint32_t* out_nodes, uint32_t* out_depth) try {
   if ((!boxes && n) || !out_boxes || !out_meta || !out_prims || !out_nodes)
     return ar::fail(AR_ERR_INVALID_ARG, "ar_bvh_build_device: NULL argument");
   std::vector<ar::HBox> b(n);
@@ -224,4 +224,6 @@ extern "C" ar_status ar_bvh_build_device(const float* boxes, uint32_t n, float* 
   *out_nodes = (uint32_t)t.nodes.size();
   if (out_depth) *out_depth = t.depth;
   return AR_OK;
+} catch (...) {
+  return ar::fail_exception("ar_bvh_build_device");
 }

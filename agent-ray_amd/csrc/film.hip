@@ -267,7 +267,7 @@ extern "C" ar_status ar_film_to_rgb8(const float* film, uint32_t width, uint32_t
 }
 
 extern "C" ar_status ar_film_save_exr(const float* film, uint32_t width, uint32_t height, float scale,
-                                      const char* path, uint32_t flags, void* stream) {
+                                      const char* path, uint32_t flags, void* stream) try {
   if (!film || !path) return fail(AR_ERR_INVALID_ARG, "ar_film_save_exr: NULL argument");
   if (width == 0 || height == 0) return fail(AR_ERR_INVALID_ARG, "ar_film_save_exr: empty film");
   if (width > (1u << 24) || height > (1u << 24) || (uint64_t)width * height >= (1ull << 31))
@@ -360,4 +360,6 @@ extern "C" ar_status ar_film_save_exr(const float* film, uint32_t width, uint32_
   ok = (std::fclose(fp) == 0) && ok;
   if (!ok) return fail(AR_ERR_INVALID_ARG, std::string("ar_film_save_exr: write failed: ") + path);
   return AR_OK;
+} catch (...) {
+  return ar::fail_exception("ar_film_save_exr");
 }

@@ -5,6 +5,7 @@
 // `tobj` crate does with GPU_LOAD_OPTIONS: polygons triangulated as fans,
 // one index per distinct position/texcoord/normal triple, a new model at
 // every `o` / `g` and `usemtl` once the current one has faces).
+#include <sys/stat.h>
 #include <zlib.h>
 
 #include <algorithm>
@@ -13,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <map>
 #include <sstream>
 #include <string>
@@ -171,10 +173,19 @@ ar_status decode_png(const std::vector<uint8_t>& f, std::vector<uint8_t>& rgb, u
   return AR_OK;
 }
 
+// Regular files only: a directory opens on Linux and fails on the first read.
 ar_status read_file(const std::string& path, std::vector<uint8_t>& out) {
-  std::ifstream f(path, std::ios::binary);
+  FILE* f = std::fopen(path.c_str(), "rb");
   if (!f) return fail(AR_ERR_NOT_FOUND, "cannot open " + path);
-  out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+  struct stat sb;
+  if (fstat(fileno(f), &sb) != 0 || !S_ISREG(sb.st_mode)) {
+    std::fclose(f);
+    return fail(AR_ERR_NOT_FOUND, "not a regular file: " + path);
+  }
+  out.resize((size_t)sb.st_size);
+  const size_t got = out.empty() ? 0 : std::fread(out.data(), 1, out.size(), f);
+  std::fclose(f);
+  if (got != out.size()) return fail(AR_ERR_NOT_FOUND, "short read: " + path);
   return AR_OK;
 }
 
@@ -295,7 +306,7 @@ struct Model {
 
 extern "C" {
 
-ar_status ar_image_load_png(const char* path, uint8_t** rgb, uint32_t* width, uint32_t* height) {
+static ar_status image_load_png_body(const char* path, uint8_t** rgb, uint32_t* width, uint32_t* height) {
   if (!path || !rgb || !width || !height) return fail(AR_ERR_INVALID_ARG, "ar_image_load_png: NULL argument");
   std::vector<uint8_t> f, out;
   ar_status e = read_file(path, f);
@@ -310,7 +321,8 @@ ar_status ar_image_load_png(const char* path, uint8_t** rgb, uint32_t* width, ui
 
 void ar_image_free(uint8_t* rgb) { std::free(rgb); }
 
-ar_status ar_scene_load_obj(ar_scene* s, const char* path, const float* transform, uint32_t* n_objects) {
+static ar_status scene_load_obj_body(ar_scene* s, const char* path, const float* transform,
+                                     uint32_t* n_objects) {
   if (!s || !path) return fail(AR_ERR_INVALID_ARG, "ar_scene_load_obj: NULL argument");
   if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
   std::vector<uint8_t> bytes;
@@ -433,6 +445,28 @@ ar_status ar_scene_load_obj(ar_scene* s, const char* path, const float* transfor
   }
   if (n_objects) *n_objects = added;
   return AR_OK;
+}
+
+// The readers' containers and streams can throw (host memory, stream
+// errors); nothing may unwind through the C ABI.
+ar_status ar_image_load_png(const char* path, uint8_t** rgb, uint32_t* width, uint32_t* height) {
+  try {
+    return image_load_png_body(path, rgb, width, height);
+  } catch (const std::bad_alloc&) {
+    return fail(AR_ERR_OUT_OF_BUDGET, "ar_image_load_png: out of host memory");
+  } catch (const std::exception& e) {
+    return fail(AR_ERR_INVALID_ARG, std::string("ar_image_load_png: ") + e.what());
+  }
+}
+
+ar_status ar_scene_load_obj(ar_scene* s, const char* path, const float* transform, uint32_t* n_objects) {
+  try {
+    return scene_load_obj_body(s, path, transform, n_objects);
+  } catch (const std::bad_alloc&) {
+    return fail(AR_ERR_OUT_OF_BUDGET, "ar_scene_load_obj: out of host memory");
+  } catch (const std::exception& e) {
+    return fail(AR_ERR_INVALID_ARG, std::string("ar_scene_load_obj: ") + e.what());
+  }
 }
 
 }  // extern "C"

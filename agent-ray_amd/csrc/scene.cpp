@@ -349,7 +349,7 @@ ar_status ar_scene_add_material(ar_scene* s, const ar_material* m, uint32_t* out
 ar_status ar_scene_add_mesh(ar_scene* s, const float* positions, uint32_t nv, const uint32_t* indices,
                             uint32_t nt, const float* normals, const uint32_t* slots, uint32_t n_slots,
                             const uint32_t* mats, uint32_t n_mats, const float* transform,
-                            uint32_t* out_object_id) {
+                            uint32_t* out_object_id) try {
   if (!s || !positions || !indices || !slots || !mats)
     return fail(AR_ERR_INVALID_ARG, "ar_scene_add_mesh: NULL argument");
   if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
@@ -377,6 +377,8 @@ ar_status ar_scene_add_mesh(ar_scene* s, const float* positions, uint32_t nv, co
   s->meshes.push_back(std::move(m));
   if (out_object_id) *out_object_id = (uint32_t)s->meshes.size() - 1;
   return AR_OK;
+} catch (...) {
+  return ar::fail_exception("ar_scene_add_mesh");
 }
 
 ar_status ar_scene_set_builder(ar_scene* s, uint32_t builder) {
@@ -387,7 +389,7 @@ ar_status ar_scene_set_builder(ar_scene* s, uint32_t builder) {
   return AR_OK;
 }
 
-ar_status ar_scene_add_texture(ar_scene* s, const uint8_t* rgb, uint32_t width, uint32_t height, uint32_t* out_id) {
+ar_status ar_scene_add_texture(ar_scene* s, const uint8_t* rgb, uint32_t width, uint32_t height, uint32_t* out_id) try {
   if (!s || !rgb) return fail(AR_ERR_INVALID_ARG, "ar_scene_add_texture: NULL argument");
   if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
   if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 28))
@@ -412,9 +414,11 @@ ar_status ar_scene_add_texture(ar_scene* s, const uint8_t* rgb, uint32_t width, 
   s->tex_info.push_back(make_uint4(first, width, height, 0u));
   if (out_id) *out_id = (uint32_t)s->tex_info.size();  // ids start at 1; 0 means no texture
   return AR_OK;
+} catch (...) {
+  return ar::fail_exception("ar_scene_add_texture");
 }
 
-ar_status ar_scene_set_mesh_texcoords(ar_scene* s, uint32_t object_id, const float* uv, uint32_t nv) {
+ar_status ar_scene_set_mesh_texcoords(ar_scene* s, uint32_t object_id, const float* uv, uint32_t nv) try {
   if (!s || !uv) return fail(AR_ERR_INVALID_ARG, "ar_scene_set_mesh_texcoords: NULL argument");
   if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
   if (object_id >= s->meshes.size()) return fail(AR_ERR_INVALID_ARG, "unknown object id");
@@ -424,6 +428,8 @@ ar_status ar_scene_set_mesh_texcoords(ar_scene* s, uint32_t object_id, const flo
     if (!std::isfinite(uv[i])) return fail(AR_ERR_INVALID_ARG, "non-finite texture coordinate");
   m.uv.assign(uv, uv + 2ull * nv);
   return AR_OK;
+} catch (...) {
+  return ar::fail_exception("ar_scene_set_mesh_texcoords");
 }
 
 ar_status ar_scene_add_light(ar_scene* s, const ar_light* l, uint32_t* out_id) {
@@ -633,7 +639,7 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
 
 extern "C" {
 
-ar_status ar_scene_commit(ar_scene* s) {
+ar_status ar_scene_commit(ar_scene* s) try {
   if (!s) return fail(AR_ERR_INVALID_ARG, "NULL scene");
   if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
   if (s->meshes.empty()) return fail(AR_ERR_INVALID_ARG, "scene has no objects");
@@ -743,6 +749,8 @@ ar_status ar_scene_commit(ar_scene* s) {
   AR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   s->committed = true;
   return AR_OK;
+} catch (...) {
+  return ar::fail_exception("ar_scene_commit");
 }
 
 ar_status ar_scene_get_info(const ar_scene* s, ar_scene_info* info) {

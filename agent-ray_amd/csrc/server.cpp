@@ -344,7 +344,7 @@ struct ar_server {
 
 extern "C" {
 
-ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_server** out) {
+ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_server** out) try {
   if (!s || !out) return fail(AR_ERR_INVALID_ARG, "ar_server_create: NULL argument");
   if (!s->committed) return fail(AR_ERR_STATE, "ar_server_create: scene not committed");
   std::unique_ptr<ar_server> srv(new ar_server());
@@ -363,6 +363,8 @@ ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_se
   p->worker = std::thread([p]() { p->loop(); });
   *out = p;
   return AR_OK;
+} catch (...) {
+  return ar::fail_exception("ar_server_create");
 }
 
 ar_status ar_server_destroy(ar_server* srv) {
@@ -390,7 +392,7 @@ ar_status ar_server_destroy(ar_server* srv) {
   return AR_OK;
 }
 
-ar_status ar_job_submit(ar_server* srv, const ar_job_desc* desc, uint64_t* out_id) {
+ar_status ar_job_submit(ar_server* srv, const ar_job_desc* desc, uint64_t* out_id) try {
   if (!srv || !desc) return fail(AR_ERR_INVALID_ARG, "ar_job_submit: NULL argument");
   auto j = std::make_shared<Job>();
   j->spec.width = desc->width;
@@ -412,6 +414,8 @@ ar_status ar_job_submit(ar_server* srv, const ar_job_desc* desc, uint64_t* out_i
   }
   srv->cv_work.notify_all();
   return AR_OK;
+} catch (...) {
+  return ar::fail_exception("ar_job_submit");
 }
 
 ar_status ar_job_poll(ar_server* srv, uint64_t id, int32_t* state) {

@@ -287,6 +287,13 @@ def test_obj_without_mtl_and_errors(ar, tmp_path):
         n = C.c_uint32()
         assert hs.lib.ar_scene_load_obj(hs.s, os.fsencode(str(tmp_path / "b.obj")), None, C.byref(n)) == -1
         assert hs.lib.ar_scene_load_obj(hs.s, os.fsencode(str(tmp_path / "none.obj")), None, C.byref(n)) == -8
+        # a directory opens on Linux and fails on the first read: a status, not a C++ exception
+        (tmp_path / "sub").mkdir()
+        assert hs.lib.ar_scene_load_obj(hs.s, os.fsencode(str(tmp_path / "sub")), None, C.byref(n)) == -8
+        (tmp_path / "c.obj").write_text("mtllib sub\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+        assert hs.lib.ar_scene_load_obj(hs.s, os.fsencode(str(tmp_path / "c.obj")), None, C.byref(n)) in (0, -8)
+        with pytest.raises(ar.ArError):
+            ar.load_png(tmp_path / "sub")
     finally:
         hs.close()
 
