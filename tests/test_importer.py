@@ -347,3 +347,78 @@ def test_emitter_material_is_untextured(ar):
         assert hs.material(0)["tex"] == -1
     finally:
         hs.close()
+
+
+def _mutate_bytes(rng, b):
+    b = bytearray(b)
+    for _ in range(int(rng.integers(1, 7))):
+        if not b:
+            break
+        at = int(rng.integers(0, len(b)))
+        op = int(rng.integers(0, 5))
+        if op == 0:
+            b[at] ^= 1 << int(rng.integers(0, 8))
+        elif op == 1:
+            b[at] = int(rng.integers(0, 256))
+        elif op == 2:
+            del b[at:]
+        elif op == 3:
+            b[at:at + 4] = b"\x7f\xff\xff\xff"  # a huge chunk length or image size
+        else:
+            seg = b[at:at + int(rng.integers(1, 65))]
+            k = int(rng.integers(0, len(b)))
+            b[k:k] = seg
+    return bytes(b)
+
+
+_HOSTILE = [b"0", b"-1", b"-2147483648", b"4294967297", b"nan", b"inf", b"1e39", b"", b"/", b"1//1", b"3/3/3", b"x"]
+
+
+def _mutate_text(rng, t):
+    lines = t.split(b"\n")
+    for _ in range(int(rng.integers(1, 5))):
+        i = int(rng.integers(0, len(lines)))
+        toks = lines[i].split(b" ")
+        op = int(rng.integers(0, 3))
+        if op == 0:
+            toks[int(rng.integers(0, len(toks)))] = _HOSTILE[int(rng.integers(0, len(_HOSTILE)))]
+            lines[i] = b" ".join(toks)
+        elif op == 1:
+            del lines[i]
+            if not lines:
+                lines = [b""]
+        else:
+            lines.insert(int(rng.integers(0, len(lines))), lines[i])
+    return b"\n".join(lines)
+
+
+def test_readers_survive_mutated_files(ar, tmp_path):
+    """Seeded mutations of the PNG, OBJ and MTL fixtures (bit flips, truncation,
+    huge lengths, hostile index and number tokens, repeated or missing lines):
+    every call returns a status -- decoded images have the size they claim --
+    and none aborts the process."""
+    rng = np.random.default_rng(7)
+    png = open(os.path.join(GOLD, "tex", "checker.png"), "rb").read()
+    obj = open(os.path.join(GOLD, "scene.obj"), "rb").read()
+    mtl = open(os.path.join(GOLD, "scene.mtl"), "rb").read()
+    (tmp_path / "tex").mkdir()
+    lib = ar.lib()
+    for _ in range(400):
+        (tmp_path / "tex" / "checker.png").write_bytes(_mutate_bytes(rng, png))
+        try:
+            img = ar.load_png(tmp_path / "tex" / "checker.png")
+            assert img.ndim == 3 and img.shape[2] == 3
+        except ar.ArError:
+            pass
+        if rng.integers(0, 2):
+            (tmp_path / "scene.obj").write_bytes(_mutate_text(rng, obj))
+            (tmp_path / "scene.mtl").write_bytes(mtl)
+        else:
+            (tmp_path / "scene.obj").write_bytes(obj)
+            (tmp_path / "scene.mtl").write_bytes(_mutate_text(rng, mtl))
+        s = C.c_void_p()
+        assert lib.ar_scene_create(C.byref(s)) == 0
+        n = C.c_uint32()
+        rc = lib.ar_scene_load_obj(s, os.fsencode(str(tmp_path / "scene.obj")), None, C.byref(n))
+        assert rc in (0, -1, -2, -3, -8), rc
+        lib.ar_scene_destroy(s)
