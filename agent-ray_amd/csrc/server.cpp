@@ -93,6 +93,7 @@ struct ar_server {
   std::vector<std::unique_ptr<Slot>> slots;
   std::vector<hipEvent_t> event_pool;
   size_t used_bytes = 0;
+  bool broken = false;  // the worker thread ended on an exception
   bool stop = false;
   uint64_t next_id = 1;
   std::thread worker;
@@ -130,6 +131,21 @@ struct ar_server {
     j.error = code;
     j.error_msg = msg;
     j.t_finish = now();
+  }
+
+  // Last resort of the worker thread: an exception escaping the scheduling
+  // loop (host memory) fails every unfinished job, so that no waiter blocks
+  // forever, and the server takes no new jobs.  Device work already issued
+  // keeps its buffers until ar_server_destroy synchronises the streams.
+  void fail_all(const std::string& msg) {
+    std::lock_guard<std::mutex> lk(mu);
+    broken = true;
+    for (auto& kv : jobs) {
+      Job& j = *kv.second;
+      if (j.state == AR_JOB_QUEUED || j.state == AR_JOB_RUNNING) fail_job(j, AR_ERR_OUT_OF_BUDGET, msg);
+    }
+    pending.clear();
+    cv_done.notify_all();
   }
 
   // Admit queued jobs into free slots within the budget (caller holds mu).
@@ -359,8 +375,15 @@ ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_se
     AR_HIP(hipStreamCreateWithFlags(&srv->slots.back()->stream, hipStreamNonBlocking));
   }
   srv->t0 = std::chrono::steady_clock::now();
-  ar_server* p = srv.release();
-  p->worker = std::thread([p]() { p->loop(); });
+  ar_server* p = srv.get();
+  p->worker = std::thread([p]() {
+    try {
+      p->loop();
+    } catch (const std::exception& e) {
+      p->fail_all(std::string("server worker: ") + e.what());
+    }
+  });
+  srv.release();
   *out = p;
   return AR_OK;
 } catch (...) {
@@ -406,6 +429,7 @@ ar_status ar_job_submit(ar_server* srv, const ar_job_desc* desc, uint64_t* out_i
   {
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->stop) return fail(AR_ERR_STATE, "server is shutting down");
+    if (srv->broken) return fail(AR_ERR_STATE, "the server's worker thread failed; destroy the server");
     j->id = srv->next_id++;
     j->t_submit = srv->now();
     srv->jobs[j->id] = j;
