@@ -370,19 +370,32 @@ ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_se
   if (srv->cfg.max_active_jobs == 0) srv->cfg.max_active_jobs = 8;
   if (srv->cfg.max_active_jobs > 64) return fail(AR_ERR_INVALID_ARG, "max_active_jobs above 64");
   (void)hipGetDevice(&srv->device);
+  auto drop_streams = [&srv]() {
+    for (auto& sp : srv->slots)
+      if (sp->stream) (void)hipStreamDestroy(sp->stream);
+  };
   for (uint32_t i = 0; i < srv->cfg.max_active_jobs; ++i) {
     srv->slots.emplace_back(new Slot());
-    AR_HIP(hipStreamCreateWithFlags(&srv->slots.back()->stream, hipStreamNonBlocking));
+    const hipError_t e = hipStreamCreateWithFlags(&srv->slots.back()->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      drop_streams();
+      return fail(AR_ERR_DEVICE, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+    }
   }
   srv->t0 = std::chrono::steady_clock::now();
   ar_server* p = srv.get();
-  p->worker = std::thread([p]() {
-    try {
-      p->loop();
-    } catch (const std::exception& e) {
-      p->fail_all(std::string("server worker: ") + e.what());
-    }
-  });
+  try {
+    p->worker = std::thread([p]() {
+      try {
+        p->loop();
+      } catch (const std::exception& e) {
+        p->fail_all(std::string("server worker: ") + e.what());
+      }
+    });
+  } catch (...) {
+    drop_streams();
+    throw;
+  }
   srv.release();
   *out = p;
   return AR_OK;
