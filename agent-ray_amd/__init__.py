@@ -38,6 +38,7 @@ AR_ASYNC = 0x2
 AR_REF_ORDER = 0x4
 AR_TUNE_CLOSEST = 0x10
 AR_TUNE_ANYHIT = 0x20
+AR_COUNT_ANYHIT = 0x40
 NO_HIT = 0xFFFFFFFF
 
 
@@ -109,6 +110,12 @@ def config_key(c: dict) -> tuple:
             c["park_slots"])
 
 
+class ArTraceCounts(C.Structure):
+    _fields_ = [("rays", C.c_uint64), ("node_visits", C.c_uint64), ("tri_tests", C.c_uint64),
+                ("inst_tests", C.c_uint64), ("spill_writes", C.c_uint64), ("spill_reads", C.c_uint64),
+                ("grid", C.c_uint32), ("workgroup_size", C.c_uint32)]
+
+
 class ArJobStats(C.Structure):
     _fields_ = [("submit_time", C.c_double), ("start_time", C.c_double), ("finish_time", C.c_double),
                 ("paths", C.c_uint64), ("rays", C.c_uint64), ("state", C.c_int32)]
@@ -142,6 +149,7 @@ def _load():
         "ar_scene_get_info": (i32, [P, C.POINTER(ArSceneInfo)]),
         "ar_trace": (i32, [P, P, u64, P, P, u32, P]),
         "ar_occluded": (i32, [P, P, u64, P, u32, P]),
+        "ar_trace_count": (i32, [P, P, u64, u32, P, C.POINTER(ArTraceCounts)]),
         "ar_camera_lookat": (i32, [P, P, P, f, f, C.POINTER(ArCamera)]),
         "ar_camera_eye_angle": (i32, [P, f, f, f, f, C.POINTER(ArCamera)]),
         "ar_render": (i32, [P, C.POINTER(ArCamera), u32, u32, C.POINTER(ArRenderConfig), P, P,
@@ -424,6 +432,16 @@ class Scene:
         _check(_lib.ar_trace(self._h, _ptr(r), n, _ptr(hits), _ptr(inst), flags | AR_HOST_PTRS, None))
         return hits[:, 0].copy(), hits[:, 1].view(np.uint32).copy(), inst
 
+    def trace_counts(self, rays, anyhit: bool = False, ref_order: bool = False) -> dict:
+        """Traversal work of the launch ar_trace (or ar_occluded with anyhit)
+        makes over these rays (torch CUDA tensor, (n, 8) float32): node,
+        triangle and instance records read and stack entries spilled, counted
+        by an instrumented build of the same kernel with the same launch."""
+        flags = (AR_COUNT_ANYHIT if anyhit else 0) | (AR_REF_ORDER if ref_order else 0)
+        c = ArTraceCounts()
+        _check(_lib.ar_trace_count(self._h, C.c_void_p(rays.data_ptr()), rays.shape[0], flags, None, C.byref(c)))
+        return {k: getattr(c, k) for k, _ in ArTraceCounts._fields_}
+
     def trace_config(self) -> dict:
         c = ArTraceConfig()
         _check(_lib.ar_scene_get_trace_config(self._h, C.byref(c)))
@@ -657,7 +675,8 @@ class GdptConfig:
 # autotuner's pick: (workgroup, LDS stack, blocks per CU, refill threshold,
 # search quorum, park slots).  The winner is then tried with the other park
 # slot count (3 <-> 4).
-TUNE_CANDIDATES = [(256, 8, 8, 8, 16, 3), (256, 8, 8, 8, 8, 3), (256, 8, 8, 8, 16, 4), (256, 8, 8, 8, 8, 4),
+TUNE_CANDIDATES = [(256, 12, 8, 8, 16, 3), (256, 12, 8, 8, 8, 4),
+                   (256, 8, 8, 8, 16, 3), (256, 8, 8, 8, 8, 3), (256, 8, 8, 8, 16, 4), (256, 8, 8, 8, 8, 4),
                    (256, 8, 8, 24, 16, 3), (128, 8, 16, 8, 16, 3), (128, 16, 8, 8, 16, 3), (128, 16, 8, 8, 8, 3),
                    (64, 16, 16, 8, 16, 3)]
 

@@ -43,6 +43,15 @@ std::unique_ptr<JobRun> make_bdpt_job(const ar_scene* s, const JobSpec& spec, Wo
                                       float* out_dev);
 // Device bytes a job of this shape needs (admission control).
 size_t job_footprint(const ar_scene* s, const JobSpec& spec);
+// Largest chunk (paths in flight) a job may use: path ids are 32-bit.
+constexpr uint64_t kMaxChunkPaths = 1ull << 31;
+// BDPT wavefront connection: bytes of connection slots per pass, bytes per
+// slot (ray origin, direction, contribution, occlusion flag, queue entry),
+// slots per path at a depth, and paths per pass.
+constexpr uint64_t kConnBudget = 12ull << 30;
+constexpr uint64_t kConnSlotBytes = 3 * 16 + 1 + 4;
+uint64_t bdpt_conn_slots(uint32_t max_depth);
+uint64_t bdpt_conn_pass(uint32_t max_depth, uint64_t cap);
 ar_status validate_job(const ar_scene* s, const JobSpec& spec);
 
 }  // namespace ar

@@ -8,11 +8,13 @@ constexpr int kTraceBlock = 128;       // 2 waves per workgroup
 constexpr int kTraceLdsStack = 16;     // LDS stack entries per lane (8 B each)
 constexpr int kTraceBlocksPerCU = 8;   // persistent grid: 16 waves per CU
 constexpr uint32_t kTraceRefillMin = 16;  // idle lanes a wave waits for before taking new rays
-// Traversal kernels with an 8-entry LDS stack are compiled for at least 6
-// waves per SIMD (<= 80 VGPRs): measured faster than the unconstrained 5
-// waves / 85 VGPRs; the few values the register limit moves to scratch sit
-// off the node loop.  A 16-entry stack is held to 5 waves by LDS anyway.
-constexpr int trace_min_waves(int lds_stack) { return lds_stack <= 8 ? 6 : 1; }
+// Traversal kernels with an 8- or 12-entry LDS stack are compiled for at
+// least 6 waves per SIMD (<= 80 VGPRs): measured faster than the
+// unconstrained 5 waves / 85 VGPRs; the few values the register limit moves
+// to scratch sit off the node loop.  At 6 waves per SIMD a CU holds six
+// 256-thread workgroups, whose 12-entry stacks (24 KiB each) fit its 160 KiB
+// of LDS.  A 16-entry stack is held to 5 waves by LDS anyway.
+constexpr int trace_min_waves(int lds_stack) { return lds_stack <= 12 ? 6 : 1; }
 
 uint32_t trace_grid(uint64_t n, int block);
 uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c);
@@ -24,8 +26,11 @@ int device_cus();
 uint32_t clamp_resident(uint32_t G, const void* kernel, int block);
 
 // Traversal variants compiled: (workgroup size, LDS stack entries per lane).
-#define AR_TRACE_VARIANTS(X) X(64, 8) X(64, 16) X(128, 8) X(128, 16) X(256, 8)
+#define AR_TRACE_VARIANTS(X) X(64, 8) X(64, 16) X(128, 8) X(128, 16) X(256, 8) X(256, 12)
 bool trace_variant_supported(uint32_t block, uint32_t stack);
+// The grid ar_trace (any = false) / ar_occluded (any = true) launches for n
+// rays under c; 0 for an unsupported variant.
+uint32_t trace_launch_grid(uint64_t n, const ar_trace_config& c, bool any, bool near);
 
 // Scene-level scratch (traversal stack spill).  acquire() orders the caller's
 // stream after the previous user of the scratch (an event, no host wait) and

@@ -39,7 +39,10 @@ struct Workspace {
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
   uint64_t graph_key[8] = {};  // what the captured bounce loop was recorded for
-  ar_status reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_entries, uint32_t grid_threads);
+  ar_status reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_entries, uint32_t grid_threads,
+                          hipStream_t st);
+  size_t bytes() const { return buf.bytes + out.bytes + aux.bytes + pois.bytes; }
+  void trim();  // frees every buffer (the caller has drained the stream that used them)
   ~Workspace();
 };
 

@@ -22,16 +22,31 @@
 #pragma once
 #include "ar_device.h"
 
-// Instrumentation hooks (empty in the library build).
+// Instrumentation hooks, empty in the library's production kernels.  The
+// counting build (trace_count.hip) defines them and compiles this header in
+// a namespace of its own (AR_TRAVERSE_NS), so its instrumented walker never
+// shares a symbol with the production one.
 #ifndef AR_COUNT_NODE
-#define AR_COUNT_NODE()
+#define AR_COUNT_NODE()  // one 64-byte node record read (4-wide or binary)
 #endif
 #ifndef AR_COUNT_TRI
-#define AR_COUNT_TRI()
+#define AR_COUNT_TRI()  // one 48-byte triangle record read and tested
+#endif
+#ifndef AR_COUNT_INST
+#define AR_COUNT_INST()  // one 96-byte instance record read at a TLAS leaf
+#endif
+#ifndef AR_COUNT_SPILL_W
+#define AR_COUNT_SPILL_W()  // one 8-byte stack entry written to the global spill
+#endif
+#ifndef AR_COUNT_SPILL_R
+#define AR_COUNT_SPILL_R()  // one 8-byte stack entry read back from the global spill
+#endif
+#ifndef AR_TRAVERSE_NS
+#define AR_TRAVERSE_NS ar
 #endif
 
-
-namespace ar {
+namespace AR_TRAVERSE_NS {
+using namespace ::ar;
 
 // Stack pointers carry their address space in device code, so LDS entries
 // are read and written with LDS instructions and spilled ones with global
@@ -56,10 +71,12 @@ struct Stack {
   int sp;
   AR_HD void push(int ref, float tnear) {
     int2 e = make_int2(ref, bits_i(tnear));
-    if (sp < S)
+    if (sp < S) {
       lds[sp * stride] = e;
-    else
+    } else {
+      AR_COUNT_SPILL_W();
       spill[(size_t)(sp - S) * spill_stride] = (unsigned long long)(uint32_t)e.x | ((unsigned long long)(uint32_t)e.y << 32);
+    }
     ++sp;
   }
   AR_HD int2 pop() {
@@ -68,6 +85,7 @@ struct Stack {
     if (sp < S) {
       e = lds[sp * stride];
     } else {
+      AR_COUNT_SPILL_R();
       const unsigned long long v = spill[(size_t)(sp - S) * spill_stride];
       e = make_int2((int)(uint32_t)v, (int)(uint32_t)(v >> 32));
     }
@@ -384,6 +402,7 @@ struct Walker {
   // root box the local ray hits.  False when the leaf has none left.
   AR_HD bool enter(const SceneView& sv, uint32_t k, const Stack& stk) {
     for (;;) {
+      AR_COUNT_INST();
       const Inst* in = sv.insts + k;
       f3 lo, ld;
       if (in->identity) {
@@ -651,4 +670,4 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
 
 #endif  // __HIP__
 
-}  // namespace ar
+}  // namespace AR_TRAVERSE_NS

@@ -24,11 +24,17 @@
 // path tracer); light-tracing (s = 1) splats are added as 64-bit fixed-point
 // integers, so the splat film is the same whatever order the atomics land in.
 //
-// Reference behaviour kept (bug-compatible): t = 0 / s = 0 not implemented;
+// Reference behaviour kept (bug-compatible): s = 0 not implemented;
 // point-light t = 1 omits 1/sel_pmf; the camera vertex normal is the first
 // ray direction; surface vertices are never delta; the first camera-side
 // MIS step reuses vertex s-1 in the loop.  Quad lights (extension) connect
-// at t = 1 to the sampled emission point with the general formula.
+// at t = 1 to the sampled emission point with the general formula, and the
+// t = 0 strategy the reference leaves as a TODO for area lights (:810-818)
+// is built: a camera subpath vertex on an emitter's front face contributes
+// beta * Le (no shadow ray), weighted by the reference's walk with the
+// light side's pdfs of generating it (the emitter point: sel_pmf / area;
+// the vertex before it: the emission pdf in area measure); with an area
+// light the light-side walk also counts the t' = 0 alternative.
 #include "ar_kernels.h"
 #include "ar_pt.h"
 #include "ar_job.h"
@@ -48,7 +54,7 @@ struct BdptParams {
 struct Verts {
   float4* pf;    // (p.xyz, pdf_fwd)          [2][V][npaths]
   float4* nr;    // (n.xyz, pdf_rev)
-  float4* bm;    // (beta.xyz, meta bits: type | delta << 2 | id << 4)
+  float4* bm;    // (beta.xyz, meta bits: type | delta << 2 | emitter << 3 | id << 4)
   float2* uv;    // tex_uv of surface vertices (textured albedo)
   uint2* count;  // (camera vertices, light vertices) per path
   uint32_t npaths;
@@ -62,6 +68,7 @@ struct PV {
   f3 p, n, beta;
   float pdf_fwd, pdf_rev;
   uint32_t type, delta, id;  // id: material (surface) or light index
+  uint32_t emit = 0;         // camera-side surface vertex on the front face of an emitter
   float tu = 0.0f, tv = 0.0f;  // tex_uv (surface vertices)
 };
 
@@ -77,6 +84,7 @@ __device__ __forceinline__ PV load_v(const Verts& Vt, uint32_t side, uint32_t k,
   uint32_t m = __float_as_uint(c.w);
   v.type = m & 3u;
   v.delta = (m >> 2) & 1u;
+  v.emit = (m >> 3) & 1u;
   v.id = m >> 4;
   const float2 t = Vt.uv[i];
   v.tu = t.x;
@@ -85,7 +93,7 @@ __device__ __forceinline__ PV load_v(const Verts& Vt, uint32_t side, uint32_t k,
 }
 __device__ __forceinline__ void store_v(const Verts& Vt, uint32_t side, uint32_t k, uint32_t p, const PV& v) {
   size_t i = Vt.at(side, k, p);
-  uint32_t m = v.type | (v.delta << 2) | (v.id << 4);
+  uint32_t m = v.type | (v.delta << 2) | (v.emit << 3) | (v.id << 4);
   Vt.pf[i] = make_float4(v.p.x, v.p.y, v.p.z, v.pdf_fwd);
   Vt.nr[i] = make_float4(v.n.x, v.n.y, v.n.z, v.pdf_rev);
   Vt.bm[i] = make_float4(v.beta.x, v.beta.y, v.beta.z, __uint_as_float(m));
@@ -193,6 +201,27 @@ __device__ __forceinline__ float light_pdf_emission_dir(const DLight& L, f3 dir)
   return c > 0.0f ? c * kInvPi : 0.0f;
 }
 
+__device__ __forceinline__ bool has_light_dist(const ShadeView& v) {
+  return v.dist_mode != AR_LIGHTDIST_NONE && v.n_lights > 0;
+}
+
+// t = 0 (extension): a camera subpath vertex sv reaching the front face of
+// emitter material m records it, with the reverse pdfs of the light side
+// generating it -- the emitter point (light-origin pdf, area measure) in sv,
+// and, from there, the previous vertex (emission pdf toward it, converted to
+// area), whose stored pdf_rev is returned through prev_pdf_rev.  Without a
+// light distribution no light subpath exists and both stay 0.
+__device__ __forceinline__ void mark_emitter(const ShadeView& v, const DMaterial& m, const ShadingPoint& sp,
+                                             const Ray& ray, const PV& prev, PV& sv, float& prev_pdf_rev) {
+  prev_pdf_rev = -1.0f;  // unchanged
+  if (m.light_id < 0 || !(dot(sp.ng, -ray.d) > 0.0f)) return;
+  sv.emit = 1u;
+  if (!has_light_dist(v)) return;
+  const DLight& L = v.lights[m.light_id];
+  sv.pdf_rev = L.pmf / L.area;
+  prev_pdf_rev = pdf_sa_to_area(light_pdf_emission_dir(L, normalize(prev.p - sp.p)), sp.p, prev.p, prev.n);
+}
+
 // Shared subpath bounce loop (:368-434, :498-556).  Vertex 0 is already
 // stored; returns the vertex count.
 template <bool NEAR>
@@ -219,11 +248,21 @@ __device__ __forceinline__ uint32_t extend_subpath(const ShadeView& v, const Bdp
     sv.beta = beta;
     sv.pdf_fwd = pdf_sa_to_area(prev_pdf_dir, prev.p, sp.p, sp.n);
     sv.pdf_rev = 0.0f;
+    const DMaterial m = material_at(v, sp.mat, sp.tu, sp.tv);
+    if (side == 0) {
+      float pr;
+      mark_emitter(v, m, sp, ray, prev, sv, pr);
+      if (pr >= 0.0f) {
+        const size_t pi = Vt.at(side, nv - 1, p);
+        const float4 nr = Vt.nr[pi];
+        Vt.nr[pi] = make_float4(nr.x, nr.y, nr.z, pr);
+        prev.pdf_rev = pr;
+      }
+    }
     store_v(Vt, side, nv, p, sv);
     before = prev;
     prev = sv;
     nv++;
-    const DMaterial m = material_at(v, sp.mat, sp.tu, sp.tv);
     f3 wo_l = to_local(sp.onb, -ray.d);
     float u_sel = rng.next();
     float ux = rng.next(), uy = rng.next();
@@ -495,9 +534,18 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_sub_shade(ShadeView v, BdptParam
         sv.beta = beta;
         sv.pdf_fwd = pdf_sa_to_area(st.w, prev.p, sp.p, sp.n);
         sv.pdf_rev = 0.0f;
+        const DMaterial m = material_at(v, sp.mat, sp.tu, sp.tv);
+        if (side == 0) {
+          float pr;
+          mark_emitter(v, m, sp, ray, prev, sv, pr);
+          if (pr >= 0.0f) {
+            const size_t pi = Vt.at(side, nv - 1, p);
+            const float4 nr = Vt.nr[pi];
+            Vt.nr[pi] = make_float4(nr.x, nr.y, nr.z, pr);
+          }
+        }
         store_v(Vt, side, nv, p, sv);
         nv++;
-        const DMaterial m = material_at(v, sp.mat, sp.tu, sp.tv);
         f3 wo_l = to_local(sp.onb, -ray.d);
         float u_sel = rng.next();
         float ux = rng.next(), uy = rng.next();
@@ -560,7 +608,7 @@ __device__ __forceinline__ double powb(double x, float beta) {
 
 // :594-770
 __device__ __forceinline__ float mis_weight(const BdptParams& P, const Verts& Vt, uint32_t p, uint32_t nc, uint32_t nl,
-                                            uint32_t s, uint32_t t, float pdf_cc, float pdf_cl) {
+                                            uint32_t s, uint32_t t, float pdf_cc, float pdf_cl, bool area_light) {
   if (P.mis_mode == 1) {  // :605-636
     uint32_t len = s + t, count = 0;
     uint32_t tmx = len < nl ? len : nl;
@@ -591,15 +639,18 @@ __device__ __forceinline__ float mis_weight(const BdptParams& P, const Verts& Vt
     }
   }
   {
+    // with an area light the t' = 0 alternative counts (extension; the
+    // reference skips it "for delta lights", :735-742)
     double ri = 1.0;
     if (t >= 1) {
       PV v = load_v(Vt, 1, t - 1, p);
       ri *= (double)pdf_cl / (double)fmaxf(v.pdf_fwd, 1e-30f);
-      bool prev_delta = t >= 2 ? false : true;
+      bool prev_delta = t >= 2 ? false : !area_light;
       if (!prev_delta) sum += powb(fabs(ri), P.mis_beta);
     }
-    for (uint32_t i = t - 1; t >= 1 && i >= 1; --i) {
-      PV v = load_v(Vt, 1, i, p);
+    const int lo = area_light ? 0 : 1;
+    for (int i = (int)t - 1; i >= lo; --i) {
+      PV v = load_v(Vt, 1, (uint32_t)i, p);
       ri *= (double)fmaxf(v.pdf_rev, 0.0f) / (double)fmaxf(v.pdf_fwd, 1e-30f);
       sum += powb(fabs(ri), P.mis_beta);
     }
@@ -621,10 +672,13 @@ constexpr uint32_t kOwnPixel = 0xffffffffu;  // target of a contribution to the 
 
 // Every connection strategy of path p in the reference order (t outer, s
 // inner; :1281-1334) whose unweighted contribution is positive, as
-// emit(shadow ray, weighted contribution, target): the light-tracing
+// emit(shadow ray, weighted contribution, target, ray): the light-tracing
 // strategies (s = 1, :1051-1155) target the pixel the light vertex projects
 // to (a splat), the others (:844-1043) kOwnPixel.  The contribution counts
 // only if the shadow ray is unoccluded; the caller decides when to trace it.
+// The t = 0 strategy (extension) has no shadow ray (ray = false); only the
+// last camera vertex can lie on an emitter, so it is a path's first strategy
+// if it exists.
 template <class Emit>
 __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptParams& P, const Verts& Vt, uint32_t p,
                                                 Emit emit) {
@@ -632,9 +686,27 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
   const uint32_t W = P.pt.width, H = P.pt.height;
   uint2 cnt = Vt.count[p];
   uint32_t smax = cnt.x, tmax = cnt.y, limit = P.pt.max_depth + 2;
+  // the path's light end lies on an area light: a t' = 0 alternative exists
+  const bool area_light = tmax >= 1 && v.lights[load_v(Vt, 1, 0, p).id].type == AR_LIGHT_QUAD;
   for (uint32_t t = 0; t <= tmax; ++t) {
     for (uint32_t s = 0; s <= smax; ++s) {
       if (s + t < 2 || s + t > limit || s == 0) continue;
+      if (t == 0) {  // the camera subpath ends on an emitter (:810-818, extension)
+        if (s < 2) continue;
+        PV z = load_v(Vt, 0, s - 1, p);
+        if (!z.emit) continue;
+        const DMaterial mz = material_at(v, z.id, z.tu, z.tv);
+        f3 c = z.beta * mk(mz.emission[0], mz.emission[1], mz.emission[2]);
+        if (maxc(c) <= 0.0f) continue;
+        float pdf_cc = 0.0f;
+        if (has_light_dist(v)) {
+          const DLight& L = v.lights[mz.light_id];
+          pdf_cc = L.pmf / L.area;
+        }
+        float w = mis_weight(P, Vt, p, smax, tmax, s, 0, pdf_cc, 0.0f, true);
+        emit(Ray{}, c * w, kOwnPixel, false);
+        continue;
+      }
       if (s == 1) {  // light tracing to the camera (:1051-1155)
         if (t < 2) continue;
         PV y = load_v(Vt, 1, t - 1, p);
@@ -657,15 +729,14 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
         float pd = bsdf_pdf(m, wi_l, wo_l);
         PV c0 = load_v(Vt, 0, 0, p);
         float pdf_cl = pdf_sa_to_area(pd, y.p, cam.origin, c0.n);
-        float w = mis_weight(P, Vt, p, smax, tmax, 1, t, 0.0f, pdf_cl);
+        float w = mis_weight(P, Vt, p, smax, tmax, 1, t, 0.0f, pdf_cl, area_light);
         c = c * w;
         uint32_t ix = (uint32_t)cs.px, iy = (uint32_t)cs.py;
         if (ix > W - 1) ix = W - 1;
         if (iy > H - 1) iy = H - 1;
-        emit(sr, c, iy * W + ix);
+        emit(sr, c, iy * W + ix, true);
         continue;
       }
-      if (t == 0) continue;  // (:810-818)
       // general / NEE (:844-1043)
       PV z = load_v(Vt, 0, s - 1, p);
       PV y = load_v(Vt, 1, t - 1, p);
@@ -725,8 +796,8 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
       f3 wi_z2 = normalize(y.p - z.p);
       float pdz = bsdf_pdf(mz, to_local(onz, wi_z2), wo_z_l);
       float pdf_cl = pdf_sa_to_area(pdz, z.p, y.p, y.n);
-      float w = mis_weight(P, Vt, p, smax, tmax, s, t, pdf_cc, pdf_cl);
-      emit(sr, c * w, kOwnPixel);
+      float w = mis_weight(P, Vt, p, smax, tmax, s, t, pdf_cc, pdf_cl, area_light);
+      emit(sr, c * w, kOwnPixel, true);
     }
   }
 }
@@ -744,10 +815,12 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams 
   uint32_t nrays = 0;
   for (uint32_t p = gtid; p < Vt.npaths; p += nthreads) {
     f3 acc = splat(0.0f);
-    bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target) {
+    bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target, bool ray) {
       WorldHit hh;
-      nrays++;
-      if (trace_scene_wave<true, true>(v.sv, sr, hh, stk)) return;
+      if (ray) {
+        nrays++;
+        if (trace_scene_wave<true, true>(v.sv, sr, hh, stk)) return;
+      }
       if (target == kOwnPixel)
         acc = acc + c;
       else
@@ -788,20 +861,26 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, BdptParam
   const uint32_t rounds = (C.m + nthreads - 1) / nthreads;
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t i = r * nthreads + gtid;
-    uint32_t j = 0;
+    uint32_t j = 0, j0 = 0;  // slots used; leading slots without a shadow ray (t = 0)
     if (i < C.m) {
       const uint32_t p = C.p0 + i;
       const size_t base = (size_t)i * C.K;
-      bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target) {
+      bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target, bool ray) {
         const size_t slot = base + j++;
-        C.o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmin);
-        C.d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.tmax);
         C.val[slot] = make_float4(c.x, c.y, c.z, __uint_as_float(target));
+        if (ray) {
+          C.o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmin);
+          C.d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, sr.tmax);
+        } else {
+          C.occl[slot] = 0;  // counts unconditionally; not queued
+          j0 = j;
+        }
       });
       C.cnt[i] = j;
     }
-    nrays += j;
-    uint32_t x = j;  // inclusive scan of the wave's counts
+    const uint32_t nq = j - j0;  // slots j0 .. j-1 need their shadow ray traced
+    nrays += nq;
+    uint32_t x = nq;  // inclusive scan of the wave's counts
     for (int o = 1; o < 64; o <<= 1) {
       const uint32_t y = __shfl_up(x, o);
       if (lane >= (uint32_t)o) x += y;
@@ -809,8 +888,8 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, BdptParam
     const uint32_t total = __shfl(x, 63);
     uint32_t start = 0;
     if (lane == 63 && total) start = atomicAdd(C.qn, total);
-    start = __shfl(start, 63) + x - j;
-    for (uint32_t k = 0; k < j; ++k) C.queue[start + k] = i * C.K + k;
+    start = __shfl(start, 63) + x - nq;
+    for (uint32_t k = 0; k < nq; ++k) C.queue[start + k] = i * C.K + j0 + k;
   }
   wave_add_u64(&stats[1], nrays);
 }
@@ -904,19 +983,6 @@ __global__ void k_bdpt_zero_stats(unsigned long long* st) {
   if (threadIdx.x < 2) st[threadIdx.x] = 0ull;
 }
 
-// Slots per path for the wavefront connection: the (s, t) strategies that
-// can emit a shadow ray with at most V vertices per subpath (s, t >= 1,
-// s + t <= limit, s = 1 only with t >= 2).
-static uint32_t bdpt_conn_slots(uint32_t V, uint32_t limit) {
-  uint32_t K = 0;
-  for (uint32_t t = 1; t <= V; ++t)
-    for (uint32_t s = 1; s <= V; ++s)
-      if (s + t <= limit && !(s == 1 && t < 2)) ++K;
-  return K;
-}
-
-constexpr uint64_t kConnBudget = 12ull << 30;  // bytes of connection slots per pass
-
 class BdptJob : public JobRun {
  public:
   BdptParams P{};
@@ -988,12 +1054,12 @@ class BdptJob : public JobRun {
       const uint64_t e_t = need > ta.lds_stack ? need - ta.lds_stack : 0;
       spill_words = std::max<uint64_t>(spill_words, e_t * (uint64_t)Gs * ta.workgroup_size);
     }
-    ar_status e = ws->reserve_paths(cap, npix, (uint32_t)((spill_words + G * BLOCK - 1) / (G * BLOCK)), G * BLOCK);
+    ar_status e =
+        ws->reserve_paths(cap, npix, (uint32_t)((spill_words + G * BLOCK - 1) / (G * BLOCK)), G * BLOCK, stream);
     if (e != AR_OK) return e;
     size_t nvert = 2ull * P.V * cap;
-    C.K = bdpt_conn_slots(P.V, cfg->max_depth + 2);
-    const uint64_t per_path = (uint64_t)C.K * (3 * sizeof(float4) + 1 + sizeof(uint32_t)) + sizeof(uint32_t);
-    pass = wavefront ? (uint32_t)std::max<uint64_t>(64, std::min<uint64_t>(cap, kConnBudget / per_path)) : 0;
+    C.K = (uint32_t)bdpt_conn_slots(cfg->max_depth);
+    pass = wavefront ? (uint32_t)bdpt_conn_pass(cfg->max_depth, cap) : 0;
     const size_t slots = (size_t)pass * C.K;
     auto al = [](size_t n) { return (n + 255) / 256 * 256; };
     size_t need = al(nvert * 3 * sizeof(float4)) + al(nvert * sizeof(float2)) + al((size_t)cap * sizeof(uint2)) +

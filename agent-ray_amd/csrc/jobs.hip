@@ -29,7 +29,29 @@ ar_status validate_job(const ar_scene* s, const JobSpec& spec) {
   if (c.integrator == AR_INTEGRATOR_BDPT && c.max_depth > 64) return fail(AR_ERR_INVALID_ARG, "BDPT max_depth > 64");
   if (c.integrator == AR_INTEGRATOR_GDPT && !(c.alpha > 0.0f)) return fail(AR_ERR_INVALID_ARG, "GDPT alpha <= 0");
   if (c.mis_mode > 1) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown MIS mode");
+  // Paths in flight per chunk index 32-bit path ids and size the per-path
+  // buffers: bound them before anything is allocated or launched.
+  const uint64_t per_chunk = (uint64_t)spec.width * spec.height * chunk_samples(spec) *
+                             (c.integrator == AR_INTEGRATOR_GDPT ? 3u : 1u);
+  if (per_chunk > kMaxChunkPaths)
+    return fail(AR_ERR_INVALID_ARG, "sample_chunk too large: width * height * sample_chunk (x3 for GDPT) above 2^31");
   return AR_OK;
+}
+
+uint64_t bdpt_conn_slots(uint32_t max_depth) {
+  // one t = 0 strategy (the camera subpath ends on an emitter), then every
+  // (s, t) with s, t >= 1 that can emit a shadow ray
+  const uint32_t V = max_depth + 1, limit = max_depth + 2;
+  uint64_t K = 1;
+  for (uint32_t t = 1; t <= V; ++t)
+    for (uint32_t s = 1; s <= V; ++s)
+      if (s + t <= limit && !(s == 1 && t < 2)) ++K;
+  return K;
+}
+
+uint64_t bdpt_conn_pass(uint32_t max_depth, uint64_t cap) {
+  const uint64_t per_path = bdpt_conn_slots(max_depth) * kConnSlotBytes + sizeof(uint32_t);
+  return std::max<uint64_t>(64, std::min<uint64_t>(cap, kConnBudget / per_path));
 }
 
 size_t job_footprint(const ar_scene* s, const JobSpec& spec) {
@@ -39,7 +61,11 @@ size_t job_footprint(const ar_scene* s, const JobSpec& spec) {
   uint64_t paths = npix * k * (c.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
   size_t bytes = paths * (5 * 16 + 16 + 8 + 16 + 4 + 4 + 48) + npix * 48 + (64ull << 20);  // state, queues, film, spill
   if (c.integrator == AR_INTEGRATOR_GDPT) bytes += npix * 12 * 3 + npix * 12 * 9 + (64 << 10);
-  if (c.integrator == AR_INTEGRATOR_BDPT) bytes += paths * (2ull * (c.max_depth + 1) * 48 + 8) + npix * 24;
+  if (c.integrator == AR_INTEGRATOR_BDPT) {
+    bytes += paths * (2ull * (c.max_depth + 1) * 56 + 8) + npix * 24;  // vertices, counts, splat film
+    if (c.schedule != AR_SCHED_MEGAKERNEL)  // wavefront connection slots of one pass
+      bytes += bdpt_conn_pass(c.max_depth, paths) * (bdpt_conn_slots(c.max_depth) * kConnSlotBytes + 4) + 256;
+  }
   bytes += npix * 12 * 4;  // output staging
   (void)s;
   return bytes;

@@ -304,7 +304,8 @@ uint32_t stack_spill_entries(const ar_scene* s) {
   return need > (uint32_t)kTraceLdsStack ? need - kTraceLdsStack : 0;
 }
 
-ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_entries, uint32_t grid_threads) {
+ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_entries, uint32_t grid_threads,
+                                   hipStream_t st) {
   size_t f4 = sizeof(float4);
   size_t bytes = 0;
   auto add = [&](size_t& off, size_t n) {
@@ -334,7 +335,9 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
   size_t o_par;
   add(o_par, sizeof(PtParams));
   if (buf.bytes < bytes) {
-    AR_HIP(hipDeviceSynchronize());
+    // the buffer's last user is this stream (callers order other streams'
+    // uses before it): drain it, not the whole device
+    AR_HIP(hipStreamSynchronize(st));
     AR_HIP(buf.alloc(bytes));
     graph_valid = false;
   }
@@ -365,6 +368,23 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
 Workspace::~Workspace() {
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
+}
+
+void Workspace::trim() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  graph_exec = nullptr;
+  graph = nullptr;
+  graph_valid = false;
+  buf.release();
+  out.release();
+  aux.release();
+  pois.release();
+  B = WaveBufs{};
+  acc = nullptr;
+  spill = nullptr;
+  params = nullptr;
+  capacity = 0;
 }
 
 // FNV-1a over the fields of the scene views a captured graph holds by value.
@@ -577,7 +597,7 @@ class PtJob : public JobRun {
       spill_words = std::max<uint64_t>(spill_words, e_t * gt);
     }
     ar_status e = ws->reserve_paths(P.variants * npix * k, npix, (uint32_t)((spill_words + G * BLOCK - 1) / (G * BLOCK)),
-                                    G * BLOCK);
+                                    G * BLOCK, stream);
     if (e != AR_OK) return e;
     if (gdpt) {
       films = aux_dev;

@@ -55,6 +55,7 @@ struct Slot {
   float* pinned = nullptr;
   size_t pinned_bytes = 0;
   bool busy = false;
+  size_t footprint = 0;  // of the job holding the slot
 };
 
 struct Job {
@@ -92,7 +93,7 @@ struct ar_server {
   std::vector<std::shared_ptr<Job>> active;
   std::vector<std::unique_ptr<Slot>> slots;
   std::vector<hipEvent_t> event_pool;
-  size_t used_bytes = 0;
+  int waiters = 0;       // threads inside ar_job_wait (ar_server_destroy waits for them)
   bool broken = false;  // the worker thread ended on an exception
   bool stop = false;
   uint64_t next_id = 1;
@@ -120,9 +121,9 @@ struct ar_server {
   void release(Job& j) {  // caller holds mu
     if (j.slot >= 0) {
       slots[j.slot]->busy = false;
+      slots[j.slot]->footprint = 0;
       j.slot = -1;
     }
-    used_bytes -= std::min(used_bytes, j.footprint);
     j.run.reset();
   }
 
@@ -148,8 +149,21 @@ struct ar_server {
     cv_done.notify_all();
   }
 
-  // Admit queued jobs into free slots within the budget (caller holds mu).
-  void admit() {
+  // Device bytes the slots hold: a busy slot at least its job's footprint,
+  // an idle one what its workspace retains (workspaces grow and are kept for
+  // the next job).  Only the worker thread changes workspaces.
+  size_t committed_bytes() const {
+    size_t total = 0;
+    for (auto& sp : slots) total += sp->busy ? std::max(sp->ws.bytes(), sp->footprint) : sp->ws.bytes();
+    return total;
+  }
+
+  // Picks queued jobs for free slots within the budget (caller holds mu) and
+  // reserves the slot; the device work of starting them (JobRun::begin, which
+  // may allocate and synchronise the slot's stream) runs in start() without
+  // the lock.  An idle slot's retained workspace is freed when the budget
+  // needs its bytes.
+  void admit(std::vector<std::shared_ptr<Job>>& starting) {
     while (!pending.empty()) {
       auto it = jobs.find(pending.front());
       if (it == jobs.end() || it->second->state != AR_JOB_QUEUED) {
@@ -163,34 +177,69 @@ struct ar_server {
         cv_done.notify_all();
         continue;
       }
-      if (used_bytes + j.footprint > cfg.hbm_budget_bytes) break;
       int free_slot = -1;
-      for (size_t s = 0; s < slots.size(); ++s)
-        if (!slots[s]->busy) {
-          free_slot = (int)s;
-          break;
+      for (int pass = 0; pass < 2 && free_slot < 0; ++pass) {
+        const size_t committed = committed_bytes();
+        for (size_t s = 0; s < slots.size(); ++s) {
+          const Slot& sl = *slots[s];
+          if (sl.busy) continue;
+          const size_t held = sl.ws.bytes();
+          if (committed - held + std::max(held, j.footprint) <= cfg.hbm_budget_bytes) {
+            free_slot = (int)s;
+            break;
+          }
         }
+        if (free_slot >= 0 || pass == 1) break;
+        // no free slot fits as things stand: free the idle workspaces (their
+        // streams have drained: a slot is idle once its job completed)
+        bool trimmed = false;
+        for (auto& sp : slots)
+          if (!sp->busy && sp->ws.bytes() > 0) {
+            (void)hipStreamSynchronize(sp->stream);
+            sp->ws.trim();
+            trimmed = true;
+          }
+        if (!trimmed) break;
+      }
       if (free_slot < 0) break;
       pending.pop_front();
       Slot& sl = *slots[free_slot];
-      size_t npix = (size_t)j.spec.width * j.spec.height;
-      size_t out_bytes = npix * 3 * sizeof(float);
-      if (sl.ws.out.bytes < out_bytes && sl.ws.out.alloc(out_bytes) != hipSuccess) {
-        fail_job(j, AR_ERR_DEVICE, "output buffer allocation failed");
-        cv_done.notify_all();
-        continue;
+      sl.busy = true;
+      sl.footprint = j.footprint;
+      j.slot = free_slot;
+      j.state = AR_JOB_RUNNING;
+      j.t_start = now();
+      starting.push_back(it->second);
+    }
+  }
+
+  // Device-side start of an admitted job (worker thread, mu not held).
+  void start(const std::shared_ptr<Job>& jp) {
+    Job& j = *jp;
+    Slot& sl = *slots[j.slot];
+    const size_t npix = (size_t)j.spec.width * j.spec.height;
+    const size_t out_bytes = npix * 3 * sizeof(float);
+    ar_status e = AR_OK;
+    std::string msg;
+    if (sl.ws.out.bytes < out_bytes) {
+      (void)hipStreamSynchronize(sl.stream);
+      if (sl.ws.out.alloc(out_bytes) != hipSuccess) {
+        e = AR_ERR_DEVICE;
+        msg = "output buffer allocation failed";
       }
-      if (sl.pinned_bytes < out_bytes) {
-        if (sl.pinned) (void)hipHostFree(sl.pinned);
-        sl.pinned = nullptr;
-        sl.pinned_bytes = 0;
-        if (hipHostMalloc((void**)&sl.pinned, out_bytes, hipHostMallocDefault) != hipSuccess) {
-          fail_job(j, AR_ERR_DEVICE, "pinned buffer allocation failed");
-          cv_done.notify_all();
-          continue;
-        }
+    }
+    if (e == AR_OK && sl.pinned_bytes < out_bytes) {
+      if (sl.pinned) (void)hipHostFree(sl.pinned);
+      sl.pinned = nullptr;
+      sl.pinned_bytes = 0;
+      if (hipHostMalloc((void**)&sl.pinned, out_bytes, hipHostMallocDefault) != hipSuccess) {
+        e = AR_ERR_DEVICE;
+        msg = "pinned buffer allocation failed";
+      } else {
         sl.pinned_bytes = out_bytes;
       }
+    }
+    if (e == AR_OK) {
       if (j.spec.cfg.sample_chunk == 0) {
         // a job of up to two chunks' work runs as one chunk; larger jobs are
         // split so that the device never holds much of them at a time
@@ -206,21 +255,18 @@ struct ar_server {
       j.work_left = w * j.spec.cfg.spp;
       j.work_total = j.work_left;
       j.work_chunk = w * chunk_samples(j.spec);
-      sl.busy = true;
-      j.slot = free_slot;
-      used_bytes += j.footprint;
-      j.state = AR_JOB_RUNNING;
-      j.t_start = now();
       j.run = make_job(scene, j.spec, &sl.ws, sl.stream, sl.ws.out.as<float>(), nullptr);
-      ar_status e = j.run->begin();
-      if (e != AR_OK) {
-        fail_job(j, e, std::string("job begin: ") + ar_last_error());
-        release(j);
-        cv_done.notify_all();
-        continue;
-      }
-      active.push_back(it->second);
+      e = j.run->begin();
+      if (e != AR_OK) msg = std::string("job begin: ") + ar_last_error();
     }
+    std::lock_guard<std::mutex> lk(mu);
+    if (e != AR_OK) {
+      fail_job(j, e, msg);
+      release(j);
+      cv_done.notify_all();
+      return;
+    }
+    active.push_back(jp);
   }
 
   // One pass: retire finished chunks and complete drained jobs, then issue
@@ -342,17 +388,20 @@ struct ar_server {
 
   void loop() {
     (void)hipSetDevice(device);
+    std::vector<std::shared_ptr<Job>> starting;
     for (;;) {
+      starting.clear();
       {
         std::unique_lock<std::mutex> lk(mu);
         if (stop && active.empty() && pending.empty()) break;
-        admit();
-        if (active.empty()) {
+        admit(starting);
+        if (active.empty() && starting.empty()) {
           if (stop) continue;
           cv_work.wait_for(lk, std::chrono::milliseconds(5));
           continue;
         }
       }
+      for (auto& jp : starting) start(jp);
       if (!pump()) std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
   }
@@ -414,9 +463,17 @@ ar_status ar_server_destroy(ar_server* srv) {
     srv->pending.clear();
     for (auto& j : srv->active) j->cancel = true;
     srv->stop = true;
+    srv->cv_done.notify_all();
   }
   srv->cv_work.notify_all();
   srv->worker.join();
+  {
+    // every job is terminal now; let threads still in ar_job_wait return
+    // before the condition variable goes away
+    std::unique_lock<std::mutex> lk(srv->mu);
+    srv->cv_done.notify_all();
+    srv->cv_done.wait(lk, [srv]() { return srv->waiters == 0; });
+  }
   for (auto& sp : srv->slots) {
     Slot& sl = *sp;
     if (sl.stream) (void)hipStreamSynchronize(sl.stream);
@@ -472,10 +529,12 @@ ar_status ar_job_wait(ar_server* srv, uint64_t id, float* out_rgb, ar_job_stats*
     auto it = srv->jobs.find(id);
     if (it == srv->jobs.end()) return fail(AR_ERR_NOT_FOUND, "unknown job id");
     j = it->second;
+    srv->waiters++;
     srv->cv_done.wait(lk, [&]() {
       return j->state == AR_JOB_DONE || j->state == AR_JOB_CANCELLED || j->state == AR_JOB_FAILED;
     });
     srv->jobs.erase(id);
+    if (--srv->waiters == 0) srv->cv_done.notify_all();
   }
   if (stats) {
     stats->submit_time = j->t_submit;

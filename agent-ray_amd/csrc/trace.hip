@@ -84,6 +84,33 @@ static void launch_variant(const ar_scene* s, uint32_t G, const ar_ray* rays, ui
     launch_slots<ANY, NEAR, BLOCK, S, 3>(s, G, rays, n, hits, inst, occ, spill, st, c);
 }
 
+// Launches over n rays in pieces of < 2^31 (32-bit ray positions inside a
+// wave) with grid G and the given spill area; false for an unsupported
+// variant.
+template <bool ANY, bool NEAR>
+static bool launch_pieces(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst,
+                          uint32_t* occ, int2* spill, hipStream_t st, const ar_trace_config& c) {
+  bool ok = false;
+  const uint64_t kMaxLaunch = 1ull << 31;
+  for (uint64_t off = 0; off < n; off += kMaxLaunch) {
+    const uint64_t m = n - off < kMaxLaunch ? n - off : kMaxLaunch;
+    const ar_ray* r = rays + off;
+    ar_hit* h = hits ? hits + off : nullptr;
+    uint32_t* in = inst ? inst + off : nullptr;
+    uint32_t* oc = occ ? occ + off : nullptr;
+    ok = false;
+#define AR_LAUNCH(B, SS)                                                           \
+  if (c.workgroup_size == B && c.lds_stack == SS) {                                \
+    launch_variant<ANY, NEAR, B, SS>(s, G, r, m, h, in, oc, spill, st, c);            \
+    ok = true;                                                                     \
+  }
+    AR_TRACE_VARIANTS(AR_LAUNCH)
+#undef AR_LAUNCH
+    if (!ok) break;
+  }
+  return ok;
+}
+
 template <bool ANY, bool NEAR>
 static ar_status launch_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst,
                               uint32_t* occ, hipStream_t st, const ar_trace_config& c) {
@@ -94,29 +121,30 @@ static ar_status launch_trace(const ar_scene* s, const ar_ray* rays, uint64_t n,
   ar_scene* ms = const_cast<ar_scene*>(s);
   ar_status e = scratch_acquire(ms, bytes, st, (void**)&spill);
   if (e != AR_OK) return e;
-  bool ok = false;
-  // one launch handles < 2^32 rays (32-bit ray positions inside a wave)
-  const uint64_t kMaxLaunch = 1ull << 31;
-  for (uint64_t off = 0; off < n; off += kMaxLaunch) {
-    const uint64_t m = n - off < kMaxLaunch ? n - off : kMaxLaunch;
-    const ar_ray* r = rays + off;
-    ar_hit* h = hits ? hits + off : nullptr;
-    uint32_t* in = inst ? inst + off : nullptr;
-    uint32_t* oc = occ ? occ + off : nullptr;
-#define AR_LAUNCH(B, SS)                                                           \
-  if (c.workgroup_size == B && c.lds_stack == SS) {                                \
-    launch_variant<ANY, NEAR, B, SS>(s, G, r, m, h, in, oc, spill, st, c);            \
-    ok = true;                                                                     \
-  }
-    AR_TRACE_VARIANTS(AR_LAUNCH)
-#undef AR_LAUNCH
-    if (!ok) break;
-  }
+  const bool ok = launch_pieces<ANY, NEAR>(s, G, rays, n, hits, inst, occ, spill, st, c);
   hipError_t le = hipGetLastError();
   scratch_release(ms, st);
   if (!ok) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
   if (le != hipSuccess) return fail(AR_ERR_DEVICE, std::string("k_trace launch: ") + hipGetErrorString(le));
   return AR_OK;
+}
+
+// The grid a launch of ar_trace / ar_occluded uses for n rays under c (its
+// kernel variant's resident-block clamp), 0 for an unsupported variant.
+uint32_t trace_launch_grid(uint64_t n, const ar_trace_config& c, bool any, bool near) {
+  const uint32_t G = trace_grid_cfg(n, c);
+  uint32_t out = 0;
+#define AR_GRID_OF(B, SS)                                                                                     \
+  if (c.workgroup_size == B && c.lds_stack == SS) {                                                          \
+    const void* k3 = any ? (near ? (const void*)&k_trace<true, true, SS, B, 3> : (const void*)&k_trace<true, false, SS, B, 3>) \
+                         : (near ? (const void*)&k_trace<false, true, SS, B, 3> : (const void*)&k_trace<false, false, SS, B, 3>); \
+    const void* k4 = any ? (near ? (const void*)&k_trace<true, true, SS, B, 4> : (const void*)&k_trace<true, false, SS, B, 4>) \
+                         : (near ? (const void*)&k_trace<false, true, SS, B, 4> : (const void*)&k_trace<false, false, SS, B, 4>); \
+    out = clamp_resident(G, c.park_slots == 4 ? k4 : k3, B);                                                 \
+  }
+  AR_TRACE_VARIANTS(AR_GRID_OF)
+#undef AR_GRID_OF
+  return out;
 }
 
 bool trace_variant_supported(uint32_t block, uint32_t stack) {

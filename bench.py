@@ -11,14 +11,13 @@ megakernel and wavefront schedules.  --workload serving: Poisson job arrivals
 against the resident room scene (p50/p99 latency).
 
 Roofline: the dominant kernel is the closest-hit traversal.  Its algorithmic
-bytes per ray are B_ray = 32 (ray) + 8 (hit) + 32 * N_node + 36 * N_tri with
-N_node / N_tri the oracle's mean box tests and triangle tests per ray
-(reference left-first traversal) measured on the CPU-baseline sample of the
-same rays; achieved = B_ray * rays / average kernel duration (HIP events on
-the stream the kernel runs on).  Node and triangle bytes re-read from cache
-count in B_ray, so achieved can exceed the HBM peak; `traffic` is the HBM
-traffic of one launch measured by two rocprofv3 --pmc child runs
-(FETCH_SIZE, WRITE_SIZE) calibrated on a 1 GiB copy.  CPU baseline: the
+bytes per ray are B_ray = 32 (ray) + 8 (hit) + 64 N_node + 48 N_tri + 96 N_inst
++ 8 (N_spill_w + N_spill_r), the records the timed kernel reads and writes,
+counted per ray by the counting build of the same kernel (ar_trace_count,
+same grid and launch configuration, same rays); achieved = B_ray * rays /
+average kernel duration (HIP events on the stream the kernel runs on).
+`traffic` is the HBM traffic of one launch from two rocprofv3 --pmc child
+runs (L2-to-memory read and write requests by size).  CPU baseline: the
 oracle (C++ restatement of the reference CPU path) on the host cores, on a
 bounded sample of the rays.
 """
@@ -233,28 +232,43 @@ def bench_traversal(args, ar, torch):
         e1.record(stream)
         torch.cuda.synchronize()
         out[f"closest_{name}_mrays_s"] = round(sub.shape[0] / (e0.elapsed_time(e1) / 1e3) / 1e6, 1)
+    # Roofline of the dominant kernel (closest hit).  Algorithmic bytes per
+    # ray from the timed kernel's own work: the counting build of the same
+    # kernel, same grid and launch configuration, over the same rays.
+    cnt = sc.trace_counts(dr)
+    per = {k: cnt[k] / n_rays for k in ("node_visits", "tri_tests", "inst_tests", "spill_writes", "spill_reads")}
+    b_ray = (32 + 8 + 64 * per["node_visits"] + 48 * per["tri_tests"] + 96 * per["inst_tests"]
+             + 8 * (per["spill_writes"] + per["spill_reads"]))
+    kernel_s = float(np.mean(t_closest)) / 1e3
+    achieved = b_ray * n_rays / kernel_s / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                       "kernel": "k_trace<closest, near-first>", "kernel_ms": round(float(np.mean(t_closest)), 3),
+                       "bytes_per_ray": round(b_ray, 1),
+                       "per_ray": {k: round(v, 3) for k, v in per.items()},
+                       "basis": "32 B ray + 8 B hit + 64 B per node record + 48 B per triangle record + 96 B per "
+                                "instance record + 8 B per spilled stack entry written or read, counts of the "
+                                "timed launch (ar_trace_count)"}
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle) on a ray sample")
         rate, nn, nt, threads, sample = traversal_cpu_baseline(desc, rays)
         out["cpu_baseline"] = {"value": round(rate, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
                                "sample": sample}
-        b_ray = 32 + 8 + 32 * nn + 36 * nt
-        achieved = b_ray * n_rays / (np.mean(t_closest) / 1e3) / 1e9
-        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                           "kernel": "k_trace<closest, near-first>",
-                           "bytes_per_ray": round(b_ray, 1), "n_node": round(nn, 2), "n_tri": round(nt, 2),
-                           "kernel_ms": round(float(np.mean(t_closest)), 3)}
-        if not args.no_traffic:
-            c = sc.trace_config()
-            cfg = ",".join(str(v) for v in ar.config_key(c))
-            traffic, detail = measure_traffic(cfg)
-            out["roofline"]["traffic"] = None if traffic is None else round(traffic)
-            out["roofline"]["traffic_detail"] = detail
-            if traffic is not None:
-                dram = traffic / (np.mean(t_closest) / 1e3) / 1e9
-                out["roofline"]["traffic_gbs"] = round(dram, 1)
-                out["roofline"]["traffic_frac"] = round(dram / HBM_PEAK_GBS, 4)
+        # the reference's own traversal (binary tree, left child first, box
+        # test at pop) on the same sample, in the reference's record sizes
+        out["roofline"]["reference_order"] = {
+            "bytes_per_ray": round(32 + 8 + 32 * nn + 36 * nt, 1), "n_node": round(nn, 2), "n_tri": round(nt, 2),
+            "note": "oracle's reference-order visits; not the timed kernel's work"}
+    if not args.no_traffic:
+        c = sc.trace_config()
+        cfg = ",".join(str(v) for v in ar.config_key(c))
+        traffic, detail = measure_traffic(cfg)
+        out["roofline"]["traffic"] = None if traffic is None else round(traffic)
+        out["roofline"]["traffic_detail"] = detail
+        if traffic is not None:
+            dram = traffic / kernel_s / 1e9
+            out["roofline"]["traffic_gbs"] = round(dram, 1)
+            out["roofline"]["traffic_frac"] = round(dram / HBM_PEAK_GBS, 4)
     return out
 
 
@@ -303,11 +317,25 @@ def pmc_probe(args, ar, torch):
     print(json.dumps({"probe": "ok", "copy_bytes": 4 * n}), flush=True)
 
 
+READ_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum")
+WRITE_COUNTERS = ("TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum")
+
+
+def _dram_bytes(v):
+    """Bytes moved between L2 and memory from request counts by size: reads
+    32 / 64 / 128 B (a request of none of those sizes, if any, as 64 B),
+    writes 64 B or else 32 B."""
+    r32, r64, r128, rall = (v.get(k, 0.0) for k in READ_COUNTERS)
+    rest = max(rall - r32 - r64 - r128, 0.0)
+    w, w64 = (v.get(k, 0.0) for k in WRITE_COUNTERS)
+    return 32 * r32 + 64 * r64 + 128 * r128 + 64 * rest, 32 * (w - w64) + 64 * w64
+
+
 def measure_traffic(cfg: str, kernel_key: str = "k_trace<false, true"):
     """HBM bytes of one closest-hit launch from two rocprofv3 --pmc child runs
-    of the probe (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch).  Each
-    counter is scaled by what it reads for the probe's 1 GiB copy, whose true
-    traffic is 2^30 bytes read and 2^30 written.  Returns (bytes, detail) or
+    of the probe (read requests by size, then write requests by size).  The
+    same counters on the probe's 1 GiB copy (2^30 bytes read, 2^30 written)
+    check the conversion; no scale is applied.  Returns (bytes, detail) or
     (None, reason)."""
     import csv
     import shutil
@@ -317,40 +345,36 @@ def measure_traffic(cfg: str, kernel_key: str = "k_trace<false, true"):
     if not os.path.exists(prof):
         return None, "rocprofv3 not found"
     env = dict(os.environ, TMPDIR="/tmp")
-    out = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    trav, copy = {}, {}
+    for counters in (READ_COUNTERS, WRITE_COUNTERS):
         d = tempfile.mkdtemp(prefix="ar_pmc_", dir="/tmp")
-        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "probe", "--",
+        cmd = [prof, "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "probe", "--",
                os.path.realpath(sys.executable), os.path.join(ROOT, "bench.py"), "--workload", "pmc-probe",
                "--trace-config", cfg]
-        log(f"traffic: rocprofv3 --pmc {counter}")
+        log(f"traffic: rocprofv3 --pmc {' '.join(counters)}")
         try:
             r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                                timeout=240)
         except subprocess.TimeoutExpired:
-            return None, f"rocprofv3 --pmc {counter} timed out"
+            return None, f"rocprofv3 --pmc {counters[0]} timed out"
         files = [os.path.join(dp, f) for dp, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
         if not files:
-            return None, f"rocprofv3 --pmc {counter}: no counter file (rc {r.returncode})"
-        trav, copy = None, None
+            return None, f"rocprofv3 --pmc {counters[0]}: no counter file (rc {r.returncode})"
         for row in csv.DictReader(open(files[0])):
-            v = float(row["Counter_Value"])
-            name = row["Kernel_Name"]
+            v, name, cn = float(row["Counter_Value"]), row["Kernel_Name"], row["Counter_Name"]
             if kernel_key in name:
-                trav = v
+                trav[cn] = trav.get(cn, 0.0) + v
             elif "copyBuffer" in name or "copy_kernel" in name:
-                copy = v if copy is None else max(copy, v)
-        if trav is None or not copy:
-            return None, f"rocprofv3 --pmc {counter}: kernels not found"
-        out[counter] = (trav, copy)
-    f_fetch = (1 << 30) / (out["FETCH_SIZE"][1] * 1024.0)
-    f_write = (1 << 30) / (out["WRITE_SIZE"][1] * 1024.0)
-    fetch_b = out["FETCH_SIZE"][0] * 1024.0 * f_fetch
-    write_b = out["WRITE_SIZE"][0] * 1024.0 * f_write
-    return fetch_b + write_b, {"fetch_bytes": round(fetch_b), "write_bytes": round(write_b),
-                               "fetch_scale": round(f_fetch, 4), "write_scale": round(f_write, 4),
-                               "source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on one closest-hit launch, "
-                                         "scaled by the same counters on a 1 GiB copy"}
+                copy[cn] = max(copy.get(cn, 0.0), v)
+        if not any(k in trav for k in counters) or not any(k in copy for k in counters):
+            return None, f"rocprofv3 --pmc {counters[0]}: kernels not found"
+    rd, wr = _dram_bytes(trav)
+    crd, cwr = _dram_bytes(copy)
+    return rd + wr, {"read_bytes": round(rd), "write_bytes": round(wr),
+                     "copy_check": {"read_bytes": round(crd), "write_bytes": round(cwr), "true_bytes": 1 << 30},
+                     "counters": {k: trav.get(k) for k in READ_COUNTERS + WRITE_COUNTERS},
+                     "source": "rocprofv3 --pmc TCC_EA0_RDREQ_{32B,64B,128B} / TCC_EA0_WRREQ{,_64B} on one "
+                               "closest-hit launch, bytes by request size"}
 
 
 def C_void(p):

@@ -44,6 +44,7 @@ typedef int32_t ar_status;
 #define AR_REF_ORDER 0x4u    /* traverse left child first (reference DFS order) */
 #define AR_TUNE_CLOSEST 0x10u /* ar_scene_autotune: tune the closest-hit launch only */
 #define AR_TUNE_ANYHIT 0x20u  /* ar_scene_autotune: tune the any-hit launch only */
+#define AR_COUNT_ANYHIT 0x40u /* ar_trace_count: count the any-hit (ar_occluded) launch */
 
 /* ---- value types ------------------------------------------------------ */
 
@@ -199,7 +200,7 @@ ar_status ar_bvh_build_device(const float* boxes, uint32_t n, float* out_boxes, 
 /* ---- launch configuration and autotuning (no reference counterpart: the
  * reference runs on rayon's CPU pool) ---------------------------------------
  * Traversal kernels come in (workgroup size, LDS stack depth) variants
- * (64/8, 64/16, 128/8, 128/16, 256/8) and a persistent grid of blocks_per_cu
+ * (64/8, 64/16, 128/8, 128/16, 256/8, 256/12) and a persistent grid of blocks_per_cu
  * workgroups per CU.  The autotuner times the candidates on a ray sample
  * (device pointers) and keeps the fastest for this scene; ar_trace,
  * ar_occluded and the wavefront stages then use it. */
@@ -229,6 +230,25 @@ ar_status ar_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hi
 /* Any hit (boolean equal to the reference's full closest-hit is_some()). */
 ar_status ar_occluded(const ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t* occluded,
                       uint32_t flags, void* stream);
+
+/* Diagnostic, no reference counterpart: the traversal work of one ar_trace
+ * (or, with AR_COUNT_ANYHIT, ar_occluded) launch over the same rays, counted
+ * by an instrumented build of the same kernel run with the same grid and
+ * launch configuration.  Device pointers only; the rays are traced once more
+ * and the results discarded.  Record sizes: node 64 B, triangle 48 B,
+ * instance 96 B, spilled stack entry 8 B. */
+typedef struct ar_trace_counts {
+  uint64_t rays;
+  uint64_t node_visits;  /* 4-wide or binary node records read */
+  uint64_t tri_tests;    /* triangle records read and tested */
+  uint64_t inst_tests;   /* instance records read at TLAS leaves */
+  uint64_t spill_writes; /* stack entries pushed beyond the LDS stack */
+  uint64_t spill_reads;  /* spilled stack entries popped */
+  uint32_t grid;         /* workgroups of the launch */
+  uint32_t workgroup_size;
+} ar_trace_counts;
+ar_status ar_trace_count(const ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t flags, void* stream,
+                         ar_trace_counts* out);
 
 /* ---- camera (cameras.rs:91-149) ----------------------------------------- */
 ar_status ar_camera_lookat(const float lookfrom[3], const float lookat[3], const float vup[3], float vfov_deg,

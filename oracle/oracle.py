@@ -51,6 +51,12 @@ def lib():
             "orc_render_gdpt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, u64, C.c_int, P, P, P]),
             "orc_poisson_sor": (d, [P, P, P, u32, u32, f, u32, f, C.c_int, P]),
             "orc_render_bdpt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, f, C.c_int, u64, C.c_int, P, P]),
+            "orc_render_bdpt_ex": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, f, C.c_int, u64, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, P, P]),
+            "orc_light_dist": (u32, [P, P, P]),
+            "orc_light_sample_index": (u32, [P, f, P]),
+            "orc_render_pt_ex": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, u64, C.c_int, u32, u32, C.c_int, u32,
+                                     P, P, P]),
             "orc_film_rgb8": (None, [P, u64, u32, f, f, f, P]),
             "orc_scene_add_texture": (C.c_int, [P, P, u32, u32]),
             "orc_scene_set_material_texture": (None, [P, u32, C.c_int]),
@@ -111,6 +117,19 @@ class OracleScene:
             L.orc_scene_add_light(self.h, l.type, _p(p))
         L.orc_scene_commit(self.h, desc.light_dist)
 
+    def light_dist(self):
+        """(pmf, cdf) of the scene's light selection (lights.rs:115-235)."""
+        n = lib().orc_light_dist(self.h, None, None)
+        pmf, cdf = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        lib().orc_light_dist(self.h, _p(pmf), _p(cdf))
+        return pmf, cdf
+
+    def light_sample_index(self, u):
+        """LightDistribution::sample_index(u) -> (index, pmf)."""
+        p = np.zeros(1, np.float32)
+        i = lib().orc_light_sample_index(self.h, float(u), _p(p))
+        return int(i), float(p[0])
+
     def texture_sample(self, tex, uv):
         """ImageTexture::sample (texture.rs:97-122) at (n, 2) uv -> (n, 3)."""
         uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
@@ -164,14 +183,17 @@ class OracleScene:
         return occ.astype(bool), cnt, secs
 
     def render_pt(self, cam16, w, h, spp, max_depth=8, rr_depth=3, rng_mode=1, seed=0, nthreads=0,
-                  rows=None, want_m2=False):
+                  rows=None, want_m2=False, pt_mode=0, only_len=0):
+        """PathTracer::render.  pt_mode / only_len are test-only estimator
+        switches: 0 MIS, 1 light sampling only, 2 BSDF hits only; only_len > 0
+        keeps paths of exactly that many vertices (camera and light included)."""
         mean = np.zeros((h, w, 3), np.float32)
         m2 = np.zeros((h, w, 3), np.float32) if want_m2 else None
         cnt = np.zeros(2, np.uint64)
         r0, r1 = (0, h) if rows is None else rows
         cam = np.ascontiguousarray(cam16, np.float32)
-        secs = lib().orc_render_pt(self.h, _p(cam), w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads, r0, r1,
-                                   _p(mean), _p(m2), _p(cnt))
+        secs = lib().orc_render_pt_ex(self.h, _p(cam), w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads, r0,
+                                      r1, pt_mode, only_len, _p(mean), _p(m2), _p(cnt))
         return mean, m2, cnt, secs
 
 
@@ -185,13 +207,17 @@ def render_gdpt(scene, cam16, w, h, spp, max_depth=8, rr_depth=3, rng_mode=1, se
 
 
 def render_bdpt(scene, cam16, w, h, spp, max_depth=8, rr_depth=3, mis_mode=0, mis_beta=2.0, rng_mode=1, seed=0,
-                nthreads=0):
-    """BidirectionalPathTracer::render: (film, splat film, seconds)."""
+                nthreads=0, strategy=None, unweighted=False):
+    """BidirectionalPathTracer::render: (film, splat film, seconds).
+    mis_mode 0 power heuristic (reference walk), 1 uniform, 2 textbook walk
+    (test only).  strategy=(s, t) evaluates that strategy alone and
+    unweighted=True sets every MIS weight to 1 (test only)."""
     out = np.zeros((h, w, 3), np.float32)
     spl = np.zeros((h, w, 3), np.float32)
     cam = np.ascontiguousarray(cam16, np.float32)
-    secs = lib().orc_render_bdpt(scene.h, _p(cam), w, h, spp, max_depth, rr_depth, mis_mode, mis_beta, rng_mode,
-                                 seed, nthreads, _p(out), _p(spl))
+    only_s, only_t = (-1, -1) if strategy is None else strategy
+    secs = lib().orc_render_bdpt_ex(scene.h, _p(cam), w, h, spp, max_depth, rr_depth, mis_mode, mis_beta, rng_mode,
+                                    seed, nthreads, only_s, only_t, int(unweighted), _p(out), _p(spl))
     return out, spl, secs
 
 

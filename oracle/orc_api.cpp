@@ -161,6 +161,29 @@ uint32_t orc_bvh_dump(void* s, int obj, float* boxes, uint32_t* meta, uint32_t* 
 // rays: 8 floats each (o.xyz, tmin, d.xyz, tmax), directions as given.
 // Closest hit: t, prim (UINT32_MAX on miss), inst.  counters[0..1] += node
 // visits and triangle tests.  Returns wall seconds.
+// Light selection (lights.rs:115-235) of a committed scene: pmf and cdf
+// (n entries each, NULL to skip); returns the light count (0 without a
+// distribution).
+uint32_t orc_light_dist(void* s, float* pmf, float* cdf) {
+  const Scene* sc = (const Scene*)s;
+  if (!sc->has_dist) return 0;
+  const uint32_t n = (uint32_t)sc->dist.pmf.size();
+  for (uint32_t i = 0; i < n; ++i) {
+    if (pmf) pmf[i] = sc->dist.pmf[i];
+    if (cdf) cdf[i] = sc->dist.cdf[i];
+  }
+  return n;
+}
+
+// LightDistribution::sample_index (lights.rs:147-150, 219-226).
+uint32_t orc_light_sample_index(void* s, float u, float* pmf) {
+  uint32_t idx = 0;
+  float p = 0.0f;
+  ((const Scene*)s)->dist.sample_index(u, &idx, &p);
+  if (pmf) *pmf = p;
+  return idx;
+}
+
 double orc_trace(void* s, const float* rays, uint64_t n, float* t, uint32_t* prim, uint32_t* inst,
                  uint64_t* counters, int nthreads) {
   Scene* sc = (Scene*)s;
@@ -308,16 +331,19 @@ int orc_bsdf_sample(uint32_t type, const float* p, const float* wo, float u_sel,
 // pixel (RGB), out_m2 (optional) = mean of squared per-sample radiance.
 // rng_mode 0: per-pixel reference stream; 1: counter sampler (seed keyed).
 // Returns wall seconds; counters: node visits, tri tests, rays traced.
-double orc_render_pt(void* s, const float* cam16, uint32_t w, uint32_t h, uint32_t spp,
-                     uint32_t max_depth, uint32_t rr_depth, int rng_mode, uint64_t seed, int nthreads,
-                     uint32_t row_begin, uint32_t row_end, float* out_mean, float* out_m2,
-                     uint64_t* counters) {
+// pt_mode / only_len: test-only estimator switches (PtConfig).
+double orc_render_pt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, uint32_t spp,
+                        uint32_t max_depth, uint32_t rr_depth, int rng_mode, uint64_t seed, int nthreads,
+                        uint32_t row_begin, uint32_t row_end, int pt_mode, uint32_t only_len, float* out_mean,
+                        float* out_m2, uint64_t* counters) {
   Scene* sc = (Scene*)s;
   Camera cam = cam_from(cam16);
   PtConfig cfg;
   cfg.spp = spp;
   cfg.max_depth = max_depth;
   cfg.rr_depth = rr_depth;
+  cfg.pt_mode = pt_mode;
+  cfg.only_len = only_len;
   if (row_end == 0 || row_end > h) row_end = h;
   auto t0 = std::chrono::steady_clock::now();
   std::atomic<uint64_t> cn(0), ct(0);
@@ -353,6 +379,13 @@ double orc_render_pt(void* s, const float* cam16, uint32_t w, uint32_t h, uint32
     counters[1] += ct.load();
   }
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+double orc_render_pt(void* s, const float* cam16, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth,
+                     uint32_t rr_depth, int rng_mode, uint64_t seed, int nthreads, uint32_t row_begin,
+                     uint32_t row_end, float* out_mean, float* out_m2, uint64_t* counters) {
+  return orc_render_pt_ex(s, cam16, w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads, row_begin, row_end, 0,
+                          0, out_mean, out_m2, counters);
 }
 
 }  // extern "C"

@@ -616,10 +616,12 @@ V3 pt_li(const Scene& sc, Ray ray, Sampler& smp, const PtConfig& cfg, TraceCount
     // Emission (extension): camera hit adds Le; BSDF-sampled hit adds Le with
     // the power-heuristic weight against light sampling.  Emitters are black.
     if (hit.mat->emissive()) {
-      if (dot(hit.ng_world, wo) > 0.0f) {
-        if (depth == 0 || !sc.has_dist) {
+      // path of depth + 2 vertices: camera, depth surfaces, the emitter
+      const bool keep = cfg.only_len == 0 || cfg.only_len == depth + 2;
+      if (dot(hit.ng_world, wo) > 0.0f && keep) {
+        if (depth == 0 || !sc.has_dist || cfg.pt_mode == 2) {
           radiance = radiance + beta * hit.mat->emission;
-        } else {
+        } else if (cfg.pt_mode == 0) {
           const Light& L = sc.lights[hit.mat->light_id];
           float pdf_l = sc.dist.pmf[hit.mat->light_id] * L.pdf_li(prev_p, hit.p, length(hit.p - prev_p));
           float w = power_heuristic(prev_pdf, pdf_l);
@@ -629,7 +631,7 @@ V3 pt_li(const Scene& sc, Ray ray, Sampler& smp, const PtConfig& cfg, TraceCount
       break;
     }
     V3 wo_l = hit.onb.to_local(wo);
-    if (sc.has_dist) {  // path_tracer.rs:79-117
+    if (sc.has_dist && cfg.pt_mode != 2) {  // path_tracer.rs:79-117
       float u_sel = smp.next_1d();
       uint32_t li;
       float sel_pmf;
@@ -647,7 +649,9 @@ V3 pt_li(const Scene& sc, Ray ray, Sampler& smp, const PtConfig& cfg, TraceCount
         // emitter is then decided by the last bit.
         if (!L.is_delta()) t_end = ls.dist + dot(off, L.n) / ls.cos_l;
         Ray sr = Ray::make(hit.p + off, ls.wi, 0.0f, t_end * (1.0f - 1e-4f));
-        if (!sc.occluded(sr, c)) {
+        // path of depth + 3 vertices: camera, depth + 1 surfaces, the light
+        const bool keep = cfg.only_len == 0 || cfg.only_len == depth + 3;
+        if (keep && !sc.occluded(sr, c)) {
           V3 wi_l = hit.onb.to_local(ls.wi);
           float cos_wi = std::fabs(wi_l.z);
           if (cos_wi > 0.0f) {
@@ -657,7 +661,7 @@ V3 pt_li(const Scene& sc, Ray ray, Sampler& smp, const PtConfig& cfg, TraceCount
             } else {
               float pdf_light = ls.pdf * sel_pmf;
               float pdf_bsdf = bsdf_pdf(*hit.mat, wi_l, wo_l);
-              float w = power_heuristic(pdf_light, pdf_bsdf);
+              float w = cfg.pt_mode == 1 ? 1.0f : power_heuristic(pdf_light, pdf_bsdf);
               radiance = radiance + beta * f * cos_wi * ls.li * w / pdf_light;
             }
           }

@@ -192,6 +192,12 @@ enum RngMode : int { RNG_REF = 0, RNG_COUNTER = 1 };
 // ---------------------------------------------------------------------------
 struct PtConfig {
   uint32_t spp = 64, max_depth = 8, rr_depth = 3;
+  // Test-only switches (pins, not reference behaviour): emitter estimator
+  // 0 = MIS (light sampling + BSDF hits, power heuristic), 1 = light sampling
+  // only, 2 = BSDF hits only; only_len > 0 keeps the contributions of paths
+  // of exactly only_len vertices (camera and light included).
+  int pt_mode = 0;
+  uint32_t only_len = 0;
 };
 
 float power_heuristic(float a, float b);

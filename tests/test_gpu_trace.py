@@ -190,7 +190,7 @@ def test_every_launch_variant_same_answer(ar, wl):
     sc = ar.Scene.from_desc(desc)
     ref_t, ref_p, _ = sc.intersect(rays)
     ref_o = sc.occluded(rays)
-    for (b, s) in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8)]:
+    for (b, s) in [(64, 8), (64, 16), (128, 8), (128, 16), (256, 8), (256, 12)]:
         for pc, rf, sm, ps in ((4, 1, 0, 3), (16, 64, 63, 4), (8, 24, 8, 4)):
             sc.set_trace_config(b, s, pc, rf, sm, ps)
             t, p, _ = sc.intersect(rays)
