@@ -149,6 +149,7 @@ def _load():
         "ar_scene_get_info": (i32, [P, C.POINTER(ArSceneInfo)]),
         "ar_trace": (i32, [P, P, u64, P, P, u32, P]),
         "ar_occluded": (i32, [P, P, u64, P, u32, P]),
+        "ar_trace_noalloc": (i32, [P, P, u64, P, P, u32, P]),
         "ar_trace_count": (i32, [P, P, u64, u32, P, C.POINTER(ArTraceCounts)]),
         "ar_camera_lookat": (i32, [P, P, P, f, f, C.POINTER(ArCamera)]),
         "ar_camera_eye_angle": (i32, [P, f, f, f, f, C.POINTER(ArCamera)]),

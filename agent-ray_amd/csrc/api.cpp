@@ -48,7 +48,7 @@ ar_status ar_init(int device) {
   if (e != hipSuccess || n == 0) return fail(AR_ERR_DEVICE, "no HIP device available");
   if (device < 0 || device >= n) return fail(AR_ERR_INVALID_ARG, "device index out of range");
   AR_HIP(hipSetDevice(device));
-  return AR_OK;
+  return noalloc_setup_current_device();
 }
 
 // PinholeCamera::from_lookat (cameras.rs:91-123).

@@ -168,6 +168,9 @@ struct ar_scene {
 
 namespace ar {
 hipStream_t pick_stream(const ar_scene* s, void* stream);
+// Looks up the current device's address of the allocation-free traversal
+// scratch (ar_trace_noalloc) once per device.
+ar_status noalloc_setup_current_device();
 
 // Host-side result of building a committed scene's acceleration structure
 // and shading records (what ar_scene_commit uploads).

@@ -232,6 +232,46 @@ void scratch_release(ar_scene* s, hipStream_t st) {
   s->mu.unlock();
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free tracing (ar_trace_noalloc).  The traversal stack spill of
+// such calls lives in `scratch`, a fixed array of kNoallocSpillBytes in every
+// GPU's memory for the life of the process.  Its device address is looked up
+// once per device (hipGetSymbolAddress, at ar_init or at the first call on
+// that device) and only that address is handed to kernels: the array's name
+// in host code is a host-side stand-in, not a device address.  The spill
+// needs no clearing (every entry is written before it is read).  Calls that
+// overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoallocSpillBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocSpillBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+ar_status noalloc_setup_current_device() {
+  int dev = 0;
+  AR_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(AR_ERR_DEVICE, "device index beyond the allocation-free table");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (g_noalloc_ptr[dev]) return AR_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(AR_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_noalloc_ptr[dev] = p;
+  return AR_OK;
+}
+
+static void* noalloc_scratch_current_device() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  return g_noalloc_ptr[dev];
+}
+
 }  // namespace ar
 
 using namespace ar;
@@ -275,6 +315,43 @@ ar_status ar_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hi
     if (inst_ids) AR_HIP(hipMemcpyAsync(inst_ids, d_inst, n * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
   }
   if (host || !(flags & AR_ASYNC)) AR_HIP(hipStreamSynchronize(st));
+  return AR_OK;
+}
+
+ar_status ar_trace_noalloc(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst_ids,
+                           uint32_t flags, void* stream) {
+  if (!s || (!rays && n) || (!hits && n)) return fail(AR_ERR_INVALID_ARG, "ar_trace_noalloc: NULL argument");
+  if (!s->committed) return fail(AR_ERR_STATE, "ar_trace_noalloc: scene not committed");
+  if (flags & AR_HOST_PTRS)
+    return fail(AR_ERR_INVALID_ARG, "ar_trace_noalloc: device pointers only (staging host buffers would allocate)");
+  if (n == 0) return AR_OK;
+  ar_status e = noalloc_setup_current_device();
+  if (e != AR_OK) return e;
+  int2* spill = static_cast<int2*>(noalloc_scratch_current_device());
+  if (!spill) return fail(AR_ERR_DEVICE, "ar_trace_noalloc: no scratch address for the current device");
+  const ar_trace_config c = s->tcfg;
+  const uint32_t need = stack_need(s);
+  const uint64_t per_thread = need > c.lds_stack ? (uint64_t)(need - c.lds_stack) * sizeof(int2) : 0;
+  uint32_t G = trace_grid_cfg(n, c);
+  if (per_thread) {
+    // the spill is [entry][thread]: fit the grid to the fixed array (the
+    // answer does not depend on the grid)
+    const uint64_t gmax = kNoallocSpillBytes / (per_thread * c.workgroup_size) / 8 * 8;
+    if (gmax < 8)
+      return fail(AR_ERR_OUT_OF_BUDGET,
+                  "ar_trace_noalloc: this scene's stack spill needs " + std::to_string(per_thread * c.workgroup_size * 8) +
+                      " bytes for the smallest grid; the allocation-free array holds " +
+                      std::to_string(kNoallocSpillBytes));
+    G = (uint32_t)std::min<uint64_t>(G, gmax);
+  }
+  hipStream_t st = pick_stream(s, stream);
+  const bool ok = (flags & AR_REF_ORDER)
+                      ? launch_pieces<false, false>(s, G, rays, n, hits, inst_ids, nullptr, spill, st, c)
+                      : launch_pieces<false, true>(s, G, rays, n, hits, inst_ids, nullptr, spill, st, c);
+  if (!ok) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
+  const hipError_t le = hipGetLastError();
+  if (le != hipSuccess) return fail(AR_ERR_DEVICE, std::string("k_trace launch: ") + hipGetErrorString(le));
+  if (!(flags & AR_ASYNC)) AR_HIP(hipStreamSynchronize(st));
   return AR_OK;
 }
 

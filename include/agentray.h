@@ -227,6 +227,18 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* sample_rays, uint64_t n, 
 /* Closest hit for n rays.  inst_ids may be NULL. */
 ar_status ar_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst_ids,
                    uint32_t flags, void* stream);
+/* No counterpart in the reference.  ar_trace (same answers, bit for bit)
+ * for callers that cannot allocate device memory: code under HIP graph
+ * capture, serving loops and harnesses that forbid hipMalloc.  It allocates
+ * nothing: the traversal stack spill lives in a fixed 64 MiB array in each
+ * GPU's memory, held until the process ends, and the grid is reduced to fit
+ * it.  A scene whose spill does not fit even the smallest grid is rejected
+ * with AR_ERR_OUT_OF_BUDGET; host pointers (AR_HOST_PTRS) with
+ * AR_ERR_INVALID_ARG.  Device pointers only.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU race and must be serialised by
+ * the caller. */
+ar_status ar_trace_noalloc(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst_ids,
+                           uint32_t flags, void* stream);
 /* Any hit (boolean equal to the reference's full closest-hit is_some()). */
 ar_status ar_occluded(const ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t* occluded,
                       uint32_t flags, void* stream);
