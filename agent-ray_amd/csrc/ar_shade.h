@@ -137,11 +137,25 @@ __device__ __forceinline__ void make_shading(const ShadeView& v, const Ray& r, f
 // ---------------------------------------------------------------------------
 AR_HD bool same_hemi(f3 a, f3 b) { return a.z * b.z > 0.0f; }
 
+// sin and cos of an fp32 angle evaluated in fp64 and rounded once to fp32.
+// Two accurate fp64 implementations (the device's and a CPU's) then give
+// the same fp32 bits except where their results straddle an fp32 rounding
+// boundary (about one angle in 2^29), so a sampled direction does not
+// depend on whose fp32 sin/cos ran.  The reference's f32 sin/cos (Rust's
+// libm, surfaces.rs:50-57, microfacet.rs:53-65, point.rs:41-46) fixes no
+// particular last bit either.
+__device__ __forceinline__ void sincos_f32(float x, float* s, float* c) {
+  double sd, cd;
+  sincos((double)x, &sd, &cd);
+  *s = (float)sd;
+  *c = (float)cd;
+}
+
 __device__ __forceinline__ f3 cosine_hemisphere(float ux, float uy) {  // surfaces.rs:50-57
   float r = sqrtf(ux);
   float phi = (2.0f * kPi) * uy;
   float s, c;
-  sincosf(phi, &s, &c);
+  sincos_f32(phi, &s, &c);
   float x = r * c, y = r * s;
   float z = sqrtf(fmaxf((1.0f - x * x) - y * y, 0.0f));
   return mk(x, y, z);
@@ -199,7 +213,7 @@ __device__ __forceinline__ f3 ggx_sample_wh(float a, f3 wo, float ux, float uy) 
   float sh = sqrtf(fmaxf(1.0f - c2, 0.0f));
   float phi = (2.0f * kPi) * uy;
   float s, c;
-  sincosf(phi, &s, &c);
+  sincos_f32(phi, &s, &c);
   f3 h = mk(sh * c, sh * s, ch);
   if (h.z * wo.z < 0.0f) h = -h;
   return h;

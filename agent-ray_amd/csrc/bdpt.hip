@@ -173,7 +173,7 @@ __device__ __forceinline__ bool light_emission(const DLight& L, float upx, float
     float st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
     float phi = (2.0f * kPi) * udy;
     float s, c;
-    sincosf(phi, &s, &c);
+    sincos_f32(phi, &s, &c);
     f3 dir = mk(st * c, st * s, ct);
     e.ray = make_ray(mk(L.pos[0], L.pos[1], L.pos[2]), dir, 1e-4f, kF32Max);
     e.le = mk(L.intensity[0], L.intensity[1], L.intensity[2]);

@@ -435,7 +435,17 @@ double orc_render_bdpt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, u
   cfg.only_t = only_t;
   cfg.unweighted = unweighted != 0;
   size_t npix = (size_t)w * h;
-  std::vector<double> splat(3 * npix, 0.0);
+  // Light-tracing splats land on any pixel from any thread; the reference
+  // adds them with float CAS loops in whatever order they arrive (film.rs:59-73,
+  // nondeterministic).  They are summed here as 64-bit fixed point with 2^-24
+  // resolution (each contribution rounded to nearest), the representation the
+  // library's deterministic film uses, so the sum is order-independent.
+  std::vector<uint64_t> splat(3 * npix, 0);
+  auto fixed = [](float c) -> uint64_t {
+    const float v = c * 16777216.0f;
+    if (!(v == v)) return 0;
+    return (uint64_t)(int64_t)std::fmin(std::fmax(std::rint(v), -4.0e18f), 4.0e18f);
+  };
   std::mutex mu;
   auto t0 = std::chrono::steady_clock::now();
   parallel_rows(h, nthreads, [&](uint32_t yr) {
@@ -456,13 +466,18 @@ double orc_render_bdpt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, u
         camera_subpath(*sc, cam, ray, cfg, smp, cvv, nullptr);
         light_subpath(*sc, cfg, smp, lvv, nullptr);
         const int smax = (int)cvv.size(), tmax = (int)lvv.size(), limit = (int)max_depth + 2;
+        // The reference adds every strategy's contribution to one running
+        // pixel sum (:1322); here a sample's contributions are summed first
+        // and the sample then added to the pixel (the same value up to float
+        // association; the library reduces per path the same way).
+        V3 ls;
         for (int t = 0; t <= tmax; ++t)
           for (int sv = 0; sv <= smax; ++sv) {
             if (sv + t < 2 || sv + t > limit || sv == 0) continue;
             if (cfg.only_s >= 0 && (sv != cfg.only_s || t != cfg.only_t)) continue;
             V3 contrib;
             if (t == 0) {  // extension (:810-818)
-              if (connect_t0(*sc, cvv, lvv, sv, cfg, &contrib)) acc = acc + contrib;
+              if (connect_t0(*sc, cvv, lvv, sv, cfg, &contrib)) ls = ls + contrib;
               continue;
             }
             if (sv == 1) {
@@ -473,8 +488,9 @@ double orc_render_bdpt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, u
               }
               continue;
             }
-            if (connect(*sc, cvv, lvv, sv, t, cfg, &contrib, nullptr)) acc = acc + contrib;
+            if (connect(*sc, cvv, lvv, sv, t, cfg, &contrib, nullptr)) ls = ls + contrib;
           }
+        acc = acc + ls;
       }
       V3 m = acc / (float)spp;
       size_t o = 3 * ((size_t)yr * w + x);
@@ -484,14 +500,15 @@ double orc_render_bdpt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, u
     }
     std::lock_guard<std::mutex> lk(mu);
     for (auto& e : local) {
-      splat[3 * e.first] += e.second.x;
-      splat[3 * e.first + 1] += e.second.y;
-      splat[3 * e.first + 2] += e.second.z;
+      splat[3 * e.first] += fixed(e.second.x);
+      splat[3 * e.first + 1] += fixed(e.second.y);
+      splat[3 * e.first + 2] += fixed(e.second.z);
     }
   });
   float inv = 1.0f / (float)spp;  // :1341-1348
   for (size_t i = 0; i < npix; ++i) {
-    V3 sp((float)splat[3 * i], (float)splat[3 * i + 1], (float)splat[3 * i + 2]);
+    auto val = [&](size_t k) { return (float)((double)(int64_t)splat[k] * (1.0 / 16777216.0)); };
+    V3 sp(val(3 * i), val(3 * i + 1), val(3 * i + 2));
     if (out_splat) {
       out_splat[3 * i] = sp.x;
       out_splat[3 * i + 1] = sp.y;

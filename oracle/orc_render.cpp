@@ -12,12 +12,18 @@ namespace orc {
 static inline bool same_hemisphere(V3 a, V3 b) { return a.z * b.z > 0.0f; }  // surfaces.rs:42
 static inline float abs_cos(V3 v) { return std::fabs(v.z); }
 
+// sin / cos of an fp32 angle in fp64, rounded once: the library's device
+// code evaluates them the same way, so both sample the same directions
+// (the reference's f32 sin/cos, Rust's libm, are statistically equivalent).
+static inline float sin_f32(float x) { return (float)std::sin((double)x); }
+static inline float cos_f32(float x) { return (float)std::cos((double)x); }
+
 // surfaces.rs:50-57
 static V3 cosine_hemisphere_sample(float ux, float uy) {
   float r = std::sqrt(ux);
   float phi = (2.0f * kPi) * uy;
-  float x = r * std::cos(phi);
-  float y = r * std::sin(phi);
+  float x = r * cos_f32(phi);
+  float y = r * sin_f32(phi);
   float z = std::sqrt(std::max((1.0f - x * x) - y * y, 0.0f));
   return V3(x, y, z);
 }
@@ -87,7 +93,7 @@ static V3 ggx_sample_wh(float alpha, V3 wo, float ux, float uy) {
   float cos_h = std::sqrt(std::max(cos2_h, 0.0f));
   float sin_h = std::sqrt(std::max(1.0f - cos2_h, 0.0f));
   float phi = (2.0f * kPi) * uy;
-  V3 h(sin_h * std::cos(phi), sin_h * std::sin(phi), cos_h);
+  V3 h(sin_h * cos_f32(phi), sin_h * sin_f32(phi), cos_h);
   if (h.z * wo.z < 0.0f) h = -h;
   return h;
 }
@@ -248,7 +254,7 @@ static V3 uniform_sphere_sample(float ux, float uy) {
   float cos_t = 1.0f - 2.0f * ux;
   float sin_t = std::sqrt(std::max(1.0f - cos_t * cos_t, 0.0f));
   float phi = (2.0f * kPi) * uy;
-  return V3(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t);
+  return V3(sin_t * cos_f32(phi), sin_t * sin_f32(phi), cos_t);
 }
 
 bool Light::sample(V3 p, float ux, float uy, LightSample* out) const {

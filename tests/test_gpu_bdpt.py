@@ -2,13 +2,18 @@
 bidirectional_path_tracer.rs:1221-1357) against the oracle restatement.
 
 The reference BDPT is reproduced bug-compatibly (its MIS walk counts the
-connection vertex twice and excludes s'=0/t'=0 inconsistently, so its image
-is biased against the path tracer; measured on the Cornell twins with the
-oracle: +21% with the point light, -40% with the quad light).  These tests
-therefore check parity with the oracle, not unbiasedness:
-* shared counter RNG: films agree to rounding (fixed-point splats vs fp64
-  splats, rare branch flips allowed);
-* both light types, power and uniform MIS modes, beta = 1 and 2;
+connection vertex twice, and its light tracing leaves out the camera's
+cosine, so its image is biased against the path tracer: +21 % with the
+Cornell box's point light, +10 % with its quad light, measured with the
+oracle; tests/test_oracle_pins.py pins the parts and shows a textbook walk
+equal to the path tracer).  These tests therefore check parity with the
+oracle, not unbiasedness:
+* shared counter RNG: films bitwise equal to the oracle's (light-tracing
+  splats summed in the same 2^-24 fixed point on both sides), including the
+  t = 0 strategy (the camera sees the quad light);
+* both light types, power and uniform MIS modes, beta = 1 and 2; the room
+  at the benchmark's BDPT settings (depth 10, power heuristic beta = 2,
+  three quad lights, power light selection);
 * determinism: identical films across runs and sample chunkings;
 * the wavefront schedule (subpaths as bounce loops of extend and shade
   stages; connection enumerated, every shadow ray traced in one any-hit
@@ -40,11 +45,10 @@ def test_bdpt_matches_oracle(ar, orc, wl, twin, mode, beta):
     g, st = bdpt(ar, sc, cam, w, h, spp, seed=4, mode=mode, beta=beta)
     osc = orc.OracleScene(desc)
     o, _, _ = orc.render_bdpt(osc, cam16, w, h, spp, 8, 3, mode, beta, rng_mode=1, seed=4)
-    rel = np.abs(g - o) / np.maximum(np.abs(o), 1e-2)
-    frac = float((rel > 1e-3).mean())
-    print(f"  twin={twin} mode={mode} beta={beta}: frac(rel>1e-3) {frac:.4%}, means {g.mean():.5f} {o.mean():.5f}")
-    assert frac < 0.03
-    assert abs(g.mean() - o.mean()) < 2e-3 * o.mean()
+    from conftest import film_mismatch
+    n, rel = film_mismatch(g, o)
+    print(f"  twin={twin} mode={mode} beta={beta}: pixels not bitwise equal {n} of {w * h}, max rel {rel:.3g}")
+    assert n <= 1
     assert st.paths == w * h * spp and st.rays > 0
 
 
@@ -60,12 +64,20 @@ def test_bdpt_deterministic(ar, wl):
     assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
 
 
-def test_bdpt_room_runs(ar, wl):
-    desc = wl.procedural_room(n_boxes=20, width=64, height=36)
+def test_bdpt_room_matches_oracle(ar, orc, wl):
+    """The benchmark's BDPT configuration (the room: ~280k triangles, three
+    quad lights, mixed Lambertian / conductor / dielectric; depth 10, power
+    heuristic beta 2) at 96 x 54 x 4 spp, against the oracle."""
+    from conftest import film_mismatch
+    desc = wl.procedural_room(width=96, height=54)
     sc = ar.Scene.from_desc(desc)
-    cam16 = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 64 / 36)
-    g, st = bdpt(ar, sc, ar.PinholeCamera.from_array(cam16), 64, 36, 2, depth=10)
-    assert np.all(np.isfinite(g)) and g.mean() > 0
+    cam16 = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 96 / 54)
+    g, st = bdpt(ar, sc, ar.PinholeCamera.from_array(cam16), 96, 54, 4, seed=6, depth=10)
+    o, _, _ = orc.render_bdpt(orc.OracleScene(desc), cam16, 96, 54, 4, 10, 3, 0, 2.0, rng_mode=1, seed=6)
+    n, rel = film_mismatch(g, o)
+    print(f"  room: pixels not bitwise equal {n} of {96 * 54}, max rel {rel:.3g}, mean {g.mean():.5f}")
+    assert n <= 1 and g.mean() > 0
+    assert st.paths == 96 * 54 * 4
 
 
 @pytest.mark.parametrize("twin,mode", [(True, 0), (False, 0), (False, 1)])
@@ -75,7 +87,7 @@ def test_bdpt_wavefront_connection_bitwise(ar, wl, twin, mode):
     cam16 = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 1.0)
     cam = ar.PinholeCamera.from_array(cam16)
     # 256 x 256 x 32 at depth 16: 2^21 paths in one chunk, more than one
-    # 12 GiB connection pass holds (152 slots of 53 B per path), so two passes
+    # 12 GiB connection pass holds (153 slots of 53 B per path), so two passes
     mega, sm = bdpt(ar, sc, cam, 256, 256, 32, seed=3, depth=16, mode=mode, schedule=ar.Schedule.MEGAKERNEL)
     wave, sw = bdpt(ar, sc, cam, 256, 256, 32, seed=3, depth=16, mode=mode)
     assert np.array_equal(mega.view(np.uint32), wave.view(np.uint32))

@@ -2,13 +2,14 @@
 
 * schedules: megakernel and wavefront films are bitwise identical, and do
   not depend on the sample chunking (deterministic accumulation);
-* shared counter RNG: GPU and oracle trace the same random numbers, so the
-  films agree to rounding except on the rare path whose branch flips on a
-  last-bit difference of sin/cos: at most 0.5% of pixel-channels off by more
-  than 1e-4 relative, total |diff| below 1e-4 of the total film;
-* independent RNG: per-pixel means agree within 3 sigma except for a
-  binomial fraction of pixels; relMSE against a high-spp oracle render is
-  within 5% of the oracle's own at equal spp;
+* shared counter RNG: GPU and oracle trace the same random numbers with the
+  same operation order, so the films are bitwise equal (Cornell box and the
+  280k-triangle room; conftest.film_mismatch explains the pixel kept in
+  reserve);
+* independent RNG: per-pixel means agree within 3 sigma (GPU replicates
+  against the oracle's own per-pixel variance) up to the binomial quantile
+  of 0.27 %; relMSE against a 4096-spp oracle render, averaged over 8 seeds,
+  is within 5 % of the oracle's own at the same spp (check 2);
 * closed form: direct lighting from a point light over a plane.
 """
 import math
@@ -69,45 +70,48 @@ def test_schedules_and_chunking_bitwise_identical(ar, cornell):
 
 
 def test_counter_rng_matches_oracle(ar, cornell):
+    from conftest import film_mismatch
     desc, sc, cam, cam16, osc = cornell
     w, h, spp = 64, 64, 16
     g, _ = render(ar, sc, cam, w, h, spp, seed=7)
     o, _, _, _ = osc.render_pt(cam16, w, h, spp, 8, 3, rng_mode=1, seed=7)
-    diff = np.abs(g - o)
-    tol = 1e-4 * np.maximum(1.0, np.abs(o))
-    frac = float((diff > tol).mean())
-    l1 = float(diff.sum() / o.sum())
-    print(f"  pixel-channels off by > 1e-4 rel: {frac:.4%}, max diff {diff.max():.3g}, l1 rel {l1:.3g}")
-    assert frac <= 0.005
-    assert l1 < 1e-4
-    assert abs(g.mean() - o.mean()) < 1e-4 * o.mean()
+    n, rel = film_mismatch(g, o)
+    print(f"  pixels not bitwise equal: {n} of {w * h}, max rel diff {rel:.3g}")
+    assert n <= 1
 
 
 def test_independent_rng_three_sigma(ar, cornell):
+    from conftest import outlier_bound
     desc, sc, cam, cam16, osc = cornell
-    w, h, spp = 48, 48, 256
-    g, _ = render(ar, sc, cam, w, h, spp, seed=1)
-    o, m2, _, _ = osc.render_pt(cam16, w, h, spp, 8, 3, rng_mode=0, want_m2=True)
-    var = np.maximum(m2 - o * o, 0.0)
-    sigma = np.sqrt(2.0 * var / spp) + 1e-6
-    out = np.abs(g - o) > 3.0 * sigma
-    frac = float(out.mean())
-    print(f"  3-sigma outliers: {frac:.4%}")
-    # 0.27% expected for Gaussian errors; heavy-tailed light paths allow more
-    assert frac < 0.015
+    w, h, reps, spp = 48, 48, 32, 16
+    gs = np.stack([render(ar, sc, cam, w, h, spp, seed=300 + r)[0] for r in range(reps)])
+    gm, gv = gs.mean(0), gs.var(0, ddof=1) / reps
+    o, m2, _, _ = osc.render_pt(cam16, w, h, 512, 8, 3, rng_mode=0, want_m2=True)
+    ov = np.maximum(m2 - o * o, 0.0) / 512
+    z = (gm - o) / np.sqrt(gv + ov + 1e-20)
+    n3 = int((np.abs(z) > 3).sum())
+    bound = outlier_bound(z.size, dof=reps - 1)  # the GPU variance from reps replicates
+    print(f"  |z| > 3: {n3} of {z.size} (bound {bound}), max |z| {np.abs(z).max():.2f}")
+    assert n3 <= bound
+    zg = (gm - o).sum() / math.sqrt((gv + ov).sum())
+    assert abs(zg) < 4.0
 
 
-def test_relmse_vs_high_spp_reference(ar, cornell):
+def test_relmse_vs_4096spp_reference(ar, cornell):
+    """Check (2): relMSE = mean over pixel-channels of (I - R)^2 / (R^2 + 1e-2)
+    against a 4096-spp oracle render R; the GPU's at 16 spp, averaged over 8
+    seeds, is within 5 % of the oracle's at 16 spp over 8 other seeds."""
     desc, sc, cam, cam16, osc = cornell
-    w, h = 32, 32
-    ref, _, _, _ = osc.render_pt(cam16, w, h, 2048, 8, 3, rng_mode=1, seed=99)
+    w, h = 64, 64
+    ref, _, _, _ = osc.render_pt(cam16, w, h, 4096, 8, 3, rng_mode=1, seed=12345)
+
     def relmse(x):
         return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
-    g, _ = render(ar, sc, cam, w, h, 16, seed=3)
-    o, _, _, _ = osc.render_pt(cam16, w, h, 16, 8, 3, rng_mode=0)
-    rg, ro = relmse(g), relmse(o)
-    print(f"  relMSE gpu {rg:.4g} cpu {ro:.4g} ratio {rg / ro:.3f}")
-    assert rg <= 1.25 * ro  # single seeds; the bench averages several
+    rg = [relmse(render(ar, sc, cam, w, h, 16, seed=100 + s)[0]) for s in range(8)]
+    ro = [relmse(osc.render_pt(cam16, w, h, 16, 8, 3, rng_mode=1, seed=200 + s)[0]) for s in range(8)]
+    ratio = float(np.mean(rg) / np.mean(ro))
+    print(f"  relMSE gpu {np.mean(rg):.5g} cpu {np.mean(ro):.5g} ratio {ratio:.4f}")
+    assert ratio <= 1.05
 
 
 def test_edge_cases(ar, cornell):
@@ -121,6 +125,19 @@ def test_edge_cases(ar, cornell):
     assert np.array_equal(odd_a, odd_b)
     with pytest.raises(ar.ArError):
         render(ar, sc, cam, 0, 4, 1)
+    # a chunk of 2^31 paths or more (32-bit path ids) is rejected before any
+    # allocation or launch, by ar_render and by the server's submit
+    for integ, w, h, spp, chunk in ((0, 2048, 2048, 1024, 1024), (2, 1920, 1080, 700, 700)):
+        cfg = ar.GdptConfig(spp=spp) if integ == 2 else ar.PathTracerConfig(spp=spp)
+        it = (ar.GradientDomainPathTracer if integ == 2 else ar.PathTracer)(cfg, sample_chunk=chunk)
+        with pytest.raises(ar.ArError) as e:
+            it.render(sc, cam, w, h)
+        assert e.value.code == -1 and "sample_chunk" in str(e.value)
+        srv = ar.RenderServer(sc, max_active_jobs=1)
+        with pytest.raises(ar.ArError) as e:
+            srv.submit(it, cam, w, h)
+        assert e.value.code == -1
+        srv.close()
 
 
 def test_point_light_plane_closed_form(ar, wl):
@@ -156,9 +173,10 @@ def test_room_scene_schedules_and_oracle(ar, orc, wl):
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
     osc = orc.OracleScene(desc)
     o, _, _, _ = osc.render_pt(cam16, 96, 54, 4, 16, 3, rng_mode=1, seed=5)
-    rel = np.abs(a - o) / np.maximum(1.0, np.abs(o))
-    print(f"  room: frac(rel>1e-3) {(rel > 1e-3).mean():.4%}  gpu mean {a.mean():.5f} cpu mean {o.mean():.5f}")
-    assert float((rel > 1e-3).mean()) < 0.03
+    from conftest import film_mismatch
+    n, rel = film_mismatch(a, o)
+    print(f"  room: pixels not bitwise equal {n} of {96 * 54}, max rel diff {rel:.3g}")
+    assert n <= 1
     # larger render: 2M paths, millions of extension and shadow rays through
     # the wave-refill traversal vs one ray per lane; films bitwise equal
     cam2 = ar.PinholeCamera.from_array(wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup,
@@ -167,3 +185,29 @@ def test_room_scene_schedules_and_oracle(ar, orc, wl):
     d, sd_ = render(ar, sc, cam2, 480, 270, 16, depth=16, schedule=1, seed=9)
     assert sc_.rays == sd_.rays
     assert np.array_equal(c.view(np.uint32), d.view(np.uint32))
+
+
+def test_room_relmse_vs_4096spp_reference(ar, orc, wl):
+    """Check (2) on the room (~280k triangles, mixed materials, three quad
+    lights, depth 16, Russian roulette from bounce 3) at 48 x 27.  With the
+    shared counter RNG the GPU's films are the oracle's bit for bit, so its
+    relMSE against the 4096-spp oracle render equals the oracle's at the same
+    spp and seeds.  Independent seeds are compared on the Cornell box only:
+    on the room, caustic paths through the dielectrics make one seed's
+    relMSE at 16 spp vary by a factor of several, far above the 5 % to
+    resolve."""
+    desc = wl.procedural_room(width=48, height=27)
+    sc = ar.Scene.from_desc(desc)
+    cam16 = wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 48 / 27)
+    cam = ar.PinholeCamera.from_array(cam16)
+    osc = orc.OracleScene(desc)
+    ref, _, _, _ = osc.render_pt(cam16, 48, 27, 4096, 16, 3, rng_mode=1, seed=54321)
+
+    def relmse(x):
+        return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+    rg = [relmse(render(ar, sc, cam, 48, 27, 16, depth=16, seed=100 + s)[0]) for s in range(8)]
+    ro = [relmse(osc.render_pt(cam16, 48, 27, 16, 16, 3, rng_mode=1, seed=100 + s)[0]) for s in range(8)]
+    ratio = float(np.mean(rg) / np.mean(ro))
+    print(f"  room relMSE gpu {np.mean(rg):.5g} cpu {np.mean(ro):.5g} ratio {ratio:.6f}, "
+          f"per seed {min(rg):.3g} .. {max(rg):.3g}")
+    assert abs(ratio - 1.0) < 1e-6
