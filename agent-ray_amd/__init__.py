@@ -85,7 +85,8 @@ class ArSceneInfo(C.Structure):
 
 
 class ArServerConfig(C.Structure):
-    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_jobs", C.c_uint32), ("path_slots", C.c_uint32)]
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_jobs", C.c_uint32),
+                ("max_concurrent_jobs", C.c_uint32)]
 
 
 class ArJobDesc(C.Structure):
@@ -830,9 +831,10 @@ class RenderServer:
     one resident scene; jobs are admitted into slots by HBM budget and run
     concurrently (see include/agentray.h, ar_server_*)."""
 
-    def __init__(self, scene: Scene, hbm_budget_bytes: int = 200 << 30, max_active_jobs: int = 8):
+    def __init__(self, scene: Scene, hbm_budget_bytes: int = 200 << 30, max_active_jobs: int = 8,
+                 max_concurrent_jobs: int = 0):
         self.scene = scene
-        cfg = ArServerConfig(hbm_budget_bytes, max_active_jobs, 0)
+        cfg = ArServerConfig(hbm_budget_bytes, max_active_jobs, max_concurrent_jobs)
         self._h = C.c_void_p()
         _check(_lib.ar_server_create(scene.handle, C.byref(cfg), C.byref(self._h)))
         self._shapes = {}

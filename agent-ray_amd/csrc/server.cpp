@@ -3,28 +3,29 @@
 //
 // Continuous batching: jobs wait in a FIFO and are admitted into job slots
 // as soon as a slot is free and the HBM budget covers the job's footprint.
-// Every slot owns a HIP stream, a grow-only workspace (path state, queues,
-// film, stack spill) and a pinned output buffer.  One scheduler thread issues
-// chunks of the admitted jobs one at a time (kInflight), always of the job
-// whose projected slowdown -- (time since submission + estimated remaining
-// device time) / estimated solo time -- is highest.  A job that arrives
-// while a long one renders therefore waits for one chunk and then runs
-// ahead of it (its projected slowdown starts at 1 and grows fastest), and a
-// long job is not starved: its own projected slowdown grows while it waits.
-// Device time is estimated from the job's work (pixels x spp, weighted per
-// integrator) and a running measurement of the device's rate; stream
-// priorities are not used.  Server jobs without an explicit sample_chunk run
-// in one chunk when small and in chunks of 2^23 path-equivalents (2^19 BDPT
-// samples, both ~20-40 ms of device time) otherwise: smaller chunks keep the
-// wait for the running chunk short but pay their bounce tails and launches
-// more often (at 2^20 / 2^16 the device spent about half its time on them),
-// larger ones make small jobs wait longer.  Several chunks
-// on different streams would not help a short job: the device interleaves
-// kernels of concurrent streams one for one, so each of the short job's
-// kernels would wait for one of the long job's.
-// A job's film depends only on its own description
-// (per-job seed, deterministic accumulation in sample order, independent of
-// chunking), never on what it shared the device with.
+// Every slot owns a grow-only workspace (path state, queues, film, stack
+// spill) and a pinned output buffer.  Admitted jobs' chunks run on up to
+// `max_concurrent_jobs` lanes at once (one HIP stream each, so their kernels
+// overlap on the device: one job's late bounces, which leave most CUs idle,
+// run beside another job's full stages).  Whenever a lane is free the
+// scheduler issues the next chunk of the admitted job with the least
+// remaining work (shortest remaining processing time first), so a small job
+// that arrives while large ones render starts at the next chunk boundary of
+// any lane and is not queued behind them.  A job of one chunk ("short") runs
+// with the device to itself: while one is on the device or waiting, long
+// jobs issue no new chunk (theirs in flight drain), unless a long job has
+// waited 0.2 s since its last chunk; the persistent kernels of long jobs
+// fill every CU, so a short job's kernels would otherwise queue behind
+// theirs one by one.  Several long jobs share the lanes, their late bounces
+// overlapping.  A job has at most one chunk on the device; consecutive chunks
+// of a job may use different lanes, since each is issued only after the
+// previous one has completed.  Stream priorities are not used.  Server jobs
+// without an explicit sample_chunk run in one chunk when their work is at
+// most two chunks and otherwise in chunks of 2^23 path-equivalents (2^19
+// BDPT samples, both ~20-40 ms of device time): smaller chunks pay their
+// bounce tails more often (2^21: p50 latency / solo 2.2 instead of 1.35).  A job's film depends only on its own description (per-job
+// seed, deterministic accumulation in sample order, independent of chunking
+// and of the lanes), never on what it shared the device with.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -37,9 +38,10 @@
 
 namespace ar {
 
-constexpr int kInflight = 1;                 // chunks on the device, all jobs together
+constexpr uint32_t kMaxLanes = 4;            // hardware queues a process gets by default
 constexpr uint64_t kServerChunkPaths = 1ull << 23;   // PT / GDPT path-equivalents per chunk
 constexpr uint64_t kServerChunkBdpt = 1ull << 19;    // BDPT samples per chunk
+constexpr double kLongWaitMax = 0.2;                 // s a long job yields to short ones at most
 
 // Relative device cost of one pixel-sample of each integrator, measured on
 // the room at 1280x720 (PT path 1; GDPT sample = base + two shifted paths;
@@ -71,10 +73,11 @@ struct Job {
   bool issued_all = false;   // every chunk + resolve + copy-out enqueued
   double work_left = 0;      // weighted paths not yet issued
   double work_chunk = 0;     // weighted paths per chunk
-  double work_total = 0;     // weighted paths of the whole job
   std::unique_ptr<JobRun> run;
-  std::deque<hipEvent_t> inflight;
-  std::deque<double> inflight_issue, inflight_work;  // host issue time, weighted paths of each chunk
+  hipEvent_t begun = nullptr;  // after the set-up work begin() enqueued on the slot's stream
+  int lane = -1;               // lane holding the job's chunk on the device, -1: none
+  bool short_job = false;      // runs as one chunk
+  double last_issue = 0;       // server clock of its last chunk
   std::vector<float> film;
   ar_render_stats stats{};
 };
@@ -82,6 +85,12 @@ struct Job {
 }  // namespace ar
 
 using namespace ar;
+
+struct Lane {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;  // recorded after the chunk in flight
+  std::shared_ptr<Job> job;   // the job whose chunk is in flight, null when free
+};
 
 struct ar_server {
   const ar_scene* scene = nullptr;
@@ -92,6 +101,7 @@ struct ar_server {
   std::map<uint64_t, std::shared_ptr<Job>> jobs;
   std::vector<std::shared_ptr<Job>> active;
   std::vector<std::unique_ptr<Slot>> slots;
+  std::vector<Lane> lanes;
   std::vector<hipEvent_t> event_pool;
   int waiters = 0;       // threads inside ar_job_wait (ar_server_destroy waits for them)
   bool broken = false;  // the worker thread ended on an exception
@@ -102,10 +112,6 @@ struct ar_server {
   int device = 0;
 
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
-  // device rate in weighted paths per second (running estimate from retired
-  // chunks) and when the device last finished a chunk
-  double rate = 150e6;
-  double last_done = 0;
 
   hipEvent_t get_event() {
     if (!event_pool.empty()) {
@@ -119,6 +125,10 @@ struct ar_server {
   }
 
   void release(Job& j) {  // caller holds mu
+    if (j.begun) {
+      event_pool.push_back(j.begun);
+      j.begun = nullptr;
+    }
     if (j.slot >= 0) {
       slots[j.slot]->busy = false;
       slots[j.slot]->footprint = 0;
@@ -209,6 +219,7 @@ struct ar_server {
       j.slot = free_slot;
       j.state = AR_JOB_RUNNING;
       j.t_start = now();
+      j.last_issue = j.t_start;
       starting.push_back(it->second);
     }
   }
@@ -245,19 +256,28 @@ struct ar_server {
         // split so that the device never holds much of them at a time
         uint64_t per_sample = npix * (j.spec.cfg.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
         uint64_t target = j.spec.cfg.integrator == AR_INTEGRATOR_BDPT ? kServerChunkBdpt : kServerChunkPaths;
-        if (per_sample * j.spec.cfg.spp <= 2 * target)
+        if (per_sample * j.spec.cfg.spp <= 2 * target) {
           j.spec.cfg.sample_chunk = j.spec.cfg.spp;
+          j.short_job = true;
+        }
         else
           j.spec.cfg.sample_chunk = (uint32_t)std::max<uint64_t>(
               1, std::min<uint64_t>(j.spec.cfg.spp, target / std::max<uint64_t>(per_sample, 1)));
       }
       const double w = path_weight(j.spec.cfg.integrator) * (double)npix;
       j.work_left = w * j.spec.cfg.spp;
-      j.work_total = j.work_left;
       j.work_chunk = w * chunk_samples(j.spec);
       j.run = make_job(scene, j.spec, &sl.ws, sl.stream, sl.ws.out.as<float>(), nullptr);
       e = j.run->begin();
-      if (e != AR_OK) msg = std::string("job begin: ") + ar_last_error();
+      if (e != AR_OK) {
+        msg = std::string("job begin: ") + ar_last_error();
+      } else {
+        j.begun = get_event();
+        if (!j.begun || hipEventRecord(j.begun, sl.stream) != hipSuccess) {
+          e = AR_ERR_DEVICE;
+          msg = "event record failed";
+        }
+      }
     }
     std::lock_guard<std::mutex> lk(mu);
     if (e != AR_OK) {
@@ -269,105 +289,93 @@ struct ar_server {
     active.push_back(jp);
   }
 
-  // One pass: retire finished chunks and complete drained jobs, then issue
-  // chunks, shortest remaining work first, up to kInflight on the device.
+  // One pass: retire finished chunks, complete drained jobs, then fill free
+  // lanes with the chunks of the jobs with the least remaining work.
   // Returns true if any progress was made.  Called without mu held; takes it
   // only for bookkeeping.
   bool pump() {
+    bool progress = false;
+    for (auto& ln : lanes) {
+      if (!ln.job || hipEventQuery(ln.done) != hipSuccess) continue;
+      Job& j = *ln.job;
+      j.lane = -1;
+      ln.job.reset();
+      progress = true;
+    }
     std::vector<std::shared_ptr<Job>> act;
     {
       std::lock_guard<std::mutex> lk(mu);
       act = active;
     }
-    bool progress = false;
-    int inflight = 0;
     for (auto& jp : act) {
       Job& j = *jp;
-      while (!j.inflight.empty() && hipEventQuery(j.inflight.front()) == hipSuccess) {
-        event_pool.push_back(j.inflight.front());
-        j.inflight.pop_front();
-        const double t = now();
-        const double dt = t - std::max(j.inflight_issue.front(), last_done);
-        if (dt > 1e-4) rate = 0.8 * rate + 0.2 * (j.inflight_work.front() / dt);
-        last_done = t;
-        j.inflight_issue.pop_front();
-        j.inflight_work.pop_front();
-        progress = true;
+      if (j.lane >= 0) continue;
+      std::lock_guard<std::mutex> lk(mu);
+      if (!j.issued_all && !j.cancel) continue;
+      // drained: complete the job
+      if (j.cancel && !j.issued_all) {
+        j.state = AR_JOB_CANCELLED;
+        j.t_finish = now();
+      } else {
+        Slot& sl = *slots[j.slot];
+        size_t n = (size_t)j.spec.width * j.spec.height * 3;
+        j.film.assign(sl.pinned, sl.pinned + n);
+        (void)j.run->read_stats(&j.stats);
+        j.state = j.cancel ? AR_JOB_CANCELLED : AR_JOB_DONE;
+        j.t_finish = now();
       }
-      bool cancel;
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        cancel = j.cancel;
-      }
-      if (j.issued_all || cancel) {
-        if (!j.inflight.empty()) {
-          inflight += (int)j.inflight.size();
-          continue;
-        }
-        // drained: complete the job
-        std::lock_guard<std::mutex> lk(mu);
-        if (cancel && !j.issued_all) {
-          j.state = AR_JOB_CANCELLED;
-          j.t_finish = now();
-        } else {
-          Slot& sl = *slots[j.slot];
-          size_t n = (size_t)j.spec.width * j.spec.height * 3;
-          j.film.assign(sl.pinned, sl.pinned + n);
-          (void)j.run->read_stats(&j.stats);
-          j.state = cancel ? AR_JOB_CANCELLED : AR_JOB_DONE;
-          j.t_finish = now();
-        }
-        release(j);
-        active.erase(std::find(active.begin(), active.end(), jp));
-        cv_done.notify_all();
-        cv_work.notify_all();
-        progress = true;
-        continue;
-      }
-      inflight += (int)j.inflight.size();
+      release(j);
+      active.erase(std::find(active.begin(), active.end(), jp));
+      cv_done.notify_all();
+      cv_work.notify_all();
+      progress = true;
     }
-    while (inflight < kInflight) {
-      // highest projected slowdown first: (waited + remaining) / solo, both
-      // in device time at the measured rate; ties go to less remaining work
+    for (size_t li = 0; li < lanes.size(); ++li) {
+      Lane& ln = lanes[li];
+      if (ln.job) continue;
+      // shortest remaining work first; ties go to the earlier submission
       std::shared_ptr<Job> pick;
-      double best = -1.0;
-      const double t = now();
       {
         std::lock_guard<std::mutex> lk(mu);
+        bool short_active = false;
+        for (auto& jp : active)
+          if (jp->short_job && !jp->cancel && (!jp->issued_all || jp->lane >= 0)) short_active = true;
+        const double tnow = now();
         for (auto& jp : active) {
           Job& j = *jp;
-          if (j.issued_all || j.cancel || j.inflight.size() >= 2) continue;
-          const double sd = ((t - j.t_submit) * rate + j.work_left) / std::max(j.work_total, 1.0);
-          if (!pick || sd > best || (sd == best && j.work_left < pick->work_left)) {
+          if (j.lane >= 0 || j.issued_all || j.cancel) continue;
+          // while a short job is on the device or waiting, long jobs issue
+          // nothing new (their chunks in flight drain) unless one has waited
+          // kLongWaitMax since its last chunk
+          if (short_active && !j.short_job && tnow - j.last_issue < kLongWaitMax) continue;
+          if (!pick || j.work_left < pick->work_left ||
+              (j.work_left == pick->work_left && j.t_submit < pick->t_submit))
             pick = jp;
-            best = sd;
-          }
         }
       }
       if (!pick) break;
       Job& j = *pick;
+      ar_status e = AR_OK;
+      if (j.begun && hipStreamWaitEvent(ln.stream, j.begun, 0) != hipSuccess)
+        e = fail(AR_ERR_DEVICE, "hipStreamWaitEvent failed");
+      j.run->stream = ln.stream;
       bool done = false;
-      const double chunk_work = std::min(j.work_left, j.work_chunk);
-      ar_status e = j.run->step(&done);
+      if (e == AR_OK) e = j.run->step(&done);
       j.work_left = std::max(0.0, j.work_left - j.work_chunk);
       if (e == AR_OK && done) {
         e = j.run->finish();
         if (e == AR_OK) {
           Slot& sl = *slots[j.slot];
           size_t bytes = (size_t)j.spec.width * j.spec.height * 3 * sizeof(float);
-          if (hipMemcpyAsync(sl.pinned, sl.ws.out.p, bytes, hipMemcpyDeviceToHost, sl.stream) != hipSuccess)
+          if (hipMemcpyAsync(sl.pinned, sl.ws.out.p, bytes, hipMemcpyDeviceToHost, ln.stream) != hipSuccess)
             e = fail(AR_ERR_DEVICE, "copy-out failed");
         }
         j.issued_all = true;
       }
+      if (e == AR_OK && hipEventRecord(ln.done, ln.stream) != hipSuccess) e = fail(AR_ERR_DEVICE, "event record failed");
       if (e != AR_OK) {
-        (void)hipStreamSynchronize(slots[j.slot]->stream);
+        (void)hipStreamSynchronize(ln.stream);
         std::lock_guard<std::mutex> lk(mu);
-        for (auto ev : j.inflight) event_pool.push_back(ev);
-        inflight -= (int)j.inflight.size();
-        j.inflight.clear();
-        j.inflight_issue.clear();
-        j.inflight_work.clear();
         fail_job(j, e, ar_last_error());
         release(j);
         active.erase(std::find(active.begin(), active.end(), pick));
@@ -375,12 +383,9 @@ struct ar_server {
         progress = true;
         continue;
       }
-      hipEvent_t ev = get_event();
-      (void)hipEventRecord(ev, slots[j.slot]->stream);
-      j.inflight.push_back(ev);
-      j.inflight_issue.push_back(t);
-      j.inflight_work.push_back(chunk_work);
-      ++inflight;
+      j.lane = (int)li;
+      j.last_issue = now();
+      ln.job = pick;
       progress = true;
     }
     return progress;
@@ -418,10 +423,17 @@ ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_se
   if (srv->cfg.hbm_budget_bytes == 0) srv->cfg.hbm_budget_bytes = 200ull << 30;
   if (srv->cfg.max_active_jobs == 0) srv->cfg.max_active_jobs = 8;
   if (srv->cfg.max_active_jobs > 64) return fail(AR_ERR_INVALID_ARG, "max_active_jobs above 64");
+  if (srv->cfg.max_concurrent_jobs == 0) srv->cfg.max_concurrent_jobs = std::min<uint32_t>(3, srv->cfg.max_active_jobs);
+  if (srv->cfg.max_concurrent_jobs > kMaxLanes)
+    return fail(AR_ERR_INVALID_ARG, "max_concurrent_jobs above 4 (the hardware queues of a process)");
   (void)hipGetDevice(&srv->device);
   auto drop_streams = [&srv]() {
     for (auto& sp : srv->slots)
       if (sp->stream) (void)hipStreamDestroy(sp->stream);
+    for (auto& ln : srv->lanes) {
+      if (ln.stream) (void)hipStreamDestroy(ln.stream);
+      if (ln.done) (void)hipEventDestroy(ln.done);
+    }
   };
   for (uint32_t i = 0; i < srv->cfg.max_active_jobs; ++i) {
     srv->slots.emplace_back(new Slot());
@@ -429,6 +441,16 @@ ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_se
     if (e != hipSuccess) {
       drop_streams();
       return fail(AR_ERR_DEVICE, std::string("hipStreamCreateWithFlags: ") + hipGetErrorString(e));
+    }
+  }
+  for (uint32_t i = 0; i < srv->cfg.max_concurrent_jobs; ++i) {
+    srv->lanes.emplace_back();
+    Lane& ln = srv->lanes.back();
+    hipError_t e = hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ln.done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+      drop_streams();
+      return fail(AR_ERR_DEVICE, std::string("lane stream: ") + hipGetErrorString(e));
     }
   }
   srv->t0 = std::chrono::steady_clock::now();
@@ -473,6 +495,11 @@ ar_status ar_server_destroy(ar_server* srv) {
     std::unique_lock<std::mutex> lk(srv->mu);
     srv->cv_done.notify_all();
     srv->cv_done.wait(lk, [srv]() { return srv->waiters == 0; });
+  }
+  for (auto& ln : srv->lanes) {
+    if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+    if (ln.stream) (void)hipStreamDestroy(ln.stream);
+    if (ln.done) (void)hipEventDestroy(ln.done);
   }
   for (auto& sp : srv->slots) {
     Slot& sl = *sp;

@@ -634,6 +634,95 @@ def bench_bdpt_gdpt(ar, torch):
     return out
 
 
+def _crop(img, x0, y0, n):
+    return img[y0:y0 + n, x0:x0 + n]
+
+
+def bench_cpu_and_quality(ar, torch):
+    """CPU baselines of the path-sampling configurations (the oracle, a C++
+    restatement of the reference CPU path, on all host threads; Poisson SOR
+    single-threaded as the reference runs it), each on a bounded sample of
+    the workload, and relMSE at equal spp: the GPU's and the oracle's 16-spp
+    renders against a 4096-spp oracle render of the same pixels
+    (mean over pixel-channels of (I - R)^2 / (R^2 + 1e-2), 4 seeds each)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    wl = ar.workloads
+    threads = cpu_threads()
+    model = cpu_model()
+    out = {}
+
+    def cam_for(desc, w, h):
+        cd = desc.camera
+        return wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, w / h)
+    # C1: the whole Cornell workload (256^2 x 16 spp, depth 8)
+    cor = wl.cornell_box()
+    osc = orc.OracleScene(cor)
+    c1 = cam_for(cor, 256, 256)
+    _, _, _, secs = osc.render_pt(c1, 256, 256, 16, 8, 3, rng_mode=0, nthreads=threads)
+    out["pt_cornell_256x256x16spp"] = {"value": round(256 * 256 * 16 / secs / 1e6, 3), "unit": "Mpaths/s",
+                                       "cores": threads, "kind": "port", "seconds": round(secs, 3),
+                                       "sample": f"the whole workload, {threads} threads, {model}"}
+    log(f"cpu: C1 {out['pt_cornell_256x256x16spp']}")
+    # relMSE on the central 64 x 64 pixels of C1
+    x0 = y0 = 96
+    n = 64
+    ref, _, _, rsecs = osc.render_pt(c1, 256, 256, 4096, 8, 3, rng_mode=1, seed=12345, nthreads=threads,
+                                     rows=(y0, y0 + n), cols=(x0, x0 + n))
+    ref = _crop(ref, x0, y0, n)
+    sc = ar.Scene.from_desc(cor)
+    cam = ar.PinholeCamera.from_array(c1)
+
+    def relmse(x):
+        return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+    rg = [relmse(_crop(ar.PathTracer(ar.PathTracerConfig(spp=16, max_depth=8, rr_depth=3), seed=100 + k)
+                       .render(sc, cam, 256, 256).pixels, x0, y0, n)) for k in range(4)]
+    ro = [relmse(_crop(osc.render_pt(c1, 256, 256, 16, 8, 3, rng_mode=1, seed=200 + k, nthreads=threads,
+                                     rows=(y0, y0 + n), cols=(x0, x0 + n))[0], x0, y0, n)) for k in range(4)]
+    out["relmse_cornell_16spp"] = {"gpu": round(float(np.mean(rg)), 6), "cpu": round(float(np.mean(ro)), 6),
+                                   "ratio": round(float(np.mean(rg) / np.mean(ro)), 4),
+                                   "reference": f"oracle 4096 spp on pixels [{x0},{x0 + n})^2 of 256^2 "
+                                                f"({rsecs:.1f} s)", "seeds": 4}
+    log(f"cpu: relMSE {out['relmse_cornell_16spp']}")
+    # C3 crops: four 64 x 64 crops of the 1920x1080 room at 64 spp, depth 16
+    room = wl.procedural_room()
+    orm = orc.OracleScene(room)
+    c3 = cam_for(room, 1920, 1080)
+    crops = [(480, 270), (1376, 270), (480, 746), (1376, 746)]
+    secs = 0.0
+    for (cx, cy) in crops:
+        secs += orm.render_pt(c3, 1920, 1080, 64, 16, 3, rng_mode=0, nthreads=threads, rows=(cy, cy + 64),
+                              cols=(cx, cx + 64))[3]
+    out["pt_room_1920x1080x64spp"] = {"value": round(4 * 64 * 64 * 64 / secs / 1e6, 3), "unit": "Mpaths/s",
+                                      "cores": threads, "kind": "port", "seconds": round(secs, 3),
+                                      "sample": f"crops 64x64 at {crops} of 1920x1080, 64 spp, {threads} threads"}
+    log(f"cpu: C3 {out['pt_room_1920x1080x64spp']}")
+    # C4: BDPT 16 spp depth 10 and GDPT 32 spp depth 16 on a 128 x 128 crop of 1280x720
+    c4 = cam_for(room, 1280, 720)
+    win = (576, 704, 296, 424)
+    _, _, secs = orc.render_bdpt(orm, c4, 1280, 720, 16, 10, 3, 0, 2.0, rng_mode=0, nthreads=threads, window=win)
+    out["bdpt_room_1280x720x16spp"] = {"value": round(128 * 128 * 16 / secs / 1e6, 4), "unit": "Msamples/s",
+                                       "cores": threads, "kind": "port", "seconds": round(secs, 3),
+                                       "sample": f"pixels {win} (x0, x1, y0, y1) of 1280x720, 16 spp, depth 10"}
+    log(f"cpu: C4 BDPT {out['bdpt_room_1280x720x16spp']}")
+    p, gx, gy, secs = orc.render_gdpt(orm, c4, 1280, 720, 32, 16, 3, rng_mode=0, nthreads=threads, window=win)
+    out["gdpt_room_1280x720x32spp"] = {"value": round(3 * 128 * 128 * 32 / secs / 1e6, 4), "unit": "Mpaths/s",
+                                       "cores": threads, "kind": "port", "seconds": round(secs, 3),
+                                       "sample": f"pixels {win} of 1280x720, 32 spp x 3 paths, depth 16"}
+    log(f"cpu: C4 GDPT {out['gdpt_room_1280x720x32spp']}")
+    # the reference's screened Poisson (red-black SOR, 200 sweeps, one thread) at 1280x720
+    full = [np.zeros((720, 1280, 3), np.float32) for _ in range(3)]
+    rng = np.random.default_rng(7)
+    for f in full:
+        f[:] = rng.random(f.shape, dtype=np.float32)
+    _, secs = orc.poisson_sor(full[0], full[1], full[2], 0.2, iterations=200, omega=1.6, sign=1)
+    out["poisson_sor200_1280x720"] = {"value": round(secs * 1e3, 1), "unit": "ms", "cores": 1, "kind": "port",
+                                      "sample": "one solve on random fp32 films (SOR time does not depend on "
+                                                "the values)"}
+    log(f"cpu: SOR-200 {out['poisson_sor200_1280x720']}")
+    return out
+
+
 def bench_serving(args, ar, torch):
     """Continuous batching against the resident room scene.  Job mix: square
     side U{64..1024 step 64}, spp round(exp(U[0, ln 256])), PT 60% / BDPT 25% /
@@ -687,7 +776,8 @@ def bench_serving(args, ar, torch):
     arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
     log(f"serving: {len(spec_jobs)} jobs at {args.rate}/s specified; mean solo {mean_solo:.3f}s -> offered load "
         f"{rho_spec:.2f}; serving the first {len(jobs)} at {rate:.3f}/s (load {args.load})")
-    srv = ar.RenderServer(sc, hbm_budget_bytes=200 << 30, max_active_jobs=args.slots)
+    srv = ar.RenderServer(sc, hbm_budget_bytes=200 << 30, max_active_jobs=args.slots,
+                          max_concurrent_jobs=args.lanes)
     warm = [srv.submit(integ(j), cam(j), j.width, j.height) for j in jobs[: args.slots]]
     for w in warm:
         srv.wait(w)
@@ -722,7 +812,7 @@ def bench_serving(args, ar, torch):
             f"{solos[k] * 1e3:.1f} ms, latency {lat[k] * 1e3:.1f} ms (admission wait {waits[k] * 1e3:.1f} ms, "
             f"{ratio[k]:.1f}x)")
     return {"jobs": len(jobs), "rate_per_s": round(rate, 3), "arrival_window_s": round(window, 2),
-            "slots": args.slots, "specified_rate_per_s": args.rate, "specified_jobs": len(spec_jobs),
+            "slots": args.slots, "lanes": args.lanes, "specified_rate_per_s": args.rate, "specified_jobs": len(spec_jobs),
             "mean_solo_s": round(mean_solo, 4), "offered_load_at_specified_rate": round(rho_spec, 2),
             "offered_load": args.load,
             "job_mix": {"pt": sum(j.integrator == 0 for j in jobs), "bdpt": sum(j.integrator == 1 for j in jobs),
@@ -754,11 +844,13 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the path-tracer throughput fields")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter runs for roofline.traffic")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg of the default run")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0, help="specified arrival window (s)")
     ap.add_argument("--load", type=float, default=0.7, help="serving: target offered load when the rate is above it")
     ap.add_argument("--serve-jobs", type=int, default=60, help="serving: jobs of the specified mix actually served")
     ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--lanes", type=int, default=0, help="serving: jobs on the device at once (0 = the server's default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
@@ -799,6 +891,12 @@ def main():
                         "pt_room_1920x1080x64spp": bench_pt(ar, torch, "room")}
         out["extra"].update(bench_bdpt_gdpt(ar, torch))
         out["extra"]["tree_builders_soup1M"] = bench_builders(ar, torch)
+        if not args.no_cpu_baseline:
+            out["extra"]["cpu_baselines"] = bench_cpu_and_quality(ar, torch)
+        if not args.no_serving:
+            sargs = argparse.Namespace(**vars(args))
+            sargs.load, sargs.serve_jobs = 0.5, 60
+            out["extra"]["serving_room"] = bench_serving(sargs, ar, torch)
     print(json.dumps(out), flush=True)
 
 

@@ -332,7 +332,8 @@ typedef struct ar_server ar_server;
 typedef struct ar_server_config {
   uint64_t hbm_budget_bytes; /* admission budget for per-job device state */
   uint32_t max_active_jobs;  /* job slots rendering concurrently */
-  uint32_t path_slots;       /* in-flight path slots shared by all jobs (0 = auto) */
+  uint32_t max_concurrent_jobs; /* jobs whose chunks are on the device at once, each on its own
+                                  stream (1..4; 0 = min(3, max_active_jobs)) */
 } ar_server_config;
 typedef struct ar_job_desc {
   uint32_t width, height;

@@ -52,11 +52,13 @@ def lib():
             "orc_poisson_sor": (d, [P, P, P, u32, u32, f, u32, f, C.c_int, P]),
             "orc_render_bdpt": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, f, C.c_int, u64, C.c_int, P, P]),
             "orc_render_bdpt_ex": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, f, C.c_int, u64, C.c_int,
-                                       C.c_int, C.c_int, C.c_int, P, P]),
+                                       C.c_int, C.c_int, C.c_int, u32, u32, u32, u32, P, P]),
+            "orc_render_gdpt_ex": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, u64, C.c_int, u32, u32, u32, u32,
+                                       P, P, P]),
             "orc_light_dist": (u32, [P, P, P]),
             "orc_light_sample_index": (u32, [P, f, P]),
-            "orc_render_pt_ex": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, u64, C.c_int, u32, u32, C.c_int, u32,
-                                     P, P, P]),
+            "orc_render_pt_ex": (d, [P, P, u32, u32, u32, u32, u32, C.c_int, u64, C.c_int, u32, u32, u32, u32,
+                                     C.c_int, u32, P, P, P]),
             "orc_film_rgb8": (None, [P, u64, u32, f, f, f, P]),
             "orc_scene_add_texture": (C.c_int, [P, P, u32, u32]),
             "orc_scene_set_material_texture": (None, [P, u32, C.c_int]),
@@ -183,41 +185,47 @@ class OracleScene:
         return occ.astype(bool), cnt, secs
 
     def render_pt(self, cam16, w, h, spp, max_depth=8, rr_depth=3, rng_mode=1, seed=0, nthreads=0,
-                  rows=None, want_m2=False, pt_mode=0, only_len=0):
+                  rows=None, want_m2=False, pt_mode=0, only_len=0, cols=None):
         """PathTracer::render.  pt_mode / only_len are test-only estimator
         switches: 0 MIS, 1 light sampling only, 2 BSDF hits only; only_len > 0
-        keeps paths of exactly that many vertices (camera and light included)."""
+        keeps paths of exactly that many vertices (camera and light included).
+        rows / cols = (begin, end) restrict the render to a window."""
         mean = np.zeros((h, w, 3), np.float32)
         m2 = np.zeros((h, w, 3), np.float32) if want_m2 else None
         cnt = np.zeros(2, np.uint64)
         r0, r1 = (0, h) if rows is None else rows
+        c0, c1 = (0, w) if cols is None else cols
         cam = np.ascontiguousarray(cam16, np.float32)
         secs = lib().orc_render_pt_ex(self.h, _p(cam), w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads, r0,
-                                      r1, pt_mode, only_len, _p(mean), _p(m2), _p(cnt))
+                                      r1, c0, c1, pt_mode, only_len, _p(mean), _p(m2), _p(cnt))
         return mean, m2, cnt, secs
 
 
-def render_gdpt(scene, cam16, w, h, spp, max_depth=8, rr_depth=3, rng_mode=1, seed=0, nthreads=0):
-    """GDPT pass 1: (primal, dx, dy) films, each (h, w, 3), and seconds."""
+def render_gdpt(scene, cam16, w, h, spp, max_depth=8, rr_depth=3, rng_mode=1, seed=0, nthreads=0, window=None):
+    """GDPT pass 1: (primal, dx, dy) films, each (h, w, 3), and seconds.
+    window = (x0, x1, y0, y1) restricts the render to those pixels."""
     films = [np.zeros((h, w, 3), np.float32) for _ in range(3)]
     cam = np.ascontiguousarray(cam16, np.float32)
-    secs = lib().orc_render_gdpt(scene.h, _p(cam), w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads,
-                                 _p(films[0]), _p(films[1]), _p(films[2]))
+    x0, x1, y0, y1 = (0, 0, 0, 0) if window is None else window
+    secs = lib().orc_render_gdpt_ex(scene.h, _p(cam), w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads,
+                                    x0, x1, y0, y1, _p(films[0]), _p(films[1]), _p(films[2]))
     return films[0], films[1], films[2], secs
 
 
 def render_bdpt(scene, cam16, w, h, spp, max_depth=8, rr_depth=3, mis_mode=0, mis_beta=2.0, rng_mode=1, seed=0,
-                nthreads=0, strategy=None, unweighted=False):
+                nthreads=0, strategy=None, unweighted=False, window=None):
     """BidirectionalPathTracer::render: (film, splat film, seconds).
     mis_mode 0 power heuristic (reference walk), 1 uniform, 2 textbook walk
     (test only).  strategy=(s, t) evaluates that strategy alone and
-    unweighted=True sets every MIS weight to 1 (test only)."""
+    unweighted=True sets every MIS weight to 1 (test only).  window = (x0, x1,
+    y0, y1) traces the camera paths of those pixels only."""
     out = np.zeros((h, w, 3), np.float32)
     spl = np.zeros((h, w, 3), np.float32)
     cam = np.ascontiguousarray(cam16, np.float32)
     only_s, only_t = (-1, -1) if strategy is None else strategy
+    x0, x1, y0, y1 = (0, 0, 0, 0) if window is None else window
     secs = lib().orc_render_bdpt_ex(scene.h, _p(cam), w, h, spp, max_depth, rr_depth, mis_mode, mis_beta, rng_mode,
-                                    seed, nthreads, only_s, only_t, int(unweighted), _p(out), _p(spl))
+                                    seed, nthreads, only_s, only_t, int(unweighted), x0, x1, y0, y1, _p(out), _p(spl))
     return out, spl, secs
 
 

@@ -334,8 +334,8 @@ int orc_bsdf_sample(uint32_t type, const float* p, const float* wo, float u_sel,
 // pt_mode / only_len: test-only estimator switches (PtConfig).
 double orc_render_pt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, uint32_t spp,
                         uint32_t max_depth, uint32_t rr_depth, int rng_mode, uint64_t seed, int nthreads,
-                        uint32_t row_begin, uint32_t row_end, int pt_mode, uint32_t only_len, float* out_mean,
-                        float* out_m2, uint64_t* counters) {
+                        uint32_t row_begin, uint32_t row_end, uint32_t col_begin, uint32_t col_end, int pt_mode,
+                        uint32_t only_len, float* out_mean, float* out_m2, uint64_t* counters) {
   Scene* sc = (Scene*)s;
   Camera cam = cam_from(cam16);
   PtConfig cfg;
@@ -345,12 +345,13 @@ double orc_render_pt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, uin
   cfg.pt_mode = pt_mode;
   cfg.only_len = only_len;
   if (row_end == 0 || row_end > h) row_end = h;
+  if (col_end == 0 || col_end > w) col_end = w;
   auto t0 = std::chrono::steady_clock::now();
   std::atomic<uint64_t> cn(0), ct(0);
   parallel_rows(row_end - row_begin, nthreads, [&](uint32_t yy) {
     uint32_t y = row_begin + yy;
     TraceCounters c;
-    for (uint32_t x = 0; x < w; ++x) {
+    for (uint32_t x = col_begin; x < col_end; ++x) {
       RefSampler ref(RefSampler::pixel_seed(x, y));
       V3 acc, acc2;
       for (uint32_t si = 0; si < spp; ++si) {
@@ -385,7 +386,7 @@ double orc_render_pt(void* s, const float* cam16, uint32_t w, uint32_t h, uint32
                      uint32_t rr_depth, int rng_mode, uint64_t seed, int nthreads, uint32_t row_begin,
                      uint32_t row_end, float* out_mean, float* out_m2, uint64_t* counters) {
   return orc_render_pt_ex(s, cam16, w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads, row_begin, row_end, 0,
-                          0, out_mean, out_m2, counters);
+                          0, 0, 0, out_mean, out_m2, counters);
 }
 
 }  // extern "C"

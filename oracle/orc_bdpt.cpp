@@ -420,7 +420,8 @@ extern "C" {
 // (-1: all); unweighted (test only): MIS weight 1.
 double orc_render_bdpt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth,
                           uint32_t rr_depth, int mis_mode, float mis_beta, int rng_mode, uint64_t seed, int nthreads,
-                          int only_s, int only_t, int unweighted, float* out, float* out_splat) {
+                          int only_s, int only_t, int unweighted, uint32_t x0, uint32_t x1, uint32_t y0,
+                          uint32_t y1, float* out, float* out_splat) {
   Scene* sc = (Scene*)s;
   Camera cam;
   const float* c = cam16;
@@ -448,10 +449,13 @@ double orc_render_bdpt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, u
   };
   std::mutex mu;
   auto t0 = std::chrono::steady_clock::now();
-  parallel_rows(h, nthreads, [&](uint32_t yr) {
+  if (x1 == 0 || x1 > w) x1 = w;
+  if (y1 == 0 || y1 > h) y1 = h;
+  parallel_rows(y1 - y0, nthreads, [&](uint32_t yy) {
+    const uint32_t yr = y0 + yy;
     std::vector<std::pair<size_t, V3>> local;
     std::vector<PVertex> cvv, lvv;
-    for (uint32_t x = 0; x < w; ++x) {
+    for (uint32_t x = x0; x < x1; ++x) {
       RefSampler ref(RefSampler::pixel_seed(x, yr));
       V3 acc;
       for (uint32_t si = 0; si < spp; ++si) {
@@ -528,7 +532,7 @@ double orc_render_bdpt(void* s, const float* cam16, uint32_t w, uint32_t h, uint
                        uint32_t rr_depth, int mis_mode, float mis_beta, int rng_mode, uint64_t seed, int nthreads,
                        float* out, float* out_splat) {
   return orc_render_bdpt_ex(s, cam16, w, h, spp, max_depth, rr_depth, mis_mode, mis_beta, rng_mode, seed, nthreads, -1,
-                            -1, 0, out, out_splat);
+                            -1, 0, 0, 0, 0, 0, out, out_splat);
 }
 
 }  // extern "C"

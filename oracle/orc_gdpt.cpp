@@ -127,9 +127,11 @@ extern "C" {
 
 // GDPT pass 1 (gradient_domain_pt.rs:346-393): primal, dx, dy films
 // (accum * (1/spp)).  Returns wall seconds.
-double orc_render_gdpt(void* s, const float* cam16, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth,
-                       uint32_t rr_depth, int rng_mode, uint64_t seed, int nthreads, float* primal, float* dxo,
-                       float* dyo) {
+// Pixels [x0, x1) x [y0, y1) only (x1, y1 = 0: to the edge); the others are
+// left untouched.
+double orc_render_gdpt_ex(void* s, const float* cam16, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth,
+                          uint32_t rr_depth, int rng_mode, uint64_t seed, int nthreads, uint32_t x0, uint32_t x1,
+                          uint32_t y0, uint32_t y1, float* primal, float* dxo, float* dyo) {
   Scene* sc = (Scene*)s;
   Camera cam;
   const float* c = cam16;
@@ -144,8 +146,11 @@ double orc_render_gdpt(void* s, const float* cam16, uint32_t w, uint32_t h, uint
   cfg.max_depth = max_depth;
   cfg.rr_depth = rr_depth;
   auto t0 = std::chrono::steady_clock::now();
-  parallel_rows(h, nthreads, [&](uint32_t y) {
-    for (uint32_t x = 0; x < w; ++x) {
+  if (x1 == 0 || x1 > w) x1 = w;
+  if (y1 == 0 || y1 > h) y1 = h;
+  parallel_rows(y1 - y0, nthreads, [&](uint32_t yy) {
+    const uint32_t y = y0 + yy;
+    for (uint32_t x = x0; x < x1; ++x) {
       RefSampler inner(RefSampler::pixel_seed(x, y));
       ReplaySampler rep(&inner);
       V3 pa, xa, ya;
@@ -170,6 +175,13 @@ double orc_render_gdpt(void* s, const float* cam16, uint32_t w, uint32_t h, uint
     }
   });
   return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+double orc_render_gdpt(void* s, const float* cam16, uint32_t w, uint32_t h, uint32_t spp, uint32_t max_depth,
+                       uint32_t rr_depth, int rng_mode, uint64_t seed, int nthreads, float* primal, float* dxo,
+                       float* dyo) {
+  return orc_render_gdpt_ex(s, cam16, w, h, spp, max_depth, rr_depth, rng_mode, seed, nthreads, 0, 0, 0, 0, primal,
+                            dxo, dyo);
 }
 
 double orc_poisson_sor(const float* P, const float* gx, const float* gy, uint32_t w, uint32_t h, float alpha,

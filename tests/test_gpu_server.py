@@ -2,6 +2,10 @@
 
 * every job's film in a concurrent mix of PT / BDPT / GDPT jobs of different
   sizes equals, bit for bit, the same job rendered alone with ar_render;
+* a sample of the specified serving mix (workloads.serving_jobs: square
+  sides up to 1024, spp log-uniform up to 256, PT / BDPT / GDPT) on the full
+  room scene, submitted at Poisson arrivals: every job completes and equals
+  its solo render bit for bit, with one lane and with several;
 * a short job submitted while a long one runs is admitted and finishes first;
 * queued and running jobs can be cancelled; the server stays usable;
 * a job larger than the HBM budget is rejected at submit.
@@ -83,3 +87,57 @@ def test_cancel_and_budget(ar, wl, room):
         tiny.submit(_integ(ar, 0, 4, 1), _cam(ar, wl, desc, 512, 512), 512, 512)
     assert e.value.code == -3
     tiny.close()
+
+
+def _sample_mix(wl, budget=1.6e8):
+    """Jobs of the specified mix, in arrival order, within a work budget (path
+    equivalents: PT 1, GDPT 3, BDPT 5 per pixel-sample), keeping every
+    integrator and the largest side (1024) represented."""
+    jobs = wl.serving_jobs(rate=20.0, duration=60.0, seed=0)
+    w = {0: 1, 1: 5, 2: 3}
+    cost = lambda j: j.width * j.height * j.spp * w[j.integrator]
+    pick, total = [], 0.0
+    for j in jobs:
+        if total + cost(j) <= budget and len(pick) < 12:
+            pick.append(j)
+            total += cost(j)
+    for need in (lambda j: j.integrator == 1, lambda j: j.integrator == 2, lambda j: j.width == 1024):
+        if not any(need(j) for j in pick):
+            pick.append(min((j for j in jobs if need(j)), key=cost))
+    pick.sort(key=lambda j: j.index)
+    assert {j.integrator for j in pick} == {0, 1, 2} and any(j.width == 1024 for j in pick)
+    return pick
+
+
+@pytest.mark.parametrize("lanes", [1, 3])
+def test_serving_mix_at_shape_equals_alone(ar, wl, lanes):
+    desc = wl.procedural_room()
+    sc = ar.Scene.from_desc(desc)
+    cd = desc.camera
+    jobs = _sample_mix(wl)
+    print(f"  {len(jobs)} jobs: " + ", ".join(f"{['pt', 'bdpt', 'gdpt'][j.integrator]} {j.width}^2 x {j.spp}"
+                                               for j in jobs), flush=True)
+
+    def integ(j):
+        if j.integrator == 0:
+            return ar.PathTracer(ar.PathTracerConfig(spp=j.spp, max_depth=16, rr_depth=3), seed=j.seed)
+        if j.integrator == 1:
+            return ar.BidirectionalPathTracer(ar.BdptConfig(spp=j.spp, max_depth=10, rr_depth=3), seed=j.seed)
+        return ar.GradientDomainPathTracer(ar.GdptConfig(spp=j.spp, max_depth=16, poisson_iterations=50),
+                                           seed=j.seed)
+
+    def cam(j):
+        return ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov,
+                                                            j.width / j.height))
+    srv = ar.RenderServer(sc, max_active_jobs=8, max_concurrent_jobs=lanes)
+    rng = np.random.default_rng(lanes)
+    ids = []
+    for j in jobs:
+        time.sleep(float(rng.exponential(0.03)))
+        ids.append(srv.submit(integ(j), cam(j), j.width, j.height))
+    served = [srv.wait(i) for i in ids]
+    srv.close()
+    for j, (film, st) in zip(jobs, served):
+        alone = integ(j).render(sc, cam(j), j.width, j.height).pixels
+        assert np.array_equal(film.pixels.view(np.uint32), alone.view(np.uint32)), (j.index, j.integrator)
+        assert st["finish"] >= st["start"] >= st["submit"] and st["paths"] > 0
