@@ -106,3 +106,24 @@ def test_builder_tree_equals_oracle(ar, orc, wl, case):
     assert np.array_equal(lm, om)
     assert np.array_equal(lp, op)
     assert depth >= 1
+
+
+def test_rust_binding_declares_the_header():
+    """INTEGRATION.md's Rust extern block declares every function of
+    include/agentray.h, with the same number of arguments, and nothing else."""
+    src = open(os.path.join(ROOT, "include", "agentray.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    header = {}
+    for name, args in re.findall(r"\b(ar_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", src, flags=re.S):
+        a = " ".join(args.split())
+        header[name] = 0 if a in ("", "void") else a.count(",") + 1
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    block = doc[doc.index('extern "C" {'):]
+    block = block[:block.index("\n}")]
+    block = re.sub(r"//[^\n]*", "", block)
+    rust = {}
+    for name, args in re.findall(r"pub fn (ar_[a-z0-9_]+)\s*\(([^)]*)\)", block, flags=re.S):
+        a = " ".join(args.split()).rstrip(",")
+        rust[name] = 0 if not a else a.count(",") + 1
+    assert set(rust) == set(header), (sorted(set(header) - set(rust)), sorted(set(rust) - set(header)))
+    assert {k: v for k, v in rust.items() if v != header[k]} == {}
