@@ -151,6 +151,8 @@ def _load():
         "ar_trace": (i32, [P, P, u64, P, P, u32, P]),
         "ar_occluded": (i32, [P, P, u64, P, u32, P]),
         "ar_trace_noalloc": (i32, [P, P, u64, P, P, u32, P]),
+        "ar_render_autotune_chunk": (i32, [P, C.POINTER(ArCamera), u32, u32, C.POINTER(ArRenderConfig), P,
+                                           C.POINTER(u64)]),
         "ar_trace_count": (i32, [P, P, u64, u32, P, C.POINTER(ArTraceCounts)]),
         "ar_camera_lookat": (i32, [P, P, P, f, f, C.POINTER(ArCamera)]),
         "ar_camera_eye_angle": (i32, [P, f, f, f, f, C.POINTER(ArCamera)]),
@@ -715,6 +717,17 @@ class _Integrator:
         c.poisson_sign = int(getattr(self.config, "poisson_sign", 1))
         c.gdpt_outputs = 1 if self.integrator_id == 2 else 0
         return c
+
+    def tune_chunk(self, scene: Scene, camera: PinholeCamera, width: int, height: int) -> int:
+        """Queue chunk of this integrator's renders on `scene`: the paths
+        (BDPT: samples) one wavefront pass holds, timed per candidate on one
+        chunk of this job (ar_render_autotune_chunk) and kept on the scene."""
+        c = self._cfg()
+        c.gdpt_outputs = 0
+        best = C.c_uint64()
+        _check(_lib.ar_render_autotune_chunk(scene.handle, C.byref(camera.c), width, height, C.byref(c), None,
+                                             C.byref(best)))
+        return best.value
 
     def tune_launch(self, scene: Scene, camera: PinholeCamera, width: int, height: int, spp: int = 16,
                     candidates=None) -> dict:

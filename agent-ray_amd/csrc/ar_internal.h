@@ -136,6 +136,9 @@ struct Workspace;
 
 struct ar_scene {
   ar::Workspace* ws = nullptr;  // render workspace used by ar_render
+  // paths (BDPT: samples) per chunk of ar_render when sample_chunk is 0, by
+  // integrator; 0 = the library default; set by ar_render_autotune_chunk
+  uint64_t chunk_target[3] = {0, 0, 0};
   std::vector<ar::DMaterial> materials;
   std::vector<ar::HostMesh> meshes;
   std::vector<float4> texels;    // linear texels of every texture, back to back

@@ -13,6 +13,7 @@ struct JobSpec {
   uint32_t width = 0, height = 0;
   ar_camera camera{};
   ar_render_config cfg{};
+  uint64_t chunk_target = 0;  // paths (BDPT: samples) per chunk when cfg.sample_chunk is 0; 0 = default
 };
 
 class JobRun {

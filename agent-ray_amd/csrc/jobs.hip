@@ -13,7 +13,9 @@ uint32_t chunk_samples(const JobSpec& spec) {
   // bounces (a handful of long paths whose traversal is one dependent chain
   // of loads) and HBM has room: 2^25 paths of wavefront state is ~6.6 GB,
   // 2^24 BDPT samples of vertices ~17 GB at depth 10.
+  // The scene's tuned target (ar_render_autotune_chunk) overrides these.
   uint64_t target = c.integrator == AR_INTEGRATOR_BDPT ? (1ull << 24) : (1ull << 25);
+  if (spec.chunk_target) target = spec.chunk_target;
   uint32_t k = c.sample_chunk;
   if (k == 0) k = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c.spp, target / std::max<uint64_t>(per_sample, 1)));
   return std::min(k, c.spp);

@@ -453,6 +453,8 @@ def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
     tuner = ar.PathTracer(ar.PathTracerConfig(spp=desc.spp, max_depth=desc.max_depth, rr_depth=3), seed=0)
     out["launch_refined"] = tuner.tune_launch(sc, cam, desc.width, desc.height, spp=min(16, desc.spp))
     out["launch_config"] = dict(sc.trace_config(), anyhit=sc.anyhit_config())
+    # and the queue chunk (paths per wavefront pass), kept on the scene
+    out["chunk_paths"] = tuner.tune_chunk(sc, cam, desc.width, desc.height)
     films = {}
     for sched in schedules:
         pt = ar.PathTracer(ar.PathTracerConfig(spp=desc.spp, max_depth=desc.max_depth, rr_depth=3), seed=0,
@@ -559,7 +561,8 @@ def bench_bdpt_gdpt(ar, torch):
     GDPT 32 spp, depth 16, base + two shifted paths per sample, screened
     Poisson alpha 0.2 with 50 CG iterations.  The scene is autotuned on
     ray samples, then each integrator refines the launches on a short render
-    of its own job (PathTracer.tune_launch).  Each timed render follows a
+    of its own job (PathTracer.tune_launch) and picks its queue chunk
+    (tune_chunk).  Each timed render follows a
     warm-up render; the Poisson solve is also timed alone (device buffers,
     HIP events, allocation of its work planes included) against its
     algorithmic bytes: 120 B per pixel and iteration (two passes over fp32
@@ -578,6 +581,7 @@ def bench_bdpt_gdpt(ar, torch):
     # refine both launches on one 4-spp chunk of the BDPT job itself (subpath
     # extension = closest hit, connection = any hit)
     bd_refined = bd.tune_launch(sc, cam, w, h, spp=4)
+    bd_chunk = bd.tune_chunk(sc, cam, w, h)
     bd.render(sc, cam, w, h)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -585,7 +589,8 @@ def bench_bdpt_gdpt(ar, torch):
     dt = time.perf_counter() - t0
     st = bd.last_stats
     out["bdpt_room_1280x720x16spp"] = {"launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()),
-                                       "launch_refined": bd_refined, "msamples_s": round(w * h * 16 / dt / 1e6, 2),
+                                       "launch_refined": bd_refined, "chunk_samples": bd_chunk // (w * h),
+                                       "msamples_s": round(w * h * 16 / dt / 1e6, 2),
                                        "mrays_s": round(st.rays / dt / 1e6, 1), "seconds": round(dt, 4),
                                        "rays": int(st.rays)}
     log(f"bdpt room: {out['bdpt_room_1280x720x16spp']}")
@@ -596,6 +601,7 @@ def bench_bdpt_gdpt(ar, torch):
     sc.set_trace_config(*sample_c)
     sc.set_anyhit_config(*sample_a)
     gd_refined = gd.tune_launch(sc, cam, w, h, spp=8)
+    gd_chunk = gd.tune_chunk(sc, cam, w, h)
     gd.render(sc, cam, w, h)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -604,7 +610,8 @@ def bench_bdpt_gdpt(ar, torch):
     st = gd.last_stats
     g = {"mpaths_s": round(st.paths / dt / 1e6, 2), "mrays_s": round(st.rays / dt / 1e6, 1),
          "seconds": round(dt, 4), "paths": int(st.paths),
-         "launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()), "launch_refined": gd_refined}
+         "launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()), "launch_refined": gd_refined,
+         "chunk_paths": gd_chunk}
     # the Poisson solve alone on the render's own films
     films = [torch.from_numpy(np.ascontiguousarray(gd.aux[k], np.float32)).cuda() for k in range(3)]
     dst = torch.empty_like(films[0])

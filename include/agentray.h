@@ -222,6 +222,15 @@ ar_status ar_scene_get_anyhit_config(const ar_scene* s, ar_trace_config* c);
 ar_status ar_scene_autotune(ar_scene* s, const ar_ray* sample_rays, uint64_t n, uint32_t flags, void* stream,
                             ar_trace_config* best);
 
+/* Queue chunk of ar_render (no reference counterpart): how many paths (BDPT:
+ * samples) one wavefront pass holds when cfg->sample_chunk is 0.  Times one
+ * chunk of the given job at each of five sizes (2^22..2^26 paths; BDPT
+ * 2^19..2^23 samples; at most spp samples per pixel), keeps the smallest
+ * within 3 % of the best rate for this scene and integrator, and returns it.
+ * The scene's ar_render calls of that integrator use it afterwards. */
+ar_status ar_render_autotune_chunk(ar_scene* s, const ar_camera* cam, uint32_t width, uint32_t height,
+                                   const ar_render_config* cfg, void* stream, uint64_t* best_target);
+
 /* ---- traversal (TLAS::intersect, bvh.rs:413-447; Scene::occluded,
  * scene.rs:166-168) ------------------------------------------------------- */
 /* Closest hit for n rays.  inst_ids may be NULL. */

@@ -56,6 +56,33 @@ def test_launch_refinement_keeps_the_film(ar, wl):
     assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
 
 
+@pytest.mark.parametrize("integ", ["pt", "bdpt", "gdpt"])
+def test_chunk_tuning_keeps_the_film(ar, wl, integ):
+    """tune_chunk times queue chunks of the job and keeps one on the scene; the
+    chosen chunk is a whole number of samples per pixel within the job, and
+    the film rendered with it equals the one-sample-per-chunk film bit for bit."""
+    desc = wl.cornell_box()
+    sc = ar.Scene.from_desc(desc)
+    w, h, spp = 512, 256, 32
+    cam = ar.PinholeCamera.from_array(
+        wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, w / h))
+
+    def make(chunk):
+        if integ == "pt":
+            return ar.PathTracer(ar.PathTracerConfig(spp=spp, max_depth=8, rr_depth=3), sample_chunk=chunk)
+        if integ == "bdpt":
+            return ar.BidirectionalPathTracer(ar.BdptConfig(spp=spp, max_depth=6, rr_depth=3), sample_chunk=chunk)
+        return ar.GradientDomainPathTracer(ar.GdptConfig(spp=spp, max_depth=6, rr_depth=3), sample_chunk=chunk)
+
+    per_sample = w * h * (3 if integ == "gdpt" else 1)
+    target = make(0).tune_chunk(sc, cam, w, h)
+    assert target % per_sample == 0 and per_sample <= target <= spp * per_sample
+    tuned = make(0).render(sc, cam, w, h).pixels.copy()
+    single = make(1).render(sc, cam, w, h).pixels.copy()
+    assert np.array_equal(tuned.view(np.uint32), single.view(np.uint32))
+    assert tuned.mean() > 0.01
+
+
 def test_schedules_and_chunking_bitwise_identical(ar, cornell):
     desc, sc, cam, _, _ = cornell
     a, sa = render(ar, sc, cam, 64, 48, 8, schedule=1)
