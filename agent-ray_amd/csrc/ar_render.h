@@ -23,8 +23,63 @@ struct WaveBufs {
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays
 };
 
+// A chunk's launch sequence captured once in a HIP graph and replayed for
+// every later chunk whose launches are identical.  The key holds everything
+// the captured launches take by value (buffers, grids, launch configuration,
+// scene views); what changes per chunk is read from device memory.
+struct GraphCache {
+  static constexpr int kKeyWords = 12;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool valid = false;
+  uint64_t key[kKeyWords] = {};
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    valid = false;
+  }
+  // Launches `record` on st through the cached graph, capturing it first when
+  // the key differs from the cached one.
+  template <class F>
+  ar_status launch(const uint64_t (&k)[kKeyWords], hipStream_t st, const char* what, F&& record) {
+    if (!valid || std::memcmp(k, key, sizeof(key)) != 0) {
+      reset();
+      AR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      record();
+      hipError_t le = hipGetLastError();
+      hipError_t ce = hipStreamEndCapture(st, &graph);
+      if (le != hipSuccess) return fail(AR_ERR_DEVICE, std::string(what) + " capture: " + hipGetErrorString(le));
+      AR_HIP(ce);
+      AR_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      std::memcpy(key, k, sizeof(key));
+      valid = true;
+    }
+    AR_HIP(hipGraphLaunch(exec, st));
+    return AR_OK;
+  }
+  ~GraphCache() { reset(); }
+};
+
+// Bytes of the device block that holds a chunk's integrator parameters.
+constexpr size_t kParamsBytes = 1024;
+
+struct ShadeView;
+// FNV-1a over every field of a scene view, as held by value in captured launches.
+uint64_t view_hash(const ShadeView& v);
+// Closest-hit and any-hit launch configurations packed into key words.
+inline uint64_t launch_key(const ar_trace_config& tc, const ar_trace_config& ta) {
+  return ((uint64_t)tc.workgroup_size << 48) | ((uint64_t)tc.lds_stack << 40) | ((uint64_t)tc.park_slots << 32) |
+         ((uint64_t)ta.workgroup_size << 16) | ((uint64_t)ta.lds_stack << 8) | ta.park_slots;
+}
+inline uint64_t launch_key_thresholds(const ar_trace_config& tc, const ar_trace_config& ta) {
+  return ((uint64_t)tc.refill_min << 48) | ((uint64_t)tc.search_min << 32) | ((uint64_t)ta.refill_min << 16) |
+         ta.search_min;
+}
+
 // Device memory of one render in flight (one per scene for ar_render, one
-// per job slot for the server).  Grows monotonically, never shrinks.
+// per job slot for the server).  Grows as jobs need; trim() frees it.
 struct Workspace {
   DevBuf buf;       // path state + queues + film accumulator + stack spill
   DevBuf out;       // staging for host output
@@ -34,16 +89,13 @@ struct Workspace {
   float4* acc = nullptr;  // per-pixel film accumulators (3 planes of npix)
   int2* spill = nullptr;  // traversal stack spill, [entry][thread]
   uint32_t capacity = 0;
-  void* params = nullptr;  // device copy of the chunk's PtParams (read by the generate / shade stages)
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t graph_exec = nullptr;
-  bool graph_valid = false;
-  uint64_t graph_key[8] = {};  // what the captured bounce loop was recorded for
+  void* params = nullptr;  // device copy of the chunk's parameters (kParamsBytes), read by captured stages
+  GraphCache pt_graph;     // path tracer / GDPT bounce loop of one chunk
+  GraphCache bdpt_graph;   // BDPT chunk: subpaths, connection passes, film
   ar_status reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_entries, uint32_t grid_threads,
                           hipStream_t st);
   size_t bytes() const { return buf.bytes + out.bytes + aux.bytes + pois.bytes; }
   void trim();  // frees every buffer (the caller has drained the stream that used them)
-  ~Workspace();
 };
 
 uint32_t stack_spill_entries(const ar_scene* s);

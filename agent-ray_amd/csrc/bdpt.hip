@@ -52,15 +52,15 @@ struct BdptParams {
 };
 
 struct Verts {
-  float4* pf;    // (p.xyz, pdf_fwd)          [2][V][npaths]
+  float4* pf;    // (p.xyz, pdf_fwd)          [2][V][stride]
   float4* nr;    // (n.xyz, pdf_rev)
   float4* bm;    // (beta.xyz, meta bits: type | delta << 2 | emitter << 3 | id << 4)
   float2* uv;    // tex_uv of surface vertices (textured albedo)
   uint2* count;  // (camera vertices, light vertices) per path
-  uint32_t npaths;
+  uint32_t stride;  // path slots per vertex plane: the workspace's capacity (a chunk uses npix * k of them)
   uint32_t V;
   __device__ __forceinline__ size_t at(uint32_t side, uint32_t k, uint32_t p) const {
-    return ((size_t)side * V + k) * npaths + p;
+    return ((size_t)side * V + k) * stride + p;
   }
 };
 
@@ -305,7 +305,8 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_subpaths(ShadeView v, BdptParams
   stk.S = kTraceLdsStack;
   stk.sp = 0;
   uint32_t nrays = 0;
-  for (uint32_t p = gtid; p < Vt.npaths; p += nthreads) {
+  const uint32_t n = P.pt.npix * P.pt.k;
+  for (uint32_t p = gtid; p < n; p += nthreads) {
     uint32_t pixel = p % P.pt.npix;
     uint32_t smp = P.pt.s0 + p / P.pt.npix;
     Rng rng;
@@ -383,9 +384,11 @@ __device__ __forceinline__ void sub_state(const WaveBufs& B, uint32_t p, const R
 
 // Camera vertex and first ray of every path (:338-361); queue 0 = all paths.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_bdpt_cam_gen(BdptParams P, Verts Vt, WaveBufs B) {
+__global__ __launch_bounds__(BLOCK) void k_bdpt_cam_gen(const BdptParams* __restrict__ Pd, Verts Vt, WaveBufs B) {
+  const BdptParams P = *Pd;
+  const uint32_t n = P.pt.npix * P.pt.k;
   const uint32_t nthreads = gridDim.x * BLOCK;
-  for (uint32_t p = xcd_block() * BLOCK + threadIdx.x; p < Vt.npaths; p += nthreads) {
+  for (uint32_t p = xcd_block() * BLOCK + threadIdx.x; p < n; p += nthreads) {
     const uint32_t pixel = p % P.pt.npix, smp = P.pt.s0 + p / P.pt.npix;
     Rng rng;
     rng.init(P.pt.seed, pixel, smp, 0);
@@ -408,7 +411,7 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_cam_gen(BdptParams P, Verts Vt, 
     B.q[0][p] = p;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    B.cnt[0] = P.pt.max_depth > 0 ? Vt.npaths : 0u;
+    B.cnt[0] = P.pt.max_depth > 0 ? n : 0u;
     B.cnt[1] = 0u;
   }
 }
@@ -417,16 +420,19 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_cam_gen(BdptParams P, Verts Vt, 
 // continuing at the camera subpath's last dimension; emitting lights go to
 // queue 0.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_bdpt_light_gen(ShadeView v, BdptParams P, Verts Vt, WaveBufs B) {
+__global__ __launch_bounds__(BLOCK) void k_bdpt_light_gen(ShadeView v, const BdptParams* __restrict__ Pd, Verts Vt,
+                                                          WaveBufs B) {
   __shared__ uint32_t qsh[2][1][BLOCK / 64];
+  const BdptParams P = *Pd;
+  const uint32_t n = P.pt.npix * P.pt.k;
   uint32_t* const ctr[1] = {&B.cnt[0]};
   const uint32_t nthreads = gridDim.x * BLOCK;
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
-  const uint32_t rounds = (Vt.npaths + nthreads - 1) / nthreads;
+  const uint32_t rounds = (n + nthreads - 1) / nthreads;
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t p = r * nthreads + gtid;
     bool push = false;
-    if (p < Vt.npaths && v.dist_mode != AR_LIGHTDIST_NONE && v.n_lights > 0) {
+    if (p < n && v.dist_mode != AR_LIGHTDIST_NONE && v.n_lights > 0) {
       const uint32_t pixel = p % P.pt.npix, smp = P.pt.s0 + p / P.pt.npix;
       Rng rng;
       rng.init(P.pt.seed, pixel, smp, 0, B.meta[p].z);
@@ -495,9 +501,10 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_sub_extend(S
 // others record their vertex count (and, on the camera side, the dimension
 // where the light subpath's random numbers start).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_bdpt_sub_shade(ShadeView v, BdptParams P, Verts Vt, WaveBufs B, int qi,
-                                                          uint32_t side) {
+__global__ __launch_bounds__(BLOCK) void k_bdpt_sub_shade(ShadeView v, const BdptParams* __restrict__ Pd, Verts Vt,
+                                                          WaveBufs B, int qi, uint32_t side) {
   __shared__ uint32_t qsh[2][1][BLOCK / 64];
+  const BdptParams P = *Pd;
   uint32_t* const ctr[1] = {&B.cnt[qi ^ 1]};
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
@@ -813,7 +820,8 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams 
   const uint32_t nthreads = gridDim.x * BLOCK;
   Stack stk = make_stack<BLOCK>(lds, spill, gtid, nthreads);
   uint32_t nrays = 0;
-  for (uint32_t p = gtid; p < Vt.npaths; p += nthreads) {
+  const uint32_t n = P.pt.npix * P.pt.k;
+  for (uint32_t p = gtid; p < n; p += nthreads) {
     f3 acc = splat(0.0f);
     bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target, bool ray) {
       WorldHit hh;
@@ -832,9 +840,11 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams 
 }
 
 // ---- wavefront connection: enumerate, trace the shadow rays together, gather
-// Paths [p0, p0 + m) of a chunk per pass.  Path p's strategies occupy slots
-// (p - p0) * K + j, j < cnt, in enumeration order; the queue lists the used
-// slots (appended per wave) for the any-hit stage, which writes occl[slot].
+// Pass i takes paths [p0, p0 + m) of the chunk, p0 = i * pass.  Path p's
+// strategies occupy slots (p - p0) * K + j, j < cnt, in enumeration order;
+// the queue lists the used slots (appended per wave) for the any-hit stage,
+// which writes occl[slot].  A pass past the chunk's paths (the last chunk
+// can be shorter than the one the launches were recorded for) has m = 0.
 struct ConnBufs {
   float4* o;        // shadow ray origin, t_min           [m * K]
   float4* d;        // direction, t_max
@@ -843,27 +853,37 @@ struct ConnBufs {
   uint32_t* queue;  // used slots                          [m * K]
   uint32_t* cnt;    // strategies per path                 [m]
   uint32_t* qn;     // queue length
-  uint32_t K, p0, m;
+  uint32_t K, pass;
 };
+
+struct PassRange {
+  uint32_t p0, m;
+};
+__device__ __forceinline__ PassRange pass_range(const BdptParams& P, const ConnBufs& C, uint32_t pi) {
+  const uint32_t n = P.pt.npix * P.pt.k, p0 = pi * C.pass;
+  return PassRange{p0, n > p0 ? min(C.pass, n - p0) : 0u};
+}
 
 __global__ void k_bdpt_conn_reset(ConnBufs C) {
   if (threadIdx.x == 0) *C.qn = 0u;
 }
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, BdptParams P, Verts Vt, ConnBufs C,
-                                                          unsigned long long* __restrict__ stats) {
+__global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, const BdptParams* __restrict__ Pd, Verts Vt,
+                                                          ConnBufs C, uint32_t pi, unsigned long long* __restrict__ stats) {
+  const BdptParams P = *Pd;
+  const PassRange R = pass_range(P, C, pi);
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
   const uint32_t lane = __lane_id();
   uint32_t nrays = 0;
   // whole waves iterate together so the wave-aggregated append sees every lane
-  const uint32_t rounds = (C.m + nthreads - 1) / nthreads;
+  const uint32_t rounds = (R.m + nthreads - 1) / nthreads;
   for (uint32_t r = 0; r < rounds; ++r) {
     const uint32_t i = r * nthreads + gtid;
     uint32_t j = 0, j0 = 0;  // slots used; leading slots without a shadow ray (t = 0)
-    if (i < C.m) {
-      const uint32_t p = C.p0 + i;
+    if (i < R.m) {
+      const uint32_t p = R.p0 + i;
       const size_t base = (size_t)i * C.K;
       bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target, bool ray) {
         const size_t slot = base + j++;
@@ -914,8 +934,10 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_conn_vis(Sce
 
 // Unoccluded contributions in enumeration order: the path's own radiance is
 // summed exactly as the megakernel schedule sums it; splats are fixed-point.
-__global__ void k_bdpt_conn_gather(ConnBufs C, float4* __restrict__ Lout, unsigned long long* __restrict__ splat_film) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < C.m; i += gridDim.x * blockDim.x) {
+__global__ void k_bdpt_conn_gather(const BdptParams* __restrict__ Pd, ConnBufs C, uint32_t pi, float4* __restrict__ Lout,
+                                   unsigned long long* __restrict__ splat_film) {
+  const PassRange R = pass_range(*Pd, C, pi);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < R.m; i += gridDim.x * blockDim.x) {
     const uint32_t n = C.cnt[i];
     f3 acc = splat(0.0f);
     for (uint32_t k = 0; k < n; ++k) {
@@ -928,7 +950,7 @@ __global__ void k_bdpt_conn_gather(ConnBufs C, float4* __restrict__ Lout, unsign
       else
         splat_add(splat_film, target, mk(c.x, c.y, c.z));
     }
-    Lout[C.p0 + i] = make_float4(acc.x, acc.y, acc.z, 0.0f);
+    Lout[R.p0 + i] = make_float4(acc.x, acc.y, acc.z, 0.0f);
   }
 }
 
@@ -953,8 +975,11 @@ __global__ void k_zero_u64(unsigned long long* p, size_t n) {
   for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0ull;
 }
 
-__global__ void k_film_accumulate_b(const float4* __restrict__ L, float4* __restrict__ acc, uint32_t npix, uint32_t k,
-                                    uint32_t first) {
+// The chunk's per-sample radiance added to the film in sample order.
+__global__ void k_film_accumulate_b(const float4* __restrict__ L, float4* __restrict__ acc,
+                                    const BdptParams* __restrict__ Pd) {
+  const uint32_t npix = Pd->pt.npix, k = Pd->pt.k;
+  const bool first = Pd->pt.s0 == 0;
   for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
     float4 a = first ? make_float4(0.0f, 0.0f, 0.0f, 0.0f) : acc[px];
     for (uint32_t s = 0; s < k; ++s) {
@@ -979,6 +1004,13 @@ static uint32_t bdpt_grid(int block) {
   return (uint32_t)(cus * kTraceBlocksPerCU / 8 * 8);
 }
 
+// The chunk's parameters, written ahead of its (possibly graph-replayed)
+// launches, which read them from this block.
+static_assert(sizeof(BdptParams) <= kParamsBytes, "chunk parameters exceed their device block");
+__global__ void k_bdpt_params(BdptParams P, BdptParams* dst) {
+  if (threadIdx.x == 0) *dst = P;
+}
+
 __global__ void k_bdpt_zero_stats(unsigned long long* st) {
   if (threadIdx.x < 2) st[threadIdx.x] = 0ull;
 }
@@ -991,7 +1023,7 @@ class BdptJob : public JobRun {
   ConnBufs C{};
   ar_trace_config tc{}, ta{};
   unsigned long long* splat = nullptr;
-  uint32_t k = 1, s0 = 0, G = 0, eg = 0, Gs = 0, Ge = 0, Gw = 0, pass = 0;
+  uint32_t k = 1, s0 = 0, G = 0, eg = 0, Gs = 0, Ge = 0, Gw = 0, pass = 0, cap = 0, spill_entries = 0;
   bool wavefront = true;
   static constexpr int kSubBlock = 512;  // subpath generate / shade stages
 
@@ -1014,7 +1046,7 @@ class BdptJob : public JobRun {
     P.mis_beta = cfg->mis_beta;
     k = chunk_samples(spec);
     G = bdpt_grid(BLOCK);
-    uint32_t cap = npix * k;
+    cap = npix * k;
     wavefront = cfg->schedule != AR_SCHED_MEGAKERNEL;
     // subpath extension stages: the scene's closest-hit launch; any-hit
     // stage of the wavefront connection: the scene's any-hit launch
@@ -1054,12 +1086,13 @@ class BdptJob : public JobRun {
       const uint64_t e_t = need > ta.lds_stack ? need - ta.lds_stack : 0;
       spill_words = std::max<uint64_t>(spill_words, e_t * (uint64_t)Gs * ta.workgroup_size);
     }
-    ar_status e =
-        ws->reserve_paths(cap, npix, (uint32_t)((spill_words + G * BLOCK - 1) / (G * BLOCK)), G * BLOCK, stream);
+    spill_entries = (uint32_t)((spill_words + G * BLOCK - 1) / (G * BLOCK));
+    ar_status e = ws->reserve_paths(cap, npix, spill_entries, G * BLOCK, stream);
     if (e != AR_OK) return e;
     size_t nvert = 2ull * P.V * cap;
     C.K = (uint32_t)bdpt_conn_slots(cfg->max_depth);
     pass = wavefront ? (uint32_t)bdpt_conn_pass(cfg->max_depth, cap) : 0;
+    C.pass = pass;
     const size_t slots = (size_t)pass * C.K;
     auto al = [](size_t n) { return (n + 255) / 256 * 256; };
     size_t need = al(nvert * 3 * sizeof(float4)) + al(nvert * sizeof(float2)) + al((size_t)cap * sizeof(uint2)) +
@@ -1093,6 +1126,7 @@ class BdptJob : public JobRun {
       C.qn = (uint32_t*)take(256);
     }
     Vt.V = P.V;
+    Vt.stride = cap;
     eg = std::min<uint32_t>((npix + 255) / 256, 4096);
     hipLaunchKernelGGL(k_bdpt_zero_stats, dim3(1), dim3(64), 0, stream, ws->B.stats);
     hipLaunchKernelGGL(k_zero_u64, dim3(eg), dim3(256), 0, stream, splat, (size_t)npix * 3);
@@ -1100,67 +1134,92 @@ class BdptJob : public JobRun {
     return AR_OK;
   }
 
+  // One chunk of the wavefront schedule: both subpaths bounce by bounce,
+  // then the connection passes, then the film.  The launches depend on the
+  // job's buffers, grids and launch configurations only (the chunk's s0 and
+  // k are read from the device block), so they are captured once per job
+  // shape and replayed per chunk; passes past a shorter last chunk are empty.
+  ar_status run_chunk_wavefront(const BdptParams* Pd) {
+    const WaveBufs& B = ws->B;
+    int2* const spill = ws->spill;
+    const uint32_t npass = (cap + pass - 1) / pass;
+    const uint32_t Gg = std::min<uint32_t>((pass + 255) / 256, 4096);
+    auto record = [&]() {
+      for (uint32_t side = 0; side < 2; ++side) {
+        if (side == 0)
+          hipLaunchKernelGGL((k_bdpt_cam_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, Pd, Vt, B);
+        else
+          hipLaunchKernelGGL((k_bdpt_light_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, Pd, Vt, B);
+        for (uint32_t d = 0; d < P.pt.max_depth; ++d) {
+          const int qi = d & 1;
+#define AR_SUB_EXT(BB, SS)                                                                                   \
+  if (tc.workgroup_size == BB && tc.lds_stack == SS) {                                                       \
+    if (tc.park_slots == 4)                                                                                  \
+      hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 4>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi, spill,  \
+                         tc.refill_min, tc.search_min);                                                      \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 3>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi, spill,  \
+                         tc.refill_min, tc.search_min);                                                      \
+  }
+          AR_TRACE_VARIANTS(AR_SUB_EXT)
+#undef AR_SUB_EXT
+          hipLaunchKernelGGL((k_bdpt_sub_shade<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, Pd, Vt, B, qi,
+                             side);
+          hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, qi);
+        }
+        hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, -1);
+      }
+      for (uint32_t pi = 0; pi < npass; ++pi) {
+        hipLaunchKernelGGL(k_bdpt_conn_reset, dim3(1), dim3(64), 0, stream, C);
+        hipLaunchKernelGGL((k_bdpt_conn_prep<kTraceBlock>), dim3(G), dim3(kTraceBlock), 0, stream, v, Pd, Vt, C, pi,
+                           B.stats);
+#define AR_VIS(BB, SS)                                                                                      \
+  if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                      \
+    if (ta.park_slots == 4)                                                                                 \
+      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 4>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, spill,       \
+                         ta.refill_min, ta.search_min);                                                     \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 3>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, spill,       \
+                         ta.refill_min, ta.search_min);                                                     \
+  }
+        AR_TRACE_VARIANTS(AR_VIS)
+#undef AR_VIS
+        hipLaunchKernelGGL(k_bdpt_conn_gather, dim3(Gg), dim3(256), 0, stream, Pd, C, pi, B.L, splat);
+      }
+      hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, B.L, ws->acc, Pd);
+    };
+    const uint64_t key[GraphCache::kKeyWords] = {(uint64_t)(uintptr_t)scene,
+                                                 (uint64_t)(uintptr_t)ws->buf.p,
+                                                 (uint64_t)(uintptr_t)ws->aux.p,
+                                                 launch_key(tc, ta),
+                                                 launch_key_thresholds(tc, ta),
+                                                 ((uint64_t)Ge << 32) | Gs,
+                                                 ((uint64_t)Gw << 32) | G,
+                                                 ((uint64_t)cap << 32) | P.pt.npix,
+                                                 ((uint64_t)pass << 32) | C.K,
+                                                 ((uint64_t)P.pt.max_depth << 32) | spill_entries,
+                                                 view_hash(v),
+                                                 0};
+    return ws->bdpt_graph.launch(key, stream, "BDPT chunk", record);
+  }
+
   ar_status step(bool* done) override {
     constexpr int BLOCK = kTraceBlock;
     if (s0 < P.pt.spp) {
       P.pt.s0 = s0;
       P.pt.k = std::min(k, P.pt.spp - s0);
-      Vt.npaths = P.pt.npix * P.pt.k;
-      if (!wavefront) {
+      BdptParams* Pd = reinterpret_cast<BdptParams*>(ws->params);
+      hipLaunchKernelGGL(k_bdpt_params, dim3(1), dim3(64), 0, stream, P, Pd);
+      if (wavefront) {
+        ar_status e = run_chunk_wavefront(Pd);
+        if (e != AR_OK) return e;
+      } else {
         hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->spill,
                            ws->B.stats);
-      } else {
-        const WaveBufs& B = ws->B;
-        for (uint32_t side = 0; side < 2; ++side) {
-          if (side == 0)
-            hipLaunchKernelGGL((k_bdpt_cam_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, P, Vt, B);
-          else
-            hipLaunchKernelGGL((k_bdpt_light_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, P, Vt, B);
-          for (uint32_t d = 0; d < P.pt.max_depth; ++d) {
-            const int qi = d & 1;
-#define AR_SUB_EXT(BB, SS)                                                                                   \
-  if (tc.workgroup_size == BB && tc.lds_stack == SS) {                                                       \
-    if (tc.park_slots == 4)                                                                                  \
-      hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 4>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi,         \
-                         ws->spill, tc.refill_min, tc.search_min);                                           \
-    else                                                                                                     \
-      hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 3>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi,         \
-                         ws->spill, tc.refill_min, tc.search_min);                                           \
-  }
-            AR_TRACE_VARIANTS(AR_SUB_EXT)
-#undef AR_SUB_EXT
-            hipLaunchKernelGGL((k_bdpt_sub_shade<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, P, Vt, B, qi,
-                               side);
-            hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, qi);
-          }
-          hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, -1);
-        }
-      }
-      if (!wavefront) {
         hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->B.L, splat,
                            ws->spill, ws->B.stats);
+        hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, Pd);
       }
-      for (uint32_t p0 = 0; wavefront && p0 < Vt.npaths; p0 += pass) {
-        C.p0 = p0;
-        C.m = std::min(pass, Vt.npaths - p0);
-        hipLaunchKernelGGL(k_bdpt_conn_reset, dim3(1), dim3(64), 0, stream, C);
-        hipLaunchKernelGGL((k_bdpt_conn_prep<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, C, ws->B.stats);
-#define AR_VIS(BB, SS)                                                                                      \
-  if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                      \
-    if (ta.park_slots == 4)                                                                                 \
-      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 4>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, ws->spill,   \
-                         ta.refill_min, ta.search_min);                                                     \
-    else                                                                                                    \
-      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 3>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, ws->spill,   \
-                         ta.refill_min, ta.search_min);                                                     \
-  }
-        AR_TRACE_VARIANTS(AR_VIS)
-#undef AR_VIS
-        hipLaunchKernelGGL(k_bdpt_conn_gather, dim3(std::min<uint32_t>((C.m + 255) / 256, 4096)), dim3(256), 0,
-                           stream, C, ws->B.L, splat);
-      }
-      hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, P.pt.npix, P.pt.k,
-                         s0 == 0 ? 1u : 0u);
       AR_HIP(hipGetLastError());
       s0 += P.pt.k;
     }

@@ -117,6 +117,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_generate(const PtParams* __restric
 
 // The chunk's parameters, written to device memory ahead of the (possibly
 // graph-replayed) bounce loop, whose kernels read them there.
+static_assert(sizeof(PtParams) <= kParamsBytes, "chunk parameters exceed their device block");
 __global__ void k_wf_params(PtParams P, PtParams* dst) {
   if (threadIdx.x == 0) *dst = P;
 }
@@ -333,13 +334,14 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
   add(o_acc, (size_t)npix * f4 * 3);
   add(o_spill, (size_t)spill_entries * grid_threads * sizeof(int2));
   size_t o_par;
-  add(o_par, sizeof(PtParams));
+  add(o_par, kParamsBytes);
   if (buf.bytes < bytes) {
     // the buffer's last user is this stream (callers order other streams'
     // uses before it): drain it, not the whole device
     AR_HIP(hipStreamSynchronize(st));
     AR_HIP(buf.alloc(bytes));
-    graph_valid = false;
+    pt_graph.reset();
+    bdpt_graph.reset();
   }
   char* b = (char*)buf.p;
   B.ray_o = (float4*)(b + o_ro);
@@ -365,17 +367,9 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
   return AR_OK;
 }
 
-Workspace::~Workspace() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
-}
-
 void Workspace::trim() {
-  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-  if (graph) (void)hipGraphDestroy(graph);
-  graph_exec = nullptr;
-  graph = nullptr;
-  graph_valid = false;
+  pt_graph.reset();
+  bdpt_graph.reset();
   buf.release();
   out.release();
   aux.release();
@@ -387,8 +381,7 @@ void Workspace::trim() {
   capacity = 0;
 }
 
-// FNV-1a over the fields of the scene views a captured graph holds by value.
-static uint64_t view_hash(const ShadeView& v) {
+uint64_t view_hash(const ShadeView& v) {
   uint64_t h = 1469598103934665603ull;
   auto mix = [&](uint64_t x) {
     for (int i = 0; i < 8; ++i) h = (h ^ ((x >> (8 * i)) & 255u)) * 1099511628211ull;
@@ -396,8 +389,10 @@ static uint64_t view_hash(const ShadeView& v) {
   const SceneView& sv = v.sv;
   for (const void* p : {(const void*)sv.nodes, (const void*)sv.qnodes, (const void*)sv.tris, (const void*)sv.insts,
                         (const void*)v.objects, (const void*)v.materials, (const void*)v.lights, (const void*)v.idx,
-                        (const void*)v.nrm, (const void*)v.slots, (const void*)v.objmats, (const void*)v.stri})
+                        (const void*)v.nrm, (const void*)v.slots, (const void*)v.objmats, (const void*)v.stri,
+                        (const void*)v.suv, (const void*)v.texels, (const void*)v.tex_info})
     mix((uint64_t)(uintptr_t)p);
+  mix(v.n_tex);
   mix(((uint64_t)(uint32_t)sv.tlas_root << 32) | (uint32_t)sv.qtlas_root);
   for (int k = 0; k < 6; ++k) mix(bits_u(sv.tlas_box[k]));
   mix(((uint64_t)sv.n_inst << 32) | v.n_lights);
@@ -477,35 +472,15 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
     return AR_OK;
   }
   // what the graph depends on besides the chunk parameters
-  const uint64_t key[8] = {(uint64_t)(uintptr_t)s,
-                           (uint64_t)(uintptr_t)ws.buf.p,
-                           ((uint64_t)tc.workgroup_size << 48) | ((uint64_t)tc.lds_stack << 40) |
-                               ((uint64_t)tc.park_slots << 32) | ((uint64_t)ta.workgroup_size << 16) |
-                               ((uint64_t)ta.lds_stack << 8) | ta.park_slots,
-                           ((uint64_t)tc.refill_min << 48) | ((uint64_t)tc.search_min << 32) |
-                               ((uint64_t)ta.refill_min << 16) | ta.search_min,
-                           ((uint64_t)Ge << 32) | Gs,
-                           ((uint64_t)G << 32) | P.max_depth,
-                           view_hash(v),
-                           (uint64_t)ws.capacity};
-  if (!ws.graph_valid || std::memcmp(key, ws.graph_key, sizeof(key)) != 0) {
-    if (ws.graph_exec) (void)hipGraphExecDestroy(ws.graph_exec);
-    if (ws.graph) (void)hipGraphDestroy(ws.graph);
-    ws.graph_exec = nullptr;
-    ws.graph = nullptr;
-    ws.graph_valid = false;
-    AR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    record();
-    hipError_t le = hipGetLastError();
-    hipError_t ce = hipStreamEndCapture(st, &ws.graph);
-    if (le != hipSuccess) return fail(AR_ERR_DEVICE, std::string("bounce loop capture: ") + hipGetErrorString(le));
-    AR_HIP(ce);
-    AR_HIP(hipGraphInstantiate(&ws.graph_exec, ws.graph, nullptr, nullptr, 0));
-    std::memcpy(ws.graph_key, key, sizeof(key));
-    ws.graph_valid = true;
-  }
-  AR_HIP(hipGraphLaunch(ws.graph_exec, st));
-  return AR_OK;
+  const uint64_t key[GraphCache::kKeyWords] = {(uint64_t)(uintptr_t)s,
+                                               (uint64_t)(uintptr_t)ws.buf.p,
+                                               launch_key(tc, ta),
+                                               launch_key_thresholds(tc, ta),
+                                               ((uint64_t)Ge << 32) | Gs,
+                                               ((uint64_t)G << 32) | P.max_depth,
+                                               view_hash(v),
+                                               (uint64_t)ws.capacity};
+  return ws.pt_graph.launch(key, st, "bounce loop", record);
 }
 
 // ---------------------------------------------------------------------------

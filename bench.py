@@ -556,6 +556,40 @@ def bench_film_output(ar, torch, pixels, spp):
     return res
 
 
+def bench_bdpt_room(args, ar, torch):
+    """BDPT alone on the room at 1280x720, 16 spp, depth 10: the launches given
+    by --trace-config / --anyhit-config (else autotuned on ray samples), the
+    queue chunk given by --chunk (samples per pixel; else the default), one
+    warm-up render, then --steps timed renders.  For per-stage profiles."""
+    wl = ar.workloads
+    w, h = 1280, 720
+    desc = wl.procedural_room(width=w, height=h)
+    sc = ar.Scene.from_desc(desc)
+    cd = desc.camera
+    cam = ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, w / h))
+    if args.trace_config:
+        sc.set_trace_config(*[int(v) for v in args.trace_config.split(",")])
+        sc.set_anyhit_config(*[int(v) for v in (args.anyhit_config or args.trace_config).split(",")])
+    else:
+        autotune_scene(ar, torch, sc, desc, w / h)
+    bd = ar.BidirectionalPathTracer(ar.BdptConfig(spp=16, max_depth=10, rr_depth=3), seed=0,
+                                    sample_chunk=args.chunk)
+    bd.render(sc, cam, w, h)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        bd.render(sc, cam, w, h)
+        times.append(time.perf_counter() - t0)
+        log(f"bdpt room render: {times[-1] * 1e3:.1f} ms")
+    dt = min(times)
+    st = bd.last_stats
+    return {"bdpt_room_1280x720x16spp": {"launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()),
+                                         "sample_chunk": args.chunk, "msamples_s": round(w * h * 16 / dt / 1e6, 2),
+                                         "mrays_s": round(st.rays / dt / 1e6, 1), "seconds": round(dt, 4),
+                                         "rays": int(st.rays)}}
+
+
 def bench_bdpt_gdpt(ar, torch):
     """The room at 1280x720: BDPT 16 spp, depth 10, power heuristic beta 2;
     GDPT 32 spp, depth 16, base + two shifted paths per sample, screened
@@ -839,7 +873,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "pt", "copy", "pmc-probe", "builders"],
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "pt", "bdpt", "copy", "pmc-probe",
+                                                          "builders"],
                     help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--trace-config", default=None,
@@ -847,6 +882,7 @@ def main():
                          "traversal launch instead of autotuning")
     ap.add_argument("--anyhit-config", default=None,
                     help="W,S,P[,R[,SM[,PS]]]: any-hit launch (with --trace-config; default: the same as closest hit)")
+    ap.add_argument("--chunk", type=int, default=0, help="bdpt workload: samples per pixel per queue chunk")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the path-tracer throughput fields")
@@ -874,6 +910,9 @@ def main():
                "pt_room_1920x1080x64spp": bench_pt(ar, torch, "room")}
         res.update(bench_bdpt_gdpt(ar, torch))
         print(json.dumps(res), flush=True)
+        return
+    if args.workload == "bdpt":
+        print(json.dumps(bench_bdpt_room(args, ar, torch)), flush=True)
         return
     if args.workload == "builders":
         print(json.dumps(bench_builders(ar, torch)), flush=True)
