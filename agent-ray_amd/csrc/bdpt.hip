@@ -613,9 +613,28 @@ __device__ __forceinline__ double powb(double x, float beta) {
   return beta == 2.0f ? x * x : pow(x, (double)beta);
 }
 
+// (pdf_fwd, pdf_rev) of vertex i of one side of path p, as the MIS walk
+// reads them: from the vertex planes, or from the lane's LDS copy.
+struct PdfsGlobal {
+  const Verts& Vt;
+  uint32_t p;
+  __device__ __forceinline__ float2 operator()(uint32_t side, uint32_t i) const {
+    const size_t a = Vt.at(side, i, p);
+    return make_float2(Vt.pf[a].w, Vt.nr[a].w);
+  }
+};
+struct PdfsLds {
+  const float2* base;  // the lane's first entry; [side][vertex] at a stride of the workgroup size
+  uint32_t V, stride;
+  __device__ __forceinline__ float2 operator()(uint32_t side, uint32_t i) const {
+    return base[(side * V + i) * stride];
+  }
+};
+
 // :594-770
-__device__ __forceinline__ float mis_weight(const BdptParams& P, const Verts& Vt, uint32_t p, uint32_t nc, uint32_t nl,
-                                            uint32_t s, uint32_t t, float pdf_cc, float pdf_cl, bool area_light) {
+template <class Pdfs>
+__device__ __forceinline__ float mis_weight(const BdptParams& P, const Pdfs& pd, uint32_t nc, uint32_t nl, uint32_t s,
+                                            uint32_t t, float pdf_cc, float pdf_cl, bool area_light) {
   if (P.mis_mode == 1) {  // :605-636
     uint32_t len = s + t, count = 0;
     uint32_t tmx = len < nl ? len : nl;
@@ -634,14 +653,14 @@ __device__ __forceinline__ float mis_weight(const BdptParams& P, const Verts& Vt
   {
     double ri = 1.0;
     if (s >= 1) {
-      PV v = load_v(Vt, 0, s - 1, p);
-      ri *= (double)pdf_cc / (double)fmaxf(v.pdf_fwd, 1e-30f);
+      const float2 v = pd(0, s - 1);
+      ri *= (double)pdf_cc / (double)fmaxf(v.x, 1e-30f);
       bool prev_delta = s >= 2 ? false : true;
       if (!prev_delta) sum += powb(fabs(ri), P.mis_beta);
     }
     for (uint32_t i = s - 1; s >= 1 && i >= 1; --i) {
-      PV v = load_v(Vt, 0, i, p);
-      ri *= (double)fmaxf(v.pdf_rev, 0.0f) / (double)fmaxf(v.pdf_fwd, 1e-30f);
+      const float2 v = pd(0, i);
+      ri *= (double)fmaxf(v.y, 0.0f) / (double)fmaxf(v.x, 1e-30f);
       sum += powb(fabs(ri), P.mis_beta);
     }
   }
@@ -650,15 +669,15 @@ __device__ __forceinline__ float mis_weight(const BdptParams& P, const Verts& Vt
     // reference skips it "for delta lights", :735-742)
     double ri = 1.0;
     if (t >= 1) {
-      PV v = load_v(Vt, 1, t - 1, p);
-      ri *= (double)pdf_cl / (double)fmaxf(v.pdf_fwd, 1e-30f);
+      const float2 v = pd(1, t - 1);
+      ri *= (double)pdf_cl / (double)fmaxf(v.x, 1e-30f);
       bool prev_delta = t >= 2 ? false : !area_light;
       if (!prev_delta) sum += powb(fabs(ri), P.mis_beta);
     }
     const int lo = area_light ? 0 : 1;
     for (int i = (int)t - 1; i >= lo; --i) {
-      PV v = load_v(Vt, 1, (uint32_t)i, p);
-      ri *= (double)fmaxf(v.pdf_rev, 0.0f) / (double)fmaxf(v.pdf_fwd, 1e-30f);
+      const float2 v = pd(1, (uint32_t)i);
+      ri *= (double)fmaxf(v.y, 0.0f) / (double)fmaxf(v.x, 1e-30f);
       sum += powb(fabs(ri), P.mis_beta);
     }
   }
@@ -686,9 +705,9 @@ constexpr uint32_t kOwnPixel = 0xffffffffu;  // target of a contribution to the 
 // The t = 0 strategy (extension) has no shadow ray (ray = false); only the
 // last camera vertex can lie on an emitter, so it is a path's first strategy
 // if it exists.
-template <class Emit>
+template <class Pdfs, class Emit>
 __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptParams& P, const Verts& Vt, uint32_t p,
-                                                Emit emit) {
+                                                const Pdfs& pdfs, Emit emit) {
   const Cam& cam = P.pt.cam;
   const uint32_t W = P.pt.width, H = P.pt.height;
   uint2 cnt = Vt.count[p];
@@ -710,7 +729,7 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
           const DLight& L = v.lights[mz.light_id];
           pdf_cc = L.pmf / L.area;
         }
-        float w = mis_weight(P, Vt, p, smax, tmax, s, 0, pdf_cc, 0.0f, true);
+        float w = mis_weight(P, pdfs, smax, tmax, s, 0, pdf_cc, 0.0f, true);
         emit(Ray{}, c * w, kOwnPixel, false);
         continue;
       }
@@ -736,7 +755,7 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
         float pd = bsdf_pdf(m, wi_l, wo_l);
         PV c0 = load_v(Vt, 0, 0, p);
         float pdf_cl = pdf_sa_to_area(pd, y.p, cam.origin, c0.n);
-        float w = mis_weight(P, Vt, p, smax, tmax, 1, t, 0.0f, pdf_cl, area_light);
+        float w = mis_weight(P, pdfs, smax, tmax, 1, t, 0.0f, pdf_cl, area_light);
         c = c * w;
         uint32_t ix = (uint32_t)cs.px, iy = (uint32_t)cs.py;
         if (ix > W - 1) ix = W - 1;
@@ -803,7 +822,7 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
       f3 wi_z2 = normalize(y.p - z.p);
       float pdz = bsdf_pdf(mz, to_local(onz, wi_z2), wo_z_l);
       float pdf_cl = pdf_sa_to_area(pdz, z.p, y.p, y.n);
-      float w = mis_weight(P, Vt, p, smax, tmax, s, t, pdf_cc, pdf_cl, area_light);
+      float w = mis_weight(P, pdfs, smax, tmax, s, t, pdf_cc, pdf_cl, area_light);
       emit(sr, c * w, kOwnPixel, true);
     }
   }
@@ -823,7 +842,7 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_connect(ShadeView v, BdptParams 
   const uint32_t n = P.pt.npix * P.pt.k;
   for (uint32_t p = gtid; p < n; p += nthreads) {
     f3 acc = splat(0.0f);
-    bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target, bool ray) {
+    bdpt_strategies(v, P, Vt, p, PdfsGlobal{Vt, p}, [&](const Ray& sr, f3 c, uint32_t target, bool ray) {
       WorldHit hh;
       if (ray) {
         nrays++;
@@ -868,9 +887,13 @@ __global__ void k_bdpt_conn_reset(ConnBufs C) {
   if (threadIdx.x == 0) *C.qn = 0u;
 }
 
-template <int BLOCK>
+// With LDS, each lane first copies its path's (pdf_fwd, pdf_rev) pairs into
+// the workgroup's dynamic LDS (2 * V * BLOCK float2, [side][vertex][lane]),
+// so the MIS walks of its strategies read them there instead of from HBM.
+template <int BLOCK, bool LDS>
 __global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, const BdptParams* __restrict__ Pd, Verts Vt,
                                                           ConnBufs C, uint32_t pi, unsigned long long* __restrict__ stats) {
+  extern __shared__ float2 pdf_lds[];
   const BdptParams P = *Pd;
   const PassRange R = pass_range(P, C, pi);
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
@@ -885,7 +908,7 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, const Bdp
     if (i < R.m) {
       const uint32_t p = R.p0 + i;
       const size_t base = (size_t)i * C.K;
-      bdpt_strategies(v, P, Vt, p, [&](const Ray& sr, f3 c, uint32_t target, bool ray) {
+      auto emit = [&](const Ray& sr, f3 c, uint32_t target, bool ray) {
         const size_t slot = base + j++;
         C.val[slot] = make_float4(c.x, c.y, c.z, __uint_as_float(target));
         if (ray) {
@@ -895,7 +918,23 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, const Bdp
           C.occl[slot] = 0;  // counts unconditionally; not queued
           j0 = j;
         }
-      });
+      };
+      if (LDS) {
+        // the walks read vertices below the subpath lengths only
+        const uint2 cnt = Vt.count[p];
+        float2* mine = pdf_lds + threadIdx.x;
+        for (uint32_t k = 0; k < cnt.x; ++k) {
+          const size_t a = Vt.at(0, k, p);
+          mine[k * BLOCK] = make_float2(Vt.pf[a].w, Vt.nr[a].w);
+        }
+        for (uint32_t k = 0; k < cnt.y; ++k) {
+          const size_t a = Vt.at(1, k, p);
+          mine[(Vt.V + k) * BLOCK] = make_float2(Vt.pf[a].w, Vt.nr[a].w);
+        }
+        bdpt_strategies(v, P, Vt, p, PdfsLds{mine, Vt.V, (uint32_t)BLOCK}, emit);
+      } else {
+        bdpt_strategies(v, P, Vt, p, PdfsGlobal{Vt, p}, emit);
+      }
       C.cnt[i] = j;
     }
     const uint32_t nq = j - j0;  // slots j0 .. j-1 need their shadow ray traced
@@ -1026,6 +1065,7 @@ class BdptJob : public JobRun {
   uint32_t k = 1, s0 = 0, G = 0, eg = 0, Gs = 0, Ge = 0, Gw = 0, pass = 0, cap = 0, spill_entries = 0;
   bool wavefront = true;
   static constexpr int kSubBlock = 512;  // subpath generate / shade stages
+  static constexpr size_t kPdfLdsMax = 64 * 1024;  // LDS for the MIS walks' pdf pairs (depth <= 31 at 128 lanes)
 
   ar_status begin() override {
     constexpr int BLOCK = kTraceBlock;
@@ -1143,6 +1183,10 @@ class BdptJob : public JobRun {
     const WaveBufs& B = ws->B;
     int2* const spill = ws->spill;
     const uint32_t npass = (cap + pass - 1) / pass;
+    // the MIS walks' pdf pairs in LDS while they fit a workgroup's share
+    const size_t pdf_lds_bytes = 2ull * Vt.V * kTraceBlock * sizeof(float2) <= kPdfLdsMax
+                                     ? 2ull * Vt.V * kTraceBlock * sizeof(float2)
+                                     : 0;
     const uint32_t Gg = std::min<uint32_t>((pass + 255) / 256, 4096);
     auto record = [&]() {
       for (uint32_t side = 0; side < 2; ++side) {
@@ -1171,8 +1215,12 @@ class BdptJob : public JobRun {
       }
       for (uint32_t pi = 0; pi < npass; ++pi) {
         hipLaunchKernelGGL(k_bdpt_conn_reset, dim3(1), dim3(64), 0, stream, C);
-        hipLaunchKernelGGL((k_bdpt_conn_prep<kTraceBlock>), dim3(G), dim3(kTraceBlock), 0, stream, v, Pd, Vt, C, pi,
-                           B.stats);
+        if (pdf_lds_bytes)
+          hipLaunchKernelGGL((k_bdpt_conn_prep<kTraceBlock, true>), dim3(G), dim3(kTraceBlock), pdf_lds_bytes, stream,
+                             v, Pd, Vt, C, pi, B.stats);
+        else
+          hipLaunchKernelGGL((k_bdpt_conn_prep<kTraceBlock, false>), dim3(G), dim3(kTraceBlock), 0, stream, v, Pd, Vt,
+                             C, pi, B.stats);
 #define AR_VIS(BB, SS)                                                                                      \
   if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                      \
     if (ta.park_slots == 4)                                                                                 \
