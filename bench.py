@@ -556,13 +556,23 @@ def bench_film_output(ar, torch, pixels, spp):
     return res
 
 
-def bench_bdpt_room(args, ar, torch):
-    """BDPT alone on the room at 1280x720, 16 spp, depth 10: the launches given
-    by --trace-config / --anyhit-config (else autotuned on ray samples), the
-    queue chunk given by --chunk (samples per pixel; else the default), one
-    warm-up render, then --steps timed renders.  For per-stage profiles."""
+ROOM_JOBS = {  # integrator: (width, height, spp, max_depth), the bench's room configurations
+    "pt": (1920, 1080, 64, 16),
+    "bdpt": (1280, 720, 16, 10),
+    "gdpt": (1280, 720, 32, 16),
+}
+
+
+def bench_render_room(args, ar, torch):
+    """One integrator alone on the room at its bench configuration (ROOM_JOBS;
+    BDPT with the power heuristic beta 2, GDPT with 50 CG iterations): the
+    launches given by --trace-config / --anyhit-config (else autotuned on ray
+    samples), the queue chunk given by --chunk (samples per pixel; else the
+    default), the wavefront schedule, one warm-up render, then --steps timed
+    renders.  For per-stage profiles."""
     wl = ar.workloads
-    w, h = 1280, 720
+    integ = args.integrator
+    w, h, spp, depth = ROOM_JOBS[integ]
     desc = wl.procedural_room(width=w, height=h)
     sc = ar.Scene.from_desc(desc)
     cd = desc.camera
@@ -572,22 +582,29 @@ def bench_bdpt_room(args, ar, torch):
         sc.set_anyhit_config(*[int(v) for v in (args.anyhit_config or args.trace_config).split(",")])
     else:
         autotune_scene(ar, torch, sc, desc, w / h)
-    bd = ar.BidirectionalPathTracer(ar.BdptConfig(spp=16, max_depth=10, rr_depth=3), seed=0,
-                                    sample_chunk=args.chunk)
-    bd.render(sc, cam, w, h)
+    if integ == "pt":
+        it = ar.PathTracer(ar.PathTracerConfig(spp=spp, max_depth=depth, rr_depth=3), seed=0,
+                           sample_chunk=args.chunk)
+    elif integ == "bdpt":
+        it = ar.BidirectionalPathTracer(ar.BdptConfig(spp=spp, max_depth=depth, rr_depth=3), seed=0,
+                                        sample_chunk=args.chunk)
+    else:
+        it = ar.GradientDomainPathTracer(ar.GdptConfig(spp=spp, max_depth=depth, rr_depth=3, alpha=0.2,
+                                                       poisson_iterations=50), seed=0, sample_chunk=args.chunk)
+    it.render(sc, cam, w, h)
     torch.cuda.synchronize()
     times = []
     for _ in range(max(1, args.steps)):
         t0 = time.perf_counter()
-        bd.render(sc, cam, w, h)
+        it.render(sc, cam, w, h)
         times.append(time.perf_counter() - t0)
-        log(f"bdpt room render: {times[-1] * 1e3:.1f} ms")
+        log(f"{integ} room render: {times[-1] * 1e3:.1f} ms")
     dt = min(times)
-    st = bd.last_stats
-    return {"bdpt_room_1280x720x16spp": {"launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()),
-                                         "sample_chunk": args.chunk, "msamples_s": round(w * h * 16 / dt / 1e6, 2),
-                                         "mrays_s": round(st.rays / dt / 1e6, 1), "seconds": round(dt, 4),
-                                         "rays": int(st.rays)}}
+    st = it.last_stats
+    name = f"{integ}_room_{w}x{h}x{spp}spp"
+    return {name: {"launch_config": dict(sc.trace_config(), anyhit=sc.anyhit_config()), "sample_chunk": args.chunk,
+                   "msamples_s": round(w * h * spp / dt / 1e6, 2), "mpaths_s": round(st.paths / dt / 1e6, 2),
+                   "mrays_s": round(st.rays / dt / 1e6, 1), "seconds": round(dt, 4), "rays": int(st.rays)}}
 
 
 def bench_bdpt_gdpt(ar, torch):
@@ -873,7 +890,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "pt", "bdpt", "copy", "pmc-probe",
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "pt", "render", "copy", "pmc-probe",
                                                           "builders"],
                     help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
@@ -882,7 +899,9 @@ def main():
                          "traversal launch instead of autotuning")
     ap.add_argument("--anyhit-config", default=None,
                     help="W,S,P[,R[,SM[,PS]]]: any-hit launch (with --trace-config; default: the same as closest hit)")
-    ap.add_argument("--chunk", type=int, default=0, help="bdpt workload: samples per pixel per queue chunk")
+    ap.add_argument("--integrator", default="pt", choices=sorted(ROOM_JOBS),
+                    help="render workload: the integrator rendered on the room")
+    ap.add_argument("--chunk", type=int, default=0, help="render workload: samples per pixel per queue chunk")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the path-tracer throughput fields")
@@ -896,8 +915,8 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="serving: jobs on the device at once (0 = the server's default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not launched", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark runs on one GPU; sharding over several GPUs is designed "
+                                   "(sample ranges or pixel tiles, one ordered film reduction) but not launched", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
@@ -911,8 +930,8 @@ def main():
         res.update(bench_bdpt_gdpt(ar, torch))
         print(json.dumps(res), flush=True)
         return
-    if args.workload == "bdpt":
-        print(json.dumps(bench_bdpt_room(args, ar, torch)), flush=True)
+    if args.workload == "render":
+        print(json.dumps(bench_render_room(args, ar, torch)), flush=True)
         return
     if args.workload == "builders":
         print(json.dumps(bench_builders(ar, torch)), flush=True)
