@@ -679,7 +679,8 @@ class GdptConfig:
 # autotuner's pick: (workgroup, LDS stack, blocks per CU, refill threshold,
 # search quorum, park slots).  The winner is then tried with the other park
 # slot count (3 <-> 4).
-TUNE_CANDIDATES = [(256, 12, 8, 8, 16, 3), (256, 12, 8, 8, 8, 4),
+TUNE_CANDIDATES = [(256, 12, 8, 8, 16, 3), (256, 12, 8, 8, 8, 4), (256, 12, 8, 8, 32, 4), (256, 12, 8, 8, 32, 3),
+                   (256, 8, 8, 8, 32, 4), (128, 16, 8, 8, 32, 3),
                    (256, 8, 8, 8, 16, 3), (256, 8, 8, 8, 8, 3), (256, 8, 8, 8, 16, 4), (256, 8, 8, 8, 8, 4),
                    (256, 8, 8, 24, 16, 3), (128, 8, 16, 8, 16, 3), (128, 16, 8, 8, 16, 3), (128, 16, 8, 8, 8, 3),
                    (64, 16, 16, 8, 16, 3)]

@@ -357,7 +357,8 @@ struct Walker {
   // testing them at once, so the lanes of a wave keep doing node work
   // together; the leaf phase then tests the parked leaves.  A lane can act
   // while it holds a node, or a triangle leaf and a free park slot; a lane
-  // that has parked nothing yet is "searching".  Every lane stays in the
+  // that has parked nothing yet is "searching".  A step that parks a leaf
+  // also visits the node popped after it.  Every lane stays in the
   // loop until go(can, searching) -- wave-uniform on the device, see
   // walk_items -- ends it for all of them.  Alone, a lane stops at its first
   // parked leaf or when it cannot act.  The visiting order then depends on
@@ -378,9 +379,7 @@ struct Walker {
       const bool searching = can && park0 == kLevelEnd;
       if (!go(can, searching)) return;
       if (can) {
-        if (node) {
-          visit(sv, stk);
-        } else {
+        if (!node) {
           if (park0 == kLevelEnd)
             park0 = cur;
           else if (park1 == kLevelEnd)
@@ -391,6 +390,11 @@ struct Walker {
             park3 = cur;
           pop_level(stk);
         }
+        // A lane that has just parked a leaf visits the node it popped in
+        // the same iteration (while a slot is still free), so parking does
+        // not cost it an iteration of node work.
+        const int free_slot = SLOTS == 4 ? park3 : park2;
+        if (cur >= 0 && free_slot == kLevelEnd) visit(sv, stk);
       }
     }
   }

@@ -446,7 +446,7 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
   float best_ms[2] = {INFINITY, INFINITY};
   const uint32_t per_cu[3] = {4, 8, 16};
   const uint32_t refills[3] = {8, 24, 48};
-  const uint32_t searches[2] = {8, 16};
+  const uint32_t searches[3] = {8, 16, 32};
   const uint32_t slot_counts[2] = {3, 4};
   const uint32_t variants[][2] = {
 #define AR_LIST(B, SS) {B, SS},
