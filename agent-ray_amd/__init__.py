@@ -647,9 +647,13 @@ class PathTracerConfig:
 
 
 class MisMode(IntEnum):
-    """bidirectional_path_tracer.rs:91-99."""
+    """bidirectional_path_tracer.rs:91-99 (Power, Uniform), plus Textbook: the
+    power heuristic with every reverse pdf taken once, the camera cosine and
+    the light pick's pmf included (no reference counterpart), whose weights
+    sum to one, so the image equals the path tracer's in expectation."""
     Power = 0
     Uniform = 1
+    Textbook = 2
 
 
 @dataclass

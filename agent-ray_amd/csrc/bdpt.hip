@@ -631,10 +631,13 @@ struct PdfsLds {
   }
 };
 
-// :594-770
+// :594-770.  rev_z2 / rev_y2: the pdfs of z[s-2] / y[t-2] generated from
+// z[s-1] / y[t-1] arriving from the connection, used by the textbook walk
+// only (mis_mode 2).
 template <class Pdfs>
 __device__ __forceinline__ float mis_weight(const BdptParams& P, const Pdfs& pd, uint32_t nc, uint32_t nl, uint32_t s,
-                                            uint32_t t, float pdf_cc, float pdf_cl, bool area_light) {
+                                            uint32_t t, float pdf_cc, float pdf_cl, bool area_light, float rev_z2,
+                                            float rev_y2) {
   if (P.mis_mode == 1) {  // :605-636
     uint32_t len = s + t, count = 0;
     uint32_t tmx = len < nl ? len : nl;
@@ -647,6 +650,30 @@ __device__ __forceinline__ float mis_weight(const BdptParams& P, const Pdfs& pd,
   }
   if (s + t == 2) return 1.0f;
   double sum = 0.0;
+  if (P.mis_mode == 2) {
+    // Textbook walk (extension): strategy (s', t') of the same path is
+    // reached by moving the connection; each vertex's reverse pdf is taken
+    // once, with the connection's own pdfs at the two vertices next to it.
+    // Alternatives the subpath lengths cannot produce (max_depth + 1
+    // vertices per subpath) or that do not exist (s' = 0; t' = 0 without an
+    // area light) are left out, so a path's weights sum to one.
+    const int k = (int)(s + t), dmax = (int)P.pt.max_depth + 1;
+    double ri = 1.0;
+    for (int i = (int)s - 1; i >= 1; --i) {  // alternative (i, k - i)
+      const float2 v = pd(0, (uint32_t)i);
+      const double rev = i == (int)s - 1 ? pdf_cc : (i == (int)s - 2 ? rev_z2 : fmaxf(v.y, 0.0f));
+      ri *= rev / (double)fmaxf(v.x, 1e-30f);
+      if (k - i <= dmax) sum += powb(fabs(ri), P.mis_beta);
+    }
+    ri = 1.0;
+    for (int i = (int)t - 1; i >= 0; --i) {  // alternative (k - i, i)
+      const float2 v = pd(1, (uint32_t)i);
+      const double rev = i == (int)t - 1 ? pdf_cl : (i == (int)t - 2 ? rev_y2 : fmaxf(v.y, 0.0f));
+      ri *= rev / (double)fmaxf(v.x, 1e-30f);
+      if ((i >= 1 || area_light) && k - i <= dmax) sum += powb(fabs(ri), P.mis_beta);
+    }
+    return (float)(1.0 / (1.0 + sum));
+  }
   // Only surface vertices can be MIS-delta, and the reference never marks
   // them delta (:268), so every "delta" test below is false except where the
   // reference forces an unimplemented s' = 0 / t' = 0 strategy.
@@ -729,7 +756,8 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
           const DLight& L = v.lights[mz.light_id];
           pdf_cc = L.pmf / L.area;
         }
-        float w = mis_weight(P, pdfs, smax, tmax, s, 0, pdf_cc, 0.0f, true);
+        const float rev_z2 = s >= 3 ? pdfs(0, s - 2).y : 0.0f;
+        float w = mis_weight(P, pdfs, smax, tmax, s, 0, pdf_cc, 0.0f, true, rev_z2, 0.0f);
         emit(Ray{}, c * w, kOwnPixel, false);
         continue;
       }
@@ -751,11 +779,22 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
         float cos_y = fabsf(dot(y.n, to_cam));
         float g = cos_y / (cs.dist * cs.dist);
         f3 c = y.beta * f * g * cs.we;
+        // the textbook walk takes the camera's cosine, which the reference's
+        // We = 1 / (A cos^4) leaves out (:1104-1108)
+        if (P.mis_mode == 2) c = c * dot(-to_cam, cam.fwd);
         if (maxc(c) <= 0.0f) continue;
-        float pd = bsdf_pdf(m, wi_l, wo_l);
-        PV c0 = load_v(Vt, 0, 0, p);
-        float pdf_cl = pdf_sa_to_area(pd, y.p, cam.origin, c0.n);
-        float w = mis_weight(P, pdfs, smax, tmax, 1, t, 0.0f, pdf_cl, area_light);
+        float pdf_cl, rev_y2 = 0.0f;
+        if (P.mis_mode == 2) {
+          // the camera's own pdf of the direction to y, in area measure at y
+          const Ray cr = make_ray(cam.origin, -to_cam, 0.0f, kF32Max);
+          pdf_cl = pdf_sa_to_area(cam_pdf_we(cam, cr.d), cam.origin, y.p, y.n);
+          rev_y2 = pdf_sa_to_area(bsdf_pdf(m, wo_l, wi_l), y.p, yp.p, yp.n);
+        } else {
+          float pd = bsdf_pdf(m, wi_l, wo_l);
+          PV c0 = load_v(Vt, 0, 0, p);
+          pdf_cl = pdf_sa_to_area(pd, y.p, cam.origin, c0.n);
+        }
+        float w = mis_weight(P, pdfs, smax, tmax, 1, t, 0.0f, pdf_cl, area_light, 0.0f, rev_y2);
         c = c * w;
         uint32_t ix = (uint32_t)cs.px, iy = (uint32_t)cs.py;
         if (ix > W - 1) ix = W - 1;
@@ -806,23 +845,30 @@ __device__ __forceinline__ void bdpt_strategies(const ShadeView& v, const BdptPa
         f3 fd = bsdf_eval(mz, to_local(onz, ls.wi), wo_z_l);
         float cos_z = fabsf(dot(z.n, ls.wi));
         c = z.beta * fd * cos_z * ls.li;
+        if (P.mis_mode == 2) c = c / Lt->pmf;  // textbook: the light pick's pmf
       } else {
         c = z.beta * f_z * g * f_y * y.beta;
       }
       if (maxc(c) <= 0.0f) continue;
-      float pdf_cc;
+      float pdf_cc, rev_y2 = 0.0f, rev_z2 = 0.0f;
       if (t == 1) {
         f3 dyz = normalize(z.p - y.p);
         pdf_cc = pdf_sa_to_area(light_pdf_emission_dir(*Lt, dyz), y.p, z.p, z.n);
       } else {
         f3 wi_y2 = normalize(z.p - y.p);
-        float pd = bsdf_pdf(material_at(v, y.id, y.tu, y.tv), to_local(ony, wi_y2), wo_y_l);
+        const DMaterial my = material_at(v, y.id, y.tu, y.tv);
+        float pd = bsdf_pdf(my, to_local(ony, wi_y2), wo_y_l);
         pdf_cc = pdf_sa_to_area(pd, y.p, z.p, z.n);
+        if (P.mis_mode == 2) {
+          const PV yp = load_v(Vt, 1, t - 2, p);
+          rev_y2 = pdf_sa_to_area(bsdf_pdf(my, wo_y_l, to_local(ony, wi_y2)), y.p, yp.p, yp.n);
+        }
       }
       f3 wi_z2 = normalize(y.p - z.p);
       float pdz = bsdf_pdf(mz, to_local(onz, wi_z2), wo_z_l);
       float pdf_cl = pdf_sa_to_area(pdz, z.p, y.p, y.n);
-      float w = mis_weight(P, pdfs, smax, tmax, s, t, pdf_cc, pdf_cl, area_light);
+      if (P.mis_mode == 2 && s >= 3) rev_z2 = pdf_sa_to_area(bsdf_pdf(mz, wo_z_l, to_local(onz, wi_z2)), z.p, zp.p, zp.n);
+      float w = mis_weight(P, pdfs, smax, tmax, s, t, pdf_cc, pdf_cl, area_light, rev_z2, rev_y2);
       emit(sr, c * w, kOwnPixel, true);
     }
   }

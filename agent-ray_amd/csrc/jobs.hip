@@ -30,7 +30,7 @@ ar_status validate_job(const ar_scene* s, const JobSpec& spec) {
   if (c.schedule > AR_SCHED_MEGAKERNEL) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown schedule");
   if (c.integrator == AR_INTEGRATOR_BDPT && c.max_depth > 64) return fail(AR_ERR_INVALID_ARG, "BDPT max_depth > 64");
   if (c.integrator == AR_INTEGRATOR_GDPT && !(c.alpha > 0.0f)) return fail(AR_ERR_INVALID_ARG, "GDPT alpha <= 0");
-  if (c.mis_mode > 1) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown MIS mode");
+  if (c.mis_mode > 2) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown MIS mode");
   // Paths in flight per chunk index 32-bit path ids and size the per-path
   // buffers: bound them before anything is allocated or launched.
   const uint64_t per_chunk = (uint64_t)spec.width * spec.height * chunk_samples(spec) *

@@ -124,7 +124,10 @@ typedef struct ar_render_config {
   uint32_t schedule;      /* AR_SCHED_* */
   uint32_t sample_chunk;  /* samples per pixel per wave (0 = auto) */
   /* BDPT (BdptConfig, bidirectional_path_tracer.rs:102-133) */
-  uint32_t mis_mode;      /* 0 = power heuristic, 1 = uniform */
+  uint32_t mis_mode;      /* 0 = power heuristic, 1 = uniform (MisWeightMode, bidirectional_path_tracer.rs:91-99);
+                             2 = textbook power heuristic (no reference counterpart): every reverse pdf taken once,
+                             the camera cosine and the light pick's pmf included, so the weights of a path sum to
+                             one and the image equals the path tracer's in expectation */
   float mis_beta;
   /* GDPT (GdptConfig, gradient_domain_pt.rs:47-76) */
   float alpha;

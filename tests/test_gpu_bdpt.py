@@ -11,9 +11,11 @@ oracle, not unbiasedness:
 * shared counter RNG: films bitwise equal to the oracle's (light-tracing
   splats summed in the same 2^-24 fixed point on both sides), including the
   t = 0 strategy (the camera sees the quad light);
-* both light types, power and uniform MIS modes, beta = 1 and 2; the room
+* both light types, power, uniform and textbook MIS modes, beta = 1 and 2; the room
   at the benchmark's BDPT settings (depth 10, power heuristic beta = 2,
   three quad lights, power light selection);
+* the textbook walk (mis_mode 2, no reference counterpart) makes the GPU's
+  BDPT image equal its path tracer's, block by block over replicates;
 * determinism: identical films across runs and sample chunkings;
 * the wavefront schedule (subpaths as bounce loops of extend and shade
   stages; connection enumerated, every shadow ray traced in one any-hit
@@ -35,7 +37,8 @@ def bdpt(ar, sc, cam, w, h, spp, seed=0, depth=8, mode=0, beta=2.0, chunk=0, sch
     return film.pixels.copy(), b.last_stats
 
 
-@pytest.mark.parametrize("twin,mode,beta", [(True, 0, 2.0), (False, 0, 2.0), (True, 1, 2.0), (False, 0, 1.0)])
+@pytest.mark.parametrize("twin,mode,beta", [(True, 0, 2.0), (False, 0, 2.0), (True, 1, 2.0), (False, 0, 1.0),
+                                             (True, 2, 2.0), (False, 2, 2.0)])
 def test_bdpt_matches_oracle(ar, orc, wl, twin, mode, beta):
     desc = wl.cornell_box(point_light_twin=twin)
     sc = ar.Scene.from_desc(desc)
@@ -80,7 +83,7 @@ def test_bdpt_room_matches_oracle(ar, orc, wl):
     assert st.paths == 96 * 54 * 4
 
 
-@pytest.mark.parametrize("twin,mode", [(True, 0), (False, 0), (False, 1)])
+@pytest.mark.parametrize("twin,mode", [(True, 0), (False, 0), (False, 1), (False, 2)])
 def test_bdpt_wavefront_connection_bitwise(ar, wl, twin, mode):
     desc = wl.cornell_box(point_light_twin=twin)
     sc = ar.Scene.from_desc(desc)
@@ -94,6 +97,36 @@ def test_bdpt_wavefront_connection_bitwise(ar, wl, twin, mode):
     assert sm.shadow_rays == sw.shadow_rays > 0
     chunked, _ = bdpt(ar, sc, cam, 256, 256, 32, seed=3, depth=16, mode=mode, chunk=5)
     assert np.array_equal(chunked.view(np.uint32), wave.view(np.uint32))
+
+
+@pytest.mark.parametrize("twin", [True, False])
+def test_textbook_bdpt_equals_path_tracer(ar, wl, twin):
+    """With the textbook walk every path's weights sum to one, so BDPT and the
+    path tracer estimate the same image: 4 x 4 block means (channels summed)
+    over 16 independent replicates each agree within 4.5 standard errors, and
+    the reference's own walk does not (it is biased by +21 % / +10 %)."""
+    desc = wl.cornell_box(point_light_twin=twin)
+    sc = ar.Scene.from_desc(desc)
+    W = 64
+    cam = ar.PinholeCamera.from_array(
+        wl.camera_lookat(desc.camera.lookfrom, desc.camera.lookat, desc.camera.vup, desc.camera.vfov, 1.0))
+
+    def blocks(img):
+        return img.sum(axis=2).reshape(4, W // 4, 4, W // 4).mean(axis=(1, 3)).ravel()
+
+    reps = 16
+    pt = np.asarray([blocks(ar.PathTracer(ar.PathTracerConfig(spp=128, max_depth=5, rr_depth=3), seed=500 + r)
+                            .render(sc, cam, W, W).pixels) for r in range(reps)], np.float64)
+    bd = np.asarray([blocks(bdpt(ar, sc, cam, W, W, 32, seed=700 + r, depth=5, mode=2)[0]) for r in range(reps)],
+                    np.float64)
+    ref = np.asarray([blocks(bdpt(ar, sc, cam, W, W, 32, seed=900 + r, depth=5, mode=0)[0]) for r in range(4)],
+                     np.float64)
+    se = np.sqrt(bd.var(axis=0, ddof=1) / reps + pt.var(axis=0, ddof=1) / reps)
+    z = (bd.mean(axis=0) - pt.mean(axis=0)) / np.maximum(se, 1e-30)
+    print(f"  twin={twin}: textbook / PT {bd.mean() / pt.mean():.4f}, max |z| {np.abs(z).max():.2f}; "
+          f"reference walk / PT {ref.mean() / pt.mean():.4f}")
+    assert np.all(np.abs(z) < 4.5)
+    assert abs(ref.mean() / pt.mean() - 1.0) > 0.05
 
 
 def test_bdpt_wavefront_connection_launches(ar, wl):
