@@ -86,7 +86,8 @@ class ArSceneInfo(C.Structure):
 
 class ArServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active_jobs", C.c_uint32),
-                ("max_concurrent_jobs", C.c_uint32)]
+                ("max_concurrent_jobs", C.c_uint32), ("pool_paths", C.c_uint32), ("pool_min_paths", C.c_uint32),
+                ("pool_fill_paths", C.c_uint32), ("chunk_paths", C.c_uint32), ("bdpt_chunk_samples", C.c_uint32)]
 
 
 class ArJobDesc(C.Structure):
@@ -849,10 +850,21 @@ class RenderServer:
     one resident scene; jobs are admitted into slots by HBM budget and run
     concurrently (see include/agentray.h, ar_server_*)."""
 
+    POOL_OFF = 0xFFFFFFFF
+
     def __init__(self, scene: Scene, hbm_budget_bytes: int = 200 << 30, max_active_jobs: int = 8,
-                 max_concurrent_jobs: int = 0):
+                 max_concurrent_jobs: int = 0, pool_paths: int = 0, pool_min_paths: int = 0,
+                 pool_fill_paths: int = 0, chunk_paths: int = 0, bdpt_chunk_samples: int = 0):
+        """pool_paths: paths in flight in the shared path-tracer queues (0 =
+        the library default, RenderServer.POOL_OFF = no pool); PT / GDPT jobs
+        of at least pool_min_paths paths (0 = default) share them; while one
+        drains its last paths the others keep at most pool_fill_paths alive
+        (0 = default).  Jobs
+        outside the pool run in chunks of chunk_paths paths (BDPT:
+        bdpt_chunk_samples samples); 0 = default."""
         self.scene = scene
-        cfg = ArServerConfig(hbm_budget_bytes, max_active_jobs, max_concurrent_jobs)
+        cfg = ArServerConfig(hbm_budget_bytes, max_active_jobs, max_concurrent_jobs, pool_paths, pool_min_paths,
+                             pool_fill_paths, chunk_paths, bdpt_chunk_samples)
         self._h = C.c_void_p()
         _check(_lib.ar_server_create(scene.handle, C.byref(cfg), C.byref(self._h)))
         self._shapes = {}

@@ -14,6 +14,7 @@ struct JobSpec {
   ar_camera camera{};
   ar_render_config cfg{};
   uint64_t chunk_target = 0;  // paths (BDPT: samples) per chunk when cfg.sample_chunk is 0; 0 = default
+  uint64_t conn_pass = 0;     // BDPT paths per connection pass at most; 0 = as many as the pass budget holds
 };
 
 class JobRun {
@@ -21,8 +22,11 @@ class JobRun {
   virtual ~JobRun() {}
   // Reserve device memory and enqueue the set-up work.
   virtual ar_status begin() = 0;
-  // Enqueue the next chunk; *done is set once every sample has been issued.
+  // Enqueue the next piece of work (a chunk, or a phase of one); *done is
+  // set once every sample has been issued.
   virtual ar_status step(bool* done) = 0;
+  // Share of a chunk's work the last step() issued.
+  virtual double last_step_share() const { return 1.0; }
   // Enqueue the resolve (and the Poisson solve for GDPT) into out/aux.
   virtual ar_status finish() = 0;
   // After the stream has drained: rays/paths counters.
@@ -52,7 +56,7 @@ constexpr uint64_t kMaxChunkPaths = 1ull << 31;
 constexpr uint64_t kConnBudget = 12ull << 30;
 constexpr uint64_t kConnSlotBytes = 3 * 16 + 1 + 4;
 uint64_t bdpt_conn_slots(uint32_t max_depth);
-uint64_t bdpt_conn_pass(uint32_t max_depth, uint64_t cap);
+uint64_t bdpt_conn_pass(uint32_t max_depth, uint64_t cap, uint64_t limit = 0);
 ar_status validate_job(const ar_scene* s, const JobSpec& spec);
 
 }  // namespace ar

@@ -91,7 +91,7 @@ struct Workspace {
   uint32_t capacity = 0;
   void* params = nullptr;  // device copy of the chunk's parameters (kParamsBytes), read by captured stages
   GraphCache pt_graph;     // path tracer / GDPT bounce loop of one chunk
-  GraphCache bdpt_graph;   // BDPT chunk: subpaths, connection passes, film
+  GraphCache bdpt_graph[2];  // BDPT chunk: camera and light subpath bounce loops
   ar_status reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_entries, uint32_t grid_threads,
                           hipStream_t st);
   size_t bytes() const { return buf.bytes + out.bytes + aux.bytes + pois.bytes; }
@@ -100,7 +100,9 @@ struct Workspace {
 
 uint32_t stack_spill_entries(const ar_scene* s);
 
-// Screened Poisson CG on device buffers (interleaved RGB films).
+// Screened Poisson CG on device buffers (interleaved RGB films); the
+// scratch grows to poisson_scratch_bytes(w, h) if smaller.
+size_t poisson_scratch_bytes(uint32_t w, uint32_t h);
 ar_status poisson_solve_device(const float* P_rgb, const float* gx_rgb, const float* gy_rgb, uint32_t w, uint32_t h,
                                float alpha, uint32_t iters, int32_t sign, float* out_rgb, double* final_residual,
                                DevBuf& scratch, hipStream_t st);

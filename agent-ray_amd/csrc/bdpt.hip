@@ -1109,6 +1109,8 @@ class BdptJob : public JobRun {
   ar_trace_config tc{}, ta{};
   unsigned long long* splat = nullptr;
   uint32_t k = 1, s0 = 0, G = 0, eg = 0, Gs = 0, Ge = 0, Gw = 0, pass = 0, cap = 0, spill_entries = 0;
+  uint32_t phase = 0;          // next piece of the wavefront chunk: 0 camera, 1 light subpaths, 2 + i pass i
+  double step_share = 1.0;     // share of the chunk the last step issued
   bool wavefront = true;
   static constexpr int kSubBlock = 512;  // subpath generate / shade stages
   static constexpr size_t kPdfLdsMax = 64 * 1024;  // LDS for the MIS walks' pdf pairs (depth <= 31 at 128 lanes)
@@ -1177,7 +1179,7 @@ class BdptJob : public JobRun {
     if (e != AR_OK) return e;
     size_t nvert = 2ull * P.V * cap;
     C.K = (uint32_t)bdpt_conn_slots(cfg->max_depth);
-    pass = wavefront ? (uint32_t)bdpt_conn_pass(cfg->max_depth, cap) : 0;
+    pass = wavefront ? (uint32_t)bdpt_conn_pass(cfg->max_depth, cap, spec.conn_pass) : 0;
     C.pass = pass;
     const size_t slots = (size_t)pass * C.K;
     auto al = [](size_t n) { return (n + 255) / 256 * 256; };
@@ -1220,28 +1222,22 @@ class BdptJob : public JobRun {
     return AR_OK;
   }
 
-  // One chunk of the wavefront schedule: both subpaths bounce by bounce,
-  // then the connection passes, then the film.  The launches depend on the
-  // job's buffers, grids and launch configurations only (the chunk's s0 and
-  // k are read from the device block), so they are captured once per job
-  // shape and replayed per chunk; passes past a shorter last chunk are empty.
-  ar_status run_chunk_wavefront(const BdptParams* Pd) {
+  // A chunk of the wavefront schedule is issued in pieces, so that a server
+  // can put other work between them: the camera subpaths, the light
+  // subpaths (each bounce loop captured once per job shape in a HIP graph
+  // and replayed, the chunk's s0 and k being read from the device block),
+  // then each connection pass, the last followed by the film.  Passes past
+  // the chunk's paths are not issued (they would read an empty range).
+  ar_status run_subpaths(const BdptParams* Pd, uint32_t side) {
     const WaveBufs& B = ws->B;
     int2* const spill = ws->spill;
-    const uint32_t npass = (cap + pass - 1) / pass;
-    // the MIS walks' pdf pairs in LDS while they fit a workgroup's share
-    const size_t pdf_lds_bytes = 2ull * Vt.V * kTraceBlock * sizeof(float2) <= kPdfLdsMax
-                                     ? 2ull * Vt.V * kTraceBlock * sizeof(float2)
-                                     : 0;
-    const uint32_t Gg = std::min<uint32_t>((pass + 255) / 256, 4096);
     auto record = [&]() {
-      for (uint32_t side = 0; side < 2; ++side) {
-        if (side == 0)
-          hipLaunchKernelGGL((k_bdpt_cam_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, Pd, Vt, B);
-        else
-          hipLaunchKernelGGL((k_bdpt_light_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, Pd, Vt, B);
-        for (uint32_t d = 0; d < P.pt.max_depth; ++d) {
-          const int qi = d & 1;
+      if (side == 0)
+        hipLaunchKernelGGL((k_bdpt_cam_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, Pd, Vt, B);
+      else
+        hipLaunchKernelGGL((k_bdpt_light_gen<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, Pd, Vt, B);
+      for (uint32_t d = 0; d < P.pt.max_depth; ++d) {
+        const int qi = d & 1;
 #define AR_SUB_EXT(BB, SS)                                                                                   \
   if (tc.workgroup_size == BB && tc.lds_stack == SS) {                                                       \
     if (tc.park_slots == 4)                                                                                  \
@@ -1251,36 +1247,13 @@ class BdptJob : public JobRun {
       hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 3>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi, spill,  \
                          tc.refill_min, tc.search_min);                                                      \
   }
-          AR_TRACE_VARIANTS(AR_SUB_EXT)
+        AR_TRACE_VARIANTS(AR_SUB_EXT)
 #undef AR_SUB_EXT
-          hipLaunchKernelGGL((k_bdpt_sub_shade<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, Pd, Vt, B, qi,
-                             side);
-          hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, qi);
-        }
-        hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, -1);
+        hipLaunchKernelGGL((k_bdpt_sub_shade<kSubBlock>), dim3(Gw), dim3(kSubBlock), 0, stream, v, Pd, Vt, B, qi,
+                           side);
+        hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, qi);
       }
-      for (uint32_t pi = 0; pi < npass; ++pi) {
-        hipLaunchKernelGGL(k_bdpt_conn_reset, dim3(1), dim3(64), 0, stream, C);
-        if (pdf_lds_bytes)
-          hipLaunchKernelGGL((k_bdpt_conn_prep<kTraceBlock, true>), dim3(G), dim3(kTraceBlock), pdf_lds_bytes, stream,
-                             v, Pd, Vt, C, pi, B.stats);
-        else
-          hipLaunchKernelGGL((k_bdpt_conn_prep<kTraceBlock, false>), dim3(G), dim3(kTraceBlock), 0, stream, v, Pd, Vt,
-                             C, pi, B.stats);
-#define AR_VIS(BB, SS)                                                                                      \
-  if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                      \
-    if (ta.park_slots == 4)                                                                                 \
-      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 4>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, spill,       \
-                         ta.refill_min, ta.search_min);                                                     \
-    else                                                                                                    \
-      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 3>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, spill,       \
-                         ta.refill_min, ta.search_min);                                                     \
-  }
-        AR_TRACE_VARIANTS(AR_VIS)
-#undef AR_VIS
-        hipLaunchKernelGGL(k_bdpt_conn_gather, dim3(Gg), dim3(256), 0, stream, Pd, C, pi, B.L, splat);
-      }
-      hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, B.L, ws->acc, Pd);
+      hipLaunchKernelGGL(k_bdpt_sub_reset, dim3(1), dim3(64), 0, stream, B, -1);
     };
     const uint64_t key[GraphCache::kKeyWords] = {(uint64_t)(uintptr_t)scene,
                                                  (uint64_t)(uintptr_t)ws->buf.p,
@@ -1293,33 +1266,93 @@ class BdptJob : public JobRun {
                                                  ((uint64_t)pass << 32) | C.K,
                                                  ((uint64_t)P.pt.max_depth << 32) | spill_entries,
                                                  view_hash(v),
-                                                 0};
-    return ws->bdpt_graph.launch(key, stream, "BDPT chunk", record);
+                                                 side};
+    return ws->bdpt_graph[side].launch(key, stream, side ? "BDPT light subpaths" : "BDPT camera subpaths", record);
+  }
+
+  // Connection pass pi of the chunk: enumerate the strategies and store the
+  // weighted contributions, trace their shadow rays (any hit), gather the
+  // unoccluded ones in enumeration order.
+  void run_conn_pass(const BdptParams* Pd, uint32_t pi) {
+    const WaveBufs& B = ws->B;
+    int2* const spill = ws->spill;
+    // the MIS walks' pdf pairs in LDS while they fit a workgroup's share
+    const size_t pdf_lds_bytes = 2ull * Vt.V * kTraceBlock * sizeof(float2) <= kPdfLdsMax
+                                     ? 2ull * Vt.V * kTraceBlock * sizeof(float2)
+                                     : 0;
+    const uint32_t Gg = std::min<uint32_t>((pass + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_bdpt_conn_reset, dim3(1), dim3(64), 0, stream, C);
+    if (pdf_lds_bytes)
+      hipLaunchKernelGGL((k_bdpt_conn_prep<kTraceBlock, true>), dim3(G), dim3(kTraceBlock), pdf_lds_bytes, stream, v,
+                         Pd, Vt, C, pi, B.stats);
+    else
+      hipLaunchKernelGGL((k_bdpt_conn_prep<kTraceBlock, false>), dim3(G), dim3(kTraceBlock), 0, stream, v, Pd, Vt, C,
+                         pi, B.stats);
+#define AR_VIS(BB, SS)                                                                                      \
+  if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                      \
+    if (ta.park_slots == 4)                                                                                 \
+      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 4>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, spill,       \
+                         ta.refill_min, ta.search_min);                                                     \
+    else                                                                                                    \
+      hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 3>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, spill,       \
+                         ta.refill_min, ta.search_min);                                                     \
+  }
+    AR_TRACE_VARIANTS(AR_VIS)
+#undef AR_VIS
+    hipLaunchKernelGGL(k_bdpt_conn_gather, dim3(Gg), dim3(256), 0, stream, Pd, C, pi, B.L, splat);
   }
 
   ar_status step(bool* done) override {
     constexpr int BLOCK = kTraceBlock;
-    if (s0 < P.pt.spp) {
+    BdptParams* Pd = reinterpret_cast<BdptParams*>(ws->params);
+    step_share = 1.0;
+    if (s0 < P.pt.spp && !wavefront) {
       P.pt.s0 = s0;
       P.pt.k = std::min(k, P.pt.spp - s0);
-      BdptParams* Pd = reinterpret_cast<BdptParams*>(ws->params);
       hipLaunchKernelGGL(k_bdpt_params, dim3(1), dim3(64), 0, stream, P, Pd);
-      if (wavefront) {
-        ar_status e = run_chunk_wavefront(Pd);
-        if (e != AR_OK) return e;
-      } else {
-        hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->spill,
-                           ws->B.stats);
-        hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->B.L, splat,
-                           ws->spill, ws->B.stats);
-        hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, Pd);
-      }
+      hipLaunchKernelGGL((k_bdpt_subpaths<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->spill,
+                         ws->B.stats);
+      hipLaunchKernelGGL((k_bdpt_connect<BLOCK>), dim3(G), dim3(BLOCK), 0, stream, v, P, Vt, ws->B.L, splat,
+                         ws->spill, ws->B.stats);
+      hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, Pd);
       AR_HIP(hipGetLastError());
       s0 += P.pt.k;
+    } else if (s0 < P.pt.spp) {
+      // subpaths take about a third of a chunk, the passes share the rest
+      const uint32_t paths = P.pt.npix * std::min(k, P.pt.spp - s0);
+      const uint32_t npass = (paths + pass - 1) / pass;
+      if (phase == 0) {
+        P.pt.s0 = s0;
+        P.pt.k = std::min(k, P.pt.spp - s0);
+        hipLaunchKernelGGL(k_bdpt_params, dim3(1), dim3(64), 0, stream, P, Pd);
+        ar_status e = run_subpaths(Pd, 0);
+        if (e != AR_OK) return e;
+        step_share = 1.0 / 6.0;
+        phase = 1;
+      } else if (phase == 1) {
+        ar_status e = run_subpaths(Pd, 1);
+        if (e != AR_OK) return e;
+        step_share = 1.0 / 6.0;
+        phase = 2;
+      } else {
+        const uint32_t pi = phase - 2;
+        run_conn_pass(Pd, pi);
+        step_share = 2.0 / 3.0 / npass;
+        if (pi + 1 < npass) {
+          phase++;
+        } else {
+          hipLaunchKernelGGL(k_film_accumulate_b, dim3(eg), dim3(256), 0, stream, ws->B.L, ws->acc, Pd);
+          phase = 0;
+          s0 += P.pt.k;
+        }
+      }
+      AR_HIP(hipGetLastError());
     }
     *done = s0 >= P.pt.spp;
     return AR_OK;
   }
+
+  double last_step_share() const override { return step_share; }
 
   ar_status finish() override {
     hipLaunchKernelGGL(k_bdpt_resolve, dim3(eg), dim3(256), 0, stream, ws->acc, (const long long*)splat, out_dev,

@@ -51,9 +51,11 @@ uint64_t bdpt_conn_slots(uint32_t max_depth) {
   return K;
 }
 
-uint64_t bdpt_conn_pass(uint32_t max_depth, uint64_t cap) {
+uint64_t bdpt_conn_pass(uint32_t max_depth, uint64_t cap, uint64_t limit) {
   const uint64_t per_path = bdpt_conn_slots(max_depth) * kConnSlotBytes + sizeof(uint32_t);
-  return std::max<uint64_t>(64, std::min<uint64_t>(cap, kConnBudget / per_path));
+  uint64_t pass = std::min<uint64_t>(cap, kConnBudget / per_path);
+  if (limit) pass = std::min<uint64_t>(pass, limit);
+  return std::max<uint64_t>(64, pass);
 }
 
 size_t job_footprint(const ar_scene* s, const JobSpec& spec) {
@@ -66,7 +68,8 @@ size_t job_footprint(const ar_scene* s, const JobSpec& spec) {
   if (c.integrator == AR_INTEGRATOR_BDPT) {
     bytes += paths * (2ull * (c.max_depth + 1) * 56 + 8) + npix * 24;  // vertices, counts, splat film
     if (c.schedule != AR_SCHED_MEGAKERNEL)  // wavefront connection slots of one pass
-      bytes += bdpt_conn_pass(c.max_depth, paths) * (bdpt_conn_slots(c.max_depth) * kConnSlotBytes + 4) + 256;
+      bytes += bdpt_conn_pass(c.max_depth, paths, spec.conn_pass) * (bdpt_conn_slots(c.max_depth) * kConnSlotBytes + 4) +
+               256;
   }
   bytes += npix * 12 * 4;  // output staging
   (void)s;

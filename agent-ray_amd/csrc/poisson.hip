@@ -252,6 +252,12 @@ __global__ void k_deinterleave(const float* in, float* out, uint32_t n) {
     for (int c = 0; c < 3; ++c) out[(size_t)c * n + i] = in[3 * (size_t)i + c];
 }
 
+size_t poisson_scratch_bytes(uint32_t w, uint32_t h) {
+  size_t plane = (size_t)w * h * 3 * sizeof(float);
+  size_t pl_al = (plane + 255) / 256 * 256;
+  return pl_al * 9 + 3 * kPoisMaxBlocks * sizeof(double) + 256;
+}
+
 // Solve with interleaved RGB device inputs; scratch is caller-provided.
 ar_status poisson_solve_device(const float* P_rgb, const float* gx_rgb, const float* gy_rgb, uint32_t w, uint32_t h,
                                float alpha, uint32_t iters, int32_t sign, float* out_rgb, double* final_residual,
@@ -259,7 +265,7 @@ ar_status poisson_solve_device(const float* P_rgb, const float* gx_rgb, const fl
   uint32_t n = w * h;
   size_t plane = (size_t)n * 3 * sizeof(float);
   size_t pl_al = (plane + 255) / 256 * 256;
-  size_t need = pl_al * 9 + 3 * kPoisMaxBlocks * sizeof(double) + 256;
+  size_t need = poisson_scratch_bytes(w, h);
   if (scratch.bytes < need) {
     AR_HIP(hipStreamSynchronize(st));
     AR_HIP(scratch.alloc(need));

@@ -3,45 +3,57 @@
 //
 // Continuous batching: jobs wait in a FIFO and are admitted into job slots
 // as soon as a slot is free and the HBM budget covers the job's footprint.
-// Every slot owns a grow-only workspace (path state, queues, film, stack
-// spill) and a pinned output buffer.  Admitted jobs' chunks run on up to
-// `max_concurrent_jobs` lanes at once (one HIP stream each, so their kernels
-// overlap on the device: one job's late bounces, which leave most CUs idle,
-// run beside another job's full stages).  Whenever a lane is free the
-// scheduler issues the next chunk of the admitted job with the least
-// remaining work (shortest remaining processing time first), so a small job
-// that arrives while large ones render starts at the next chunk boundary of
-// any lane and is not queued behind them.  A job of one chunk ("short") runs
-// with the device to itself: while one is on the device or waiting, long
-// jobs issue no new chunk (theirs in flight drain), unless a long job has
-// waited 0.2 s since its last chunk; the persistent kernels of long jobs
-// fill every CU, so a short job's kernels would otherwise queue behind
-// theirs one by one.  Several long jobs share the lanes, their late bounces
-// overlapping.  A job has at most one chunk on the device; consecutive chunks
-// of a job may use different lanes, since each is issued only after the
-// previous one has completed.  Stream priorities are not used.  Server jobs
-// without an explicit sample_chunk run in one chunk when their work is at
-// most two chunks and otherwise in chunks of 2^23 path-equivalents (2^19
-// BDPT samples, both ~20-40 ms of device time): smaller chunks pay their
-// bounce tails more often (2^21: p50 latency / solo 2.2 instead of 1.35).  A job's film depends only on its own description (per-job
-// seed, deterministic accumulation in sample order, independent of chunking
-// and of the lanes), never on what it shared the device with.
+// Every slot owns a grow-only workspace and a pinned output buffer.
+//
+// Path-tracer and gradient-domain jobs of at least pool_min_paths paths
+// render in the pool (pool.hip): one set of wavefront queues holding the
+// paths of every such job, refilled every bounce from the jobs in order of
+// remaining work.  Smaller ones, BDPT jobs and megakernel jobs render in
+// chunks on up to `max_concurrent_jobs` lanes (one HIP stream each); a BDPT
+// chunk is issued in pieces (camera subpaths, light subpaths, connection
+// passes of a quarter chunk), each about 10 ms of device time at the
+// default chunk, and a job of at most two chunks' work runs as one chunk.
+//
+// Scheduling is shortest remaining processing time first over all jobs:
+// whenever the pool holds the job with the least remaining work it issues
+// iterations (two in flight), otherwise that job issues its next piece and
+// the pool issues nothing new; the others wait, so the preemption points are
+// pool iterations and pieces.  A job that has issued nothing for kStarveMax
+// issues anyway.  Inside the pool a draining job (every path generated) is
+// the top one, and the other jobs keep at most pool_fill_paths paths alive
+// meanwhile.  Stream priorities are not used.  A job's film depends only on
+// its own description (per-job seed, deterministic accumulation in sample
+// order, independent of chunking, lanes and what shared the pool), never on
+// what it shared the device with.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <limits>
 #include <map>
 #include <memory>
 #include <thread>
 
 #include "ar_job.h"
+#include "ar_pool.h"
+
+// Built with -DAR_SERVER_TRACE, the scheduler prints its events (server
+// clock, job, what happened) to stderr.
+#ifdef AR_SERVER_TRACE
+#include <cstdio>
+#define AR_STRACE(...) std::fprintf(stderr, __VA_ARGS__)
+#else
+#define AR_STRACE(...) \
+  do {                 \
+  } while (0)
+#endif
 
 namespace ar {
 
 constexpr uint32_t kMaxLanes = 4;            // hardware queues a process gets by default
-constexpr uint64_t kServerChunkPaths = 1ull << 23;   // PT / GDPT path-equivalents per chunk
-constexpr uint64_t kServerChunkBdpt = 1ull << 19;    // BDPT samples per chunk
-constexpr double kLongWaitMax = 0.2;                 // s a long job yields to short ones at most
+constexpr uint64_t kServerChunkPaths = 1ull << 23;   // PT / GDPT path-equivalents per chunk (default)
+constexpr uint64_t kServerChunkBdpt = 1ull << 21;    // BDPT samples per chunk (default)
+constexpr double kStarveMax = 1.0;                   // s a job waits for higher-priority work at most
 
 // Relative device cost of one pixel-sample of each integrator, measured on
 // the room at 1280x720 (PT path 1; GDPT sample = base + two shifted paths;
@@ -78,6 +90,8 @@ struct Job {
   int lane = -1;               // lane holding the job's chunk on the device, -1: none
   bool short_job = false;      // runs as one chunk
   double last_issue = 0;       // server clock of its last chunk
+  bool pooled = false;         // PT / GDPT job rendered in the shared pool
+  int pool_h = -1;             // its pool handle once started
   std::vector<float> film;
   ar_render_stats stats{};
 };
@@ -110,6 +124,18 @@ struct ar_server {
   std::thread worker;
   std::chrono::steady_clock::time_point t0;
   int device = 0;
+  std::unique_ptr<PathPool> pool;  // shared path-tracer queues (null: every job runs in chunks)
+  uint32_t pool_min_paths = 0;
+  double pool_last_issue = 0;  // server clock of the pool's last iteration (or last idle moment)
+
+  // PT / GDPT wavefront jobs of at least pool_min_paths paths share the pool.
+  bool poolable(const JobSpec& spec) const {
+    const ar_render_config& c = spec.cfg;
+    if (!pool || c.schedule != AR_SCHED_WAVEFRONT) return false;
+    if (c.integrator != AR_INTEGRATOR_PT && c.integrator != AR_INTEGRATOR_GDPT) return false;
+    const uint64_t paths = (uint64_t)spec.width * spec.height * c.spp * (c.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
+    return paths >= pool_min_paths;
+  }
 
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 
@@ -128,6 +154,10 @@ struct ar_server {
     if (j.begun) {
       event_pool.push_back(j.begun);
       j.begun = nullptr;
+    }
+    if (j.pool_h >= 0) {
+      pool->release(j.pool_h);
+      j.pool_h = -1;
     }
     if (j.slot >= 0) {
       slots[j.slot]->busy = false;
@@ -250,12 +280,15 @@ struct ar_server {
         sl.pinned_bytes = out_bytes;
       }
     }
-    if (e == AR_OK) {
+    if (e == AR_OK && j.pooled) {
+      e = pool->add(j.spec, &sl.ws, sl.ws.out.as<float>(), sl.pinned, sl.stream, j.id, &j.pool_h);
+      if (e != AR_OK) msg = std::string("pool: ") + ar_last_error();
+    } else if (e == AR_OK) {
       if (j.spec.cfg.sample_chunk == 0) {
         // a job of up to two chunks' work runs as one chunk; larger jobs are
         // split so that the device never holds much of them at a time
         uint64_t per_sample = npix * (j.spec.cfg.integrator == AR_INTEGRATOR_GDPT ? 3 : 1);
-        uint64_t target = j.spec.cfg.integrator == AR_INTEGRATOR_BDPT ? kServerChunkBdpt : kServerChunkPaths;
+        uint64_t target = j.spec.cfg.integrator == AR_INTEGRATOR_BDPT ? cfg.bdpt_chunk_samples : cfg.chunk_paths;
         if (per_sample * j.spec.cfg.spp <= 2 * target) {
           j.spec.cfg.sample_chunk = j.spec.cfg.spp;
           j.short_job = true;
@@ -289,10 +322,37 @@ struct ar_server {
     active.push_back(jp);
   }
 
-  // One pass: retire finished chunks, complete drained jobs, then fill free
-  // lanes with the chunks of the jobs with the least remaining work.
-  // Returns true if any progress was made.  Called without mu held; takes it
-  // only for bookkeeping.
+  // Remaining work of an active job in path-equivalents, the order of
+  // shortest remaining processing time first: a pooled job counts the paths
+  // it has still to generate (0 while its last paths drain), a chunked job
+  // the work of its chunks not yet issued.
+  double remaining(const Job& j) const {
+    if (j.pooled) {
+      if (j.pool_h < 0) return 0.0;
+      const double per = j.spec.cfg.integrator == AR_INTEGRATOR_GDPT ? path_weight(AR_INTEGRATOR_GDPT) / 3.0 : 1.0;
+      return per * (double)pool->remaining_paths(j.pool_h);
+    }
+    return j.work_left;
+  }
+
+  // Completes a job whose work has drained (caller holds mu).
+  void complete(const std::shared_ptr<Job>& jp) {
+    Job& j = *jp;
+    release(j);
+    active.erase(std::find(active.begin(), active.end(), jp));
+    cv_done.notify_all();
+    cv_work.notify_all();
+  }
+
+  // One pass: retire finished chunks and pool iterations, complete drained
+  // jobs, then let the job with the least remaining work issue work: the
+  // pool when one of its jobs is that job, otherwise that job's next chunk
+  // on a free lane (shortest remaining processing time first, preempting at
+  // pool iterations and chunk boundaries).  The pool fills what its top job
+  // leaves of each iteration with its other jobs' paths.  A job that has
+  // issued nothing for kStarveMax issues anyway.  Returns true if any
+  // progress was made.  Called without mu held; takes it only for
+  // bookkeeping.
   bool pump() {
     bool progress = false;
     for (auto& ln : lanes) {
@@ -307,10 +367,50 @@ struct ar_server {
       std::lock_guard<std::mutex> lk(mu);
       act = active;
     }
+    auto pool_failed = [&](ar_status e) {
+      // a device error in the shared queues ends every pooled job
+      (void)hipStreamSynchronize(pool->stream());
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto& jp : act)
+        if (jp->pooled) {
+          fail_job(*jp, e, std::string("pool: ") + ar_last_error());
+          complete(jp);
+        }
+      return true;
+    };
+    if (pool) {
+      // cancellations reach the pool before it plans another iteration
+      for (auto& jp : act)
+        if (jp->pooled && jp->pool_h >= 0 && jp->cancel) pool->cancel(jp->pool_h);
+      const ar_status e = pool->pump(false, &progress);
+      if (e != AR_OK) return pool_failed(e);
+    }
     for (auto& jp : act) {
       Job& j = *jp;
       if (j.lane >= 0) continue;
       std::lock_guard<std::mutex> lk(mu);
+      if (j.pooled) {
+        // film resolved and copied (state 1, done event complete) or
+        // cancelled and drained (state 2)
+        if (j.pool_h < 0) continue;
+        const int ps = pool->state(j.pool_h);
+        if (ps == 1 && hipEventQuery(pool->done_event(j.pool_h)) == hipSuccess) {
+          Slot& sl = *slots[j.slot];
+          size_t n = (size_t)j.spec.width * j.spec.height * 3;
+          j.film.assign(sl.pinned, sl.pinned + n);
+          pool->stats(j.pool_h, &j.stats);
+          j.state = j.cancel ? AR_JOB_CANCELLED : AR_JOB_DONE;
+        } else if (ps == 2) {
+          j.state = AR_JOB_CANCELLED;
+        } else {
+          continue;
+        }
+        j.t_finish = now();
+        AR_STRACE("T %.4f done pooled job %llu\n", j.t_finish, (unsigned long long)j.id);
+        complete(jp);
+        progress = true;
+        continue;
+      }
       if (!j.issued_all && !j.cancel) continue;
       // drained: complete the job
       if (j.cancel && !j.issued_all) {
@@ -324,30 +424,58 @@ struct ar_server {
         j.state = j.cancel ? AR_JOB_CANCELLED : AR_JOB_DONE;
         j.t_finish = now();
       }
-      release(j);
-      active.erase(std::find(active.begin(), active.end(), jp));
-      cv_done.notify_all();
-      cv_work.notify_all();
+      AR_STRACE("T %.4f done job %llu\n", j.t_finish, (unsigned long long)j.id);
+      complete(jp);
       progress = true;
+    }
+    // priorities: the least remaining work of the pool's jobs and of the
+    // chunked jobs (those with a chunk in flight included)
+    double pool_min = std::numeric_limits<double>::infinity();
+    double chunk_min = std::numeric_limits<double>::infinity();
+    std::shared_ptr<Job> chunk_top;
+    const double tnow = now();
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      for (auto& jp : active) {
+        Job& j = *jp;
+        if (j.cancel) continue;
+        if (j.pooled) {
+          if (j.pool_h >= 0 && pool->state(j.pool_h) == 0) pool_min = std::min(pool_min, remaining(j));
+          continue;
+        }
+        if (!j.run || (j.issued_all && j.lane < 0)) continue;
+        const double r = remaining(j);
+        if (!chunk_top || r < chunk_min || (r == chunk_min && j.t_submit < chunk_top->t_submit)) {
+          chunk_min = r;
+          chunk_top = jp;
+        }
+      }
+    }
+    if (pool && !pool->idle()) {
+      const bool turn = pool_min <= chunk_min || tnow - pool_last_issue >= kStarveMax;
+      const uint32_t inflight0 = pool->in_flight();
+      const ar_status e = pool->pump(turn, &progress);
+      if (e != AR_OK) return pool_failed(e);
+      if (turn) pool_last_issue = tnow;
+      if (pool->in_flight() > inflight0)
+        AR_STRACE("T %.4f pool iterations %u in flight, pool_min %.0f chunk_min %.0f\n", tnow, pool->in_flight(),
+                  pool_min, chunk_min);
+    } else {
+      pool_last_issue = tnow;
     }
     for (size_t li = 0; li < lanes.size(); ++li) {
       Lane& ln = lanes[li];
       if (ln.job) continue;
-      // shortest remaining work first; ties go to the earlier submission
+      // the chunked job with the least remaining work, if no pooled job has
+      // less; any job that has waited kStarveMax since its last chunk
       std::shared_ptr<Job> pick;
       {
         std::lock_guard<std::mutex> lk(mu);
-        bool short_active = false;
-        for (auto& jp : active)
-          if (jp->short_job && !jp->cancel && (!jp->issued_all || jp->lane >= 0)) short_active = true;
-        const double tnow = now();
         for (auto& jp : active) {
           Job& j = *jp;
-          if (j.lane >= 0 || j.issued_all || j.cancel) continue;
-          // while a short job is on the device or waiting, long jobs issue
-          // nothing new (their chunks in flight drain) unless one has waited
-          // kLongWaitMax since its last chunk
-          if (short_active && !j.short_job && tnow - j.last_issue < kLongWaitMax) continue;
+          if (j.pooled || j.lane >= 0 || j.issued_all || j.cancel || !j.run) continue;
+          const bool top = jp == chunk_top && chunk_min < pool_min;
+          if (!top && tnow - j.last_issue < kStarveMax) continue;
           if (!pick || j.work_left < pick->work_left ||
               (j.work_left == pick->work_left && j.t_submit < pick->t_submit))
             pick = jp;
@@ -361,7 +489,7 @@ struct ar_server {
       j.run->stream = ln.stream;
       bool done = false;
       if (e == AR_OK) e = j.run->step(&done);
-      j.work_left = std::max(0.0, j.work_left - j.work_chunk);
+      j.work_left = std::max(0.0, j.work_left - j.work_chunk * j.run->last_step_share());
       if (e == AR_OK && done) {
         e = j.run->finish();
         if (e == AR_OK) {
@@ -377,14 +505,14 @@ struct ar_server {
         (void)hipStreamSynchronize(ln.stream);
         std::lock_guard<std::mutex> lk(mu);
         fail_job(j, e, ar_last_error());
-        release(j);
-        active.erase(std::find(active.begin(), active.end(), pick));
-        cv_done.notify_all();
+        complete(pick);
         progress = true;
         continue;
       }
       j.lane = (int)li;
       j.last_issue = now();
+      AR_STRACE("T %.4f issue job %llu lane %zu share %.3f left %.0f pool_min %.0f\n", j.last_issue,
+                (unsigned long long)j.id, li, j.run->last_step_share(), j.work_left, pool_min);
       ln.job = pick;
       progress = true;
     }
@@ -406,7 +534,13 @@ struct ar_server {
           continue;
         }
       }
-      for (auto& jp : starting) start(jp);
+      for (auto& jp : starting) {
+        const double t0s = now();
+        start(jp);
+        AR_STRACE("T %.4f start job %llu integ %u %ux%u spp %u pooled %d took %.2f ms\n", t0s,
+                  (unsigned long long)jp->id, jp->spec.cfg.integrator, jp->spec.width, jp->spec.height, jp->spec.cfg.spp,
+                  (int)jp->pooled, (now() - t0s) * 1e3);
+      }
       if (!pump()) std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
   }
@@ -421,12 +555,28 @@ ar_status ar_server_create(const ar_scene* s, const ar_server_config* cfg, ar_se
   srv->scene = s;
   if (cfg) srv->cfg = *cfg;
   if (srv->cfg.hbm_budget_bytes == 0) srv->cfg.hbm_budget_bytes = 200ull << 30;
+  if (srv->cfg.chunk_paths == 0) srv->cfg.chunk_paths = (uint32_t)kServerChunkPaths;
+  if (srv->cfg.bdpt_chunk_samples == 0) srv->cfg.bdpt_chunk_samples = (uint32_t)kServerChunkBdpt;
   if (srv->cfg.max_active_jobs == 0) srv->cfg.max_active_jobs = 8;
   if (srv->cfg.max_active_jobs > 64) return fail(AR_ERR_INVALID_ARG, "max_active_jobs above 64");
   if (srv->cfg.max_concurrent_jobs == 0) srv->cfg.max_concurrent_jobs = std::min<uint32_t>(3, srv->cfg.max_active_jobs);
   if (srv->cfg.max_concurrent_jobs > kMaxLanes)
     return fail(AR_ERR_INVALID_ARG, "max_concurrent_jobs above 4 (the hardware queues of a process)");
   (void)hipGetDevice(&srv->device);
+  // The shared path-tracer queues take their device memory out of the
+  // budget; a budget too small for them leaves every job in chunks.
+  if (srv->cfg.pool_paths != AR_POOL_OFF) {
+    const uint32_t cap = srv->cfg.pool_paths ? srv->cfg.pool_paths : kPoolDefaultPaths;
+    if (cap > (1u << 26)) return fail(AR_ERR_INVALID_ARG, "pool_paths above 2^26");
+    const size_t fp = PathPool::footprint(cap);
+    if (fp < srv->cfg.hbm_budget_bytes / 2) {
+      const uint32_t fill = srv->cfg.pool_fill_paths ? srv->cfg.pool_fill_paths : cap / 8;
+      const ar_status e = PathPool::create(s, cap, fill, &srv->pool);
+      if (e != AR_OK) return e;
+      srv->cfg.hbm_budget_bytes -= fp;
+      srv->pool_min_paths = srv->cfg.pool_min_paths ? srv->cfg.pool_min_paths : (1u << 22);
+    }
+  }
   auto drop_streams = [&srv]() {
     for (auto& sp : srv->slots)
       if (sp->stream) (void)hipStreamDestroy(sp->stream);
@@ -496,6 +646,7 @@ ar_status ar_server_destroy(ar_server* srv) {
     srv->cv_done.notify_all();
     srv->cv_done.wait(lk, [srv]() { return srv->waiters == 0; });
   }
+  srv->pool.reset();  // drains its stream
   for (auto& ln : srv->lanes) {
     if (ln.stream) (void)hipStreamSynchronize(ln.stream);
     if (ln.stream) (void)hipStreamDestroy(ln.stream);
@@ -521,7 +672,12 @@ ar_status ar_job_submit(ar_server* srv, const ar_job_desc* desc, uint64_t* out_i
   j->spec.cfg = desc->config;
   ar_status e = validate_job(srv->scene, j->spec);
   if (e != AR_OK) return e;
-  j->footprint = job_footprint(srv->scene, j->spec);
+  // BDPT connection passes of at most a quarter chunk: the pieces the
+  // server issues between its scheduling decisions stay short
+  if (j->spec.cfg.integrator == AR_INTEGRATOR_BDPT)
+    j->spec.conn_pass = std::max<uint64_t>(64, srv->cfg.bdpt_chunk_samples / 4);
+  j->pooled = srv->poolable(j->spec);
+  j->footprint = j->pooled ? PathPool::job_bytes(j->spec) : job_footprint(srv->scene, j->spec);
   if (j->footprint > srv->cfg.hbm_budget_bytes) return fail(AR_ERR_OUT_OF_BUDGET, "job exceeds the HBM budget");
   {
     std::lock_guard<std::mutex> lk(srv->mu);

@@ -835,7 +835,10 @@ def bench_serving(args, ar, torch):
     log(f"serving: {len(spec_jobs)} jobs at {args.rate}/s specified; mean solo {mean_solo:.3f}s -> offered load "
         f"{rho_spec:.2f}; serving the first {len(jobs)} at {rate:.3f}/s (load {args.load})")
     srv = ar.RenderServer(sc, hbm_budget_bytes=200 << 30, max_active_jobs=args.slots,
-                          max_concurrent_jobs=args.lanes)
+                          max_concurrent_jobs=args.lanes, pool_paths=args.pool_paths,
+                          pool_min_paths=args.pool_min_paths, pool_fill_paths=args.pool_fill,
+                          chunk_paths=args.chunk_paths,
+                          bdpt_chunk_samples=args.bdpt_chunk)
     warm = [srv.submit(integ(j), cam(j), j.width, j.height) for j in jobs[: args.slots]]
     for w in warm:
         srv.wait(w)
@@ -849,12 +852,14 @@ def bench_serving(args, ar, torch):
         if now - last_log > 10.0:
             log(f"serving: {len(ids)}/{len(jobs)} jobs submitted at t={now:.1f}s")
             last_log = now
-    lat, waits, paths = [], [], 0
-    for j, jid in ids:
+    lat, waits, paths, detail = [], [], 0, []
+    for k, (j, jid) in enumerate(ids):
         _, st = srv.wait(jid)
         lat.append(st["latency"])
         waits.append(st["start"] - st["submit"])
         paths += st["paths"]
+        detail.append([int(j.index), int(jid), int(j.integrator), int(j.width), int(j.spp), round(solos[k] * 1e3, 2),
+                       round(st["submit"], 4), round(st["start"], 4), round(st["finish"], 4)])
     wall = time.perf_counter() - t0
     srv.close()
     lat = np.asarray(lat)
@@ -870,7 +875,8 @@ def bench_serving(args, ar, torch):
             f"{solos[k] * 1e3:.1f} ms, latency {lat[k] * 1e3:.1f} ms (admission wait {waits[k] * 1e3:.1f} ms, "
             f"{ratio[k]:.1f}x)")
     return {"jobs": len(jobs), "rate_per_s": round(rate, 3), "arrival_window_s": round(window, 2),
-            "slots": args.slots, "lanes": args.lanes, "specified_rate_per_s": args.rate, "specified_jobs": len(spec_jobs),
+            "slots": args.slots, "lanes": args.lanes, "pool_paths": args.pool_paths,
+            "pool_min_paths": args.pool_min_paths, "pool_fill_paths": args.pool_fill, "chunk_paths": args.chunk_paths, "bdpt_chunk_samples": args.bdpt_chunk, "specified_rate_per_s": args.rate, "specified_jobs": len(spec_jobs),
             "mean_solo_s": round(mean_solo, 4), "offered_load_at_specified_rate": round(rho_spec, 2),
             "offered_load": args.load,
             "job_mix": {"pt": sum(j.integrator == 0 for j in jobs), "bdpt": sum(j.integrator == 1 for j in jobs),
@@ -882,7 +888,8 @@ def bench_serving(args, ar, torch):
             "p99_solo_s": round(float(np.percentile(solos, 99)), 4),
             "p99_latency_over_p99_solo": round(float(np.percentile(lat, 99) / np.percentile(solos, 99)), 3),
             "worst_jobs_by_ratio": worst,
-            "mpaths_s": round(paths / wall / 1e6, 2), "wall_s": round(wall, 2)}
+            "mpaths_s": round(paths / wall / 1e6, 2), "wall_s": round(wall, 2),
+            **({"jobs_detail": detail} if getattr(args, "detail", False) else {})}
 
 
 def main():
@@ -913,6 +920,18 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=60, help="serving: jobs of the specified mix actually served")
     ap.add_argument("--slots", type=int, default=8)
     ap.add_argument("--lanes", type=int, default=0, help="serving: jobs on the device at once (0 = the server's default)")
+    ap.add_argument("--pool-paths", type=lambda v: int(v, 0), default=0,
+                    help="serving: paths in flight in the shared path-tracer queues (0 = default, 0xFFFFFFFF = off)")
+    ap.add_argument("--detail", action="store_true",
+                    help="serving: per-job [index, server id, integrator, side, spp, solo ms, submit, start, finish]")
+    ap.add_argument("--pool-fill", type=lambda v: int(v, 0), default=0,
+                    help="serving: live paths of the other pooled jobs while one drains (0 = default)")
+    ap.add_argument("--chunk-paths", type=lambda v: int(v, 0), default=0,
+                    help="serving: PT / GDPT paths per chunk of a job outside the pool (0 = default)")
+    ap.add_argument("--bdpt-chunk", type=lambda v: int(v, 0), default=0,
+                    help="serving: BDPT samples per chunk (0 = default)")
+    ap.add_argument("--pool-min-paths", type=lambda v: int(v, 0), default=0,
+                    help="serving: PT / GDPT jobs of at least this many paths share the queues (0 = default)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; sharding over several GPUs is designed "

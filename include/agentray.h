@@ -346,7 +346,16 @@ typedef struct ar_server_config {
   uint32_t max_active_jobs;  /* job slots rendering concurrently */
   uint32_t max_concurrent_jobs; /* jobs whose chunks are on the device at once, each on its own
                                   stream (1..4; 0 = min(3, max_active_jobs)) */
+  uint32_t pool_paths;       /* paths in flight in the shared path-tracer queues (0 = 2^22;
+                                AR_POOL_OFF: every job renders in chunks of its own) */
+  uint32_t pool_min_paths;   /* PT / GDPT wavefront jobs of at least this many paths share the
+                                pool; smaller ones run alone as one chunk (0 = 2^22) */
+  uint32_t pool_fill_paths;  /* while a pooled job drains its last paths, the other pooled jobs
+                                keep at most this many alive (0 = pool_paths / 8) */
+  uint32_t chunk_paths;      /* PT / GDPT paths per chunk of a job outside the pool (0 = 2^23) */
+  uint32_t bdpt_chunk_samples; /* BDPT samples per chunk (0 = 2^21); connection passes of a quarter */
 } ar_server_config;
+#define AR_POOL_OFF 0xFFFFFFFFu
 typedef struct ar_job_desc {
   uint32_t width, height;
   ar_camera camera;

@@ -1,7 +1,12 @@
 """Render server (ar_server_*, continuous batching over a resident scene).
 
 * every job's film in a concurrent mix of PT / BDPT / GDPT jobs of different
-  sizes equals, bit for bit, the same job rendered alone with ar_render;
+  sizes equals, bit for bit, the same job rendered alone with ar_render:
+  with every PT / GDPT job in the shared pool (paths of all jobs in one set
+  of queues, refilled every bounce), with the server's default split, and
+  with the pool off (every job in chunks of its own);
+* a pooled job cancelled while its paths are in flight drains and ends as
+  cancelled while the other pooled jobs still equal their solo renders;
 * a sample of the specified serving mix (workloads.serving_jobs: square
   sides up to 1024, spp log-uniform up to 256, PT / BDPT / GDPT) on the full
   room scene, submitted at Poisson arrivals: every job completes and equals
@@ -37,11 +42,17 @@ def _integ(ar, kind, spp, seed):
     return ar.GradientDomainPathTracer(ar.GdptConfig(spp=spp, max_depth=8, poisson_iterations=30), seed=seed)
 
 
-def test_mix_equals_alone(ar, wl, room):
+POOL_MODES = {"all_pooled": dict(pool_min_paths=1), "default": {}, "pool_off": dict(pool_paths=0xFFFFFFFF),
+              "small_pool": dict(pool_paths=4096, pool_min_paths=1)}
+
+
+@pytest.mark.parametrize("mode", sorted(POOL_MODES))
+def test_mix_equals_alone(ar, wl, room, mode):
     desc, sc = room
     specs = [(0, 64, 48, 8, 1), (1, 32, 32, 4, 2), (2, 48, 40, 4, 3), (0, 96, 64, 16, 4), (1, 40, 24, 2, 5),
              (2, 16, 16, 8, 6), (0, 128, 128, 4, 7), (0, 8, 8, 1, 8)]
-    srv = ar.RenderServer(sc, max_active_jobs=4)
+    print(f"  mode {mode}", flush=True)
+    srv = ar.RenderServer(sc, max_active_jobs=4, **POOL_MODES[mode])
     ids = []
     for kind, w, h, spp, seed in specs:
         ids.append(srv.submit(_integ(ar, kind, spp, seed), _cam(ar, wl, desc, w, h), w, h))
@@ -65,6 +76,30 @@ def test_short_job_overtakes_long_job(ar, wl, room):
     srv.close()
     assert s_short["start"] < s_long["finish"]
     assert s_short["finish"] < s_long["finish"]
+
+
+def test_pooled_cancel_drains(ar, wl, room):
+    desc, sc = room
+    srv = ar.RenderServer(sc, max_active_jobs=4, pool_min_paths=1)
+    keep = [(0, 96, 96, 64, 11), (2, 64, 64, 16, 12)]
+    a = srv.submit(_integ(ar, 0, 1024, 10), _cam(ar, wl, desc, 256, 256), 256, 256)
+    ids = [srv.submit(_integ(ar, k, spp, seed), _cam(ar, wl, desc, w, h), w, h) for k, w, h, spp, seed in keep]
+    time.sleep(0.05)
+    srv.cancel(a)
+    with pytest.raises(ar.ArError) as e:
+        srv.wait(a)
+    assert e.value.code == -7
+    for (k, w, h, spp, seed), i in zip(keep, ids):
+        film, st = srv.wait(i)
+        alone = _integ(ar, k, spp, seed).render(sc, _cam(ar, wl, desc, w, h), w, h).pixels
+        assert np.array_equal(film.pixels.view(np.uint32), alone.view(np.uint32)), (k, w, h)
+        assert st["paths"] == w * h * spp * (3 if k == 2 else 1)
+    # the server keeps serving after the cancellation
+    c = srv.submit(_integ(ar, 0, 8, 13), _cam(ar, wl, desc, 32, 32), 32, 32)
+    film, _ = srv.wait(c)
+    alone = _integ(ar, 0, 8, 13).render(sc, _cam(ar, wl, desc, 32, 32), 32, 32).pixels
+    assert np.array_equal(film.pixels.view(np.uint32), alone.view(np.uint32))
+    srv.close()
 
 
 def test_cancel_and_budget(ar, wl, room):
