@@ -55,12 +55,12 @@ constexpr uint64_t kServerChunkPaths = 1ull << 23;   // PT / GDPT path-equivalen
 constexpr uint64_t kServerChunkBdpt = 1ull << 21;    // BDPT samples per chunk (default)
 constexpr double kStarveMax = 1.0;                   // s a job waits for higher-priority work at most
 
-// Relative device cost of one pixel-sample of each integrator, measured on
-// the room at 1280x720 (PT path 1; GDPT sample = base + two shifted paths;
-// BDPT sample = both subpaths and all connections), used to estimate device
-// time for scheduling.
+// Relative device cost of one pixel-sample of each integrator on the room
+// (PT path 1 at 223 Mpaths/s; GDPT sample = base + two shifted paths at 215
+// Mpaths/s; BDPT sample = both subpaths and all connections at 37 Msamples/s),
+// used to estimate device time for scheduling.
 static double path_weight(uint32_t integrator) {
-  return integrator == AR_INTEGRATOR_BDPT ? 5.0 : (integrator == AR_INTEGRATOR_GDPT ? 2.8 : 1.0);
+  return integrator == AR_INTEGRATOR_BDPT ? 6.0 : (integrator == AR_INTEGRATOR_GDPT ? 3.1 : 1.0);
 }
 
 struct Slot {
