@@ -101,6 +101,12 @@ struct PoolDev {
   uint32_t cap;
 };
 
+// Pointers loaded from memory (window buffers, shadow targets) are generic;
+// casting them to the global address space keeps their accesses global
+// loads and stores instead of flat ones, which inside the traversal loop
+// would also wait on the LDS counter of the stack.
+__device__ __forceinline__ AR_GLOBAL_AS float4* as_global(float4* p) { return (AR_GLOBAL_AS float4*)p; }
+
 __device__ __forceinline__ void pool_load(const PoolState& S, uint32_t i, PathState& s, uint32_t& job, uint32_t& p) {
   float4 o = S.ray_o[i], d = S.ray_d[i], b = S.beta[i], l = S.L[i], pp = S.prev_p[i];
   uint4 m = S.meta[i];
@@ -252,7 +258,7 @@ __global__ __launch_bounds__(BLOCK) void k_pool_generate(PoolDev D, int cur) {
         Rng rng;
         live = gen_path(P, p, s, rng) && P.max_depth > 0;
         if (!live) {
-          J->Lout[ring][p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          as_global(J->Lout[ring])[p] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
           atomicAdd(&dead[j * 2 + ring], 1u);
         }
       }
@@ -320,7 +326,7 @@ __global__ __launch_bounds__(BLOCK) void k_pool_shade(ShadeView v, PoolDev D, in
       rng.init(J->P.seed, s.pixel, s.sample, 0, s.dim);
       cont = pt_shade(v, P, s, h, rng, sh);
       s.dim = rng.dim;
-      if (!cont) J->Lout[jr >> 16][p] = make_float4(s.L.x, s.L.y, s.L.z, 0.0f);
+      if (!cont) as_global(J->Lout[jr >> 16])[p] = make_float4(s.L.x, s.L.y, s.L.z, 0.0f);
     }
     const uint32_t j = jr & 0xffffu;
     wave_keyed_add(rays, active, j, sh.valid ? 2u : 1u);
@@ -360,7 +366,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_pool_shadow(Scene
       [&](uint64_t i) { return make_ray4(D.sh_o[i], D.sh_d[i]); },
       [&](uint64_t i, const Walker<true, true>& w) {
         if (w.bpos == kNoHit) {
-          float4* t = D.sh_tgt[i];
+          AR_GLOBAL_AS float4* t = as_global(D.sh_tgt[i]);
           const float4 c = D.sh_c[i], l = *t;
           *t = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
         }

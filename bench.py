@@ -892,13 +892,57 @@ def bench_serving(args, ar, torch):
             **({"jobs_detail": detail} if getattr(args, "detail", False) else {})}
 
 
+def bench_server_alone(args, ar, torch):
+    """Each job shape rendered alone with ar_render and submitted alone to a
+    server (default configuration: PT / GDPT jobs of at least 2^22 paths in
+    the pool, the rest in chunks), best of three each: what serving costs a
+    job when nothing else runs."""
+    wl = ar.workloads
+    desc = wl.procedural_room()
+    sc = ar.Scene.from_desc(desc)
+    cd = desc.camera
+
+    def integ(kind, spp):
+        if kind == 0:
+            return ar.PathTracer(ar.PathTracerConfig(spp=spp, max_depth=16, rr_depth=3), seed=1)
+        if kind == 1:
+            return ar.BidirectionalPathTracer(ar.BdptConfig(spp=spp, max_depth=10, rr_depth=3), seed=1)
+        return ar.GradientDomainPathTracer(ar.GdptConfig(spp=spp, max_depth=16, poisson_iterations=50), seed=1)
+
+    def cam(side):
+        return ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, 1.0))
+    shapes = [(0, 384, 32), (0, 960, 22), (0, 1024, 64), (2, 704, 32), (1, 704, 16), (1, 128, 132)]
+    integ(0, 4).render(sc, cam(256), 256, 256)
+    srv = ar.RenderServer(sc, hbm_budget_bytes=200 << 30, max_active_jobs=4, pool_paths=args.pool_paths,
+                          pool_min_paths=args.pool_min_paths, bdpt_chunk_samples=args.bdpt_chunk)
+    srv.wait(srv.submit(integ(0, 4), cam(256), 256, 256))
+    out = []
+    for kind, side, spp in shapes:
+        solo, served = [], []
+        for _ in range(3):
+            if not args.served_only:
+                it = integ(kind, spp)
+                it.render(sc, cam(side), side, side)
+                solo.append(it.last_stats.seconds)
+            _, st = srv.wait(srv.submit(integ(kind, spp), cam(side), side, side))
+            served.append(st["latency"])
+        row = {"integrator": ["pt", "bdpt", "gdpt"][kind], "side": side, "spp": spp,
+               "served_ms": round(min(served) * 1e3, 2)}
+        if solo:
+            row.update(solo_ms=round(min(solo) * 1e3, 2), ratio=round(min(served) / min(solo), 3))
+        log(f"server alone: {row}")
+        out.append(row)
+    srv.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "pt", "render", "copy", "pmc-probe",
-                                                          "builders"],
+    ap.add_argument("--workload", default="traversal", choices=["traversal", "serving", "server-alone", "pt", "render",
+                                                          "copy", "pmc-probe", "builders"],
                     help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--trace-config", default=None,
@@ -922,6 +966,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=0, help="serving: jobs on the device at once (0 = the server's default)")
     ap.add_argument("--pool-paths", type=lambda v: int(v, 0), default=0,
                     help="serving: paths in flight in the shared path-tracer queues (0 = default, 0xFFFFFFFF = off)")
+    ap.add_argument("--served-only", action="store_true", help="server-alone: skip the ar_render renders")
     ap.add_argument("--detail", action="store_true",
                     help="serving: per-job [index, server id, integrator, side, spp, solo ms, submit, start, finish]")
     ap.add_argument("--pool-fill", type=lambda v: int(v, 0), default=0,
@@ -960,6 +1005,9 @@ def main():
         return
     if args.workload == "pmc-probe":
         pmc_probe(args, ar, torch)
+        return
+    if args.workload == "server-alone":
+        print(json.dumps({"server_alone": bench_server_alone(args, ar, torch)}), flush=True)
         return
     if args.workload == "serving":
         srv = bench_serving(args, ar, torch)
