@@ -78,6 +78,26 @@ def test_short_job_overtakes_long_job(ar, wl, room):
     assert s_short["finish"] < s_long["finish"]
 
 
+def test_pooled_jobs_over_several_windows_equal_alone(ar, wl, room):
+    """Pooled jobs larger than one accumulation window (2^24 paths): their
+    windows reuse the two ring slots, each accumulated in sample order.
+    PT 1024^2 x 40 spp = 2.5 windows; GDPT 512^2 x 50 spp x 3 variants =
+    2.4 windows (21 spp per window), with a small PT job arriving between."""
+    desc, sc = room
+    srv = ar.RenderServer(sc, max_active_jobs=4, pool_min_paths=1)
+    specs = [(0, 1024, 40, 21), (2, 512, 50, 22), (0, 64, 3, 23)]
+    ids = []
+    for kind, side, spp, seed in specs:
+        ids.append(srv.submit(_integ(ar, kind, spp, seed), _cam(ar, wl, desc, side, side), side, side))
+        time.sleep(0.01)
+    served = [srv.wait(i) for i in ids]
+    srv.close()
+    for (kind, side, spp, seed), (film, st) in zip(specs, served):
+        alone = _integ(ar, kind, spp, seed).render(sc, _cam(ar, wl, desc, side, side), side, side).pixels
+        assert np.array_equal(film.pixels.view(np.uint32), alone.view(np.uint32)), (kind, side, spp)
+        assert st["paths"] == side * side * spp * (3 if kind == 2 else 1)
+
+
 def test_pooled_cancel_drains(ar, wl, room):
     desc, sc = room
     srv = ar.RenderServer(sc, max_active_jobs=4, pool_min_paths=1)
