@@ -408,17 +408,14 @@ struct HJob {
   uint32_t win_samples(uint32_t w) const { return std::min<uint32_t>(k, spec.cfg.spp - w * k); }
   uint32_t win_paths(uint32_t w) const { return variants * npix * win_samples(w); }
   uint64_t total_paths() const { return (uint64_t)variants * npix * spec.cfg.spp; }
-  uint64_t generated() const {
-    uint64_t g = 0;
-    for (uint32_t w = 0; w < gen_win && w < nwin; ++w) g += win_paths(w);
-    return g + cursor;
-  }
+  // windows before the last are full (k samples each)
+  uint64_t full_paths(uint32_t windows) const { return (uint64_t)windows * variants * npix * k; }
+  uint64_t generated() const { return full_paths(std::min(gen_win, nwin)) + cursor; }
   bool generating() const { return !cancel && state == 0 && generated() < total_paths(); }
   // every path generated, some still in flight
   bool draining() const { return !cancel && state == 0 && generated() >= total_paths(); }
   uint64_t retired_total() const {
-    uint64_t r = 0;
-    for (uint32_t w = 0; w < acc_win; ++w) r += win_paths(w);
+    uint64_t r = full_paths(std::min(acc_win, nwin - 1)) + (acc_win == nwin ? win_paths(nwin - 1) : 0);
     for (uint32_t w = acc_win; w < set_win; ++w)
       if (win[w & 1] == w) r += retired[w & 1];
     return r;
