@@ -11,7 +11,7 @@
 // remaining work.  Smaller ones, BDPT jobs and megakernel jobs render in
 // chunks on up to `max_concurrent_jobs` lanes (one HIP stream each); a BDPT
 // chunk is issued in pieces (camera subpaths, light subpaths, connection
-// passes of a quarter chunk), each about 10 ms of device time at the
+// passes of a quarter chunk), each about 20 ms of device time at the
 // default chunk, and a job of at most two chunks' work runs as one chunk.
 //
 // Scheduling is shortest remaining processing time first over all jobs:
@@ -52,7 +52,7 @@ namespace ar {
 
 constexpr uint32_t kMaxLanes = 4;            // hardware queues a process gets by default
 constexpr uint64_t kServerChunkPaths = 1ull << 23;   // PT / GDPT path-equivalents per chunk (default)
-constexpr uint64_t kServerChunkBdpt = 1ull << 21;    // BDPT samples per chunk (default)
+constexpr uint64_t kServerChunkBdpt = 1ull << 22;    // BDPT samples per chunk (default)
 constexpr double kStarveMax = 1.0;                   // s a job waits for higher-priority work at most
 
 // Relative device cost of one pixel-sample of each integrator on the room

@@ -353,7 +353,7 @@ typedef struct ar_server_config {
   uint32_t pool_fill_paths;  /* while a pooled job drains its last paths, the other pooled jobs
                                 keep at most this many alive (0 = pool_paths / 8) */
   uint32_t chunk_paths;      /* PT / GDPT paths per chunk of a job outside the pool (0 = 2^23) */
-  uint32_t bdpt_chunk_samples; /* BDPT samples per chunk (0 = 2^21); connection passes of a quarter */
+  uint32_t bdpt_chunk_samples; /* BDPT samples per chunk (0 = 2^22); connection passes of a quarter */
 } ar_server_config;
 #define AR_POOL_OFF 0xFFFFFFFFu
 typedef struct ar_job_desc {
