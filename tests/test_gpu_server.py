@@ -78,6 +78,26 @@ def test_short_job_overtakes_long_job(ar, wl, room):
     assert s_short["finish"] < s_long["finish"]
 
 
+def test_pool_serves_least_remaining_work_first(ar, wl, room):
+    """Inside the pool the job with the fewest paths left to generate takes
+    the freed places first: a 2^23-path job submitted while a 2^26-path job
+    renders finishes first, and both equal their solo renders."""
+    desc, sc = room
+    srv = ar.RenderServer(sc, max_active_jobs=4, pool_min_paths=1)
+    big = (0, 1024, 64, 31)
+    small = (0, 512, 32, 32)
+    ids = []
+    for kind, side, spp, seed in (big, small):
+        ids.append(srv.submit(_integ(ar, kind, spp, seed), _cam(ar, wl, desc, side, side), side, side))
+        time.sleep(0.03)
+    (fb, sb), (fs, ss) = [srv.wait(i) for i in ids]
+    srv.close()
+    assert ss["finish"] < sb["finish"]
+    for (kind, side, spp, seed), film in ((big, fb), (small, fs)):
+        alone = _integ(ar, kind, spp, seed).render(sc, _cam(ar, wl, desc, side, side), side, side).pixels
+        assert np.array_equal(film.pixels.view(np.uint32), alone.view(np.uint32)), (side, spp)
+
+
 def test_pooled_jobs_over_several_windows_equal_alone(ar, wl, room):
     """Pooled jobs larger than one accumulation window (2^24 paths): their
     windows reuse the two ring slots, each accumulated in sample order.
