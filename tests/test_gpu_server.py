@@ -7,6 +7,10 @@
   with the pool off (every job in chunks of its own);
 * a pooled job cancelled while its paths are in flight drains and ends as
   cancelled while the other pooled jobs still equal their solo renders;
+  under churn (random submissions and cancellations into a small pool)
+  every completed job still equals its solo render;
+* the pool serves the job with the least remaining work first, and jobs
+  spanning several accumulation windows equal their solo renders;
 * a sample of the specified serving mix (workloads.serving_jobs: square
   sides up to 1024, spp log-uniform up to 256, PT / BDPT / GDPT) on the full
   room scene, submitted at Poisson arrivals: every job completes and equals
@@ -140,6 +144,49 @@ def test_pooled_cancel_drains(ar, wl, room):
     alone = _integ(ar, 0, 8, 13).render(sc, _cam(ar, wl, desc, 32, 32), 32, 32).pixels
     assert np.array_equal(film.pixels.view(np.uint32), alone.view(np.uint32))
     srv.close()
+
+
+def test_churn_with_cancellations(ar, wl, room):
+    """24 small jobs of all three integrators into a 2^16-place pool (every
+    PT / GDPT job pooled), submitted at random moments, a third of them
+    cancelled at random moments: every job ends (done or cancelled), every
+    completed film equals its solo render bit for bit, and the server keeps
+    serving afterwards."""
+    desc, sc = room
+    rng = np.random.default_rng(5)
+    srv = ar.RenderServer(sc, max_active_jobs=6, pool_paths=1 << 16, pool_min_paths=1)
+    specs, ids, cancelled = [], [], set()
+    for k in range(24):
+        kind = int(rng.integers(0, 3))
+        w, h = int(rng.integers(8, 97)), int(rng.integers(8, 97))
+        spp = int(rng.integers(1, 9))
+        specs.append((kind, w, h, spp, 100 + k))
+        ids.append(srv.submit(_integ(ar, kind, spp, 100 + k), _cam(ar, wl, desc, w, h), w, h))
+        if rng.random() < 0.35:
+            victim = int(rng.integers(0, len(ids)))
+            srv.cancel(ids[victim])
+            cancelled.add(victim)
+        time.sleep(float(rng.exponential(0.004)))
+    results = []
+    for k, jid in enumerate(ids):
+        try:
+            results.append(srv.wait(jid))
+        except ar.ArError as e:
+            assert k in cancelled and e.code == -7, (k, e.code)
+            results.append(None)
+    c = srv.submit(_integ(ar, 0, 4, 7), _cam(ar, wl, desc, 32, 32), 32, 32)
+    film_c, _ = srv.wait(c)
+    srv.close()
+    done = 0
+    for (kind, w, h, spp, seed), res in zip(specs, results):
+        if res is None:
+            continue
+        alone = _integ(ar, kind, spp, seed).render(sc, _cam(ar, wl, desc, w, h), w, h).pixels
+        assert np.array_equal(res[0].pixels.view(np.uint32), alone.view(np.uint32)), (kind, w, h, spp)
+        done += 1
+    assert done >= 12
+    alone = _integ(ar, 0, 4, 7).render(sc, _cam(ar, wl, desc, 32, 32), 32, 32).pixels
+    assert np.array_equal(film_c.pixels.view(np.uint32), alone.view(np.uint32))
 
 
 def test_cancel_and_budget(ar, wl, room):
