@@ -97,19 +97,20 @@ class ArJobDesc(C.Structure):
 class ArTraceConfig(C.Structure):
     _fields_ = [("workgroup_size", C.c_uint32), ("lds_stack", C.c_uint32), ("blocks_per_cu", C.c_uint32),
                 ("refill_min", C.c_uint32), ("search_min", C.c_uint32), ("park_slots", C.c_uint32),
-                ("ms", C.c_float)]
+                ("balance", C.c_uint32), ("ms", C.c_float)]
 
     def as_dict(self) -> dict:
         return {"workgroup_size": self.workgroup_size, "lds_stack": self.lds_stack,
                 "blocks_per_cu": self.blocks_per_cu, "refill_min": self.refill_min, "search_min": self.search_min,
-                "park_slots": self.park_slots, "ms": self.ms}
+                "park_slots": self.park_slots, "balance": self.balance, "ms": self.ms}
 
 
 def config_key(c: dict) -> tuple:
     """A launch configuration dict as the argument tuple of
-    Scene.set_trace_config / set_anyhit_config."""
+    Scene.set_trace_config / set_anyhit_config: (workgroup, LDS stack, blocks
+    per CU, refill threshold, search quorum, park slots, balance)."""
     return (c["workgroup_size"], c["lds_stack"], c["blocks_per_cu"], c["refill_min"], c["search_min"],
-            c["park_slots"])
+            c["park_slots"], c["balance"])
 
 
 class ArTraceCounts(C.Structure):
@@ -453,15 +454,21 @@ class Scene:
         return c.as_dict()
 
     def set_trace_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16,
-                         search_min: int = 8, park_slots: int = 3):
-        """Launch configuration of closest-hit and any-hit traversal."""
-        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, park_slots, 0.0)
+                         search_min: int = 8, park_slots: int = 3, balance: int = None):
+        """Launch configuration of closest-hit and any-hit traversal.  balance
+        None keeps the closest-hit launch's current setting."""
+        if balance is None:
+            balance = self.trace_config()["balance"]
+        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, park_slots, balance, 0.0)
         _check(_lib.ar_scene_set_trace_config(self._h, C.byref(c)))
 
     def set_anyhit_config(self, workgroup_size: int, lds_stack: int, blocks_per_cu: int, refill_min: int = 16,
-                          search_min: int = 8, park_slots: int = 3):
-        """Launch configuration of any-hit traversal only (occlusion, shadow rays)."""
-        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, park_slots, 0.0)
+                          search_min: int = 8, park_slots: int = 3, balance: int = None):
+        """Launch configuration of any-hit traversal only (occlusion, shadow
+        rays).  balance None keeps the any-hit launch's current setting."""
+        if balance is None:
+            balance = self.anyhit_config()["balance"]
+        c = ArTraceConfig(workgroup_size, lds_stack, blocks_per_cu, refill_min, search_min, park_slots, balance, 0.0)
         _check(_lib.ar_scene_set_anyhit_config(self._h, C.byref(c)))
 
     def anyhit_config(self) -> dict:
@@ -682,8 +689,9 @@ class GdptConfig:
 
 # Launch candidates the on-the-job refinements try around the ray-sample
 # autotuner's pick: (workgroup, LDS stack, blocks per CU, refill threshold,
-# search quorum, park slots).  The winner is then tried with the other park
-# slot count (3 <-> 4).
+# search quorum, park slots); each is tried with the stage's current balance
+# setting.  The winner is then tried with the other park slot count (3 <-> 4)
+# and with balance switched.
 TUNE_CANDIDATES = [(256, 12, 8, 8, 16, 3), (256, 12, 8, 8, 8, 4), (256, 12, 8, 8, 32, 4), (256, 12, 8, 8, 32, 3),
                    (256, 8, 8, 8, 32, 4), (128, 16, 8, 8, 32, 3),
                    (256, 8, 8, 8, 16, 3), (256, 8, 8, 8, 8, 3), (256, 8, 8, 8, 16, 4), (256, 8, 8, 8, 8, 4),
@@ -693,7 +701,17 @@ TUNE_CANDIDATES = [(256, 12, 8, 8, 16, 3), (256, 12, 8, 8, 8, 4), (256, 12, 8, 8
 
 def other_park_slots(c: tuple) -> tuple:
     """The launch configuration tuple `c` with the other park slot count."""
-    return tuple(c[:5]) + (7 - c[5],)
+    return tuple(c[:5]) + (7 - c[5],) + tuple(c[6:])
+
+
+def other_balance(c: tuple) -> tuple:
+    """The launch configuration tuple `c` (7 entries) with balance switched."""
+    return tuple(c[:6]) + (1 - c[6],)
+
+
+def with_balance(c: tuple, balance: int) -> tuple:
+    """A 6- or 7-entry launch configuration tuple with the given balance."""
+    return tuple(c[:6]) + (balance,)
 
 
 class _Integrator:
@@ -740,12 +758,13 @@ class _Integrator:
         """Refines the scene's launch configurations on the job itself: times
         a render of `spp` samples per pixel (one chunk of the real workload,
         wavefront schedule) for each candidate (workgroup, LDS stack, blocks
-        per CU, refill threshold, search quorum, park slots), the scene's current one
-        first; first for the closest-hit launch (extension stage), then, with
-        the winner, for the any-hit launch (shadow stage); each stage's
-        winner is then timed with the other park slot count.  Keeps the
-        fastest of each and returns them with the timings.  Results do not depend on
-        the launch configuration, so this only changes speed."""
+        per CU, refill threshold, search quorum, park slots, with the stage's
+        current balance setting), the scene's current one first; first for the
+        closest-hit launch (extension stage), then, with the winner, for the
+        any-hit launch (shadow stage); each stage's winner is then timed with
+        the other park slot count and with balance switched.  Keeps the
+        fastest of each and returns them with the timings.  Results do not
+        depend on the launch configuration, so this only changes speed."""
         if candidates is None:
             candidates = TUNE_CANDIDATES
         best_c, best_a = config_key(scene.trace_config()), config_key(scene.anyhit_config())
@@ -762,14 +781,15 @@ class _Integrator:
         try:
             for stage in ("closest", "anyhit"):
                 base = best_c if stage == "closest" else best_a
-                cands = [base] + [tuple(c) for c in candidates if tuple(c) != base]
+                cands = [base] + [with_balance(c, base[6]) for c in candidates if with_balance(c, base[6]) != base]
                 times = [timed(c, best_a) if stage == "closest" else timed(best_c, c) for c in cands]
                 win = cands[int(np.argmin(times))]
-                flip = other_park_slots(win)
-                if flip not in cands:
-                    cands.append(flip)
-                    times.append(timed(flip, best_a) if stage == "closest" else timed(best_c, flip))
-                    win = cands[int(np.argmin(times))]
+                for other in (other_park_slots, other_balance):
+                    flip = other(win)
+                    if flip not in cands:
+                        cands.append(flip)
+                        times.append(timed(flip, best_a) if stage == "closest" else timed(best_c, flip))
+                        win = cands[int(np.argmin(times))]
                 tried[stage] = {"x".join(map(str, c)): round(t * 1e3, 2) for c, t in zip(cands, times)}
                 if stage == "closest":
                     best_c = win

@@ -156,9 +156,11 @@ struct ar_scene {
   uint32_t stack_depth = 0;  // traversal stack entries a query can need (either tree form)
   // Launch configurations until the scene is tuned: the 12-entry LDS stack
   // with refills of 8 lanes renders the room's PT, BDPT and GDPT jobs 12-13%
-  // faster than the 8-entry stack with refills of 24.
-  ar_trace_config tcfg{256, 12, 8, 8, 8, 3, 0.0f};      // closest-hit launch configuration (autotuned)
-  ar_trace_config tcfg_any{256, 12, 8, 8, 8, 3, 0.0f};  // any-hit (occlusion, shadow rays), tuned separately
+  // faster than the 8-entry stack with refills of 24.  `balance` is set at
+  // commit from the triangle count unless a configuration was set before.
+  ar_trace_config tcfg{256, 12, 8, 8, 8, 3, 1, 0.0f};      // closest-hit launch configuration (autotuned)
+  ar_trace_config tcfg_any{256, 12, 8, 8, 8, 3, 1, 0.0f};  // any-hit (occlusion, shadow rays), tuned separately
+  bool tcfg_set = false;  // a configuration was set or tuned
   ar::DevBuf d_nodes, d_qnodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
       d_tri_obj, d_stri, d_suv, d_tex, d_texinfo;
   ar::SceneView view{};

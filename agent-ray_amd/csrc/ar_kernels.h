@@ -32,9 +32,19 @@ bool trace_variant_supported(uint32_t block, uint32_t stack);
 // rays under c; 0 for an unsupported variant.
 uint32_t trace_launch_grid(uint64_t n, const ar_trace_config& c, bool any, bool near);
 
-// Scene-level scratch (traversal stack spill).  acquire() orders the caller's
-// stream after the previous user of the scratch (an event, no host wait) and
-// grows the buffer if needed; release() records the new last use.
+// Work counters of walker launches (walk_items): kWalkWorkBytes of device
+// memory that are zero when a launch starts and that its last wave sets back
+// to zero.  Launches that run one after another may share them.  Newly
+// allocated counters are cleared by clear_walk_work, a kernel on `st` (no
+// memset node if the stream is being captured).
+constexpr size_t kWalkWorkBytes = 256;
+hipError_t clear_walk_work(void* p, hipStream_t st);
+
+// Scene-level scratch: the walker counters of ar_trace / ar_occluded
+// (kWalkWorkBytes) followed by the traversal stack spill.  acquire() orders
+// the caller's stream after the previous user of the scratch (an event, no
+// host wait) and grows the buffer to `bytes` (the counters included) if
+// needed; release() records the new last use.
 ar_status scratch_acquire(ar_scene* s, size_t bytes, hipStream_t st, void** out);
 void scratch_release(ar_scene* s, hipStream_t st);
 

@@ -21,6 +21,8 @@ struct WaveBufs {
   float4* sh_c;       // contribution if unoccluded
   uint32_t* cnt;      // [0],[1] queue counts, [2] shadow count
   unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+  uint32_t* work;     // walker counters of the traversal launches (kWalkWorkBytes; walk_items), passed
+                      // to a launch whose configuration has balance set
 };
 
 // A chunk's launch sequence captured once in a HIP graph and replayed for
@@ -70,8 +72,9 @@ struct ShadeView;
 uint64_t view_hash(const ShadeView& v);
 // Closest-hit and any-hit launch configurations packed into key words.
 inline uint64_t launch_key(const ar_trace_config& tc, const ar_trace_config& ta) {
-  return ((uint64_t)tc.workgroup_size << 48) | ((uint64_t)tc.lds_stack << 40) | ((uint64_t)tc.park_slots << 32) |
-         ((uint64_t)ta.workgroup_size << 16) | ((uint64_t)ta.lds_stack << 8) | ta.park_slots;
+  return ((uint64_t)tc.workgroup_size << 48) | ((uint64_t)tc.lds_stack << 40) | ((uint64_t)tc.balance << 38) |
+         ((uint64_t)tc.park_slots << 32) | ((uint64_t)ta.workgroup_size << 16) | ((uint64_t)ta.lds_stack << 8) |
+         ((uint64_t)ta.balance << 6) | ta.park_slots;
 }
 inline uint64_t launch_key_thresholds(const ar_trace_config& tc, const ar_trace_config& ta) {
   return ((uint64_t)tc.refill_min << 48) | ((uint64_t)tc.search_min << 32) | ((uint64_t)ta.refill_min << 16) |

@@ -622,43 +622,92 @@ __device__ __forceinline__ bool trace_scene_wave(const SceneView& sv, const Ray&
   return w.bpos != kNoHit;
 }
 
-// Items [0, n) are split in chunks of 64; wave `wave` of `waves` owns chunks
-// wave, wave + waves, ... and walks their items in order.  A lane that
-// finishes takes the next unassigned item of its wave; idle lanes are
-// refilled together once at least `refill_min` (1..64) of them wait, so
-// items fetched together are neighbours (coherent rays stay together).
-// Every item is started exactly once, and the loop ends when the wave's
-// items are exhausted and every lane is idle.  In its node phase a lane
-// parks up to SLOTS (3 or 4) triangle leaves.  n < 2^32.
+// Items [0, n) are split in chunks of 64, and each wave walks a sequence of
+// chunks.  Wave `wave` of `waves` starts with a static share, chunks wave,
+// wave + waves, ..., wave + (ns - 1) waves, where ns is half of the chunks
+// per wave.  The chunks from ns * waves on are then handed out one at a
+// time, in order, from a counter that all waves of the launch share: a wave
+// whose rays were cheap takes more of them, so the waves end together.  Ray
+// costs vary widely (misses next to deep rays), and a fixed stride can alias
+// with the image layout: with static shares alone, a closest-hit launch over
+// 2^24 rays had its waves end anywhere between 41% and 100% of its
+// duration.  Same-address atomics serialise at about 10 ns each, so the
+// counter is used only from 4 chunks per wave on; a smaller launch (few or
+// cheap rays: a 36-triangle scene at 2.7 chunks per wave ran 40% slower
+// with it) keeps every chunk static and touches no counter.
+//
+// `work` holds two counters that are zero when the launch starts: [0] the
+// chunks taken beyond the static shares, [1] the waves that have finished.
+// The last wave to finish sets both back to zero, so launches that follow
+// on the same stream, and replays of a captured graph, find them zero
+// again.  Launches that share a `work` must not overlap.  Every value taken
+// from [0] is used before the wave counts itself in [1], so each taking
+// atomic has been performed when the last wave clears the counters.  Every
+// wave of the grid must call walk_items exactly once with the same n,
+// waves and work.  With work == nullptr every chunk is static.
+//
+// A lane that finishes takes the next unassigned item of its wave; idle
+// lanes are refilled together once at least `refill_min` (1..64) of them
+// wait, so items fetched together are neighbours (coherent rays stay
+// together).  Every item is started exactly once, and the loop ends when the
+// wave's chunks are exhausted and every lane is idle.  In its node phase a
+// lane parks up to SLOTS (3 or 4) triangle leaves.  n < 2^32.
 //   src(i)      -> Ray for item i
 //   sink(i, w)  consumes the finished walker of item i
 template <bool ANY, bool NEAR, uint32_t SLOTS, class Src, class Sink>
 __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint64_t n, uint64_t wave,
-                                           uint64_t waves, uint32_t refill_min, uint32_t search_min, Src src,
-                                           Sink sink) {
+                                           uint64_t waves, uint32_t* work, uint32_t refill_min,
+                                           uint32_t search_min, Src src, Sink sink) {
   const uint32_t lane = __lane_id();
   const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-  uint32_t next = 0;  // wave-uniform: position of the next unassigned item
+  // Chunk ids and positions are wave-uniform and below 2^32 (n < 2^32):
+  // kept in scalar registers.
+  const uint32_t wv = __builtin_amdgcn_readfirstlane((uint32_t)wave);
+  const uint32_t nw = __builtin_amdgcn_readfirstlane((uint32_t)waves);
+  const uint32_t nchunks = (uint32_t)((n + 63) >> 6);
+  // Counters are used from 4 chunks per wave on; smaller launches are
+  // static (no atomics at all, the counters stay zero).
+  const bool dyn = work != nullptr && nchunks >= 4u * nw;
+  const uint32_t ns = dyn ? (nchunks / nw) >> 1 : 0xffffffffu;
+  // Chunk k of this wave's sequence (called in wave-uniform code).
+  auto chunk_at = [&](uint32_t k) -> uint32_t {
+    if (k < ns) return wv + k * nw;
+    const uint64_t act = __ballot(1);
+    const int l0 = __ffsll((unsigned long long)act) - 1;
+    uint32_t v = 0;
+    if ((int)lane == l0) v = atomicAdd(&work[0], 1u);
+    return ns * nw + __builtin_amdgcn_readfirstlane(__shfl(v, l0));
+  };
+  uint32_t ka = 0;            // sequence index of the current chunk
+  uint32_t ca = chunk_at(0);  // its chunk
+  uint32_t off = 0;           // next unassigned item within it
   uint32_t item = 0;
   bool alive = false;
   Walker<ANY, NEAR> w;
   for (;;) {
     uint64_t idle = __ballot(!alive);
     uint32_t nidle = (uint32_t)__popcll(idle);
-    uint64_t first = (wave + (next >> 6) * waves) * 64 + (next & 63);
-    bool more = first < n;
+    bool more = (uint64_t)ca * 64 + off < n;
     if (!more && nidle == 64) break;
     if (more && nidle >= refill_min) {
+      // A refill that reaches past the current chunk takes the next one now
+      // (its lanes use it at once), so no chunk is taken without being used.
+      const uint32_t cb = off + nidle > 64 ? chunk_at(ka + 1) : 0u;
       if (!alive) {
-        uint32_t p = next + (uint32_t)__popcll(idle & below);
-        uint64_t it = (wave + (p >> 6) * waves) * 64 + (p & 63);
+        const uint32_t p = off + (uint32_t)__popcll(idle & below);
+        const uint64_t it = (uint64_t)(p < 64 ? ca : cb) * 64 + (p & 63);
         if (it < n) {
           item = (uint32_t)it;
           alive = !w.start(sv, src(it), stk);
           if (!alive) sink(it, w);
         }
       }
-      next += nidle;
+      off += nidle;
+      if (off >= 64) {
+        off -= 64;
+        ++ka;
+        ca = off > 0 ? cb : chunk_at(ka);
+      }
     }
     if (alive) {
       w.template descend<SLOTS>(sv, stk, [search_min](bool, bool searching) {
@@ -668,6 +717,15 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
         sink(item, w);
         alive = false;
       }
+    }
+  }
+  if (dyn) {
+    // The wave is done; the last wave of the launch clears the counters.
+    const uint64_t act = __ballot(1);
+    const int l0 = __ffsll((unsigned long long)act) - 1;
+    if ((int)lane == l0 && atomicAdd(&work[1], 1u) == nw - 1u) {
+      atomicExch(&work[0], 0u);
+      atomicExch(&work[1], 0u);
     }
   }
 }

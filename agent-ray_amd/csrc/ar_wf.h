@@ -39,7 +39,8 @@ __device__ __forceinline__ void store_state(const WaveBufs& B, uint32_t i, const
 // lanes busy until its share of the queue is done.
 template <int BLOCK, int S, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneView sv, WaveBufs B, int qi, int2* __restrict__ spill,
-                                                     uint32_t refill_min, uint32_t search_min) {
+                                                     uint32_t* __restrict__ work, uint32_t refill_min,
+                                                     uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
@@ -47,7 +48,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneVi
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
   walk_items<false, true, SLOTS>(
-      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
+      sv, stk, n, gtid / 64, nthreads / 64, work, refill_min, search_min,
       [&](uint64_t i) {
         uint32_t p = q[i];
         return make_ray4(B.ray_o[p], B.ray_d[p]);
@@ -63,14 +64,15 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_extend(SceneVi
 // contribution to its path's radiance (one shadow ray per path per bounce).
 template <int BLOCK, int S, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_wf_shadow(SceneView sv, WaveBufs B, int2* __restrict__ spill,
-                                                     uint32_t refill_min, uint32_t search_min) {
+                                                     uint32_t* __restrict__ work, uint32_t refill_min,
+                                                     uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
   const uint32_t nthreads = gridDim.x * BLOCK;
   Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = B.cnt[2];
   walk_items<true, true, SLOTS>(
-      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
+      sv, stk, n, gtid / 64, nthreads / 64, work, refill_min, search_min,
       [&](uint64_t i) { return make_ray4(B.sh_o[i], B.sh_d[i]); },
       [&](uint64_t i, const Walker<true, true>& w) {
         if (w.bpos == kNoHit) {

@@ -475,6 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_light_gen(ShadeView v, const Bdp
 template <int BLOCK, int S, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_sub_extend(SceneView sv, WaveBufs B, int qi,
                                                                                int2* __restrict__ spill,
+                                                                               uint32_t* __restrict__ work,
                                                                                uint32_t refill_min,
                                                                                uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
@@ -484,7 +485,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_sub_extend(S
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
   walk_items<false, true, SLOTS>(
-      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
+      sv, stk, n, gtid / 64, nthreads / 64, work, refill_min, search_min,
       [&](uint64_t i) {
         const uint32_t p = q[i];
         return make_ray4(B.ray_o[p], B.ray_d[p]);
@@ -1002,6 +1003,7 @@ __global__ __launch_bounds__(BLOCK) void k_bdpt_conn_prep(ShadeView v, const Bdp
 template <int BLOCK, int S, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_conn_vis(SceneView sv, ConnBufs C,
                                                                              int2* __restrict__ spill,
+                                                                             uint32_t* __restrict__ work,
                                                                              uint32_t refill_min, uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
@@ -1009,7 +1011,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_bdpt_conn_vis(Sce
   Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = *C.qn;
   walk_items<true, true, SLOTS>(
-      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
+      sv, stk, n, gtid / 64, nthreads / 64, work, refill_min, search_min,
       [&](uint64_t i) {
         const uint32_t slot = C.queue[i];
         return make_ray4(C.o[slot], C.d[slot]);
@@ -1242,10 +1244,10 @@ class BdptJob : public JobRun {
   if (tc.workgroup_size == BB && tc.lds_stack == SS) {                                                       \
     if (tc.park_slots == 4)                                                                                  \
       hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 4>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi, spill,  \
-                         tc.refill_min, tc.search_min);                                                      \
+                         tc.balance ? B.work : nullptr, tc.refill_min, tc.search_min);                       \
     else                                                                                                     \
       hipLaunchKernelGGL((k_bdpt_sub_extend<BB, SS, 3>), dim3(Ge), dim3(BB), 0, stream, v.sv, B, qi, spill,  \
-                         tc.refill_min, tc.search_min);                                                      \
+                         tc.balance ? B.work : nullptr, tc.refill_min, tc.search_min);                       \
   }
         AR_TRACE_VARIANTS(AR_SUB_EXT)
 #undef AR_SUB_EXT
@@ -1292,10 +1294,10 @@ class BdptJob : public JobRun {
   if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                      \
     if (ta.park_slots == 4)                                                                                 \
       hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 4>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, spill,       \
-                         ta.refill_min, ta.search_min);                                                     \
+                         ta.balance ? B.work : nullptr, ta.refill_min, ta.search_min);                      \
     else                                                                                                    \
       hipLaunchKernelGGL((k_bdpt_conn_vis<BB, SS, 3>), dim3(Gs), dim3(BB), 0, stream, v.sv, C, spill,       \
-                         ta.refill_min, ta.search_min);                                                     \
+                         ta.balance ? B.work : nullptr, ta.refill_min, ta.search_min);                      \
   }
     AR_TRACE_VARIANTS(AR_VIS)
 #undef AR_VIS

@@ -95,6 +95,7 @@ struct PoolDev {
   uint32_t* cnt;                     // [0], [1] paths in S[0], S[1]; [2] shadow rays; [3] generated;
                                      // [4] paths alive when the iteration began
   unsigned long long* stats;         // [0] extension, [1] shadow rays (whole pool)
+  uint32_t* work;                    // walker counters of the extension and shadow launches (walk_items)
   PoolJob* jobs;
   PoolProg* prog;
   PoolPlan* plan;
@@ -159,6 +160,7 @@ __global__ void k_pool_init(PoolDev D) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < 5) D.cnt[t] = 0;
   if (t < 2) D.stats[t] = 0ull;
+  if (t < kWalkWorkBytes / sizeof(uint32_t)) D.work[t] = 0u;
   if (t < kPoolMaxJobs) {
     PoolProg p{};
     p.win[0] = p.win[1] = 0xffffffffu;
@@ -355,6 +357,7 @@ __global__ __launch_bounds__(BLOCK) void k_pool_shade(ShadeView v, PoolDev D, in
 template <int BLOCK, int S, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_pool_shadow(SceneView sv, PoolDev D,
                                                                            int2* __restrict__ spill,
+                                                                           uint32_t* __restrict__ work,
                                                                            uint32_t refill_min, uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t gtid = xcd_block() * BLOCK + threadIdx.x;
@@ -362,7 +365,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_pool_shadow(Scene
   Stack stk = make_stack<BLOCK, S>(lds, spill, gtid, nthreads);
   const uint32_t n = D.cnt[2];
   walk_items<true, true, SLOTS>(
-      sv, stk, n, gtid / 64, nthreads / 64, refill_min, search_min,
+      sv, stk, n, gtid / 64, nthreads / 64, work, refill_min, search_min,
       [&](uint64_t i) { return make_ray4(D.sh_o[i], D.sh_d[i]); },
       [&](uint64_t i, const Walker<true, true>& w) {
         if (w.bpos == kNoHit) {
@@ -486,6 +489,7 @@ class PoolImpl : public PathPool {
     add((void**)&D.sh_tgt, (size_t)c * sizeof(float4*));
     add((void**)&D.cnt, 64);
     add((void**)&D.stats, 64);
+    add((void**)&D.work, kWalkWorkBytes);
     add((void**)&D.jobs, sizeof(PoolJob) * kPoolMaxJobs);
     add((void**)&D.prog, sizeof(PoolProg) * kPoolMaxJobs);
     add((void**)&D.plan, sizeof(PoolPlan));
@@ -761,6 +765,7 @@ class PoolImpl : public PathPool {
     B.hit_inst = D.hit_inst;
     B.cnt = D.cnt;
     B.stats = D.stats;
+    B.work = D.work;
     int2* spill = spill_buf.as<int2>();
     hipLaunchKernelGGL(k_pool_begin, dim3(1), dim3(64), 0, st, D, plan, cur);
     hipLaunchKernelGGL((k_pool_generate<kPoolBlock>), dim3(Gp), dim3(kPoolBlock), 0, st, D, cur);
@@ -770,11 +775,11 @@ class PoolImpl : public PathPool {
       const void* kfn = tc.park_slots == 4 ? (const void*)&k_wf_extend<BB, SS, 4> : (const void*)&k_wf_extend<BB, SS, 3>;
       const uint32_t G = clamp_resident((uint32_t)(device_cus() * tc.blocks_per_cu / 8 * 8), kfn, BB);
       if (tc.park_slots == 4)
-        hipLaunchKernelGGL((k_wf_extend<BB, SS, 4>), dim3(G), dim3(BB), 0, st, view.sv, B, cur, spill, tc.refill_min,
-                           tc.search_min);
+        hipLaunchKernelGGL((k_wf_extend<BB, SS, 4>), dim3(G), dim3(BB), 0, st, view.sv, B, cur, spill,
+                           tc.balance ? D.work : nullptr, tc.refill_min, tc.search_min);
       else
-        hipLaunchKernelGGL((k_wf_extend<BB, SS, 3>), dim3(G), dim3(BB), 0, st, view.sv, B, cur, spill, tc.refill_min,
-                           tc.search_min);
+        hipLaunchKernelGGL((k_wf_extend<BB, SS, 3>), dim3(G), dim3(BB), 0, st, view.sv, B, cur, spill,
+                           tc.balance ? D.work : nullptr, tc.refill_min, tc.search_min);
     });
     hipLaunchKernelGGL((k_pool_shade<kPoolBlock>), dim3(Gp), dim3(kPoolBlock), 0, st, view, D, cur);
     const bool ok_s = pick_variant(ta, [&](auto bb, auto ss) {
@@ -783,11 +788,11 @@ class PoolImpl : public PathPool {
           ta.park_slots == 4 ? (const void*)&k_pool_shadow<BB, SS, 4> : (const void*)&k_pool_shadow<BB, SS, 3>;
       const uint32_t G = clamp_resident((uint32_t)(device_cus() * ta.blocks_per_cu / 8 * 8), kfn, BB);
       if (ta.park_slots == 4)
-        hipLaunchKernelGGL((k_pool_shadow<BB, SS, 4>), dim3(G), dim3(BB), 0, st, view.sv, D, spill, ta.refill_min,
-                           ta.search_min);
+        hipLaunchKernelGGL((k_pool_shadow<BB, SS, 4>), dim3(G), dim3(BB), 0, st, view.sv, D, spill,
+                           ta.balance ? D.work : nullptr, ta.refill_min, ta.search_min);
       else
-        hipLaunchKernelGGL((k_pool_shadow<BB, SS, 3>), dim3(G), dim3(BB), 0, st, view.sv, D, spill, ta.refill_min,
-                           ta.search_min);
+        hipLaunchKernelGGL((k_pool_shadow<BB, SS, 3>), dim3(G), dim3(BB), 0, st, view.sv, D, spill,
+                           ta.balance ? D.work : nullptr, ta.refill_min, ta.search_min);
     });
     if (!ok_e || !ok_s) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
     AR_HIP(hipGetLastError());

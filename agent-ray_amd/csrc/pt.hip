@@ -240,8 +240,12 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
     off = bytes;
     bytes += (n + 255) / 256 * 256;
   };
-  size_t o_ro, o_rd, o_b, o_L, o_pp, o_meta, o_q0, o_q1, o_hit, o_hi, o_shp, o_sho, o_shd, o_shc, o_cnt, o_st, o_acc,
-      o_spill;
+  size_t o_ro, o_rd, o_b, o_L, o_pp, o_meta, o_q0, o_q1, o_hit, o_hi, o_shp, o_sho, o_shd, o_shc, o_cnt, o_st, o_work,
+      o_acc, o_spill;
+  // the walker counters first: their place must not move when the same
+  // buffer is laid out again for another capacity (they stay zero between
+  // launches and are cleared only when the buffer is allocated)
+  add(o_work, kWalkWorkBytes);
   add(o_ro, cap * f4);
   add(o_rd, cap * f4);
   add(o_b, cap * f4);
@@ -267,6 +271,7 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
     // uses before it): drain it, not the whole device
     AR_HIP(hipStreamSynchronize(st));
     AR_HIP(buf.alloc(bytes));
+    AR_HIP(clear_walk_work((char*)buf.p + o_work, st));
     pt_graph.reset();
     bdpt_graph[0].reset();
     bdpt_graph[1].reset();
@@ -288,6 +293,7 @@ ar_status Workspace::reserve_paths(uint32_t cap, uint32_t npix, uint32_t spill_e
   B.sh_c = (float4*)(b + o_shc);
   B.cnt = (uint32_t*)(b + o_cnt);
   B.stats = (unsigned long long*)(b + o_st);
+  B.work = (uint32_t*)(b + o_work);
   acc = (float4*)(b + o_acc);
   spill = (int2*)(b + o_spill);
   params = b + o_par;
@@ -363,6 +369,8 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   AR_TRACE_VARIANTS(AR_GRIDS)
 #undef AR_GRIDS
   if (Ge == 0 || Gs == 0) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
+  uint32_t* const work_c = tc.balance ? B.work : nullptr;
+  uint32_t* const work_a = ta.balance ? B.work : nullptr;
   hipLaunchKernelGGL(k_wf_params, dim3(1), dim3(64), 0, st, P, reinterpret_cast<PtParams*>(ws.params));
   auto record = [&]() {
     hipLaunchKernelGGL(k_wf_begin, dim3(1), dim3(64), 0, st, B);
@@ -373,10 +381,10 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   if (tc.workgroup_size == BB && tc.lds_stack == SS) {                                                       \
     if (tc.park_slots == 4)                                                                                  \
       hipLaunchKernelGGL((k_wf_extend<BB, SS, 4>), dim3(Ge), dim3(BB), 0, st, v.sv, B, qi, ws.spill,         \
-                         tc.refill_min, tc.search_min);                                                      \
+                         work_c, tc.refill_min, tc.search_min);                                              \
     else                                                                                                     \
       hipLaunchKernelGGL((k_wf_extend<BB, SS, 3>), dim3(Ge), dim3(BB), 0, st, v.sv, B, qi, ws.spill,         \
-                         tc.refill_min, tc.search_min);                                                      \
+                         work_c, tc.refill_min, tc.search_min);                                              \
   }
       AR_TRACE_VARIANTS(AR_EXT)
 #undef AR_EXT
@@ -385,10 +393,10 @@ static ar_status run_chunk_wavefront(const ar_scene* s, Workspace& ws, const Sha
   if (ta.workgroup_size == BB && ta.lds_stack == SS) {                                                       \
     if (ta.park_slots == 4)                                                                                  \
       hipLaunchKernelGGL((k_wf_shadow<BB, SS, 4>), dim3(Gs), dim3(BB), 0, st, v.sv, B, ws.spill,             \
-                         ta.refill_min, ta.search_min);                                                      \
+                         work_a, ta.refill_min, ta.search_min);                                              \
     else                                                                                                     \
       hipLaunchKernelGGL((k_wf_shadow<BB, SS, 3>), dim3(Gs), dim3(BB), 0, st, v.sv, B, ws.spill,             \
-                         ta.refill_min, ta.search_min);                                                      \
+                         work_a, ta.refill_min, ta.search_min);                                              \
   }
       AR_TRACE_VARIANTS(AR_SHD)
 #undef AR_SHD

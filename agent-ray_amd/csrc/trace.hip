@@ -2,9 +2,11 @@
 // (TLAS::intersect, agent-ray src/accel/bvh.rs:413-447; Scene::occluded,
 // src/scene.rs:166-168).
 //
-// One ray per lane, persistent grid: each wave owns every W-th chunk of 64
-// rays (W = waves in the grid) and its lanes take the wave's rays in order as
-// they finish (walk_items), so a wave never waits for its slowest ray.  The
+// One ray per lane, persistent grid: each wave walks chunks of 64 rays, a
+// static share (every W-th chunk, W = waves in the grid) and then chunks
+// taken from a counter shared by the launch, and its lanes take the wave's
+// rays in order as they finish (walk_items), so a wave never waits for its
+// slowest ray and the waves of a launch end together.  The
 // grid is a multiple of 8 and block ids are remapped so that each XCD
 // (blocks are dispatched round-robin over the 8 XCDs) sweeps a contiguous
 // range of rays, keeping neighbouring (coherent) rays in the same L2.
@@ -31,7 +33,8 @@ template <bool ANY, bool NEAR, int S, int BLOCK, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_trace(SceneView sv, const ar_ray* __restrict__ rays, uint64_t n,
                                                  ar_hit* __restrict__ hits, uint32_t* __restrict__ inst_ids,
                                                  uint32_t* __restrict__ occ, int2* __restrict__ spill,
-                                                 uint32_t refill_min, uint32_t search_min) {
+                                                 uint32_t* __restrict__ work, uint32_t refill_min,
+                                                 uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t G = gridDim.x;
   const uint32_t b = blockIdx.x;
@@ -47,7 +50,7 @@ __global__ __launch_bounds__(BLOCK, trace_min_waves(S)) void k_trace(SceneView s
   const uint64_t waves = (uint64_t)G * (BLOCK / 64);
   const uint64_t wave = (uint64_t)lb * (BLOCK / 64) + threadIdx.x / 64;
   walk_items<ANY, NEAR, SLOTS>(
-      sv, stk, n, wave, waves, refill_min, search_min, [&](uint64_t i) { return load_ray(rays, i); },
+      sv, stk, n, wave, waves, work, refill_min, search_min, [&](uint64_t i) { return load_ray(rays, i); },
       [&](uint64_t i, const Walker<ANY, NEAR>& w) {
         if (ANY) {
           occ[i] = w.bpos != kNoHit ? 1u : 0u;
@@ -69,27 +72,30 @@ static uint32_t stack_need(const ar_scene* s) { return s->stack_depth; }
 
 template <bool ANY, bool NEAR, int BLOCK, int S, uint32_t SLOTS>
 static void launch_slots(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits,
-                         uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st, const ar_trace_config& c) {
+                         uint32_t* inst, uint32_t* occ, int2* spill, uint32_t* work, hipStream_t st,
+                         const ar_trace_config& c) {
   G = clamp_resident(G, reinterpret_cast<const void*>(&k_trace<ANY, NEAR, S, BLOCK, SLOTS>), BLOCK);
   hipLaunchKernelGGL((k_trace<ANY, NEAR, S, BLOCK, SLOTS>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits,
-                     inst, occ, spill, c.refill_min, c.search_min);
+                     inst, occ, spill, work, c.refill_min, c.search_min);
 }
 
 template <bool ANY, bool NEAR, int BLOCK, int S>
 static void launch_variant(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits,
-                           uint32_t* inst, uint32_t* occ, int2* spill, hipStream_t st, const ar_trace_config& c) {
+                           uint32_t* inst, uint32_t* occ, int2* spill, uint32_t* work, hipStream_t st,
+                           const ar_trace_config& c) {
   if (c.park_slots == 4)
-    launch_slots<ANY, NEAR, BLOCK, S, 4>(s, G, rays, n, hits, inst, occ, spill, st, c);
+    launch_slots<ANY, NEAR, BLOCK, S, 4>(s, G, rays, n, hits, inst, occ, spill, work, st, c);
   else
-    launch_slots<ANY, NEAR, BLOCK, S, 3>(s, G, rays, n, hits, inst, occ, spill, st, c);
+    launch_slots<ANY, NEAR, BLOCK, S, 3>(s, G, rays, n, hits, inst, occ, spill, work, st, c);
 }
 
 // Launches over n rays in pieces of < 2^31 (32-bit ray positions inside a
-// wave) with grid G and the given spill area; false for an unsupported
+// wave) with grid G, the given spill area and walker counters (the pieces
+// run one after another on st and share them); false for an unsupported
 // variant.
 template <bool ANY, bool NEAR>
 static bool launch_pieces(const ar_scene* s, uint32_t G, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst,
-                          uint32_t* occ, int2* spill, hipStream_t st, const ar_trace_config& c) {
+                          uint32_t* occ, int2* spill, uint32_t* work, hipStream_t st, const ar_trace_config& c) {
   bool ok = false;
   const uint64_t kMaxLaunch = 1ull << 31;
   for (uint64_t off = 0; off < n; off += kMaxLaunch) {
@@ -101,7 +107,7 @@ static bool launch_pieces(const ar_scene* s, uint32_t G, const ar_ray* rays, uin
     ok = false;
 #define AR_LAUNCH(B, SS)                                                           \
   if (c.workgroup_size == B && c.lds_stack == SS) {                                \
-    launch_variant<ANY, NEAR, B, SS>(s, G, r, m, h, in, oc, spill, st, c);            \
+    launch_variant<ANY, NEAR, B, SS>(s, G, r, m, h, in, oc, spill, work, st, c);      \
     ok = true;                                                                     \
   }
     AR_TRACE_VARIANTS(AR_LAUNCH)
@@ -115,13 +121,15 @@ template <bool ANY, bool NEAR>
 static ar_status launch_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* inst,
                               uint32_t* occ, hipStream_t st, const ar_trace_config& c) {
   uint32_t G = trace_grid_cfg(n, c);
-  int2* spill = nullptr;
   uint32_t need = stack_need(s);
   size_t bytes = need > c.lds_stack ? (size_t)(need - c.lds_stack) * G * c.workgroup_size * sizeof(int2) : 0;
   ar_scene* ms = const_cast<ar_scene*>(s);
-  ar_status e = scratch_acquire(ms, bytes, st, (void**)&spill);
+  char* base = nullptr;
+  ar_status e = scratch_acquire(ms, kWalkWorkBytes + bytes, st, (void**)&base);
   if (e != AR_OK) return e;
-  const bool ok = launch_pieces<ANY, NEAR>(s, G, rays, n, hits, inst, occ, spill, st, c);
+  uint32_t* work = (uint32_t*)base;
+  int2* spill = (int2*)(base + kWalkWorkBytes);
+  const bool ok = launch_pieces<ANY, NEAR>(s, G, rays, n, hits, inst, occ, spill, c.balance ? work : nullptr, st, c);
   hipError_t le = hipGetLastError();
   scratch_release(ms, st);
   if (!ok) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
@@ -195,8 +203,15 @@ uint32_t trace_grid_cfg(uint64_t n, const ar_trace_config& c) {
   return (uint32_t)g;
 }
 
+__global__ void k_clear_walk_work(uint32_t* p) { p[threadIdx.x] = 0u; }
+
+hipError_t clear_walk_work(void* p, hipStream_t st) {
+  hipLaunchKernelGGL(k_clear_walk_work, dim3(1), dim3(kWalkWorkBytes / sizeof(uint32_t)), 0, st, (uint32_t*)p);
+  return hipGetLastError();
+}
+
 uint32_t trace_grid(uint64_t n, int block) {
-  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, kTraceRefillMin, 8, 3, 0.0f};
+  ar_trace_config c{(uint32_t)block, (uint32_t)kTraceLdsStack, (uint32_t)kTraceBlocksPerCU, kTraceRefillMin, 8, 3, 0, 0.0f};
   return trace_grid_cfg(n, c);
 }
 
@@ -218,6 +233,7 @@ ar_status scratch_acquire(ar_scene* s, size_t bytes, hipStream_t st, void** out)
   if (bytes > 0 && s->d_scratch.bytes < bytes) {
     hipError_t e = hipStreamSynchronize(st);
     if (e == hipSuccess) e = s->d_scratch.alloc(bytes);
+    if (e == hipSuccess) e = clear_walk_work(s->d_scratch.p, st);
     if (e != hipSuccess) {
       s->mu.unlock();
       return fail(AR_ERR_DEVICE, std::string("scratch alloc: ") + hipGetErrorString(e));
@@ -233,14 +249,17 @@ void scratch_release(ar_scene* s, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Allocation-free tracing (ar_trace_noalloc).  The traversal stack spill of
-// such calls lives in `scratch`, a fixed array of kNoallocSpillBytes in every
-// GPU's memory for the life of the process.  Its device address is looked up
-// once per device (hipGetSymbolAddress, at ar_init or at the first call on
-// that device) and only that address is handed to kernels: the array's name
-// in host code is a host-side stand-in, not a device address.  The spill
-// needs no clearing (every entry is written before it is read).  Calls that
-// overlap on one GPU share the array and race.
+// Allocation-free tracing (ar_trace_noalloc).  The walker counters and the
+// traversal stack spill of such calls live in `scratch`, a fixed array of
+// kNoallocSpillBytes in every GPU's memory for the life of the process: the
+// counters in its first kWalkWorkBytes (zero when the code object is loaded,
+// and set back to zero by every launch's last wave), the spill after them.
+// Its device address is looked up once per device (hipGetSymbolAddress, at
+// ar_init or at the first call on that device) and only that address is
+// handed to kernels: the array's name in host code is a host-side stand-in,
+// not a device address.  The spill needs no clearing (every entry is
+// written before it is read).  Calls that overlap on one GPU share the
+// array and race.
 // ---------------------------------------------------------------------------
 constexpr size_t kNoallocSpillBytes = 64ull << 20;
 static __device__ __align__(256) unsigned char scratch[kNoallocSpillBytes];
@@ -327,8 +346,11 @@ ar_status ar_trace_noalloc(const ar_scene* s, const ar_ray* rays, uint64_t n, ar
   if (n == 0) return AR_OK;
   ar_status e = noalloc_setup_current_device();
   if (e != AR_OK) return e;
-  int2* spill = static_cast<int2*>(noalloc_scratch_current_device());
-  if (!spill) return fail(AR_ERR_DEVICE, "ar_trace_noalloc: no scratch address for the current device");
+  char* base = static_cast<char*>(noalloc_scratch_current_device());
+  if (!base) return fail(AR_ERR_DEVICE, "ar_trace_noalloc: no scratch address for the current device");
+  uint32_t* work = (uint32_t*)base;
+  int2* spill = (int2*)(base + kWalkWorkBytes);
+  constexpr size_t kSpillBytes = kNoallocSpillBytes - kWalkWorkBytes;
   const ar_trace_config c = s->tcfg;
   const uint32_t need = stack_need(s);
   const uint64_t per_thread = need > c.lds_stack ? (uint64_t)(need - c.lds_stack) * sizeof(int2) : 0;
@@ -336,18 +358,20 @@ ar_status ar_trace_noalloc(const ar_scene* s, const ar_ray* rays, uint64_t n, ar
   if (per_thread) {
     // the spill is [entry][thread]: fit the grid to the fixed array (the
     // answer does not depend on the grid)
-    const uint64_t gmax = kNoallocSpillBytes / (per_thread * c.workgroup_size) / 8 * 8;
+    const uint64_t gmax = kSpillBytes / (per_thread * c.workgroup_size) / 8 * 8;
     if (gmax < 8)
       return fail(AR_ERR_OUT_OF_BUDGET,
                   "ar_trace_noalloc: this scene's stack spill needs " + std::to_string(per_thread * c.workgroup_size * 8) +
                       " bytes for the smallest grid; the allocation-free array holds " +
-                      std::to_string(kNoallocSpillBytes));
+                      std::to_string(kSpillBytes));
     G = (uint32_t)std::min<uint64_t>(G, gmax);
   }
   hipStream_t st = pick_stream(s, stream);
   const bool ok = (flags & AR_REF_ORDER)
-                      ? launch_pieces<false, false>(s, G, rays, n, hits, inst_ids, nullptr, spill, st, c)
-                      : launch_pieces<false, true>(s, G, rays, n, hits, inst_ids, nullptr, spill, st, c);
+                      ? launch_pieces<false, false>(s, G, rays, n, hits, inst_ids, nullptr, spill,
+                                                    c.balance ? work : nullptr, st, c)
+                      : launch_pieces<false, true>(s, G, rays, n, hits, inst_ids, nullptr, spill,
+                                                   c.balance ? work : nullptr, st, c);
   if (!ok) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
   const hipError_t le = hipGetLastError();
   if (le != hipSuccess) return fail(AR_ERR_DEVICE, std::string("k_trace launch: ") + hipGetErrorString(le));
@@ -392,6 +416,7 @@ static ar_status check_trace_config(const ar_scene* s, const ar_trace_config* c,
   if (c->refill_min == 0 || c->refill_min > 64) return fail(AR_ERR_INVALID_ARG, "refill_min out of range (1..64)");
   if (c->search_min > 63) return fail(AR_ERR_INVALID_ARG, "search_min out of range (0..63)");
   if (c->park_slots != 3 && c->park_slots != 4) return fail(AR_ERR_INVALID_ARG, "park_slots must be 3 or 4");
+  if (c->balance > 1) return fail(AR_ERR_INVALID_ARG, "balance must be 0 or 1");
   return AR_OK;
 }
 
@@ -402,6 +427,7 @@ ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c) {
   s->tcfg = *c;
   s->tcfg.ms = 0.0f;
   s->tcfg_any = s->tcfg;
+  s->tcfg_set = true;
   return AR_OK;
 }
 
@@ -411,6 +437,7 @@ ar_status ar_scene_set_anyhit_config(ar_scene* s, const ar_trace_config* c) {
   std::lock_guard<std::mutex> lk(s->mu);
   s->tcfg_any = *c;
   s->tcfg_any.ms = 0.0f;
+  s->tcfg_set = true;
   return AR_OK;
 }
 
@@ -429,7 +456,9 @@ ar_status ar_scene_get_anyhit_config(const ar_scene* s, ar_trace_config* c) {
 // Times every (variant, blocks per CU, refill threshold, search quorum, park
 // slots) candidate on the sample, near-first order, one warm-up launch each,
 // once for closest hit and once for any hit (their best launches differ), and
-// keeps the fastest of each.
+// keeps the fastest of each.  `balance` keeps its current value: its effect
+// shows on launches of at least 4 chunks of 64 rays per wave, which a ray
+// sample is usually too small for (the job-level tuner times it).
 ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_t flags, void* stream,
                             ar_trace_config* best) {
   if (!s || !rays || n == 0) return fail(AR_ERR_INVALID_ARG, "ar_scene_autotune: NULL or empty sample");
@@ -443,6 +472,7 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
   AR_HIP(hipEventCreate(&e0));
   AR_HIP(hipEventCreate(&e1));
   ar_trace_config winners[2] = {s->tcfg, s->tcfg_any};
+  const uint32_t balance[2] = {s->tcfg.balance, s->tcfg_any.balance};
   float best_ms[2] = {INFINITY, INFINITY};
   const uint32_t per_cu[3] = {4, 8, 16};
   const uint32_t refills[3] = {8, 24, 48};
@@ -484,7 +514,7 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
         for (uint32_t rf : refills)
           for (uint32_t sm : searches)
             for (uint32_t ps : slot_counts)
-              if (err == AR_OK) consider(ar_trace_config{vv[0], vv[1], pc, rf, sm, ps, 0.0f});
+              if (err == AR_OK) consider(ar_trace_config{vv[0], vv[1], pc, rf, sm, ps, balance[q], 0.0f});
   }
   (void)hipStreamSynchronize(st);
   (void)hipEventDestroy(e0);
@@ -496,6 +526,7 @@ ar_status ar_scene_autotune(ar_scene* s, const ar_ray* rays, uint64_t n, uint32_
     std::lock_guard<std::mutex> lk(s->mu);
     if (!only_a) s->tcfg = winners[0];
     if (!only_c) s->tcfg_any = winners[1];
+    s->tcfg_set = true;
   }
   if (best) *best = only_a ? winners[1] : winners[0];
   return AR_OK;

@@ -54,8 +54,8 @@ __device__ __forceinline__ Ray load_ray(const ar_ray* rays, uint64_t i) {
 template <bool ANY, bool NEAR, int S, int BLOCK, uint32_t SLOTS>
 __global__ __launch_bounds__(BLOCK) void k_trace_count(SceneView sv, const ar_ray* __restrict__ rays, uint64_t n,
                                                        ar_hit* __restrict__ hits, uint32_t* __restrict__ occ,
-                                                       int2* __restrict__ spill, uint32_t refill_min,
-                                                       uint32_t search_min) {
+                                                       int2* __restrict__ spill, uint32_t* __restrict__ work,
+                                                       uint32_t refill_min, uint32_t search_min) {
   __shared__ int2 lds[S * BLOCK];
   const uint32_t G = gridDim.x;
   const uint32_t b = blockIdx.x;
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_count(SceneView sv, const ar_ra
   const uint64_t waves = (uint64_t)G * (BLOCK / 64);
   const uint64_t wave = (uint64_t)lb * (BLOCK / 64) + threadIdx.x / 64;
   walk_items<ANY, NEAR, SLOTS>(
-      sv, stk, n, wave, waves, refill_min, search_min, [&](uint64_t i) { return load_ray(rays, i); },
+      sv, stk, n, wave, waves, work, refill_min, search_min, [&](uint64_t i) { return load_ray(rays, i); },
       [&](uint64_t i, const Walker<ANY, NEAR>& w) {
         if (ANY) {
           occ[i] = w.bpos != kNoHit ? 1u : 0u;
@@ -89,13 +89,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace_count(SceneView sv, const ar_ra
 
 template <bool ANY, bool NEAR, int S, int BLOCK>
 static void launch(uint32_t G, const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hits, uint32_t* occ,
-                   int2* spill, hipStream_t st, const ar_trace_config& c) {
+                   int2* spill, uint32_t* work, hipStream_t st, const ar_trace_config& c) {
   if (c.park_slots == 4)
     hipLaunchKernelGGL((k_trace_count<ANY, NEAR, S, BLOCK, 4>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits,
-                       occ, spill, c.refill_min, c.search_min);
+                       occ, spill, work, c.refill_min, c.search_min);
   else
     hipLaunchKernelGGL((k_trace_count<ANY, NEAR, S, BLOCK, 3>), dim3(G), dim3(BLOCK), 0, st, s->view, rays, n, hits,
-                       occ, spill, c.refill_min, c.search_min);
+                       occ, spill, work, c.refill_min, c.search_min);
 }
 
 }  // namespace ar_count
@@ -114,26 +114,29 @@ extern "C" ar_status ar_trace_count(const ar_scene* s, const ar_ray* rays, uint6
   if (G == 0) return fail(AR_ERR_INVALID_ARG, "unsupported traversal configuration");
   const uint64_t threads = (uint64_t)G * c.workgroup_size;
   hipStream_t st = pick_stream(s, stream);
-  DevBuf counts, res, spill;
+  DevBuf counts, res, spill, work;
   AR_HIP(counts.alloc(threads * ar_count::kKinds * sizeof(uint32_t)));
   AR_HIP(res.alloc(n * (any ? sizeof(uint32_t) : sizeof(ar_hit))));
   const uint32_t need = s->stack_depth;
   const size_t spill_bytes = need > c.lds_stack ? (size_t)(need - c.lds_stack) * threads * sizeof(int2) : 0;
   AR_HIP(spill.alloc(spill_bytes));
+  AR_HIP(work.alloc(kWalkWorkBytes));
   AR_HIP(hipMemsetAsync(counts.p, 0, counts.bytes, st));
+  AR_HIP(hipMemsetAsync(work.p, 0, work.bytes, st));
+  uint32_t* const wk = c.balance ? work.as<uint32_t>() : nullptr;
   uint32_t* cp = counts.as<uint32_t>();
   AR_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(ar_count::g_counts), &cp, sizeof(cp), 0, hipMemcpyHostToDevice, st));
   bool ok = false;
 #define AR_COUNT_LAUNCH(B, SS)                                                                                \
   if (c.workgroup_size == B && c.lds_stack == SS) {                                                          \
     if (any && near)                                                                                          \
-      ar_count::launch<true, true, SS, B>(G, s, rays, n, nullptr, res.as<uint32_t>(), spill.as<int2>(), st, c); \
+      ar_count::launch<true, true, SS, B>(G, s, rays, n, nullptr, res.as<uint32_t>(), spill.as<int2>(), wk, st, c); \
     else if (any)                                                                                             \
-      ar_count::launch<true, false, SS, B>(G, s, rays, n, nullptr, res.as<uint32_t>(), spill.as<int2>(), st, c); \
+      ar_count::launch<true, false, SS, B>(G, s, rays, n, nullptr, res.as<uint32_t>(), spill.as<int2>(), wk, st, c); \
     else if (near)                                                                                            \
-      ar_count::launch<false, true, SS, B>(G, s, rays, n, res.as<ar_hit>(), nullptr, spill.as<int2>(), st, c); \
+      ar_count::launch<false, true, SS, B>(G, s, rays, n, res.as<ar_hit>(), nullptr, spill.as<int2>(), wk, st, c); \
     else                                                                                                      \
-      ar_count::launch<false, false, SS, B>(G, s, rays, n, res.as<ar_hit>(), nullptr, spill.as<int2>(), st, c); \
+      ar_count::launch<false, false, SS, B>(G, s, rays, n, res.as<ar_hit>(), nullptr, spill.as<int2>(), wk, st, c); \
     ok = true;                                                                                                \
   }
   AR_TRACE_VARIANTS(AR_COUNT_LAUNCH)

@@ -159,11 +159,12 @@ def bench_traversal(args, ar, torch):
             sc.set_anyhit_config(*best_a)
             closest()
             times_c[c] = launch_ms(closest)
-        for c in [best_c] + [c for c in cands if c != best_c]:
+        for c in [best_c] + [ar.with_balance(c, best_c[6]) for c in cands if ar.with_balance(c, best_c[6]) != best_c]:
             try_c(c)
-        flip = ar.other_park_slots(min(times_c, key=times_c.get))
-        if flip not in times_c:
-            try_c(flip)
+        for other in (ar.other_park_slots, ar.other_balance):
+            flip = other(min(times_c, key=times_c.get))
+            if flip not in times_c:
+                try_c(flip)
         best_c = min(times_c, key=times_c.get)
         best_t = times_c[best_c]
         sc.set_trace_config(*best_c)
@@ -173,11 +174,12 @@ def bench_traversal(args, ar, torch):
             sc.set_anyhit_config(*a)
             anyhit()
             times_a[a] = launch_ms(anyhit)
-        for a in [best_a] + [a for a in cands if a != best_a]:
+        for a in [best_a] + [ar.with_balance(a, best_a[6]) for a in cands if ar.with_balance(a, best_a[6]) != best_a]:
             try_a(a)
-        flip = ar.other_park_slots(min(times_a, key=times_a.get))
-        if flip not in times_a:
-            try_a(flip)
+        for other in (ar.other_park_slots, ar.other_balance):
+            flip = other(min(times_a, key=times_a.get))
+            if flip not in times_a:
+                try_a(flip)
         best_a = min(times_a, key=times_a.get)
         best_ta = times_a[best_a]
         sc.set_trace_config(*best_c)
@@ -946,10 +948,11 @@ def main():
                     help="copy: a plain 1 GiB device copy, the reference point that scales rocprof's "
                          "FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--trace-config", default=None,
-                    help="WORKGROUP,LDS_STACK,BLOCKS_PER_CU[,REFILL_MIN[,SEARCH_MIN[,PARK_SLOTS]]]: fixed "
-                         "traversal launch instead of autotuning")
+                    help="WORKGROUP,LDS_STACK,BLOCKS_PER_CU[,REFILL_MIN[,SEARCH_MIN[,PARK_SLOTS[,BALANCE]]]]: "
+                         "fixed traversal launch instead of autotuning")
     ap.add_argument("--anyhit-config", default=None,
-                    help="W,S,P[,R[,SM[,PS]]]: any-hit launch (with --trace-config; default: the same as closest hit)")
+                    help="W,S,P[,R[,SM[,PS[,BAL]]]]: any-hit launch (with --trace-config; default: the same as "
+                         "closest hit)")
     ap.add_argument("--integrator", default="pt", choices=sorted(ROOM_JOBS),
                     help="render workload: the integrator rendered on the room")
     ap.add_argument("--chunk", type=int, default=0, help="render workload: samples per pixel per queue chunk")

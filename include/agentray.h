@@ -214,6 +214,11 @@ typedef struct ar_trace_config {
   uint32_t refill_min; /* 1..64: idle lanes a wave gathers before it takes new rays */
   uint32_t search_min; /* 0..63: a wave's node phase goes on while more lanes than this have no leaf */
   uint32_t park_slots; /* 3 or 4: triangle leaves a lane parks in one node phase */
+  uint32_t balance;    /* 1: a wave that has walked its static half of the rays takes further chunks of 64
+                          from a counter shared by the launch, so the waves end together (launches of at
+                          least 4 chunks per wave); 0: static shares only, no atomics.  Default at commit:
+                          1 for scenes of at least 4096 triangles, 0 below (cheap rays make the counter's
+                          atomics cost more than the balance gains). */
   float ms; /* measured time of the winning candidate (autotune only) */
 } ar_trace_config;
 ar_status ar_scene_set_trace_config(ar_scene* s, const ar_trace_config* c);  /* closest and any hit */

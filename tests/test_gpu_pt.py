@@ -48,12 +48,47 @@ def test_launch_refinement_keeps_the_film(ar, wl):
     assert len(res["tried"]["anyhit"]) >= 6
     c, a = sc.trace_config(), sc.anyhit_config()
     assert list(ar.config_key(c)) == res["closest"] and list(ar.config_key(a)) == res["anyhit"]
-    # the winners were also timed with the other park slot count
+    # the winners were also timed with balance switched, and (before that
+    # switch) with the other park slot count
+    key = lambda c: "x".join(map(str, c))
     for stage, win in (("closest", res["closest"]), ("anyhit", res["anyhit"])):
-        assert "x".join(map(str, ar.other_park_slots(tuple(win)))) in res["tried"][stage]
+        tried, win = res["tried"][stage], tuple(win)
+        assert key(ar.other_balance(win)) in tried
+        assert key(ar.other_park_slots(win)) in tried or key(ar.other_park_slots(ar.other_balance(win))) in tried
     assert pt.config.spp == 4
     after, _ = render(ar, sc, cam, 96, 64, 4)
     assert np.array_equal(before.view(np.uint32), after.view(np.uint32))
+
+
+@pytest.mark.parametrize("integ", ["pt", "bdpt", "gdpt"])
+def test_balanced_launches_keep_the_film(ar, wl, integ):
+    """Extension, shadow and connection launches large enough for the shared
+    chunk counter (256x256 on the 280k-triangle room with 1024-wave launches:
+    8 or more chunks of 64 rays per wave) give the film of static shares bit
+    for bit, render after render (each launch leaves the counters at zero)."""
+    desc = wl.procedural_room()
+    sc = ar.Scene.from_desc(desc)
+    w = h = 256
+    cd = desc.camera
+    cam = ar.PinholeCamera.from_array(wl.camera_lookat(cd.lookfrom, cd.lookat, cd.vup, cd.vfov, w / h))
+
+    def make():
+        if integ == "pt":
+            return ar.PathTracer(ar.PathTracerConfig(spp=8, max_depth=8, rr_depth=3), seed=5)
+        if integ == "bdpt":
+            return ar.BidirectionalPathTracer(ar.BdptConfig(spp=8, max_depth=6, rr_depth=3), seed=5)
+        return ar.GradientDomainPathTracer(ar.GdptConfig(spp=4, max_depth=8, rr_depth=3, poisson_iterations=20),
+                                           seed=5)
+
+    films = {}
+    for bal in (0, 1):
+        sc.set_trace_config(64, 8, 4, 8, 8, 3, bal)
+        films[bal] = [make().render(sc, cam, w, h).pixels.copy() for _ in range(2)]
+    ref = films[0][0]
+    assert ref.mean() > 0.01
+    for bal in (0, 1):
+        for f in films[bal]:
+            assert np.array_equal(f.view(np.uint32), ref.view(np.uint32)), bal
 
 
 @pytest.mark.parametrize("integ", ["pt", "bdpt", "gdpt"])

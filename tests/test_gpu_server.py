@@ -122,6 +122,28 @@ def test_pooled_jobs_over_several_windows_equal_alone(ar, wl, room):
         assert st["paths"] == side * side * spp * (3 if kind == 2 else 1)
 
 
+def test_pool_with_balanced_launches_equals_alone(ar, wl):
+    """Pool iterations whose extension and shadow launches use the shared
+    chunk counter (up to 2^22 live paths over 1024-wave launches) serve films
+    equal to solo renders with static shares, bit for bit."""
+    desc = wl.procedural_room(n_boxes=20, width=64, height=64)
+    sc = ar.Scene.from_desc(desc)
+    specs = [(0, 1024, 8, 31), (2, 256, 8, 32), (0, 96, 4, 33)]
+    sc.set_trace_config(64, 8, 4, 8, 8, 3, 0)
+    alone = [_integ(ar, kind, spp, seed).render(sc, _cam(ar, wl, desc, side, side), side, side).pixels.copy()
+             for kind, side, spp, seed in specs]
+    sc.set_trace_config(64, 8, 4, 8, 8, 3, 1)
+    srv = ar.RenderServer(sc, max_active_jobs=4, pool_min_paths=1)
+    ids = []
+    for kind, side, spp, seed in specs:
+        ids.append(srv.submit(_integ(ar, kind, spp, seed), _cam(ar, wl, desc, side, side), side, side))
+        time.sleep(0.005)
+    served = [srv.wait(i) for i in ids]
+    srv.close()
+    for (kind, side, spp, seed), (film, st), ref in zip(specs, served, alone):
+        assert np.array_equal(film.pixels.view(np.uint32), ref.view(np.uint32)), (kind, side, spp)
+
+
 def test_pooled_cancel_drains(ar, wl, room):
     desc, sc = room
     srv = ar.RenderServer(sc, max_active_jobs=4, pool_min_paths=1)

@@ -219,6 +219,31 @@ def test_every_launch_variant_same_answer(ar, wl):
     for ps in (0, 2, 5):
         with pytest.raises(ar.ArError):
             sc.set_trace_config(64, 8, 4, 24, 8, ps)
+    with pytest.raises(ar.ArError):
+        sc.set_trace_config(64, 8, 4, 24, 8, 3, 2)
+
+
+def test_balanced_launches_same_answer(ar, wl):
+    """Launches with enough chunks of 64 rays per wave for the shared chunk
+    counter (balance 1: 2^20 rays over 1024 or 2048 waves, 8-16 chunks per
+    wave) return the hits of static shares (balance 0) bit for bit, for
+    closest and any hit, and so do the launches after them: each launch's
+    last wave leaves the counters at zero for the next one."""
+    desc = wl.triangle_soup(1 << 16, seed=63)
+    rays = wl.soup_rays(1 << 20, seed=64, image=1024)
+    sc = ar.Scene.from_desc(desc)
+    assert sc.trace_config()["balance"] == 1  # 2^16 triangles: on by default
+    sc.set_trace_config(64, 8, 4, 8, 8, 3, 0)
+    ref_t, ref_p, ref_i = sc.intersect(rays)
+    ref_o = sc.occluded(rays)
+    assert (ref_p != 0xFFFFFFFF).mean() > 0.2 and ref_o.mean() > 0.2
+    for cfg in [(64, 8, 4, 8, 8, 3, 1), (128, 16, 4, 8, 16, 4, 1), (64, 16, 4, 1, 0, 4, 1)]:
+        sc.set_trace_config(*cfg)
+        for rep in range(3):
+            t, p, i = sc.intersect(rays)
+            assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)), (cfg, rep)
+            assert np.array_equal(p, ref_p) and np.array_equal(i, ref_i), (cfg, rep)
+            assert np.array_equal(sc.occluded(rays), ref_o), (cfg, rep)
 
 
 @pytest.mark.slow
