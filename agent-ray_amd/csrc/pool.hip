@@ -289,8 +289,9 @@ __global__ void k_pool_advance(PoolDev D) {
   }
 }
 
+// At least 4 waves per SIMD, as k_wf_shade (two 512-thread workgroups per CU).
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_pool_shade(ShadeView v, PoolDev D, int cur) {
+__global__ __launch_bounds__(BLOCK, 4) void k_pool_shade(ShadeView v, PoolDev D, int cur) {
   __shared__ uint32_t qsh[2][2][BLOCK / 64];
   __shared__ uint32_t ret[kPoolMaxJobs * 2];
   __shared__ uint32_t rays[kPoolMaxJobs];

@@ -104,8 +104,12 @@ __global__ void k_wf_begin(WaveBufs B) {
 }
 
 
+// At least 4 waves per SIMD (<= 128 VGPRs; 28 B of scratch per lane): two
+// 512-thread workgroups per CU instead of one at the unconstrained 138
+// VGPRs.  The room's 1920x1080x64 spp render measured 516 -> 509 ms;
+// 256-thread workgroups at 3 waves per SIMD measured 523 ms.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_wf_shade(ShadeView v, const PtParams* __restrict__ Pd, WaveBufs B, int qi) {
+__global__ __launch_bounds__(BLOCK, 4) void k_wf_shade(ShadeView v, const PtParams* __restrict__ Pd, WaveBufs B, int qi) {
   const PtParams P = *Pd;
   const uint32_t n = B.cnt[qi];
   const uint32_t* q = B.q[qi];
