@@ -1,8 +1,10 @@
 """ar_trace_noalloc, the allocation-free variant of ar_trace (its traversal
 stack spill lives in a fixed per-GPU array instead of memory the library
 allocates), against ar_trace on the same rays, bit for bit: two calls in a
-row on one scene, the second with fewer rays, then a second scene, and the
-argument errors (host pointers, which would need staging)."""
+row on one scene, the second with fewer rays, then launches large enough
+for the walker's shared chunk counters (kept in the same fixed array), a
+second scene, and the argument errors (host pointers, which would need
+staging)."""
 import ctypes as C
 
 import numpy as np
@@ -42,6 +44,14 @@ def test_noalloc_equals_ar_trace(ar, wl):
     small = wl.soup_rays(1 << 12, seed=43, image=64)
     _trace_both(ar, torch, sc, small, ref_order=False)
     _trace_both(ar, torch, sc, small, ref_order=True)
+    # 1024-wave launches over 2^18 rays (4 chunks of 64 per wave): the walker
+    # counters in the fixed array are used, twice in a row, then a launch too
+    # small for them
+    print("noalloc: balanced launches, 2^18 rays over 1024 waves, then 2^12", flush=True)
+    sc.set_trace_config(64, 8, 4, 8, 8, 3, 1)
+    for _ in range(2):
+        _trace_both(ar, torch, sc, big, ref_order=False)
+    _trace_both(ar, torch, sc, small, ref_order=False)
     print("noalloc: Cornell instances (rotated, scaled)", flush=True)
     desc = wl.cornell_box()
     sc2 = ar.Scene.from_desc(desc)
