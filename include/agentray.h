@@ -247,7 +247,8 @@ ar_status ar_trace(const ar_scene* s, const ar_ray* rays, uint64_t n, ar_hit* hi
 /* No counterpart in the reference.  ar_trace (same answers, bit for bit)
  * for callers that cannot allocate device memory: code under HIP graph
  * capture, serving loops and harnesses that forbid hipMalloc.  It allocates
- * nothing: the traversal stack spill lives in a fixed 64 MiB array in each
+ * nothing: the walker's chunk counters (its first 256 bytes, zero between
+ * calls) and the traversal stack spill live in a fixed 64 MiB array in each
  * GPU's memory, held until the process ends, and the grid is reduced to fit
  * it.  A scene whose spill does not fit even the smallest grid is rejected
  * with AR_ERR_OUT_OF_BUDGET; host pointers (AR_HOST_PTRS) with
