@@ -590,6 +590,60 @@ AR_HD bool trace_scene(const SceneView& sv, const Ray& r, WorldHit& best, Stack&
   return w.bpos != kNoHit;
 }
 
+// The sequence of chunks of 64 items one wave walks in walk_items (below),
+// kept apart from the wave intrinsics so that the host emulation of the
+// walker runs this same bookkeeping.  Wave `wv` of `nw` first walks its
+// static share, chunks wv, wv + nw, ..., wv + (ns - 1) nw; chunk k >= ns of
+// its sequence is ns * nw + take(), take() handing out 0, 1, 2, ... across
+// the launch (a counter).  ns is half of the chunks per wave when the
+// launch has counters and at least 4 chunks per wave; otherwise every chunk
+// is static (take() is never called).  `off` is the next unassigned item of
+// the current chunk `ca`, sequence index `ka`.  All values are wave-uniform.
+struct ChunkCursor {
+  uint32_t wv, nw, ns;
+  uint32_t ka, ca, off;
+  bool dyn;
+
+  AR_HD static bool uses_counters(uint64_t n, uint32_t nw, bool counters) {
+    return counters && (uint32_t)((n + 63) >> 6) >= 4u * nw;
+  }
+  template <class Take>
+  AR_HD uint32_t chunk_at(uint32_t k, Take& take) const {
+    return k < ns ? wv + k * nw : ns * nw + take();
+  }
+  template <class Take>
+  AR_HD void init(uint32_t wave, uint32_t waves, uint64_t n, bool counters, Take& take) {
+    wv = wave;
+    nw = waves;
+    dyn = uses_counters(n, waves, counters);
+    ns = dyn ? ((uint32_t)((n + 63) >> 6) / nw) >> 1 : 0xffffffffu;
+    ka = 0;
+    off = 0;
+    ca = chunk_at(0, take);
+  }
+  // Whether items remain for this wave.
+  AR_HD bool more(uint64_t n) const { return (uint64_t)ca * 64 + off < n; }
+  // Start of a refill of `nidle` lanes: a refill that reaches past the
+  // current chunk takes the next chunk now (its lanes use it at once), so no
+  // chunk is taken without being used.  Returns that chunk (0 if none).
+  template <class Take>
+  AR_HD uint32_t refill_begin(uint32_t nidle, Take& take) const {
+    return off + nidle > 64 ? chunk_at(ka + 1, take) : 0u;
+  }
+  // The item of refill position p = off + (rank of the lane among the idle
+  // ones); cb is refill_begin's chunk.
+  AR_HD uint64_t item(uint32_t p, uint32_t cb) const { return (uint64_t)(p < 64 ? ca : cb) * 64 + (p & 63); }
+  template <class Take>
+  AR_HD void refill_end(uint32_t nidle, uint32_t cb, Take& take) {
+    off += nidle;
+    if (off >= 64) {
+      off -= 64;
+      ++ka;
+      ca = off > 0 ? cb : chunk_at(ka, take);
+    }
+  }
+};
+
 #ifdef __HIP__  // wave intrinsics: HIP translation units only
 // Node-phase rule of a wave whose lanes walk together: the wave keeps
 // walking nodes while more than search_min of its active lanes have parked
@@ -662,52 +716,35 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
   const uint64_t below = lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
   // Chunk ids and positions are wave-uniform and below 2^32 (n < 2^32):
   // kept in scalar registers.
-  const uint32_t wv = __builtin_amdgcn_readfirstlane((uint32_t)wave);
-  const uint32_t nw = __builtin_amdgcn_readfirstlane((uint32_t)waves);
-  const uint32_t nchunks = (uint32_t)((n + 63) >> 6);
-  // Counters are used from 4 chunks per wave on; smaller launches are
-  // static (no atomics at all, the counters stay zero).
-  const bool dyn = work != nullptr && nchunks >= 4u * nw;
-  const uint32_t ns = dyn ? (nchunks / nw) >> 1 : 0xffffffffu;
-  // Chunk k of this wave's sequence (called in wave-uniform code).
-  auto chunk_at = [&](uint32_t k) -> uint32_t {
-    if (k < ns) return wv + k * nw;
+  auto take = [&]() -> uint32_t {
     const uint64_t act = __ballot(1);
     const int l0 = __ffsll((unsigned long long)act) - 1;
     uint32_t v = 0;
     if ((int)lane == l0) v = atomicAdd(&work[0], 1u);
-    return ns * nw + __builtin_amdgcn_readfirstlane(__shfl(v, l0));
+    return __builtin_amdgcn_readfirstlane(__shfl(v, l0));
   };
-  uint32_t ka = 0;            // sequence index of the current chunk
-  uint32_t ca = chunk_at(0);  // its chunk
-  uint32_t off = 0;           // next unassigned item within it
+  ChunkCursor cc;
+  cc.init(__builtin_amdgcn_readfirstlane((uint32_t)wave), __builtin_amdgcn_readfirstlane((uint32_t)waves), n,
+          work != nullptr, take);
   uint32_t item = 0;
   bool alive = false;
   Walker<ANY, NEAR> w;
   for (;;) {
     uint64_t idle = __ballot(!alive);
     uint32_t nidle = (uint32_t)__popcll(idle);
-    bool more = (uint64_t)ca * 64 + off < n;
+    bool more = cc.more(n);
     if (!more && nidle == 64) break;
     if (more && nidle >= refill_min) {
-      // A refill that reaches past the current chunk takes the next one now
-      // (its lanes use it at once), so no chunk is taken without being used.
-      const uint32_t cb = off + nidle > 64 ? chunk_at(ka + 1) : 0u;
+      const uint32_t cb = cc.refill_begin(nidle, take);
       if (!alive) {
-        const uint32_t p = off + (uint32_t)__popcll(idle & below);
-        const uint64_t it = (uint64_t)(p < 64 ? ca : cb) * 64 + (p & 63);
+        const uint64_t it = cc.item(cc.off + (uint32_t)__popcll(idle & below), cb);
         if (it < n) {
           item = (uint32_t)it;
           alive = !w.start(sv, src(it), stk);
           if (!alive) sink(it, w);
         }
       }
-      off += nidle;
-      if (off >= 64) {
-        off -= 64;
-        ++ka;
-        ca = off > 0 ? cb : chunk_at(ka);
-      }
+      cc.refill_end(nidle, cb, take);
     }
     if (alive) {
       w.template descend<SLOTS>(sv, stk, [search_min](bool, bool searching) {
@@ -719,11 +756,11 @@ __device__ __forceinline__ void walk_items(const SceneView& sv, Stack& stk, uint
       }
     }
   }
-  if (dyn) {
+  if (cc.dyn) {
     // The wave is done; the last wave of the launch clears the counters.
     const uint64_t act = __ballot(1);
     const int l0 = __ffsll((unsigned long long)act) - 1;
-    if ((int)lane == l0 && atomicAdd(&work[1], 1u) == nw - 1u) {
+    if ((int)lane == l0 && atomicAdd(&work[1], 1u) == cc.nw - 1u) {
       atomicExch(&work[0], 0u);
       atomicExch(&work[1], 0u);
     }

@@ -3,9 +3,13 @@
 // -- over host copies of a scene's trees, so the CPU suite can compare it
 // with the oracle ray by ray.  Two drivers per query type:
 //   * trace_scene, one ray after another (no leaf parking across lanes);
-//   * an emulation of walk_items for 64-lane waves with wave-local refill
-//     and maximal leaf parking (3 or 4 park slots), which checks that every ray is started once
-//     and that the visiting order does not change the answer.
+//   * an emulation of walk_items for 64-lane waves with wave-local refill,
+//     the library's chunk bookkeeping (ChunkCursor, with its shared counter)
+//     and maximal leaf parking (3 or 4 park slots), which checks that every
+//     ray is started once and that the visiting order does not change the
+//     answer;
+//   * host_chunk_schedule: the chunk bookkeeping alone, for many waves
+//     advancing in a random interleaving.
 #include <cstdio>
 #include <vector>
 
@@ -55,35 +59,37 @@ int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, ui
     record(w, sv, i, t0, p0, i0);
   }
   std::vector<uint8_t> seen(n, 0);
+  uint32_t counter = 0;  // the launch's shared chunk counter (waves run one after another here)
+  auto take = [&]() { return counter++; };
   for (int wave = 0; wave < waves; ++wave) {
     Walker<ANY, NEAR> w[64];
     Stack stk[64];
     bool alive[64] = {};
     uint64_t item[64] = {};
     for (int l = 0; l < 64; ++l) stk[l] = Stack{lds.data() + 64 * l, spill.data() + 512 * l, 1, 1, S, 0};
-    uint64_t next = 0;
+    ChunkCursor cc;
+    cc.init((uint32_t)wave, (uint32_t)waves, n, true, take);
     const uint32_t refill_min = 16;
     for (;;) {
       uint64_t idle = 0;
       for (int l = 0; l < 64; ++l)
         if (!alive[l]) idle |= 1ull << l;
       uint32_t nidle = (uint32_t)__builtin_popcountll(idle);
-      uint64_t first = ((uint64_t)wave + (next >> 6) * waves) * 64 + (next & 63);
-      bool more = first < n;
+      bool more = cc.more(n);
       if (!more && nidle == 64) break;
       if (more && nidle >= refill_min) {
+        const uint32_t cb = cc.refill_begin(nidle, take);
         for (int l = 0; l < 64; ++l) {
           if (alive[l]) continue;
           uint64_t below = l == 0 ? 0ull : ((~0ull) >> (64 - l));
-          uint64_t p = next + (uint64_t)__builtin_popcountll(idle & below);
-          uint64_t it = ((uint64_t)wave + (p >> 6) * waves) * 64 + (p & 63);
+          uint64_t it = cc.item(cc.off + (uint32_t)__builtin_popcountll(idle & below), cb);
           if (it >= n) continue;
           if (seen[it]++) return -2;
           item[l] = it;
           alive[l] = !w[l].start(sv, ray_at(rays, it), stk[l]);
           if (!alive[l]) record(w[l], sv, it, t1, p1, i1);
         }
-        next += nidle;
+        cc.refill_end(nidle, cb, take);
       }
       for (int l = 0; l < 64; ++l) {
         if (!alive[l]) continue;
@@ -106,6 +112,77 @@ int run(const SceneView& sv, const float* rays, uint64_t n, int S, int waves, ui
 }
 
 }  // namespace
+
+// The chunk bookkeeping of walk_items (ChunkCursor) for `waves` waves of 64
+// lanes that advance in a random interleaving (xorshift, `seed`): at each
+// step a random unfinished wave refills its idle lanes if it has items left
+// and at least refill_min idle lanes, as walk_items does, and otherwise
+// retires a random subset of its busy lanes.  A finished wave counts itself,
+// and the last one clears the shared counter, as on the device.  Returns 0
+// when every item in [0, n) was started exactly once and the counter ends at
+// zero; *takes = chunks taken from the counter (0 when the launch keeps
+// static shares).
+extern "C" int host_chunk_schedule(uint64_t n, uint32_t waves, int counters, uint32_t refill_min, uint64_t seed,
+                                   uint64_t* takes) {
+  struct Wave {
+    ChunkCursor cc;
+    uint64_t busy;
+    bool done;
+  };
+  uint32_t counter = 0, finished = 0;
+  uint64_t ntake = 0;
+  auto take = [&]() {
+    ++ntake;
+    return counter++;
+  };
+  uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+  auto rnd = [&]() {
+    x ^= x << 13;
+    x ^= x >> 7;
+    x ^= x << 17;
+    return x;
+  };
+  std::vector<Wave> ws(waves);
+  for (uint32_t w = 0; w < waves; ++w) {
+    ws[w].cc.init(w, waves, n, counters != 0, take);
+    ws[w].busy = 0;
+    ws[w].done = false;
+  }
+  std::vector<uint8_t> seen(n, 0);
+  uint32_t left = waves;
+  while (left) {
+    Wave& v = ws[rnd() % waves];
+    if (v.done) continue;
+    const uint64_t idle = ~v.busy;
+    const uint32_t nidle = (uint32_t)__builtin_popcountll(idle);
+    const bool more = v.cc.more(n);
+    if (!more && nidle == 64) {
+      v.done = true;
+      --left;
+      if (v.cc.dyn && ++finished == waves) counter = finished = 0;
+      continue;
+    }
+    if (more && nidle >= refill_min) {
+      const uint32_t cb = v.cc.refill_begin(nidle, take);
+      for (int l = 0; l < 64; ++l) {
+        if (v.busy >> l & 1) continue;
+        const uint64_t below = l == 0 ? 0ull : ((~0ull) >> (64 - l));
+        const uint64_t it = v.cc.item(v.cc.off + (uint32_t)__builtin_popcountll(idle & below), cb);
+        if (it >= n) continue;
+        if (seen[it]++) return -1;
+        v.busy |= 1ull << l;
+      }
+      v.cc.refill_end(nidle, cb, take);
+    } else {
+      v.busy &= rnd();  // some lanes finish their rays
+    }
+  }
+  for (uint64_t i = 0; i < n; ++i)
+    if (seen[i] != 1) return -2;
+  if (counter != 0) return -3;
+  *takes = ntake;
+  return 0;
+}
 
 extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any, int near, int lds_stack, int slots,
                              float* t0, uint32_t* p0, uint32_t* i0, float* t1, uint32_t* p1, uint32_t* i1) {

@@ -44,6 +44,9 @@ def _helper():
     h = C.CDLL(OUT)
     h.host_traverse.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64] + [C.c_int] * 4 + [C.c_void_p] * 6
     h.host_traverse.restype = C.c_int
+    h.host_chunk_schedule.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, C.c_uint64,
+                                      C.POINTER(C.c_uint64)]
+    h.host_chunk_schedule.restype = C.c_int
     return h
 
 
@@ -168,3 +171,26 @@ def test_duplicate_geometry_ties(ar, orc, wl):
                       materials=m.materials, slots=m.slots, normals=None, transform=m.transform)
     desc = wl.SceneDesc(materials=base.materials, meshes=[dup, dup], lights=base.lights, camera=base.camera)
     _check(ar, orc, desc, wl.soup_rays(1 << 14, seed=13, image=128))
+
+
+@pytest.mark.parametrize("waves", [1, 3, 16, 96])
+def test_chunk_schedule_every_item_once(waves):
+    """The walker's chunk bookkeeping (ChunkCursor, the code walk_items runs)
+    under random interleavings of many waves: every item is started exactly
+    once, with static shares and with the shared counter, at sizes around the
+    4-chunks-per-wave threshold and ragged ends, for refill thresholds from 1
+    to 64; the counter ends at zero (the last wave clears it), and it is used
+    exactly when the launch has counters and at least 4 chunks per wave."""
+    h = _helper()
+    c = 4 * 64 * waves
+    sizes = [0, 1, 63, 64, 65, 64 * waves - 1, 64 * waves + 7, c - 1, c, c + 1, 3 * c + 29, 9 * c + 64]
+    for n in sizes:
+        for counters in (0, 1):
+            for refill_min in (1, 8, 64):
+                for seed in (1, 2):
+                    takes = C.c_uint64()
+                    rc = h.host_chunk_schedule(n, waves, counters, refill_min, seed, C.byref(takes))
+                    assert rc == 0, (n, waves, counters, refill_min, seed, rc)
+                    dyn = counters and (n + 63) // 64 >= 4 * waves
+                    assert (takes.value > 0) == bool(dyn), (n, waves, counters, takes.value)
+
