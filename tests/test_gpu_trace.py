@@ -197,8 +197,10 @@ def test_every_launch_variant_same_answer(ar, wl):
             assert np.array_equal(t.view(np.uint32), ref_t.view(np.uint32)) and np.array_equal(p, ref_p), \
                 (b, s, pc, rf, sm, ps)
             assert np.array_equal(sc.occluded(rays), ref_o)
+    bal = sc.trace_config()["balance"]
     cfg = sc.autotune(torch.from_numpy(rays).cuda())
     assert cfg["ms"] > 0 and cfg["workgroup_size"] in (64, 128, 256)
+    assert cfg["balance"] == bal and cfg["anyhit"]["balance"] == bal  # the ray sample does not tune it
     assert cfg["refill_min"] in (8, 24, 48) and cfg["park_slots"] in (3, 4)
     # any hit keeps its own (separately tuned) launch configuration
     assert cfg["anyhit"]["ms"] > 0 and cfg["anyhit"]["workgroup_size"] in (64, 128, 256)
@@ -233,6 +235,8 @@ def test_balanced_launches_same_answer(ar, wl):
     rays = wl.soup_rays(1 << 20, seed=64, image=1024)
     sc = ar.Scene.from_desc(desc)
     assert sc.trace_config()["balance"] == 1  # 2^16 triangles: on by default
+    small = ar.Scene.from_desc(wl.cornell_box())  # 36 triangles: off by default
+    assert small.trace_config()["balance"] == 0 and small.anyhit_config()["balance"] == 0
     sc.set_trace_config(64, 8, 4, 8, 8, 3, 0)
     ref_t, ref_p, ref_i = sc.intersect(rays)
     ref_o = sc.occluded(rays)
