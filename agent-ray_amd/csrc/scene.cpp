@@ -3,6 +3,7 @@
 // SAH (accel/bvh.rs:317-410), flattens them into the wide-node layout and
 // uploads everything once.  The committed scene stays resident in HBM.
 #include <algorithm>
+#include <array>
 #include <cmath>
 
 #include "ar_internal.h"
@@ -227,29 +228,120 @@ static QNode quantize_node(const HBox& nb, const std::vector<const HBox*>& cb, c
   return q;
 }
 
-// Collapses a binary tree into 4-wide nodes: each node takes its binary
-// children and repeatedly opens the internal slot of largest surface area
-// (keeping left-to-right order) until it has four slots or only leaves.
-// Leaves keep their ranges (ref ~(prim_base + start)), so leaf positions and
-// the Tri records are shared with the binary form.  With prim_boxes (bottom
-// level trees), a leaf of two or more triangles becomes a 4-wide node whose
-// children are its triangles, each with its own quantized box (ref
-// ~(record | kSingleTri)): a ray then runs the full triangle test only for the
-// triangles whose box it enters.  Nodes are stored in preorder.
+// Collapses a binary tree into 4-wide nodes, choosing each node's slots by
+// a surface-area cost model solved exactly over the binary tree (dynamic
+// programming, bottom up).  A ray is taken to visit a 4-wide node with
+// probability proportional to its box's area.  Every triangle sits in a
+// slot of its own box, wherever the collapse puts it, so its expected tests
+// are the same for every collapse, and the cost to minimise is the total
+// area of the 4-wide nodes.  F(m, k) is the least such area in the subtree
+// of binary node m placed in at most k slots of its parent:
+//   * an internal node in one slot is a 4-wide node: its area plus its
+//     children's best split of four slots, min over j of
+//     F(left, j) + F(right, 4 - j);
+//   * with k >= 2 slots it may instead be opened: min over j of
+//     F(left, j) + F(right, k - j);
+//   * a leaf of one triangle is a triangle slot (0);
+//   * a bottom-level leaf of two to four triangles in one slot is a 4-wide
+//     node whose children are its triangles (ref ~(record | kSingleTri)),
+//     and with as many slots as triangles its triangles sit directly in the
+//     parent's slots instead (0);
+//   * a top-level leaf (instances) is a leaf slot (0).
+// Against opening the slot of largest area until four slots are filled, on
+// a 2^20-triangle soup: 30.7 instead of 32.6 node visits per ray, closest
+// hit 5.73 instead of 6.08 ms per 2^24 rays.
+// Slots keep the binary tree's left-to-right order.  Leaves keep their
+// ranges (ref ~(prim_base + start)), so leaf positions and the Tri records
+// are shared with the binary form, and the answers do not depend on the
+// collapse (near-first ties are broken by leaf position).  Nodes are stored
+// in preorder.
 struct QFlat {
   int root_ref;
   uint32_t depth;
 };
-static int qbvh_rec(const BinTree& t, uint32_t i, uint32_t depth, std::vector<QNode>& out, uint32_t prim_base,
-                    QFlat& f, const std::vector<HBox>* prim_boxes) {
+struct QPlan {
+  const BinTree& t;
+  const std::vector<HBox>* prim_boxes;
+  // per binary node: F(m, k) for k = 1..4, and the choice that reaches it:
+  // 0 keep as one slot, j in 1..3 open with j slots for the left child,
+  // 4 the leaf's triangles in slots of their own
+  std::vector<std::array<float, 5>> F;
+  std::vector<std::array<uint8_t, 5>> choice;
+  std::vector<uint8_t> node_split;  // j of the 4-wide node made of m (internal m)
+
+  QPlan(const BinTree& tree, const std::vector<HBox>* pb) : t(tree), prim_boxes(pb) {
+    const size_t n = t.nodes.size();
+    F.resize(n);
+    choice.resize(n);
+    node_split.assign(n, 1);
+    // preorder: children come after their parent
+    for (size_t ii = n; ii-- > 0;) solve((uint32_t)ii);
+  }
+  void solve(uint32_t m) {
+    const BinNode& nd = t.nodes[m];
+    const float A = box_area(nd.box);
+    auto& f = F[m];
+    auto& c = choice[m];
+    if (nd.count != 0) {
+      const bool node = prim_boxes && nd.count >= 2;  // one slot: a node of triangle slots
+      for (int k = 1; k <= 4; ++k) {
+        const bool inline_tris = node && (uint32_t)k >= nd.count;
+        f[k] = node && !inline_tris ? A : 0.0f;
+        c[k] = inline_tris ? 4 : 0;
+      }
+      return;
+    }
+    const auto& L = F[nd.left_or_start];
+    const auto& R = F[nd.right_or_end];
+    float best = INFINITY;
+    for (int j = 1; j <= 3; ++j)
+      if (L[j] + R[4 - j] < best) {
+        best = L[j] + R[4 - j];
+        node_split[m] = (uint8_t)j;
+      }
+    f[1] = A + best;
+    c[1] = 0;
+    for (int k = 2; k <= 4; ++k) {
+      f[k] = f[1];
+      c[k] = 0;
+      for (int j = 1; j < k; ++j)
+        if (L[j] + R[k - j] < f[k]) {
+          f[k] = L[j] + R[k - j];
+          c[k] = (uint8_t)j;
+        }
+    }
+  }
+  // The slots binary node m occupies when given k of them.
+  struct Slot {
+    uint32_t node;
+    int tri;  // >= 0: the leaf's triangle `tri` in a slot of its own
+  };
+  void slots(uint32_t m, int k, std::vector<Slot>& out) const {
+    const BinNode& nd = t.nodes[m];
+    const uint8_t ch = choice[m][k];
+    if (ch == 0) {
+      out.push_back({m, -1});
+    } else if (ch == 4) {
+      for (uint32_t j = 0; j < nd.count; ++j) out.push_back({m, (int)j});
+    } else {
+      slots(nd.left_or_start, ch, out);
+      slots(nd.right_or_end, k - ch, out);
+    }
+  }
+};
+
+static int qbvh_rec(const QPlan& plan, uint32_t i, uint32_t depth, std::vector<QNode>& out, uint32_t prim_base,
+                    QFlat& f) {
+  const BinTree& t = plan.t;
+  const std::vector<HBox>* prim_boxes = plan.prim_boxes;
   const BinNode& n = t.nodes[i];
   if (n.count == 1 || (n.count != 0 && !prim_boxes)) return ~(int)(prim_base + n.left_or_start);
+  f.depth = std::max(f.depth, depth + 1);
+  const uint32_t me = (uint32_t)out.size();
+  out.emplace_back();
+  int refs[4] = {kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef};
+  std::vector<const HBox*> boxes;
   if (n.count != 0) {  // a leaf of 2..4 triangles: one node with a slot per triangle
-    f.depth = std::max(f.depth, depth + 1);
-    uint32_t me = (uint32_t)out.size();
-    out.emplace_back();
-    int refs[4] = {kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef};
-    std::vector<const HBox*> boxes;
     for (uint32_t j = 0; j < n.count; ++j) {
       refs[j] = ~(int)((prim_base + n.left_or_start + j) | kSingleTri);
       boxes.push_back(&(*prim_boxes)[t.prims[n.left_or_start + j]]);
@@ -257,33 +349,19 @@ static int qbvh_rec(const BinTree& t, uint32_t i, uint32_t depth, std::vector<QN
     out[me] = quantize_node(n.box, boxes, refs);
     return (int)me;
   }
-  f.depth = std::max(f.depth, depth + 1);
-  std::vector<uint32_t> slots = {n.left_or_start, n.right_or_end};
-  while (slots.size() < 4) {
-    int best = -1;
-    float best_a = -1.0f;
-    for (size_t k = 0; k < slots.size(); ++k) {
-      const BinNode& c = t.nodes[slots[k]];
-      if (c.count != 0) continue;
-      float a = box_area(c.box);
-      if (a > best_a) {
-        best_a = a;
-        best = (int)k;
-      }
+  std::vector<QPlan::Slot> sl;
+  const int j = plan.node_split[i];
+  plan.slots(n.left_or_start, j, sl);
+  plan.slots(n.right_or_end, 4 - j, sl);
+  for (size_t k = 0; k < sl.size(); ++k) {
+    const BinNode& c = t.nodes[sl[k].node];
+    if (sl[k].tri >= 0) {
+      refs[k] = ~(int)((prim_base + c.left_or_start + (uint32_t)sl[k].tri) | kSingleTri);
+      boxes.push_back(&(*prim_boxes)[t.prims[c.left_or_start + (uint32_t)sl[k].tri]]);
+    } else {
+      refs[k] = qbvh_rec(plan, sl[k].node, depth + 1, out, prim_base, f);
+      boxes.push_back(&c.box);
     }
-    if (best < 0) break;
-    const BinNode& c = t.nodes[slots[best]];
-    uint32_t l = c.left_or_start, r = c.right_or_end;
-    slots[best] = l;
-    slots.insert(slots.begin() + best + 1, r);
-  }
-  uint32_t me = (uint32_t)out.size();
-  out.emplace_back();
-  int refs[4] = {kEmptyRef, kEmptyRef, kEmptyRef, kEmptyRef};
-  std::vector<const HBox*> boxes;
-  for (size_t k = 0; k < slots.size(); ++k) {
-    refs[k] = qbvh_rec(t, slots[k], depth + 1, out, prim_base, f, prim_boxes);
-    boxes.push_back(&t.nodes[slots[k]].box);
   }
   out[me] = quantize_node(n.box, boxes, refs);
   return (int)me;
@@ -291,7 +369,8 @@ static int qbvh_rec(const BinTree& t, uint32_t i, uint32_t depth, std::vector<QN
 static QFlat build_qbvh(const BinTree& t, std::vector<QNode>& out, uint32_t prim_base,
                         const std::vector<HBox>* prim_boxes) {
   QFlat f{0, 0};
-  f.root_ref = qbvh_rec(t, 0, 0, out, prim_base, f, prim_boxes);
+  const QPlan plan(t, prim_boxes);
+  f.root_ref = qbvh_rec(plan, 0, 0, out, prim_base, f);
   return f;
 }
 
