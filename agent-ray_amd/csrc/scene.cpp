@@ -242,6 +242,7 @@ static QNode quantize_node(const HBox& nb, const std::vector<const HBox*>& cb, c
 //   * with k >= 2 slots it may instead be opened: min over j of
 //     F(left, j) + F(right, k - j);
 //   * a leaf of one triangle is a triangle slot (0);
+//   * a bottom-level leaf of more than four triangles is a leaf slot (0);
 //   * a bottom-level leaf of two to four triangles in one slot is a 4-wide
 //     node whose children are its triangles (ref ~(record | kSingleTri)),
 //     and with as many slots as triangles its triangles sit directly in the
@@ -283,7 +284,9 @@ struct QPlan {
     auto& f = F[m];
     auto& c = choice[m];
     if (nd.count != 0) {
-      const bool node = prim_boxes && nd.count >= 2;  // one slot: a node of triangle slots
+      // one slot: a node of triangle slots (two to four triangles); a leaf of
+      // more (the SAH leaf rule allows them) stays a range of Tri records
+      const bool node = prim_boxes && nd.count >= 2 && nd.count <= 4;
       for (int k = 1; k <= 4; ++k) {
         const bool inline_tris = node && (uint32_t)k >= nd.count;
         f[k] = node && !inline_tris ? A : 0.0f;
@@ -335,7 +338,7 @@ static int qbvh_rec(const QPlan& plan, uint32_t i, uint32_t depth, std::vector<Q
   const BinTree& t = plan.t;
   const std::vector<HBox>* prim_boxes = plan.prim_boxes;
   const BinNode& n = t.nodes[i];
-  if (n.count == 1 || (n.count != 0 && !prim_boxes)) return ~(int)(prim_base + n.left_or_start);
+  if (n.count == 1 || (n.count != 0 && (!prim_boxes || n.count > 4))) return ~(int)(prim_base + n.left_or_start);
   f.depth = std::max(f.depth, depth + 1);
   const uint32_t me = (uint32_t)out.size();
   out.emplace_back();

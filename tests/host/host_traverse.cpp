@@ -204,3 +204,22 @@ extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any
   if (near) return run<false, true>(sv, rays, n, lds_stack, 3, (uint32_t)slots, t0, p0, i0, t1, p1, i1);
   return run<false, false>(sv, rays, n, lds_stack, 3, (uint32_t)slots, t0, p0, i0, t1, p1, i1);
 }
+
+// sizes[o] = the largest bottom-level leaf of object o (Tri records between
+// two leaf-end tags, in the binary trees' leaf order).
+extern "C" int host_max_leaf(ar_scene* s, uint32_t* sizes) {
+  HostAccel h;
+  if (build_host_accel(s, h) != AR_OK) return -10;
+  for (size_t o = 0; o < h.objs.size(); ++o) {
+    uint32_t run = 0, best = 0;
+    for (uint32_t r = 0; r < h.objs[o].n_tris; ++r) {
+      ++run;
+      if (bits_u(h.tris[h.objs[o].tri_base + r].v0.w) & kLastBit) {
+        best = run > best ? run : best;
+        run = 0;
+      }
+    }
+    sizes[o] = best;
+  }
+  return 0;
+}

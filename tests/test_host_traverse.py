@@ -47,6 +47,8 @@ def _helper():
     h.host_chunk_schedule.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, C.c_uint64,
                                       C.POINTER(C.c_uint64)]
     h.host_chunk_schedule.restype = C.c_int
+    h.host_max_leaf.argtypes = [C.c_void_p, C.c_void_p]
+    h.host_max_leaf.restype = C.c_int
     return h
 
 
@@ -171,6 +173,34 @@ def test_duplicate_geometry_ties(ar, orc, wl):
                       materials=m.materials, slots=m.slots, normals=None, transform=m.transform)
     desc = wl.SceneDesc(materials=base.materials, meshes=[dup, dup], lights=base.lights, camera=base.camera)
     _check(ar, orc, desc, wl.soup_rays(1 << 14, seed=13, image=128))
+
+
+def test_leaves_of_more_than_four_triangles(ar, orc, wl):
+    """The SAH leaf rule keeps a leaf of n > 4 triangles when no split costs
+    less than n (here: seven coincident copies of each triangle, where every
+    split costs 1 + n) or the node's area is below f32::EPSILON (a cluster of
+    point triangles).  The 4-wide collapse must keep such a leaf as a range
+    of Tri records, not as a node with a slot per triangle."""
+    base = wl.triangle_soup(1 << 9, seed=14)
+    m = base.meshes[0]
+    idx = m.indices.reshape(-1, 3)
+    sev = np.repeat(idx, 7, axis=0).astype(np.uint32)
+    pts = np.repeat(np.float32([[0.25, 0.5, 0.5], [0.75, 0.5, 0.5]]), 3, axis=0)
+    pidx = np.tile(np.arange(6, dtype=np.uint32).reshape(2, 3), (6, 1))  # 6 point triangles at each of 2 points
+    meshes = [wl.MeshDesc(positions=m.positions, indices=sev, materials=m.materials, slots=m.slots, normals=None,
+                          transform=m.transform),
+              wl.MeshDesc(positions=pts, indices=pidx, materials=m.materials, slots=None, normals=None,
+                          transform=None)]
+    desc = wl.SceneDesc(materials=base.materials, meshes=meshes, lights=base.lights, camera=base.camera)
+    h = _helper()
+    s = _scene(ar, desc)
+    try:
+        sizes = np.zeros(2, np.uint32)
+        assert h.host_max_leaf(s, sizes.ctypes.data) == 0
+        assert sizes[0] >= 7 and sizes[1] >= 5, sizes  # the scene really has such leaves
+    finally:
+        ar.lib().ar_scene_destroy(s)
+    _check(ar, orc, desc, wl.soup_rays(1 << 13, seed=15, image=64))
 
 
 @pytest.mark.parametrize("waves", [1, 3, 16, 96])
