@@ -339,15 +339,16 @@ class Scene:
     reference's SAH, uploads the flattened scene to HBM and keeps it there."""
 
     def __init__(self, objects: List[SceneObject], lights: List[object], light_dist: str = "power",
-                 obj_files: Sequence = (), builder: str = "sah"):
+                 obj_files: Sequence = (), builder: str = "split"):
         """`obj_files`: (path, transform or None) pairs loaded with
         load_obj_scene (importer.rs:167-241) after `objects`.  `builder`:
-        "sah" (the reference's host SAH) or "lbvh" (device linear BVH) for the
-        meshes' trees."""
+        "sah" (the reference's host SAH), "lbvh" (device linear BVH) or "split"
+        (the reference's SAH for reference order, spatial-split SAH for the
+        near-first trees) for the meshes' trees."""
         init()
         self._h = C.c_void_p()
         _check(_lib.ar_scene_create(C.byref(self._h)))
-        _check(_lib.ar_scene_set_builder(self._h, {"sah": 0, "lbvh": 1}[builder]))
+        _check(_lib.ar_scene_set_builder(self._h, {"sah": 0, "lbvh": 1, "split": 2}[builder]))
         self._keep = []
         mat_ids, tex_ids = {}, {}
         for obj in objects:
@@ -399,7 +400,7 @@ class Scene:
             _check(_lib.ar_scene_set_mesh_texcoords(self._h, oid.value, _ptr(uv), uv.shape[0]))
 
     @classmethod
-    def from_desc(cls, desc: SceneDesc, builder: str = "sah") -> "Scene":
+    def from_desc(cls, desc: SceneDesc, builder: str = "split") -> "Scene":
         textures = [ImageTexture(t) for t in getattr(desc, "textures", [])]
         mats = [_mat_from_desc(m, textures) for m in desc.materials]
         objs = []

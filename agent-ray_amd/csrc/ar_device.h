@@ -111,7 +111,8 @@ static_assert(sizeof(Inst) == 96, "instance layout");
 struct SceneView {
   const WNode* nodes;   // binary trees (reference order)
   const QNode* qnodes;  // 4-wide trees (near-first order)
-  const Tri* tris;
+  const Tri* tris;       // binary trees' leaf order; hits name these records
+  const Tri* qtris;      // 4-wide trees' leaf order (the same array as tris unless spatial splits)
   const Inst* insts;
   int tlas_root;        // TLAS root ref (node index or ~first instance record)
   int qtlas_root;       // the same in the 4-wide node array

@@ -76,6 +76,15 @@ BinTree build_sah(const std::vector<HBox>& boxes, int nthreads);
 // Linear BVH built on the device (Morton order, Karras hierarchy), the same
 // form as build_sah's output with leaves of up to four primitives.
 ar_status build_lbvh(const std::vector<HBox>& boxes, BinTree& out);
+// SAH tree with spatial splits over triangle references, for near-first
+// traversal only: a triangle may be referenced from several leaves, each
+// time with the box of its part inside that leaf's region.  `pos` / `idx`
+// are the mesh (3 floats per vertex, 3 indices per triangle), `boxes` the
+// triangles' boxes.  Leaf position p of the result holds triangle
+// ref_tri[p] with box ref_box[p]; `prims` is the identity.  Spatial splits
+// stop once the references reach max_refs.
+BinTree build_split_sah(const std::vector<HBox>& boxes, const float* pos, const uint32_t* idx, size_t max_refs,
+                        std::vector<uint32_t>& ref_tri, std::vector<HBox>& ref_box);
 
 // ---- materials / lights as stored on the device -----------------------------
 struct alignas(16) DMaterial {
@@ -145,7 +154,7 @@ struct ar_scene {
   std::vector<uint4> tex_info;   // per texture: (first texel, width, height, -)
   std::vector<ar_light> lights;
   uint32_t dist_mode = AR_LIGHTDIST_POWER;
-  uint32_t builder = AR_BUILDER_SAH;  // bottom-level tree builder
+  uint32_t builder = AR_BUILDER_SAH_SPLIT;  // bottom-level tree builder
   bool committed = false;
   // host copies of what was uploaded (used by tests through the info call)
   ar_scene_info info{};
@@ -161,7 +170,7 @@ struct ar_scene {
   ar_trace_config tcfg{256, 12, 8, 8, 8, 3, 1, 0.0f};      // closest-hit launch configuration (autotuned)
   ar_trace_config tcfg_any{256, 12, 8, 8, 8, 3, 1, 0.0f};  // any-hit (occlusion, shadow rays), tuned separately
   bool tcfg_set = false;  // a configuration was set or tuned
-  ar::DevBuf d_nodes, d_qnodes, d_tris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
+  ar::DevBuf d_nodes, d_qnodes, d_tris, d_qtris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
       d_tri_obj, d_stri, d_suv, d_tex, d_texinfo;
   ar::SceneView view{};
   uint32_t n_lights = 0;
@@ -186,6 +195,10 @@ struct HostAccel {
   std::vector<WNode> nodes;
   std::vector<QNode> qnodes;
   std::vector<Tri> tris;
+  // Tri records of the 4-wide trees when they are built with spatial splits
+  // (AR_BUILDER_SAH_SPLIT): leaf order of those trees, a triangle possibly
+  // more than once; empty otherwise (the 4-wide trees then use `tris`)
+  std::vector<Tri> qtris;
   std::vector<Inst> insts;
   std::vector<DObject> objs;
   std::vector<uint32_t> gidx, gslots, gmats, tri_obj;

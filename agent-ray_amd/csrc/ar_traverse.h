@@ -446,15 +446,15 @@ struct Walker {
     }
   }
 
-  // Tests the triangles of BLAS leaf `ref`.  Any hit: true at the first hit
-  // in range (the query is then done).
-  // One triangle (Tri record k).  Returns true when an any-hit query is
-  // answered.
+  // One triangle: record k of the level's tree form (4-wide trees: qtris,
+  // binary: tris).  Hits and near-first ties use the record's key (e1.w),
+  // its index in tris.  Returns true when an any-hit query is answered.
   AR_HD bool tri(const SceneView& sv, Stack& stk, uint32_t k, uint32_t& tag) {
     AR_COUNT_TRI();
-    const Tri* tp = sv.tris + k;
+    const Tri* tp = (NEAR && wide ? sv.qtris : sv.tris) + k;
     float4 v0 = tp->v0, e1 = tp->e1, e2 = tp->e2;
     tag = bits_u(v0.w);
+    if (NEAR) k = bits_u(e1.w);
     float th, uh, vh;
     if (tri_hit(mk(v0.x, v0.y, v0.z), mk(e1.x, e1.y, e1.z), mk(e2.x, e2.y, e2.z), o, d, tmin, tcull(), &th, &uh,
                 &vh)) {

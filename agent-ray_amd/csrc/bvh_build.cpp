@@ -193,6 +193,451 @@ BinTree build_sah(const std::vector<HBox>& boxes, int nthreads) {
   return t;
 }
 
+// ---- spatial-split SAH (near-first trees only) ------------------------------
+// The object-split SAH puts every triangle in exactly one leaf, so a large or
+// slanted triangle inflates the box of every node above it.  Here a node may
+// instead be split by a plane: the triangles crossing it are referenced from
+// both sides, each with the box of its part on that side (Stich, Friedrich
+// and Dietrich, "Spatial Splits in Bounding Volume Hierarchies", HPG 2009).
+// Per node the cheaper of
+//   * the best object split (full sweep over the references' box centroids
+//     up to 4096 references, 256 centroid bins above), and
+//   * the best of 32 planes per axis, tried when the object split's children
+//     overlap by more than 1e-5 of the root's area,
+// is taken when it is cheaper than a leaf, with the SAH constants and leaf
+// rule of the reference's builder (C_trav = C_isect = 1; n <= 4, a node area
+// below f32::EPSILON, or no cheaper split makes a leaf).  A reference that
+// straddles the chosen plane goes to one side whole when that is cheaper
+// ("reference unsplitting").
+//
+// Boxes stay conservative: clip points are computed in double and rounded
+// outward, and every reference box is widened by 2^-16 of its triangle's
+// extent.  The device tests the triangle (v0, v0 + fl(v1 - v0), v0 +
+// fl(v2 - v0)), within 2^-24 of its extent of the exact one, so a point it
+// can report as a hit lies inside the box of a reference whose region holds
+// it, however small that region is against the triangle.
+namespace {
+
+struct SRef {
+  uint32_t tri;
+  HBox box;
+};
+
+inline bool box_ok(const HBox& b) {
+  return b.lo[0] <= b.hi[0] && b.lo[1] <= b.hi[1] && b.lo[2] <= b.hi[2];
+}
+inline float down_f(double d) {
+  float f = (float)d;
+  return (double)f > d ? std::nextafter(f, -INFINITY) : f;
+}
+inline float up_f(double d) {
+  float f = (float)d;
+  return (double)f < d ? std::nextafter(f, INFINITY) : f;
+}
+inline float overlap_area(const HBox& a, const HBox& b) {
+  HBox c;
+  for (int k = 0; k < 3; ++k) {
+    c.lo[k] = sse_max(a.lo[k], b.lo[k]);
+    c.hi[k] = sse_min(a.hi[k], b.hi[k]);
+  }
+  return box_ok(c) ? surface_area(c) : 0.0f;
+}
+
+struct SplitBuild {
+  const float* pos;
+  const uint32_t* idx;
+  std::vector<float> margin;  // per triangle: 2^-16 of its extent
+  float min_overlap = 0.0f;   // area of overlap that makes spatial splits worth trying
+
+  static constexpr uint32_t kSweepMax = 4096, kObjBins = 256, kSpatialBins = 32;
+
+  struct Sub {
+    std::vector<BinNode> nodes;
+    std::vector<SRef> refs;
+    uint32_t depth = 1;
+  };
+
+  // Parts of reference r's triangle inside r.box on each side of the plane
+  // x_axis = p; a side the part does not reach comes back empty (!box_ok).
+  void clip(const SRef& r, int axis, float p, HBox& L, HBox& R) const {
+    double v[3][3];
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 3; ++k) v[c][k] = pos[3 * (size_t)idx[3 * (size_t)r.tri + c] + k];
+    double llo[3], lhi[3], rlo[3], rhi[3];
+    for (int k = 0; k < 3; ++k) {
+      llo[k] = rlo[k] = INFINITY;
+      lhi[k] = rhi[k] = -INFINITY;
+    }
+    auto add = [](double* lo, double* hi, const double* q) {
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], q[k]);
+        hi[k] = std::max(hi[k], q[k]);
+      }
+    };
+    const double pd = p;
+    for (int e = 0; e < 3; ++e) {
+      const double* a = v[e];
+      const double* b = v[(e + 1) % 3];
+      if (a[axis] <= pd) add(llo, lhi, a);
+      if (a[axis] >= pd) add(rlo, rhi, a);
+      if ((a[axis] < pd && b[axis] > pd) || (a[axis] > pd && b[axis] < pd)) {
+        const double t = (pd - a[axis]) / (b[axis] - a[axis]);
+        double q[3];
+        for (int k = 0; k < 3; ++k) q[k] = a[k] + t * (b[k] - a[k]);
+        q[axis] = pd;
+        add(llo, lhi, q);
+        add(rlo, rhi, q);
+      }
+    }
+    const float m = margin[r.tri];
+    for (int k = 0; k < 3; ++k) {
+      L.lo[k] = sse_max(down_f(llo[k]) - m, r.box.lo[k]);
+      L.hi[k] = sse_min(up_f(lhi[k]) + m, r.box.hi[k]);
+      R.lo[k] = sse_max(down_f(rlo[k]) - m, r.box.lo[k]);
+      R.hi[k] = sse_min(up_f(rhi[k]) + m, r.box.hi[k]);
+    }
+  }
+
+  static float centroid(const SRef& r, int a) { return (r.box.lo[a] + r.box.hi[a]) * 0.5f; }
+
+  struct ObjSplit {
+    float cost = INFINITY;
+    int axis = -1;
+    uint32_t k = 0;       // sweep: first k of the axis order go left
+    float cmin = 0, cinv = 0;  // binned: bin of a centroid
+    uint32_t bin = 0;     // binned: bins < bin go left
+    bool binned = false;
+    HBox lb, rb;
+  };
+
+  uint32_t obj_bin(const ObjSplit& s, float c) const {
+    float f = (c - s.cmin) * s.cinv;
+    int b = (int)f;
+    return (uint32_t)std::min(std::max(b, 0), (int)kObjBins - 1);
+  }
+
+  ObjSplit object_split(const std::vector<SRef>& refs, float parent_sa) const {
+    ObjSplit best;
+    const uint32_t n = (uint32_t)refs.size();
+    if (n <= kSweepMax) {
+      std::vector<uint32_t> ord(n);
+      std::vector<float> suf(n);
+      for (int axis = 0; axis < 3; ++axis) {
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+          const float cx = centroid(refs[x], axis), cy = centroid(refs[y], axis);
+          return cx < cy || (cx == cy && x < y);
+        });
+        HBox acc = empty_box();
+        for (uint32_t i = n; i-- > 0;) {
+          merge_box(acc, refs[ord[i]].box);
+          suf[i] = surface_area(acc);
+        }
+        HBox pre = empty_box();
+        for (uint32_t k = 1; k < n; ++k) {
+          merge_box(pre, refs[ord[k - 1]].box);
+          const float cost = 1.0f + (surface_area(pre) * (float)k + suf[k] * (float)(n - k)) / parent_sa;
+          if (cost < best.cost) {
+            best.cost = cost;
+            best.axis = axis;
+            best.k = k;
+          }
+        }
+      }
+      if (best.axis >= 0) {  // the two children's boxes
+        const int axis = best.axis;
+        std::iota(ord.begin(), ord.end(), 0u);
+        std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+          const float cx = centroid(refs[x], axis), cy = centroid(refs[y], axis);
+          return cx < cy || (cx == cy && x < y);
+        });
+        best.lb = best.rb = empty_box();
+        for (uint32_t i = 0; i < n; ++i) merge_box(i < best.k ? best.lb : best.rb, refs[ord[i]].box);
+      }
+      return best;
+    }
+    HBox cb = empty_box();
+    for (const SRef& r : refs)
+      for (int a = 0; a < 3; ++a) {
+        const float c = centroid(r, a);
+        cb.lo[a] = sse_min(cb.lo[a], c);
+        cb.hi[a] = sse_max(cb.hi[a], c);
+      }
+    for (int axis = 0; axis < 3; ++axis) {
+      const float ext = cb.hi[axis] - cb.lo[axis];
+      if (!(ext > 0.0f)) continue;
+      ObjSplit s;
+      s.binned = true;
+      s.axis = axis;
+      s.cmin = cb.lo[axis];
+      s.cinv = (float)kObjBins / ext;
+      HBox bb[kObjBins];
+      uint32_t cnt[kObjBins] = {};
+      for (auto& b : bb) b = empty_box();
+      for (const SRef& r : refs) {
+        const uint32_t b = obj_bin(s, centroid(r, axis));
+        merge_box(bb[b], r.box);
+        ++cnt[b];
+      }
+      float suf[kObjBins];
+      HBox acc = empty_box();
+      for (uint32_t i = kObjBins; i-- > 0;) {
+        merge_box(acc, bb[i]);
+        suf[i] = surface_area(acc);
+      }
+      HBox pre = empty_box();
+      uint32_t nl = 0;
+      for (uint32_t b = 1; b < kObjBins; ++b) {
+        merge_box(pre, bb[b - 1]);
+        nl += cnt[b - 1];
+        if (nl == 0 || nl == n) continue;
+        const float cost = 1.0f + (surface_area(pre) * (float)nl + suf[b] * (float)(n - nl)) / parent_sa;
+        if (cost < best.cost) {
+          best = s;
+          best.cost = cost;
+          best.bin = b;
+          best.lb = pre;
+          best.rb = empty_box();
+          for (uint32_t i = b; i < kObjBins; ++i) merge_box(best.rb, bb[i]);
+        }
+      }
+    }
+    return best;
+  }
+
+  struct SpatialSplit {
+    float cost = INFINITY;
+    int axis = -1;
+    float plane = 0;
+    HBox lb, rb;
+    uint32_t nl = 0, nr = 0;
+  };
+
+  SpatialSplit spatial_split(const std::vector<SRef>& refs, const HBox& box, float parent_sa) const {
+    SpatialSplit best;
+    constexpr uint32_t NB = kSpatialBins;
+    for (int axis = 0; axis < 3; ++axis) {
+      const float lo = box.lo[axis], ext = box.hi[axis] - lo;
+      if (!(ext > 0.0f)) continue;
+      float plane[NB + 1];
+      for (uint32_t i = 0; i <= NB; ++i) plane[i] = lo + ext * ((float)i / (float)NB);
+      auto bin_of = [&](float x) {
+        int b = (int)((x - lo) * ((float)NB / ext));
+        b = std::min(std::max(b, 0), (int)NB - 1);
+        return (uint32_t)b;
+      };
+      HBox bb[NB];
+      uint32_t enter[NB] = {}, leave[NB] = {};
+      for (auto& b : bb) b = empty_box();
+      for (const SRef& r : refs) {
+        const uint32_t b0 = bin_of(r.box.lo[axis]), b1 = bin_of(r.box.hi[axis]);
+        ++enter[b0];
+        ++leave[b1];
+        SRef cur = r;
+        uint32_t b = b0;
+        for (; b < b1; ++b) {
+          HBox L, R;
+          clip(cur, axis, plane[b + 1], L, R);
+          if (box_ok(L)) merge_box(bb[b], L);
+          if (!box_ok(R)) break;
+          cur.box = R;
+        }
+        if (b == b1) merge_box(bb[b1], cur.box);
+      }
+      HBox suf[NB];
+      uint32_t nsuf[NB];
+      HBox acc = empty_box();
+      uint32_t nacc = 0;
+      for (uint32_t i = NB; i-- > 0;) {
+        merge_box(acc, bb[i]);
+        nacc += leave[i];
+        suf[i] = acc;
+        nsuf[i] = nacc;
+      }
+      HBox pre = empty_box();
+      uint32_t nl = 0;
+      for (uint32_t i = 1; i < NB; ++i) {
+        merge_box(pre, bb[i - 1]);
+        nl += enter[i - 1];
+        const uint32_t nr = nsuf[i];
+        if (nl == 0 || nr == 0 || !box_ok(pre) || !box_ok(suf[i])) continue;
+        const float cost = 1.0f + (surface_area(pre) * (float)nl + surface_area(suf[i]) * (float)nr) / parent_sa;
+        if (cost < best.cost) {
+          best.cost = cost;
+          best.axis = axis;
+          best.plane = plane[i];
+          best.lb = pre;
+          best.rb = suf[i];
+          best.nl = nl;
+          best.nr = nr;
+        }
+      }
+    }
+    return best;
+  }
+
+  Sub leaf(const HBox& box, std::vector<SRef>&& refs) const {
+    Sub s;
+    BinNode nd;
+    nd.box = box;
+    nd.left_or_start = 0;
+    nd.right_or_end = 0;
+    nd.count = (uint32_t)refs.size();
+    s.nodes.push_back(nd);
+    s.refs = std::move(refs);
+    return s;
+  }
+
+  // `extra`: references this subtree may add by spatial splits.  What a node
+  // leaves unused is divided between its children in proportion to their
+  // references, so the tree does not depend on the order of the parallel
+  // subtree builds.
+  Sub build(std::vector<SRef>&& refs, uint32_t depth, size_t extra) {
+    const uint32_t n = (uint32_t)refs.size();
+    HBox box = empty_box();
+    for (const SRef& r : refs) merge_box(box, r.box);
+    const float parent_sa = surface_area(box);
+    if (n <= 4 || depth >= 64 || parent_sa < 1.1920929e-07f) return leaf(box, std::move(refs));
+    ObjSplit os = object_split(refs, parent_sa);
+    SpatialSplit ss;
+    if (extra > 0 && (os.axis < 0 || overlap_area(os.lb, os.rb) > min_overlap))
+      ss = spatial_split(refs, box, parent_sa);
+    if (!(std::min(os.cost, ss.cost) < (float)n)) return leaf(box, std::move(refs));
+    std::vector<SRef> L, R;
+    L.reserve(n / 2 + 1);
+    R.reserve(n / 2 + 1);
+    if (ss.cost < os.cost) {
+      const int a = ss.axis;
+      const float p = ss.plane;
+      const float al = surface_area(ss.lb), ar = surface_area(ss.rb);
+      const float nl = (float)ss.nl, nr = (float)ss.nr;
+      for (const SRef& r : refs) {
+        if (r.box.hi[a] <= p) {
+          L.push_back(r);
+        } else if (r.box.lo[a] >= p) {
+          R.push_back(r);
+        } else {
+          HBox lu = ss.lb, ru = ss.rb;
+          merge_box(lu, r.box);
+          merge_box(ru, r.box);
+          const float c_split = al * nl + ar * nr;
+          const float c_left = surface_area(lu) * nl + ar * (nr - 1.0f);
+          const float c_right = al * (nl - 1.0f) + surface_area(ru) * nr;
+          if (extra > 0 && c_split < c_left && c_split < c_right) {
+            HBox lb, rb;
+            clip(r, a, p, lb, rb);
+            const bool lk = box_ok(lb), rk = box_ok(rb);
+            if (lk) L.push_back({r.tri, lb});
+            if (rk) R.push_back({r.tri, rb});
+            if (!lk && !rk) L.push_back(r);  // not reached: r.box meets the triangle
+            if (lk && rk) --extra;
+          } else if (c_left <= c_right) {
+            L.push_back(r);
+          } else {
+            R.push_back(r);
+          }
+        }
+      }
+    } else if (os.binned) {
+      for (const SRef& r : refs) (obj_bin(os, centroid(r, os.axis)) < os.bin ? L : R).push_back(r);
+    } else {
+      const int axis = os.axis;
+      std::vector<uint32_t> ord(n);
+      std::iota(ord.begin(), ord.end(), 0u);
+      std::sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+        const float cx = centroid(refs[x], axis), cy = centroid(refs[y], axis);
+        return cx < cy || (cx == cy && x < y);
+      });
+      for (uint32_t i = 0; i < n; ++i) (i < os.k ? L : R).push_back(refs[ord[i]]);
+    }
+    if (L.empty() || R.empty()) return leaf(box, std::move(refs));
+    std::vector<SRef>().swap(refs);
+    const size_t extra_l = (size_t)((double)extra * (double)L.size() / (double)(L.size() + R.size()));
+    const size_t extra_r = extra - extra_l;
+    Sub A, B;
+    if (depth < 4 && n > 8192) {
+      auto fut = std::async(std::launch::async, [&]() { return build(std::move(R), depth + 1, extra_r); });
+      A = build(std::move(L), depth + 1, extra_l);
+      B = fut.get();
+    } else {
+      A = build(std::move(L), depth + 1, extra_l);
+      B = build(std::move(R), depth + 1, extra_r);
+    }
+    Sub out;
+    const uint32_t an = (uint32_t)A.nodes.size(), ap = (uint32_t)A.refs.size();
+    out.nodes.reserve(1 + A.nodes.size() + B.nodes.size());
+    BinNode root;
+    root.box = box;
+    root.left_or_start = 1;
+    root.right_or_end = 1 + an;
+    root.count = 0;
+    out.nodes.push_back(root);
+    for (auto nd : A.nodes) {
+      if (nd.count == 0) {
+        nd.left_or_start += 1;
+        nd.right_or_end += 1;
+      }
+      out.nodes.push_back(nd);
+    }
+    for (auto nd : B.nodes) {
+      if (nd.count == 0) {
+        nd.left_or_start += 1 + an;
+        nd.right_or_end += 1 + an;
+      } else {
+        nd.left_or_start += ap;
+      }
+      out.nodes.push_back(nd);
+    }
+    out.refs = std::move(A.refs);
+    out.refs.insert(out.refs.end(), B.refs.begin(), B.refs.end());
+    out.depth = 1 + std::max(A.depth, B.depth);
+    return out;
+  }
+};
+
+}  // namespace
+
+BinTree build_split_sah(const std::vector<HBox>& boxes, const float* pos, const uint32_t* idx, size_t max_refs,
+                        std::vector<uint32_t>& ref_tri, std::vector<HBox>& ref_box) {
+  BinTree t;
+  ref_tri.clear();
+  ref_box.clear();
+  const uint32_t n = (uint32_t)boxes.size();
+  if (n == 0) return t;
+  SplitBuild sb;
+  sb.pos = pos;
+  sb.idx = idx;
+  sb.margin.resize(n);
+  std::vector<SRef> refs(n);
+  HBox root = empty_box();
+  for (uint32_t i = 0; i < n; ++i) {
+    const HBox& b = boxes[i];
+    const float ext = sse_max(sse_max(b.hi[0] - b.lo[0], b.hi[1] - b.lo[1]), b.hi[2] - b.lo[2]);
+    const float m = ext * 0x1p-16f;
+    sb.margin[i] = m;
+    refs[i].tri = i;
+    for (int k = 0; k < 3; ++k) {
+      refs[i].box.lo[k] = b.lo[k] - m;
+      refs[i].box.hi[k] = b.hi[k] + m;
+    }
+    merge_box(root, refs[i].box);
+  }
+  sb.min_overlap = 1e-5f * surface_area(root);
+  SplitBuild::Sub s = sb.build(std::move(refs), 0, max_refs > n ? max_refs - n : 0);
+  t.nodes = std::move(s.nodes);
+  t.depth = s.depth;
+  const size_t r = s.refs.size();
+  ref_tri.resize(r);
+  ref_box.resize(r);
+  t.prims.resize(r);
+  for (size_t i = 0; i < r; ++i) {
+    ref_tri[i] = s.refs[i].tri;
+    ref_box[i] = s.refs[i].box;
+    t.prims[i] = (uint32_t)i;
+  }
+  return t;
+}
+
 }  // namespace ar
 
 extern "C" ar_status ar_bvh_build_host(const float* boxes, uint32_t n, float* out_boxes, uint32_t* out_meta,

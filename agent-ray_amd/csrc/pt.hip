@@ -326,7 +326,8 @@ uint64_t view_hash(const ShadeView& v) {
     for (int i = 0; i < 8; ++i) h = (h ^ ((x >> (8 * i)) & 255u)) * 1099511628211ull;
   };
   const SceneView& sv = v.sv;
-  for (const void* p : {(const void*)sv.nodes, (const void*)sv.qnodes, (const void*)sv.tris, (const void*)sv.insts,
+  for (const void* p : {(const void*)sv.nodes, (const void*)sv.qnodes, (const void*)sv.tris, (const void*)sv.qtris,
+                        (const void*)sv.insts,
                         (const void*)v.objects, (const void*)v.materials, (const void*)v.lights, (const void*)v.idx,
                         (const void*)v.nrm, (const void*)v.slots, (const void*)v.objmats, (const void*)v.stri,
                         (const void*)v.suv, (const void*)v.texels, (const void*)v.tex_info})

@@ -466,7 +466,7 @@ ar_status ar_scene_add_mesh(ar_scene* s, const float* positions, uint32_t nv, co
 ar_status ar_scene_set_builder(ar_scene* s, uint32_t builder) {
   if (!s) return fail(AR_ERR_INVALID_ARG, "ar_scene_set_builder: NULL scene");
   if (s->committed) return fail(AR_ERR_STATE, "scene already committed");
-  if (builder > AR_BUILDER_LBVH) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown tree builder");
+  if (builder > AR_BUILDER_SAH_SPLIT) return fail(AR_ERR_UNSUPPORTED_TYPE, "unknown tree builder");
   s->builder = builder;
   return AR_OK;
 }
@@ -604,8 +604,12 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
         uint32_t tag = prim | (k + 1 == n.count ? kLastBit : 0u);
         float tagf;
         memcpy(&tagf, &tag, 4);
+        // e1.w: the record's own index, the key near-first ties and hits use
+        const uint32_t key = tri_base + n.left_or_start + k;
+        float keyf;
+        memcpy(&keyf, &key, 4);
         tr.v0 = make_float4(v0[0], v0[1], v0[2], tagf);
-        tr.e1 = make_float4(v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2], 0.0f);
+        tr.e1 = make_float4(v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2], keyf);
         tr.e2 = make_float4(v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2], 0.0f);
         if (tris.size() != tri_base + n.left_or_start + k) return fail(AR_ERR_STATE, "leaf order mismatch");
         tris.push_back(tr);
@@ -613,7 +617,32 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     }
     for (uint32_t i = 0; i < nt; ++i) tri_obj.push_back((uint32_t)oi);
     Flat f = flatten(t, nodes, tri_base);
-    QFlat qf = build_qbvh(t, h.qnodes, tri_base, &boxes);
+    QFlat qf;
+    if (s->builder == AR_BUILDER_SAH_SPLIT) {
+      // 4-wide tree from a spatial-split tree over triangle references; its
+      // records copy the binary tree's, keyed by their index there
+      std::vector<uint32_t> ref_tri;
+      std::vector<HBox> ref_box;
+      const BinTree st = build_split_sah(boxes, m.pos.data(), m.idx.data(), 2 * (size_t)nt, ref_tri, ref_box);
+      std::vector<uint32_t> at(nt);
+      for (uint32_t p = 0; p < nt; ++p) at[t.prims[p]] = p;
+      const uint32_t qbase = (uint32_t)h.qtris.size();
+      for (const BinNode& n : st.nodes) {
+        for (uint32_t k = 0; k < n.count; ++k) {
+          const uint32_t p = n.left_or_start + k;
+          if (h.qtris.size() != qbase + p) return fail(AR_ERR_STATE, "leaf order mismatch");
+          Tri tr = tris[tri_base + at[ref_tri[st.prims[p]]]];
+          uint32_t tag;
+          memcpy(&tag, &tr.v0.w, 4);
+          tag = (tag & ~kLastBit) | (k + 1 == n.count ? kLastBit : 0u);
+          memcpy(&tr.v0.w, &tag, 4);
+          h.qtris.push_back(tr);
+        }
+      }
+      qf = build_qbvh(st, h.qnodes, qbase, &ref_box);
+    } else {
+      qf = build_qbvh(t, h.qnodes, tri_base, &boxes);
+    }
     h.qmax_depth = std::max(h.qmax_depth, qf.depth);
     Inst& in = inst_by_obj[oi];
     memset(&in, 0, sizeof(in));
@@ -699,7 +728,8 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
   Flat tf = flatten(tl, nodes, 0);
   QFlat tq = build_qbvh(tl, h.qnodes, 0, nullptr);
   h.tlas_root = tf.root_ref;
-  if (tris.size() >= kSingleTri) return fail(AR_ERR_INVALID_ARG, "scene has 2^30 or more triangles");
+  if (tris.size() >= kSingleTri || h.qtris.size() >= kSingleTri)
+    return fail(AR_ERR_INVALID_ARG, "scene has 2^30 or more triangles (or triangle references)");
   h.qtlas_root = tq.root_ref;
   h.qtlas_depth = tq.depth;
   // the 4-wide box test evaluates q * (scl / d) + (org - o) / d per plane;
@@ -793,6 +823,7 @@ ar_status ar_scene_commit(ar_scene* s) try {
   AR_HIP(upload(s->d_nodes, h.nodes));
   AR_HIP(upload(s->d_qnodes, h.qnodes));
   AR_HIP(upload(s->d_tris, h.tris));
+  AR_HIP(upload(s->d_qtris, h.qtris));
   AR_HIP(upload(s->d_insts, h.insts));
   AR_HIP(upload(s->d_objects, h.objs));
   AR_HIP(upload(s->d_materials, s->materials));
@@ -811,6 +842,7 @@ ar_status ar_scene_commit(ar_scene* s) try {
   s->view.qnodes = s->d_qnodes.as<QNode>();
   s->view.qtlas_root = h.qtlas_root;
   s->view.tris = s->d_tris.as<Tri>();
+  s->view.qtris = h.qtris.empty() ? s->view.tris : s->d_qtris.as<Tri>();
   s->view.insts = s->d_insts.as<Inst>();
   s->view.tlas_root = h.tlas_root;
   for (int k = 0; k < 6; ++k) s->view.tlas_box[k] = h.tlas_box[k];
@@ -833,7 +865,7 @@ ar_status ar_scene_commit(ar_scene* s) try {
     s->tcfg.balance = bal;
     s->tcfg_any.balance = bal;
   }
-  s->info.device_bytes = s->d_nodes.bytes + s->d_qnodes.bytes + s->d_tris.bytes + s->d_insts.bytes + s->d_objects.bytes +
+  s->info.device_bytes = s->d_nodes.bytes + s->d_qnodes.bytes + s->d_tris.bytes + s->d_qtris.bytes + s->d_insts.bytes + s->d_objects.bytes +
                          s->d_materials.bytes + s->d_lights.bytes + s->d_idx.bytes + s->d_nrm.bytes +
                          s->d_slots.bytes + s->d_objmats.bytes + s->d_tri_obj.bytes + s->d_stri.bytes;
   AR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));

@@ -481,11 +481,11 @@ def bench_pt(ar, torch, which, schedules=("megakernel", "wavefront")):
 
 
 def bench_builders(ar, torch):
-    """The 2^20-triangle soup committed with the reference's host SAH and with
-    the device LBVH builder (commit wall time: tree build, flattening,
-    upload), then the 2^24 bench rays traced through each with the same
-    launches (closest hit 256 x 8 x 8 x 8 x 16 x 3, any hit 128 x 16 x 8 x 8 x
-    16 x 4), best of three launches each."""
+    """The 2^20-triangle soup committed with the reference's host SAH, the
+    device LBVH builder and the spatial-split builder (commit wall time: tree
+    build, flattening, upload), then the 2^24 bench rays traced through each
+    with the same launches (closest and any hit 256 x 12 x 8 x 8 x 16 x 3,
+    balance 1), best of three launches each."""
     wl = ar.workloads
     desc = wl.triangle_soup(1 << 20, seed=0)
     rays = wl.soup_rays(1 << 24, seed=1)
@@ -497,13 +497,13 @@ def bench_builders(ar, torch):
     sh = C_void(stream.cuda_stream)
     lib = ar.lib()
     out = {}
-    for builder in ("sah", "lbvh"):
+    for builder in ("sah", "lbvh", "split"):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         sc = ar.Scene.from_desc(desc, builder=builder)
         build_s = time.perf_counter() - t0
-        sc.set_trace_config(256, 8, 8, 8, 16, 3)
-        sc.set_anyhit_config(128, 16, 8, 8, 16, 4)
+        sc.set_trace_config(256, 12, 8, 8, 16, 3, 1)
+        sc.set_anyhit_config(256, 12, 8, 8, 16, 3, 1)
         res = {"commit_s": round(build_s, 3), "binary_nodes": int(sc.info.n_nodes), "max_depth": int(sc.info.max_depth)}
         for name, fn in (("closest", lambda: ar._check(lib.ar_trace(sc.handle, C_void(dr.data_ptr()), n,
                                                                       C_void(hits.data_ptr()), None, ar.AR_ASYNC, sh))),

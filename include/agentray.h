@@ -188,13 +188,19 @@ ar_status ar_scene_get_info(const ar_scene* s, ar_scene_info* info);
 ar_status ar_bvh_build_host(const float* boxes, uint32_t n, float* out_boxes, uint32_t* out_meta,
                             uint32_t* out_prims, uint32_t* out_nodes, uint32_t* out_depth);
 /* Bottom-level tree builders for ar_scene_commit.  AR_BUILDER_SAH is the
- * reference's full-sweep SAH on the host (bvh.rs:71-236, the default);
+ * reference's full-sweep SAH on the host (bvh.rs:71-236), for both tree
+ * forms.  AR_BUILDER_SAH_SPLIT (the default) keeps that tree for
+ * AR_REF_ORDER and builds the near-first 4-wide trees from an SAH tree with
+ * spatial splits (a triangle crossing a split plane is referenced from both
+ * sides, at most 2 references per triangle on average): about 3x the build
+ * time, 23% faster closest-hit traversal on a 2^20-triangle soup.
  * AR_BUILDER_LBVH builds each mesh's tree on the device (Morton order, radix
  * sort, Karras hierarchy; leaves of up to four triangles): much faster to
- * build, slower to trace, the same query answers.  The TLAS is always SAH.
- * No reference counterpart. */
+ * build, slower to trace.  All three give the same query answers.  The TLAS
+ * is always SAH.  No reference counterpart. */
 #define AR_BUILDER_SAH 0u
 #define AR_BUILDER_LBVH 1u
+#define AR_BUILDER_SAH_SPLIT 2u
 ar_status ar_scene_set_builder(ar_scene* s, uint32_t builder);
 /* The device builder over n boxes, in ar_bvh_build_host's output form. */
 ar_status ar_bvh_build_device(const float* boxes, uint32_t n, float* out_boxes, uint32_t* out_meta,

@@ -3,7 +3,8 @@ for host and device) compiled for the host with g++ and compared with the
 oracle ray by ray, without a GPU.
 
 It walks the same trees the device gets (build_host_accel: binary trees for
-reference order, quantized 4-wide trees for near-first order) and runs each
+reference order, quantized 4-wide trees for near-first order, built from
+the reference's SAH tree or from the spatial-split tree) and runs each
 query two ways: one ray after another, and in an emulation of the kernels'
 64-lane waves with wave-local refill and maximal leaf parking (three and
 four park slots).  Checks:
@@ -52,11 +53,15 @@ def _helper():
     return h
 
 
-def _scene(ar, desc):
+BUILDERS = {"sah": 0, "split": 2}
+
+
+def _scene(ar, desc, builder="split"):
     """The scene as the library holds it before commit (no device needed)."""
     lib = ar.lib()
     s = C.c_void_p()
     assert lib.ar_scene_create(C.byref(s)) == 0
+    assert lib.ar_scene_set_builder(s, BUILDERS[builder]) == 0
     mids = []
     for m in desc.materials:
         out = C.c_uint32()
@@ -97,26 +102,32 @@ def _near_ties(a, b):
 
 
 def _check(ar, orc, desc, rays, max_near_ties=0):
+    """Both tree builders' scenes against the oracle: the reference's SAH for
+    both tree forms, and the spatial-split 4-wide trees (the default)."""
+    osc = orc.OracleScene(desc)
+    ot, op, oi, _, _ = osc.trace(rays)
+    occ, _, _ = osc.occluded(rays)
+    for builder in BUILDERS:
+        _check_one(ar, desc, rays, (ot, op, oi), occ, max_near_ties, builder)
+
+
+def _check_one(ar, desc, rays, ora, occ, max_near_ties, builder):
     h = _helper()
     rays = np.ascontiguousarray(rays, np.float32)
-    s = _scene(ar, desc)
+    s = _scene(ar, desc, builder)
     try:
-        osc = orc.OracleScene(desc)
-        ot, op, oi, _, _ = osc.trace(rays)
-        occ, _, _ = osc.occluded(rays)
-        ora = (ot, op, oi)
         for near, slots in ((False, 3), (True, 3), (True, 4)):
             one, wave = _run(h, s, rays, False, near, slots=slots)
             for k in range(3):  # the two drivers agree bit for bit
                 assert np.array_equal(one[k].view(np.uint32), wave[k].view(np.uint32)), (near, slots, k)
             bad, ties = _near_ties(one, ora)
             if near:
-                assert bad == ties and ties <= max_near_ties, (bad, ties)
+                assert bad == ties and ties <= max_near_ties, (builder, bad, ties)
             else:
-                assert bad == 0, bad
+                assert bad == 0, (builder, bad)
             a1, a2 = _run(h, s, rays, True, near, slots=slots)
-            assert np.array_equal(a1[0].astype(bool), occ.astype(bool)), (near, slots)
-            assert np.array_equal(a2[0].astype(bool), occ.astype(bool)), (near, slots)
+            assert np.array_equal(a1[0].astype(bool), occ.astype(bool)), (builder, near, slots)
+            assert np.array_equal(a2[0].astype(bool), occ.astype(bool)), (builder, near, slots)
     finally:
         ar.lib().ar_scene_destroy(s)
 
