@@ -192,6 +192,7 @@ extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any
   sv.nodes = h.nodes.data();
   sv.qnodes = h.qnodes.data();
   sv.tris = h.tris.data();
+  sv.qtris = h.qtris.empty() ? sv.tris : h.qtris.data();
   sv.insts = h.insts.data();
   sv.tlas_root = h.tlas_root;
   sv.qtlas_root = h.qtlas_root;
