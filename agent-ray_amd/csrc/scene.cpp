@@ -260,6 +260,15 @@ struct QFlat {
   int root_ref;
   uint32_t depth;
 };
+
+// Expected SAH cost of a binary tree (C_trav = C_isect = 1) relative to its
+// root's area: the node and triangle tests of a random ray through the root.
+static double sah_cost(const BinTree& t) {
+  const double root = box_area(t.nodes[0].box);
+  double c = 0.0;
+  for (const BinNode& n : t.nodes) c += (double)box_area(n.box) * (n.count ? (double)n.count : 1.0);
+  return root > 0.0 ? c / root : 0.0;
+}
 struct QPlan {
   const BinTree& t;
   const std::vector<HBox>* prim_boxes;
@@ -570,9 +579,8 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
   uint32_t max_depth = 0;
   bool any_uv = false;
   for (const HostMesh& m : s->meshes) any_uv = any_uv || !m.uv.empty();
-  for (size_t oi = 0; oi < s->meshes.size(); ++oi) {
-    const HostMesh& m = s->meshes[oi];
-    uint32_t nt = (uint32_t)(m.idx.size() / 3);
+  auto mesh_boxes = [](const HostMesh& m) {
+    const uint32_t nt = (uint32_t)(m.idx.size() / 3);
     std::vector<HBox> boxes(nt);
     for (uint32_t i = 0; i < nt; ++i) {  // mesh.rs:81-94
       const float* v0 = &m.pos[3 * m.idx[3 * i]];
@@ -583,10 +591,48 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
         boxes[i].hi[k] = smax(smax(v0[k], v1[k]), v2[k]);
       }
     }
+    return boxes;
+  };
+  // Spatial-split trees (AR_BUILDER_SAH_SPLIT), built per object first and
+  // kept where they lower the SAH cost by at least 5%: their references
+  // cost a second copy of the triangle records, which the 4-wide walk then
+  // reads while shading reads the first (a room of boxes, whose split trees
+  // split nothing, rendered 3-4% slower with them).  When some object keeps
+  // one, every object's 4-wide tree reads records of its own (qtris), in the
+  // leaf order of the tree it is collapsed from.
+  struct SplitTree {
+    BinTree bin, st;
+    std::vector<uint32_t> ref_tri;
+    std::vector<HBox> ref_box;
+    bool keep = false;
+  };
+  std::vector<SplitTree> split(s->builder == AR_BUILDER_SAH_SPLIT ? s->meshes.size() : 0);
+  bool any_split = false;
+  for (size_t oi = 0; oi < split.size(); ++oi) {
+    const HostMesh& m = s->meshes[oi];
+    const std::vector<HBox> boxes = mesh_boxes(m);
+    const uint32_t nt = (uint32_t)boxes.size();
+    SplitTree& sp = split[oi];
+    sp.bin = build_sah(boxes, 0);
+    sp.st = build_split_sah(boxes, m.pos.data(), m.idx.data(), 2 * (size_t)nt, sp.ref_tri, sp.ref_box);
+    sp.keep = nt > 0 && sah_cost(sp.st) < 0.95 * sah_cost(sp.bin);
+    if (!sp.keep) {
+      sp.st = BinTree();
+      std::vector<uint32_t>().swap(sp.ref_tri);
+      std::vector<HBox>().swap(sp.ref_box);
+    }
+    any_split = any_split || sp.keep;
+  }
+  for (size_t oi = 0; oi < s->meshes.size(); ++oi) {
+    const HostMesh& m = s->meshes[oi];
+    uint32_t nt = (uint32_t)(m.idx.size() / 3);
+    std::vector<HBox> boxes = mesh_boxes(m);
     BinTree t;
     if (s->builder == AR_BUILDER_LBVH) {
       ar_status be = build_lbvh(boxes, t);
       if (be != AR_OK) return be;
+    } else if (!split.empty()) {
+      t = std::move(split[oi].bin);
     } else {
       t = build_sah(boxes, 0);
     }
@@ -618,20 +664,21 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     for (uint32_t i = 0; i < nt; ++i) tri_obj.push_back((uint32_t)oi);
     Flat f = flatten(t, nodes, tri_base);
     QFlat qf;
-    if (s->builder == AR_BUILDER_SAH_SPLIT) {
-      // 4-wide tree from a spatial-split tree over triangle references; its
-      // records copy the binary tree's, keyed by their index there
-      std::vector<uint32_t> ref_tri;
-      std::vector<HBox> ref_box;
-      const BinTree st = build_split_sah(boxes, m.pos.data(), m.idx.data(), 2 * (size_t)nt, ref_tri, ref_box);
+    if (any_split) {
+      // the 4-wide tree's own records: copies of the binary tree's, keyed by
+      // their index there, in the leaf order of the split tree (kept) or of
+      // the binary tree itself
+      const SplitTree& sp = split[oi];
+      const BinTree& qt = sp.keep ? sp.st : t;
       std::vector<uint32_t> at(nt);
       for (uint32_t p = 0; p < nt; ++p) at[t.prims[p]] = p;
       const uint32_t qbase = (uint32_t)h.qtris.size();
-      for (const BinNode& n : st.nodes) {
+      for (const BinNode& n : qt.nodes) {
         for (uint32_t k = 0; k < n.count; ++k) {
           const uint32_t p = n.left_or_start + k;
           if (h.qtris.size() != qbase + p) return fail(AR_ERR_STATE, "leaf order mismatch");
-          Tri tr = tris[tri_base + at[ref_tri[st.prims[p]]]];
+          const uint32_t tri = sp.keep ? sp.ref_tri[qt.prims[p]] : qt.prims[p];
+          Tri tr = tris[tri_base + at[tri]];
           uint32_t tag;
           memcpy(&tag, &tr.v0.w, 4);
           tag = (tag & ~kLastBit) | (k + 1 == n.count ? kLastBit : 0u);
@@ -639,7 +686,7 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
           h.qtris.push_back(tr);
         }
       }
-      qf = build_qbvh(st, h.qnodes, qbase, &ref_box);
+      qf = build_qbvh(qt, h.qnodes, qbase, sp.keep ? &sp.ref_box : &boxes);
     } else {
       qf = build_qbvh(t, h.qnodes, tri_base, &boxes);
     }

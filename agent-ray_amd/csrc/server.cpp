@@ -43,8 +43,9 @@
 #include <cstdio>
 #define AR_STRACE(...) std::fprintf(stderr, __VA_ARGS__)
 #else
-#define AR_STRACE(...) \
-  do {                 \
+#define AR_STRACE(...)                                 \
+  do {                                                 \
+    if (false) std::fprintf(stderr, __VA_ARGS__);      \
   } while (0)
 #endif
 

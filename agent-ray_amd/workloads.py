@@ -269,12 +269,13 @@ def cornell_box(seed: int = 0, width: int = 256, height: int = 256, spp: int = 1
 # ---------------------------------------------------------------------------
 # Triangle soup + ray sets (traversal microbenchmark)
 # ---------------------------------------------------------------------------
-def triangle_soup(n_tris: int = 1 << 20, seed: int = 0) -> SceneDesc:
-    """n random triangles: centroid U[0,1]^3, size s = exp(U[ln 1e-3, ln 2e-2]),
-    vertices c + s * (independent uniform unit vectors)."""
+def triangle_soup(n_tris: int = 1 << 20, seed: int = 0, size_range=(1e-3, 2e-2)) -> SceneDesc:
+    """n random triangles: centroid U[0,1]^3, size s = exp(U[ln lo, ln hi])
+    for size_range (lo, hi), vertices c + s * (independent uniform unit
+    vectors)."""
     rng = np.random.default_rng(seed)
     c = rng.uniform(0.0, 1.0, size=(n_tris, 1, 3))
-    s = np.exp(rng.uniform(math.log(1e-3), math.log(2e-2), size=(n_tris, 1, 1)))
+    s = np.exp(rng.uniform(math.log(size_range[0]), math.log(size_range[1]), size=(n_tris, 1, 1)))
     g = rng.standard_normal(size=(n_tris, 3, 3))
     g /= np.linalg.norm(g, axis=2, keepdims=True)
     v = (c + s * g).astype(np.float32).reshape(-1, 3)

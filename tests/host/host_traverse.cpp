@@ -224,3 +224,13 @@ extern "C" int host_max_leaf(ar_scene* s, uint32_t* sizes) {
   }
   return 0;
 }
+
+// counts[0] = Tri records, counts[1] = the 4-wide trees' own records (0 when
+// they share the binary trees' records).
+extern "C" int host_record_counts(ar_scene* s, uint64_t* counts) {
+  HostAccel h;
+  if (build_host_accel(s, h) != AR_OK) return -10;
+  counts[0] = h.tris.size();
+  counts[1] = h.qtris.size();
+  return 0;
+}

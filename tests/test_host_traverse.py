@@ -50,6 +50,8 @@ def _helper():
     h.host_chunk_schedule.restype = C.c_int
     h.host_max_leaf.argtypes = [C.c_void_p, C.c_void_p]
     h.host_max_leaf.restype = C.c_int
+    h.host_record_counts.argtypes = [C.c_void_p, C.c_void_p]
+    h.host_record_counts.restype = C.c_int
     return h
 
 
@@ -212,6 +214,33 @@ def test_leaves_of_more_than_four_triangles(ar, orc, wl):
     finally:
         ar.lib().ar_scene_destroy(s)
     _check(ar, orc, desc, wl.soup_rays(1 << 13, seed=15, image=64))
+
+
+def test_split_and_binary_objects_in_one_scene(ar, orc, wl):
+    """The split builder keeps a spatial-split tree only where it lowers the
+    SAH cost by 5%: here for a soup of large triangles (sizes 0.02-0.2 in the
+    unit cube: 1.9 references per triangle, SAH cost 114 -> 92), not for the
+    Cornell instances beside it.  The scene's 4-wide trees then read their own
+    records, the Cornell objects' in their binary leaf order; answers equal
+    the oracle's for both builders."""
+    cb = wl.cornell_box()
+    desc = wl.SceneDesc(materials=cb.materials, meshes=wl.triangle_soup(4096, seed=16, size_range=(0.02, 0.2)).meshes + list(cb.meshes), lights=cb.lights,
+                        camera=cb.camera)
+    h = _helper()
+    counts = np.zeros(2, np.uint64)
+    for builder, more in (("sah", False), ("split", True)):
+        s = _scene(ar, desc, builder)
+        try:
+            assert h.host_record_counts(s, counts.ctypes.data) == 0
+        finally:
+            ar.lib().ar_scene_destroy(s)
+        assert (counts[1] > counts[0]) == more, (builder, counts)
+    s = _scene(ar, wl.SceneDesc(materials=cb.materials, meshes=list(cb.meshes), lights=cb.lights, camera=cb.camera))
+    try:  # no object keeps a split tree: one set of records
+        assert h.host_record_counts(s, counts.ctypes.data) == 0 and counts[1] == 0, counts
+    finally:
+        ar.lib().ar_scene_destroy(s)
+    _check(ar, orc, desc, _interior_rays(wl, 0.02, 0.98, 12000, 17), max_near_ties=2)
 
 
 @pytest.mark.parametrize("waves", [1, 3, 16, 96])

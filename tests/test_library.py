@@ -127,3 +127,14 @@ def test_rust_binding_declares_the_header():
         rust[name] = 0 if not a else a.count(",") + 1
     assert set(rust) == set(header), (sorted(set(header) - set(rust)), sorted(set(rust) - set(header)))
     assert {k: v for k, v in rust.items() if v != header[k]} == {}
+
+
+def test_rust_binding_constants_match_the_header():
+    """Every constant INTEGRATION.md's Rust binding declares has the header's
+    value."""
+    src = open(os.path.join(ROOT, "include", "agentray.h")).read()
+    defs = {k: int(v, 0) for k, v in re.findall(r"#define\s+(AR_[A-Z0-9_]+)\s+\(?(0x[0-9a-fA-F]+|\d+)u?\)?", src)}
+    doc = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    consts = {k: int(v) for k, v in re.findall(r"pub const (AR_[A-Z0-9_]+): u32 = (\d+);", doc)}
+    assert consts and all(k in defs for k in consts), sorted(set(consts) - set(defs))
+    assert {k: v for k, v in consts.items() if defs[k] != v} == {}
