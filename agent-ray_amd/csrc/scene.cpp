@@ -4,6 +4,9 @@
 // uploads everything once.  The committed scene stays resident in HBM.
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <future>
+#include <thread>
 #include <cmath>
 
 #include "ar_internal.h"
@@ -607,22 +610,34 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     bool keep = false;
   };
   std::vector<SplitTree> split(s->builder == AR_BUILDER_SAH_SPLIT ? s->meshes.size() : 0);
-  bool any_split = false;
-  for (size_t oi = 0; oi < split.size(); ++oi) {
-    const HostMesh& m = s->meshes[oi];
-    const std::vector<HBox> boxes = mesh_boxes(m);
-    const uint32_t nt = (uint32_t)boxes.size();
-    SplitTree& sp = split[oi];
-    sp.bin = build_sah(boxes, 0);
-    sp.st = build_split_sah(boxes, m.pos.data(), m.idx.data(), 2 * (size_t)nt, sp.ref_tri, sp.ref_box);
-    sp.keep = nt > 0 && sah_cost(sp.st) < 0.95 * sah_cost(sp.bin);
-    if (!sp.keep) {
-      sp.st = BinTree();
-      std::vector<uint32_t>().swap(sp.ref_tri);
-      std::vector<HBox>().swap(sp.ref_box);
-    }
-    any_split = any_split || sp.keep;
+  {
+    // objects are built by a few threads in parallel (a room of 117 objects
+    // committed in 2.7 s with one); each big build also splits its own work
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+      for (size_t oi; (oi = next.fetch_add(1)) < split.size();) {
+        const HostMesh& m = s->meshes[oi];
+        const std::vector<HBox> boxes = mesh_boxes(m);
+        const uint32_t nt = (uint32_t)boxes.size();
+        SplitTree& sp = split[oi];
+        sp.bin = build_sah(boxes, 0);
+        sp.st = build_split_sah(boxes, m.pos.data(), m.idx.data(), 2 * (size_t)nt, sp.ref_tri, sp.ref_box);
+        sp.keep = nt > 0 && sah_cost(sp.st) < 0.95 * sah_cost(sp.bin);
+        if (!sp.keep) {
+          sp.st = BinTree();
+          std::vector<uint32_t>().swap(sp.ref_tri);
+          std::vector<HBox>().swap(sp.ref_box);
+        }
+      }
+    };
+    const size_t nthr = std::min<size_t>(split.size(), std::max(1u, std::min(8u, std::thread::hardware_concurrency())));
+    std::vector<std::future<void>> fs;
+    for (size_t i = 1; i < nthr; ++i) fs.push_back(std::async(std::launch::async, work));
+    work();
+    for (auto& f : fs) f.get();
   }
+  bool any_split = false;
+  for (const SplitTree& sp : split) any_split = any_split || sp.keep;
   for (size_t oi = 0; oi < s->meshes.size(); ++oi) {
     const HostMesh& m = s->meshes[oi];
     uint32_t nt = (uint32_t)(m.idx.size() / 3);
