@@ -93,6 +93,8 @@ constexpr uint32_t kLastBit = 0x80000000u;
 // A 4-wide tree leaf ref ~(k | kSingleTri) stands for Tri record k alone (its
 // own quantized box in the parent); ~k without it for the records from k to
 // the one tagged kLastBit (a binary leaf's range).  Records stay below 2^30.
+// In the TLAS's 4-wide tree the same bit marks one instance record alone,
+// in a slot holding its world box.
 constexpr uint32_t kSingleTri = 0x40000000u;
 
 // One instance (TLAS leaf record), in TLAS leaf order.

@@ -171,6 +171,7 @@ struct Walker {
   int cur;                // node ref, leaf ref, or kLevelEnd
   int park0, park1, park2, park3;  // parked triangle leaves in the order reached (kLevelEnd: free)
   bool in_blas;
+  bool one_inst;          // the instance came from a single-instance TLAS slot: no next record to try
   bool fast;              // 1/d of this level is finite: IEEE min / max slab
   bool wide;              // this level walks the 4-wide tree (near-first, fast)
 
@@ -194,6 +195,7 @@ struct Walker {
     tmin = wtmin;
     tmax = bt;
     in_blas = false;
+    one_inst = false;
     park0 = park1 = park2 = park3 = kLevelEnd;
     fast = finite_inv(inv);
     wide = NEAR && fast && wide_ray(sv, o, inv);
@@ -202,7 +204,7 @@ struct Walker {
     const float* tb = sv.tlas_box;
     if (!slab(tb[0], tb[1], tb[2], tb[3], tb[4], tb[5], o, inv, tmin, tmax, &tn)) return true;
     // a TLAS that is a single leaf: enter its instances right away
-    if (cur < 0 && !enter(sv, (uint32_t)~cur, stk)) return true;
+    if (cur < 0 && !enter(sv, (uint32_t)~cur, stk)) return true;  // (a range: single slots sit in nodes)
     return false;
   }
 
@@ -403,8 +405,11 @@ struct Walker {
   }
 
   // Enters the first instance from record k on (same TLAS leaf) whose BLAS
-  // root box the local ray hits.  False when the leaf has none left.
+  // root box the local ray hits; with kSingleTri in k, record k only.  False
+  // when the leaf has none left.
   AR_HD bool enter(const SceneView& sv, uint32_t k, const Stack& stk) {
+    one_inst = (k & kSingleTri) != 0u;
+    k &= ~kSingleTri;
     for (;;) {
       AR_COUNT_INST();
       const Inst* in = sv.insts + k;
@@ -441,7 +446,7 @@ struct Walker {
           return true;
         }
       }
-      if (in->id & kLastBit) return false;
+      if (one_inst || (in->id & kLastBit)) return false;
       ++k;
     }
   }
@@ -564,7 +569,7 @@ struct Walker {
       wide = NEAR && fast && wide_ray(sv, o, inv);
       tmin = wtmin;
       tmax = bt;
-      if (!(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
+      if (!one_inst && !(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
       pop_level(stk);
     }
     return false;

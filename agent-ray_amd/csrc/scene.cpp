@@ -788,7 +788,10 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
     }
   }
   Flat tf = flatten(tl, nodes, 0);
-  QFlat tq = build_qbvh(tl, h.qnodes, 0, nullptr);
+  // TLAS leaves of two to four instances become nodes with a world-box slot
+  // per instance record (kSingleTri refs), as bottom-level leaves do with
+  // triangles: the box test culls instances before their records are read
+  QFlat tq = build_qbvh(tl, h.qnodes, 0, &world);
   h.tlas_root = tf.root_ref;
   if (tris.size() >= kSingleTri || h.qtris.size() >= kSingleTri)
     return fail(AR_ERR_INVALID_ARG, "scene has 2^30 or more triangles (or triangle references)");
