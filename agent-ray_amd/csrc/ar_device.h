@@ -94,7 +94,8 @@ constexpr uint32_t kLastBit = 0x80000000u;
 // own quantized box in the parent); ~k without it for the records from k to
 // the one tagged kLastBit (a binary leaf's range).  Records stay below 2^30.
 // In the TLAS's 4-wide tree the same bit marks one instance record alone,
-// in a slot holding its world box.
+// in a slot holding its world box (SceneView::qinsts marks it last in its
+// leaf).
 constexpr uint32_t kSingleTri = 0x40000000u;
 
 // One instance (TLAS leaf record), in TLAS leaf order.
@@ -115,7 +116,8 @@ struct SceneView {
   const QNode* qnodes;  // 4-wide trees (near-first order)
   const Tri* tris;       // binary trees' leaf order; hits name these records
   const Tri* qtris;      // 4-wide trees' leaf order (the same array as tris unless spatial splits)
-  const Inst* insts;
+  const Inst* insts;     // TLAS leaf order; hits name these records
+  const Inst* qinsts;    // the same records for the 4-wide TLAS, kLastBit on every single-instance slot's
   int tlas_root;        // TLAS root ref (node index or ~first instance record)
   int qtlas_root;       // the same in the 4-wide node array
   float tlas_box[6];    // TLAS root box (world)

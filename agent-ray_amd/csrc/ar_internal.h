@@ -170,7 +170,7 @@ struct ar_scene {
   ar_trace_config tcfg{256, 12, 8, 8, 8, 3, 1, 0.0f};      // closest-hit launch configuration (autotuned)
   ar_trace_config tcfg_any{256, 12, 8, 8, 8, 3, 1, 0.0f};  // any-hit (occlusion, shadow rays), tuned separately
   bool tcfg_set = false;  // a configuration was set or tuned
-  ar::DevBuf d_nodes, d_qnodes, d_tris, d_qtris, d_insts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
+  ar::DevBuf d_nodes, d_qnodes, d_tris, d_qtris, d_insts, d_qinsts, d_objects, d_materials, d_lights, d_idx, d_nrm, d_slots, d_objmats,
       d_tri_obj, d_stri, d_suv, d_tex, d_texinfo;
   ar::SceneView view{};
   uint32_t n_lights = 0;
@@ -200,6 +200,9 @@ struct HostAccel {
   // more than once; empty otherwise (the 4-wide trees then use `tris`)
   std::vector<Tri> qtris;
   std::vector<Inst> insts;
+  // the instance records as the 4-wide TLAS reads them: kLastBit also on
+  // every record of a TLAS leaf of two to four instances (one slot each)
+  std::vector<Inst> qinsts;
   std::vector<DObject> objs;
   std::vector<uint32_t> gidx, gslots, gmats, tri_obj;
   std::vector<float> gnrm;

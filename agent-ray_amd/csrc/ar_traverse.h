@@ -171,7 +171,6 @@ struct Walker {
   int cur;                // node ref, leaf ref, or kLevelEnd
   int park0, park1, park2, park3;  // parked triangle leaves in the order reached (kLevelEnd: free)
   bool in_blas;
-  bool one_inst;          // the instance came from a single-instance TLAS slot: no next record to try
   bool fast;              // 1/d of this level is finite: IEEE min / max slab
   bool wide;              // this level walks the 4-wide tree (near-first, fast)
 
@@ -195,7 +194,6 @@ struct Walker {
     tmin = wtmin;
     tmax = bt;
     in_blas = false;
-    one_inst = false;
     park0 = park1 = park2 = park3 = kLevelEnd;
     fast = finite_inv(inv);
     wide = NEAR && fast && wide_ray(sv, o, inv);
@@ -404,15 +402,20 @@ struct Walker {
     descend(sv, stk, [](bool can, bool searching) { return searching; });
   }
 
+  // The instance records of the current TLAS form: the 4-wide TLAS's copy
+  // ends every single-instance slot's record with kLastBit.
+  AR_HD const Inst* tlas_insts(const SceneView& sv) const { return (NEAR && wide) ? sv.qinsts : sv.insts; }
+
   // Enters the first instance from record k on (same TLAS leaf) whose BLAS
-  // root box the local ray hits; with kSingleTri in k, record k only.  False
-  // when the leaf has none left.
+  // root box the local ray hits (a single-instance slot's ref carries
+  // kSingleTri; its record is the last of its leaf).  False when the leaf
+  // has none left.  Called at the TLAS level.
   AR_HD bool enter(const SceneView& sv, uint32_t k, const Stack& stk) {
-    one_inst = (k & kSingleTri) != 0u;
+    const Inst* ib = tlas_insts(sv);
     k &= ~kSingleTri;
     for (;;) {
       AR_COUNT_INST();
-      const Inst* in = sv.insts + k;
+      const Inst* in = ib + k;
       f3 lo, ld;
       if (in->identity) {
         lo = wo;
@@ -446,7 +449,7 @@ struct Walker {
           return true;
         }
       }
-      if (one_inst || (in->id & kLastBit)) return false;
+      if (in->id & kLastBit) return false;
       ++k;
     }
   }
@@ -569,7 +572,7 @@ struct Walker {
       wide = NEAR && fast && wide_ray(sv, o, inv);
       tmin = wtmin;
       tmax = bt;
-      if (!one_inst && !(sv.insts[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
+      if (!(tlas_insts(sv)[ik].id & kLastBit) && enter(sv, ik + 1, stk)) return false;
       pop_level(stk);
     }
     return false;

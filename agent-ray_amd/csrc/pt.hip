@@ -327,7 +327,7 @@ uint64_t view_hash(const ShadeView& v) {
   };
   const SceneView& sv = v.sv;
   for (const void* p : {(const void*)sv.nodes, (const void*)sv.qnodes, (const void*)sv.tris, (const void*)sv.qtris,
-                        (const void*)sv.insts,
+                        (const void*)sv.insts, (const void*)sv.qinsts,
                         (const void*)v.objects, (const void*)v.materials, (const void*)v.lights, (const void*)v.idx,
                         (const void*)v.nrm, (const void*)v.slots, (const void*)v.objmats, (const void*)v.stri,
                         (const void*)v.suv, (const void*)v.texels, (const void*)v.tex_info})

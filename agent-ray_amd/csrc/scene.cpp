@@ -792,6 +792,10 @@ ar_status build_host_accel(const ar_scene* s, HostAccel& h) {
   // per instance record (kSingleTri refs), as bottom-level leaves do with
   // triangles: the box test culls instances before their records are read
   QFlat tq = build_qbvh(tl, h.qnodes, 0, &world);
+  h.qinsts = insts;
+  for (const BinNode& n : tl.nodes)
+    if (n.count >= 2 && n.count <= 4)
+      for (uint32_t k = 0; k < n.count; ++k) h.qinsts[n.left_or_start + k].id |= kLastBit;
   h.tlas_root = tf.root_ref;
   if (tris.size() >= kSingleTri || h.qtris.size() >= kSingleTri)
     return fail(AR_ERR_INVALID_ARG, "scene has 2^30 or more triangles (or triangle references)");
@@ -890,6 +894,7 @@ ar_status ar_scene_commit(ar_scene* s) try {
   AR_HIP(upload(s->d_tris, h.tris));
   AR_HIP(upload(s->d_qtris, h.qtris));
   AR_HIP(upload(s->d_insts, h.insts));
+  AR_HIP(upload(s->d_qinsts, h.qinsts));
   AR_HIP(upload(s->d_objects, h.objs));
   AR_HIP(upload(s->d_materials, s->materials));
   AR_HIP(upload(s->d_lights, dl));
@@ -909,6 +914,7 @@ ar_status ar_scene_commit(ar_scene* s) try {
   s->view.tris = s->d_tris.as<Tri>();
   s->view.qtris = h.qtris.empty() ? s->view.tris : s->d_qtris.as<Tri>();
   s->view.insts = s->d_insts.as<Inst>();
+  s->view.qinsts = s->d_qinsts.as<Inst>();
   s->view.tlas_root = h.tlas_root;
   for (int k = 0; k < 6; ++k) s->view.tlas_box[k] = h.tlas_box[k];
   s->view.n_inst = (uint32_t)h.insts.size();
@@ -930,7 +936,7 @@ ar_status ar_scene_commit(ar_scene* s) try {
     s->tcfg.balance = bal;
     s->tcfg_any.balance = bal;
   }
-  s->info.device_bytes = s->d_nodes.bytes + s->d_qnodes.bytes + s->d_tris.bytes + s->d_qtris.bytes + s->d_insts.bytes + s->d_objects.bytes +
+  s->info.device_bytes = s->d_nodes.bytes + s->d_qnodes.bytes + s->d_tris.bytes + s->d_qtris.bytes + s->d_insts.bytes + s->d_qinsts.bytes + s->d_objects.bytes +
                          s->d_materials.bytes + s->d_lights.bytes + s->d_idx.bytes + s->d_nrm.bytes +
                          s->d_slots.bytes + s->d_objmats.bytes + s->d_tri_obj.bytes + s->d_stri.bytes;
   AR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));

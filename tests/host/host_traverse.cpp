@@ -194,6 +194,7 @@ extern "C" int host_traverse(ar_scene* s, const float* rays, uint64_t n, int any
   sv.tris = h.tris.data();
   sv.qtris = h.qtris.empty() ? sv.tris : h.qtris.data();
   sv.insts = h.insts.data();
+  sv.qinsts = h.qinsts.data();
   sv.tlas_root = h.tlas_root;
   sv.qtlas_root = h.qtlas_root;
   for (int k = 0; k < 6; ++k) sv.tlas_box[k] = h.tlas_box[k];
