@@ -13,6 +13,15 @@
 #include <cstdio>
 #include <vector>
 
+// Built a second time with -DHOST_COUNTS: the walker's counting hooks then
+// count node, triangle and instance records read (host_counts).
+#ifdef HOST_COUNTS
+static unsigned long long g_counts[3];
+#define AR_COUNT_NODE() (++g_counts[0])
+#define AR_COUNT_TRI() (++g_counts[1])
+#define AR_COUNT_INST() (++g_counts[2])
+#endif
+
 #include "../../agent-ray_amd/csrc/ar_internal.h"
 #include "../../agent-ray_amd/csrc/ar_traverse.h"
 
@@ -235,3 +244,14 @@ extern "C" int host_record_counts(ar_scene* s, uint64_t* counts) {
   counts[1] = h.qtris.size();
   return 0;
 }
+
+#ifdef HOST_COUNTS
+// counts[0..2] = node, triangle and instance records read since the last call
+// (both drivers of host_traverse count), then reset.
+extern "C" void host_counts(unsigned long long* counts) {
+  for (int i = 0; i < 3; ++i) {
+    counts[i] = g_counts[i];
+    g_counts[i] = 0;
+  }
+}
+#endif

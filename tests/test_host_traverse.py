@@ -31,18 +31,25 @@ OUT = os.path.join(ROOT, "tests", "host", "_build", "libhost_traverse.so")
 LIB = os.path.join(ROOT, "agent-ray_amd", "libagentray.so")
 
 
-def _helper():
+def _helper(counts=False):
+    """The host harness library; with counts=True the build whose walker
+    counts the records it reads (host_counts)."""
+    out = OUT.replace(".so", "_counts.so") if counts else OUT
     deps = [SRC, LIB] + [os.path.join(CSRC, f) for f in ("ar_traverse.h", "ar_device.h", "ar_internal.h")]
-    if not os.path.exists(OUT) or os.path.getmtime(OUT) < max(os.path.getmtime(d) for d in deps):
+    if not os.path.exists(out) or os.path.getmtime(out) < max(os.path.getmtime(d) for d in deps):
         if not os.path.exists("/opt/rocm/include/hip/hip_runtime.h"):
             pytest.skip("HIP headers not installed")
-        os.makedirs(os.path.dirname(OUT), exist_ok=True)
+        os.makedirs(os.path.dirname(out), exist_ok=True)
         subprocess.check_call(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
-                               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"),
-                               SRC, "-o", OUT + ".tmp", "-L" + os.path.dirname(LIB), "-lagentray",
-                               "-Wl,-rpath," + os.path.dirname(LIB)])
-        os.replace(OUT + ".tmp", OUT)
-    h = C.CDLL(OUT)
+                               "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include")]
+                              + (["-DHOST_COUNTS"] if counts else [])
+                              + [SRC, "-o", out + ".tmp", "-L" + os.path.dirname(LIB), "-lagentray",
+                                 "-Wl,-rpath," + os.path.dirname(LIB)])
+        os.replace(out + ".tmp", out)
+    h = C.CDLL(out)
+    if counts:
+        h.host_counts.argtypes = [C.POINTER(C.c_ulonglong)]
+        h.host_counts.restype = None
     h.host_traverse.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64] + [C.c_int] * 4 + [C.c_void_p] * 6
     h.host_traverse.restype = C.c_int
     h.host_chunk_schedule.argtypes = [C.c_uint64, C.c_uint32, C.c_int, C.c_uint32, C.c_uint64,
