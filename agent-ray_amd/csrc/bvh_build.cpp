@@ -10,6 +10,7 @@
 // list is sorted once and stably partitioned at every split, which keeps each
 // list sorted: O(n) work per tree level.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <future>
 #include <numeric>
@@ -244,8 +245,9 @@ inline float overlap_area(const HBox& a, const HBox& b) {
 }
 
 struct SplitBuild {
-  const float* pos;
-  const uint32_t* idx;
+  // per triangle: its three vertices, contiguous (the clip loop reads them
+  // once per reference and axis instead of through the index array)
+  std::vector<std::array<float, 9>> tv;
   std::vector<float> margin;  // per triangle: 2^-16 of its extent
   float min_overlap = 0.0f;   // area of overlap that makes spatial splits worth trying
 
@@ -257,12 +259,20 @@ struct SplitBuild {
     uint32_t depth = 1;
   };
 
-  // Parts of reference r's triangle inside r.box on each side of the plane
-  // x_axis = p; a side the part does not reach comes back empty (!box_ok).
+  void load(uint32_t tri, double v[3][3]) const {
+    const std::array<float, 9>& t = tv[tri];
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 3; ++k) v[c][k] = t[3 * c + k];
+  }
+  // Parts of reference r's triangle (vertices v) inside r.box on each side
+  // of the plane x_axis = p; a side the part does not reach comes back empty
+  // (!box_ok).
   void clip(const SRef& r, int axis, float p, HBox& L, HBox& R) const {
     double v[3][3];
-    for (int c = 0; c < 3; ++c)
-      for (int k = 0; k < 3; ++k) v[c][k] = pos[3 * (size_t)idx[3 * (size_t)r.tri + c] + k];
+    load(r.tri, v);
+    clip(v, r, axis, p, L, R);
+  }
+  void clip(const double v[3][3], const SRef& r, int axis, float p, HBox& L, HBox& R) const {
     double llo[3], lhi[3], rlo[3], rhi[3];
     for (int k = 0; k < 3; ++k) {
       llo[k] = rlo[k] = INFINITY;
@@ -435,9 +445,11 @@ struct SplitBuild {
         ++leave[b1];
         SRef cur = r;
         uint32_t b = b0;
+        double v[3][3];
+        if (b0 < b1) load(r.tri, v);
         for (; b < b1; ++b) {
           HBox L, R;
-          clip(cur, axis, plane[b + 1], L, R);
+          clip(v, cur, axis, plane[b + 1], L, R);
           if (box_ok(L)) merge_box(bb[b], L);
           if (!box_ok(R)) break;
           cur.box = R;
@@ -605,8 +617,10 @@ BinTree build_split_sah(const std::vector<HBox>& boxes, const float* pos, const 
   const uint32_t n = (uint32_t)boxes.size();
   if (n == 0) return t;
   SplitBuild sb;
-  sb.pos = pos;
-  sb.idx = idx;
+  sb.tv.resize(n);
+  for (uint32_t i = 0; i < n; ++i)
+    for (int c = 0; c < 3; ++c)
+      for (int k = 0; k < 3; ++k) sb.tv[i][3 * c + k] = pos[3 * (size_t)idx[3 * (size_t)i + c] + k];
   sb.margin.resize(n);
   std::vector<SRef> refs(n);
   HBox root = empty_box();
